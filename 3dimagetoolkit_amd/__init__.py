@@ -1,0 +1,355 @@
+"""Python binding of libtk.so, the MI355X RSFE Split-Bregman segmentation library.
+
+A thin ctypes layer over the C ABI declared in include/tk.h.  Volumes are numpy
+arrays of shape (nx, ny, nz) in C order, i.e. index (i*ny + j)*nz + k with z
+fastest -- the layout of the reference's im3d::image3d (image3d_imp.hxx:141-152).
+
+The package directory name starts with a digit, so import it with
+``importlib.import_module("3dimagetoolkit_amd")``.
+
+There is no CPU fallback: if libtk.so is missing or cannot be loaded, importing
+the binding raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtk.so")
+
+TK_OK = 0
+TK_ERR_ARG, TK_ERR_HIP, TK_ERR_NOMEM, TK_ERR_SHAPE = -1, -2, -3, -4
+TK_ERR_DTYPE, TK_ERR_UNSUPPORTED, TK_ERR_SWEEPS = -5, -6, -7
+TK_F32, TK_F64, TK_U8 = 1, 2, 3
+TK_GS_LEX, TK_GS_REDBLACK = 0, 1
+# tk_kernel_tag order
+KERNEL_TAGS = ("conv_x1", "conv_y", "conv_z2", "conv_x2", "gs", "sb", "div", "other")
+
+_NP2TK = {np.dtype(np.float32): TK_F32, np.dtype(np.float64): TK_F64, np.dtype(np.uint8): TK_U8}
+_TK2NP = {v: k for k, v in _NP2TK.items()}
+
+
+class TkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tk error {code}: {msg}")
+        self.code = code
+
+
+class RsfeParams(C.Structure):
+    """tk_rsfe_params (include/tk.h); defaults of rsfe_splitbregman_imp.hxx:25-41."""
+    _fields_ = [(n, C.c_double) for n in (
+        "sigma", "lamda", "lamda1", "lamda2", "beta", "dt", "epsilon", "a0", "b0", "nu",
+        "tol", "ls_tol", "radius")] + [(n, C.c_int) for n in (
+            "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "gs_order",
+            "max_sweeps", "warm_start")]
+
+
+class IterStats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("ls_err", C.c_double), ("tollerance", C.c_double),
+                ("guess_used", C.c_int)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python 3dimagetoolkit_amd/build.py")
+    lib = C.CDLL(LIB_PATH)
+    vp, ip, dp, i, d = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_double
+    u64 = C.c_uint64
+    sig = {
+        "tk_last_error": (C.c_char_p, []),
+        "tk_version": (C.c_char_p, []),
+        "tk_ctx_create": (i, [i, C.POINTER(vp)]),
+        "tk_ctx_destroy": (i, [vp]),
+        "tk_ctx_sync": (i, [vp]),
+        "tk_ctx_set_stream": (i, [vp, vp]),
+        "tk_ctx_stream": (vp, [vp]),
+        "tk_ctx_set_timing": (i, [vp, i]),
+        "tk_ctx_kernel_times": (i, [vp, dp, C.POINTER(C.c_long), i]),
+        "tk_vol_alloc": (i, [vp, i, i, i, d, d, d, i, C.POINTER(vp)]),
+        "tk_vol_wrap": (i, [vp, vp, i, i, i, d, d, d, i, C.POINTER(vp)]),
+        "tk_vol_free": (i, [vp]),
+        "tk_vol_info": (i, [vp, ip, dp, ip]),
+        "tk_vol_device_ptr": (vp, [vp]),
+        "tk_vol_upload": (i, [vp, vp, i]),
+        "tk_vol_download": (i, [vp, vp, i]),
+        "tk_vol_fill": (i, [vp, d]),
+        "tk_vol_copy": (i, [vp, vp]),
+        "tk_filter_gaussian": (i, [vp, i, i, i, d, d, d, d, d, i, i, C.POINTER(vp)]),
+        "tk_filter_free": (i, [vp]),
+        "tk_filter_taps": (i, [vp, ip]),
+        "tk_filter_axis_spectrum": (i, [vp, i, dp]),
+        "tk_conv_cyclic": (i, [vp, vp, vp, vp]),
+        "tk_conv_cyclic_pair": (i, [vp, vp, vp, vp, vp, vp]),
+        "tk_poisson_dir_sweeps": (i, [vp, vp, vp, d, d, i, i]),
+        "tk_poisson_dir_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
+        "tk_poisson_residual": (i, [vp, vp, vp, dp]),
+        "tk_heaviside": (i, [vp, vp, d, d, d, vp]),
+        "tk_grad": (i, [vp, vp, vp, vp, vp]),
+        "tk_div": (i, [vp, vp, vp, vp, vp]),
+        "tk_edge_detect": (i, [vp, vp, d, d, d, vp]),
+        "tk_shrink": (i, [vp, vp, vp, vp]),
+        "tk_norm_l2": (i, [vp, vp, dp]),
+        "tk_norm_l2_diff": (i, [vp, vp, vp, dp]),
+        "tk_minmax": (i, [vp, vp, dp, dp]),
+        "tk_change_range_of_intensity": (i, [vp, vp, d, d]),
+        "tk_cut": (i, [vp, vp, d, d]),
+        "tk_im_to_black_and_white": (i, [vp, vp, d, i, vp]),
+        "tk_cc_seeded": (i, [vp, vp, i, i, i, d, i, vp, ip]),
+        "tk_cc_label": (i, [vp, vp, vp, ip]),
+        "tk_rsfe_default_params": (i, [C.POINTER(RsfeParams)]),
+        "tk_rsfe_create": (i, [vp, vp, vp, C.POINTER(RsfeParams), C.POINTER(vp)]),
+        "tk_rsfe_iterate": (i, [vp, i, C.POINTER(IterStats), ip]),
+        "tk_rsfe_phi": (vp, [vp]),
+        "tk_rsfe_destroy": (i, [vp]),
+        "tk_batch_create": (i, [vp, i, ip, i, C.POINTER(RsfeParams), C.POINTER(vp)]),
+        "tk_batch_set_volume": (i, [vp, i, vp, vp, i]),
+        "tk_batch_iterate": (i, [vp, i, i]),
+        "tk_batch_get_phi": (i, [vp, i, vp, i]),
+        "tk_batch_destroy": (i, [vp]),
+        "tk_gen_ellipsoids": (i, [i, i, i, u64, i, dp, i, d, dp, dp]),
+        "tk_gen_standard": (i, [i, i, i, u64, i, i, dp, dp]),
+        "tk_gen_batch_dims": (i, [u64, i, ip, ip]),
+        "tk_gen_ct": (i, [i, i, i, u64, i, dp, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != TK_OK:
+        raise TkError(rc, lib.tk_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---- synthetic volumes (host) ----------------------------------------------------
+def gen_standard(dims, seed, scale=1, n_ell=2):
+    """The standard ellipsoid phantom and its sphere initial contour (float64)."""
+    nx, ny, nz = dims
+    img = np.empty((nx, ny, nz), np.float64)
+    phi0 = np.empty((nx, ny, nz), np.float64)
+    _check(lib.tk_gen_standard(nx, ny, nz, seed, scale, n_ell, _dptr(img), _dptr(phi0)))
+    return img, phi0
+
+
+def gen_ct(dims, seed, n_vessels=12):
+    nx, ny, nz = dims
+    img = np.empty((nx, ny, nz), np.float64)
+    sd = np.zeros(3, np.int32)
+    _check(lib.tk_gen_ct(nx, ny, nz, seed, n_vessels, _dptr(img),
+                         sd.ctypes.data_as(C.POINTER(C.c_int))))
+    return img, tuple(int(x) for x in sd)
+
+
+def gen_batch_dims(seed, n_vol):
+    dims = np.zeros((n_vol, 3), np.int32)
+    nell = np.zeros(n_vol, np.int32)
+    _check(lib.tk_gen_batch_dims(seed, n_vol, dims.ctypes.data_as(C.POINTER(C.c_int)),
+                                 nell.ctypes.data_as(C.POINTER(C.c_int))))
+    return dims, nell
+
+
+def default_params(**kw):
+    p = RsfeParams()
+    _check(lib.tk_rsfe_default_params(C.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+# ---- device objects ------------------------------------------------------------------
+class Context:
+    """tk_ctx: one device and one HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib.tk_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        _check(lib.tk_ctx_sync(self.h))
+
+    def set_timing(self, on):
+        _check(lib.tk_ctx_set_timing(self.h, int(bool(on))))
+
+    def kernel_times(self, reset=True):
+        """{tag: (total ms, launches)} from HIP events on this context's stream."""
+        ms = (C.c_double * len(KERNEL_TAGS))()
+        n = (C.c_long * len(KERNEL_TAGS))()
+        _check(lib.tk_ctx_kernel_times(self.h, ms, n, int(bool(reset))))
+        return {t: (ms[q], n[q]) for q, t in enumerate(KERNEL_TAGS)}
+
+    def close(self):
+        if self.h:
+            lib.tk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # convenience constructors
+    def volume(self, shape, dtype=np.float64, h=(1.0, 1.0, 1.0)):
+        return Volume(self, shape, dtype, h)
+
+    def from_numpy(self, a, dtype=None, h=(1.0, 1.0, 1.0)):
+        a = np.ascontiguousarray(a)
+        v = Volume(self, a.shape, dtype or a.dtype, h)
+        v.upload(a)
+        return v
+
+
+class Volume:
+    """tk_vol: a device-resident volume (im3d::image3d storage)."""
+
+    def __init__(self, ctx, shape, dtype=np.float64, h=(1.0, 1.0, 1.0)):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        hv = C.c_void_p()
+        _check(lib.tk_vol_alloc(ctx.h, *self.shape, *[float(x) for x in h],
+                                _NP2TK[self.dtype], C.byref(hv)))
+        self.h = hv
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        _check(lib.tk_vol_upload(self.h, _ptr(a), _NP2TK[a.dtype]))
+        return self
+
+    def numpy(self, dtype=None):
+        out = np.empty(self.shape, np.dtype(dtype) if dtype else self.dtype)
+        _check(lib.tk_vol_download(self.h, _ptr(out), _NP2TK[out.dtype]))
+        return out
+
+    def fill(self, value):
+        _check(lib.tk_vol_fill(self.h, float(value)))
+        return self
+
+    def free(self):
+        if self.h:
+            lib.tk_vol_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Filter:
+    """tk_filter: Gaussian time filter and cyclic spectrum (conv::filtering statics)."""
+
+    def __init__(self, ctx, shape, sigma, radius=3.0, pixel_approach=False, dtype=np.float64,
+                 h=(1.0, 1.0, 1.0)):
+        hv = C.c_void_p()
+        _check(lib.tk_filter_gaussian(ctx.h, *shape, *h, float(sigma), float(radius),
+                                      int(bool(pixel_approach)), _NP2TK[np.dtype(dtype)],
+                                      C.byref(hv)))
+        self.h = hv
+        self.ctx = ctx
+        self.shape = tuple(shape)
+
+    def taps(self):
+        nh = (C.c_int * 3)()
+        _check(lib.tk_filter_taps(self.h, nh))
+        return tuple(nh)
+
+    def axis_spectrum(self, axis):
+        out = np.empty(self.shape[axis], np.float64)
+        _check(lib.tk_filter_axis_spectrum(self.h, axis, _dptr(out)))
+        return out
+
+    def conv(self, out, inp):
+        _check(lib.tk_conv_cyclic(self.ctx.h, self.h, out.h, inp.h))
+        return out
+
+    def conv_pair(self, oa, ob, ia, ib):
+        _check(lib.tk_conv_cyclic_pair(self.ctx.h, self.h, oa.h, ob.h, ia.h, ib.h))
+
+    def free(self):
+        if self.h:
+            lib.tk_filter_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def poisson_sweeps(ctx, phi, rhs, n, dt=1.0, bc=0.0, order=TK_GS_LEX):
+    _check(lib.tk_poisson_dir_sweeps(ctx.h, phi.h, rhs.h, dt, bc, order, n))
+
+
+def poisson_solve(ctx, phi, rhs, ls_steps, ls_tol, dt=1.0, bc=0.0, order=TK_GS_LEX,
+                  max_sweeps=0, allow_cap=False):
+    s = C.c_int()
+    e = C.c_double()
+    rc = lib.tk_poisson_dir_solve(ctx.h, phi.h, rhs.h, dt, bc, order, ls_steps, ls_tol,
+                                  max_sweeps, C.byref(s), C.byref(e))
+    if not (allow_cap and rc == TK_ERR_SWEEPS):
+        _check(rc)
+    return s.value, e.value
+
+
+def poisson_residual(ctx, phi, rhs):
+    r = C.c_double()
+    _check(lib.tk_poisson_residual(ctx.h, phi.h, rhs.h, C.byref(r)))
+    return r.value
+
+
+class Rsfe:
+    """tk_rsfe: one segm::rsfe_splitbregman run resident on the device."""
+
+    def __init__(self, ctx, u0, init, params):
+        hv = C.c_void_p()
+        _check(lib.tk_rsfe_create(ctx.h, u0.h, init.h, C.byref(params), C.byref(hv)))
+        self.h = hv
+        self.ctx = ctx
+        self.shape = u0.shape
+        self.dtype = u0.dtype
+
+    def iterate(self, n):
+        st = (IterStats * max(n, 1))()
+        done = C.c_int()
+        _check(lib.tk_rsfe_iterate(self.h, n, st, C.byref(done)))
+        return [dict(sweeps=st[q].sweeps, ls_err=st[q].ls_err, tollerance=st[q].tollerance,
+                     guess_used=st[q].guess_used) for q in range(done.value)]
+
+    def phi(self, dtype=np.float64):
+        out = np.empty(self.shape, dtype)
+        _check(lib.tk_vol_download(lib.tk_rsfe_phi(self.h), _ptr(out), _NP2TK[np.dtype(dtype)]))
+        return out
+
+    def close(self):
+        if self.h:
+            lib.tk_rsfe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

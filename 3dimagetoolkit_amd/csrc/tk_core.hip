@@ -1,0 +1,747 @@
+// tk_core.hip -- context, volumes, conversions and the reductions/pointwise
+// operators of im3d::image3d used on the Split-Bregman path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+#include "tk_device.hpp"
+#include "tk_internal.hpp"
+
+namespace tk {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int ctx_work(tk_ctx ctx, int slot, size_t bytes, void** out)
+{
+    if (ctx->work_bytes[slot] < bytes) {
+        if (ctx->work[slot]) {
+            TK_HIP(hipStreamSynchronize(ctx->stream));
+            TK_HIP(hipFree(ctx->work[slot]));
+            ctx->work[slot] = nullptr;
+            ctx->work_bytes[slot] = 0;
+        }
+        TK_HIP(hipMalloc(&ctx->work[slot], bytes));
+        ctx->work_bytes[slot] = bytes;
+    }
+    *out = ctx->work[slot];
+    return TK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <class D, class S>
+__global__ void k_convert(D* __restrict__ dst, const S* __restrict__ src, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = (D)src[i];
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ a, T v, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+// partial sums of (a-b)^2 and b^2 (b may be null: then a^2 and 0)
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_sumsq_partial(const T* __restrict__ a,
+                                                               const T* __restrict__ b, long n,
+                                                               double* __restrict__ part)
+{
+    double s1 = 0, s2 = 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        double x = (double)a[i];
+        if (b) {
+            double y = (double)b[i];
+            double d = x - y;
+            s1 += d * d;
+            s2 += y * y;
+        } else {
+            s1 += x * x;
+        }
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+// fixed-order final sum of `nb` pairs (deterministic)
+__global__ __launch_bounds__(1024) void k_sum_pairs(const double* __restrict__ part, int nb,
+                                                    double* __restrict__ out)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        s1 += part[2 * i];
+        s2 += part[2 * i + 1];
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_minmax_partial(const T* __restrict__ a, long n,
+                                                                double* __restrict__ part)
+{
+    __shared__ double smin[kRedThreads], smax[kRedThreads];
+    double mn = (double)a[0], mx = (double)a[0];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        double x = (double)a[i];
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + o]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smin[0];
+        part[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+__global__ void k_minmax_final(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double mn = part[0], mx = part[1];
+        for (int i = 1; i < nb; ++i) {
+            mn = fmin(mn, part[2 * i]);
+            mx = fmax(mx, part[2 * i + 1]);
+        }
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
+// image3d_imp.hxx:1627-1635 as one pass: x = (x - mn) / (mx' / (max - min)) + min,
+// where mx' = max of (x - mn) -- computed by the caller from the same min/max pass
+template <class T>
+__global__ void k_change_range(T* __restrict__ a, long n, T mn, T div, T add)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        T x = a[i];
+        x -= mn;
+        x /= div;
+        x += add;
+        a[i] = x;
+    }
+}
+
+template <class T>
+__global__ void k_max_shifted_partial(const T* __restrict__ a, long n, T mn, double* part)
+{
+    __shared__ double smax[kRedThreads];
+    double mx = (double)(a[0] - mn);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        double x = (double)(T)(a[i] - mn);
+        mx = x > mx ? x : mx;
+    }
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smax[0];
+        part[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+// rsfe_splitbregman_imp.hxx:320-339
+template <class T>
+__global__ void k_cut(T* __restrict__ a, long n, T a0, T b0)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        T x = a[i];
+        if (x < a0) x = a0;
+        else if (x > b0) x = b0;
+        a[i] = x;
+    }
+}
+
+// rsfe_splitbregman_imp.hxx:174-192
+template <class T>
+__global__ void k_heaviside(const T* __restrict__ phi, T* __restrict__ H, long n, T center, T eps)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        H[i] = (T)(0.5 * (1. + 2. / M_PI * atan((double)((phi[i] - center) / eps))));
+}
+
+__global__ void k_heaviside_f32(const float* __restrict__ phi, float* __restrict__ H, long n,
+                                float center, float eps)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        H[i] = 0.5f * (1.f + (float)(2. / M_PI) * atanf((phi[i] - center) / eps));
+}
+
+// rsfe_splitbregman_imp.hxx:343-365
+template <class T>
+__global__ void k_shrink(const T* __restrict__ b, const T* __restrict__ g, T* __restrict__ d, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        T f = b[i];
+        T a = fabs(f) - g[i];
+        T z = 0;
+        d[i] = (T)((int)(f > 0) - (int)(f < 0)) * ((a < z) ? z : a);
+    }
+}
+
+// image3d_imp.hxx:1591-1623
+template <class T>
+__global__ void k_bw(const T* __restrict__ a, uint8_t* __restrict__ out, long n, T pivot, int neg)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = (a[i] < pivot) ? (uint8_t)(0 + neg) : (uint8_t)(1 - neg);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+int reduce_sumsq(tk_ctx ctx, tk_vol a, tk_vol b, double* s1, double* s2)
+{
+    long n = a->n();
+    int nb = (int)blocks_for(n, kRedThreads, kRedBlocks);
+    double* part = ctx->acc + 64;  // slots [64, 64 + 2*kRedBlocks)
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL(k_sumsq_partial<double>, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                           (const double*)a->d, b ? (const double*)b->d : nullptr, n, part);
+    else
+        hipLaunchKernelGGL(k_sumsq_partial<float>, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                           (const float*)a->d, b ? (const float*)b->d : nullptr, n, part);
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(1024), 0, ctx->stream, part, nb, ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    *s1 = ctx->h_acc[0];
+    if (s2) *s2 = ctx->h_acc[1];
+    return TK_OK;
+}
+
+static int minmax(tk_ctx ctx, tk_vol a, double* mn, double* mx)
+{
+    long n = a->n();
+    int nb = (int)blocks_for(n, kRedThreads, kRedBlocks);
+    double* part = ctx->acc + 64;
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL(k_minmax_partial<double>, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                           (const double*)a->d, n, part);
+    else if (a->dtype == TK_F32)
+        hipLaunchKernelGGL(k_minmax_partial<float>, dim3(nb), dim3(kRedThreads), 0, ctx->stream,
+                           (const float*)a->d, n, part);
+    else
+        TK_CHECK(false, TK_ERR_DTYPE, "minmax: U8 volume");
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, part, nb, ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    *mn = ctx->h_acc[0];
+    *mx = ctx->h_acc[1];
+    return TK_OK;
+}
+
+// ---- per-kernel event timing -------------------------------------------------------
+constexpr size_t kMaxPending = 4096;
+
+int timing_flush(tk_ctx ctx)
+{
+    if (ctx->pending.empty()) return TK_OK;
+    TK_HIP(hipEventSynchronize(ctx->pending.back().b));
+    for (auto& p : ctx->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ctx->tag_ms[p.tag] += ms;
+            ctx->tag_n[p.tag] += 1;
+        }
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return TK_OK;
+}
+
+static hipEvent_t take_event(tk_ctx ctx)
+{
+    if (ctx->ev_pool.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+}
+
+int timing_begin(tk_ctx ctx, int tag)
+{
+    if (!ctx->timing) return -1;
+    if (ctx->pending.size() >= kMaxPending) timing_flush(ctx);
+    hipEvent_t a = take_event(ctx), b = take_event(ctx);
+    if (!a || !b) return -1;
+    if (hipEventRecord(a, ctx->stream) != hipSuccess) return -1;
+    ctx->pending.push_back({a, b, tag});
+    return (int)ctx->pending.size() - 1;
+}
+
+void timing_end(tk_ctx ctx, int slot)
+{
+    if (slot < 0 || slot >= (int)ctx->pending.size()) return;
+    (void)hipEventRecord(ctx->pending[slot].b, ctx->stream);
+}
+
+int launch_cut(tk_ctx ctx, tk_vol phi, double a0, double b0)
+{
+    long n = phi->n();
+    unsigned nb = blocks_for(n, 256, 8192);
+    if (phi->dtype == TK_F64)
+        hipLaunchKernelGGL(k_cut<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)phi->d, n,
+                           a0, b0);
+    else
+        hipLaunchKernelGGL(k_cut<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)phi->d, n,
+                           (float)a0, (float)b0);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+#define CHECK_VOL(v) TK_CHECK((v) && (v)->d, TK_ERR_ARG, "%s: null volume", __func__)
+#define CHECK_FP(v) \
+    TK_CHECK((v)->dtype == TK_F32 || (v)->dtype == TK_F64, TK_ERR_DTYPE, "%s: not FP", __func__)
+
+extern "C" {
+
+const char* tk_last_error(void) { return tk::g_err; }
+const char* tk_version(void) { return "tk 0.1 gfx950"; }
+
+int tk_ctx_create(int device, tk_ctx* out)
+{
+    TK_CHECK(out, TK_ERR_ARG, "tk_ctx_create: null out");
+    int ndev = 0;
+    TK_HIP(hipGetDeviceCount(&ndev));
+    TK_CHECK(device >= 0 && device < ndev, TK_ERR_ARG, "tk_ctx_create: device %d of %d", device,
+             ndev);
+    TK_HIP(hipSetDevice(device));
+    tk_ctx c = new tk_ctx_s();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->acc, sizeof(double) * kAccSlots) != hipSuccess ||
+        hipMalloc(&c->cnt, sizeof(unsigned) * kCntSlots) != hipSuccess ||
+        hipHostMalloc(&c->h_acc, sizeof(double) * kAccSlots, hipHostMallocDefault) != hipSuccess) {
+        set_error("tk_ctx_create: allocation failed");
+        tk_ctx_destroy(c);
+        return TK_ERR_NOMEM;
+    }
+    c->stream = c->own_stream;
+    TK_HIP(hipMemsetAsync(c->acc, 0, sizeof(double) * kAccSlots, c->stream));
+    TK_HIP(hipMemsetAsync(c->cnt, 0, sizeof(unsigned) * kCntSlots, c->stream));
+    TK_HIP(hipStreamSynchronize(c->stream));
+    *out = c;
+    return TK_OK;
+}
+
+int tk_ctx_destroy(tk_ctx c)
+{
+    if (!c) return TK_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 4; ++i)
+        if (c->work[i]) (void)hipFree(c->work[i]);
+    if (c->acc) (void)hipFree(c->acc);
+    if (c->cnt) (void)hipFree(c->cnt);
+    if (c->h_acc) (void)hipHostFree(c->h_acc);
+    timing_flush(c);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return TK_OK;
+}
+
+int tk_ctx_sync(tk_ctx c)
+{
+    TK_CHECK(c, TK_ERR_ARG, "tk_ctx_sync: null ctx");
+    TK_HIP(hipStreamSynchronize(c->stream));
+    return timing_flush(c);
+}
+
+int tk_ctx_set_timing(tk_ctx c, int enable)
+{
+    TK_CHECK(c, TK_ERR_ARG, "tk_ctx_set_timing: null ctx");
+    TK_TRY(timing_flush(c));
+    c->timing = enable != 0;
+    return TK_OK;
+}
+
+int tk_ctx_kernel_times(tk_ctx c, double* ms, long* n, int reset)
+{
+    TK_CHECK(c, TK_ERR_ARG, "tk_ctx_kernel_times: null ctx");
+    TK_TRY(timing_flush(c));
+    for (int t = 0; t < TK_K_COUNT; ++t) {
+        if (ms) ms[t] = c->tag_ms[t];
+        if (n) n[t] = c->tag_n[t];
+        if (reset) {
+            c->tag_ms[t] = 0;
+            c->tag_n[t] = 0;
+        }
+    }
+    return TK_OK;
+}
+
+int tk_ctx_set_stream(tk_ctx c, void* s)
+{
+    TK_CHECK(c, TK_ERR_ARG, "tk_ctx_set_stream: null ctx");
+    TK_HIP(hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return TK_OK;
+}
+
+void* tk_ctx_stream(tk_ctx c) { return c ? (void*)c->stream : nullptr; }
+
+int tk_vol_alloc(tk_ctx ctx, int nx, int ny, int nz, double hx, double hy, double hz,
+                 tk_dtype dtype, tk_vol* out)
+{
+    TK_CHECK(ctx && out, TK_ERR_ARG, "tk_vol_alloc: null argument");
+    TK_CHECK(nx > 0 && ny > 0 && nz > 0, TK_ERR_ARG, "tk_vol_alloc: bad dims %d %d %d", nx, ny,
+             nz);
+    TK_CHECK(dtype == TK_F32 || dtype == TK_F64 || dtype == TK_U8, TK_ERR_ARG,
+             "tk_vol_alloc: bad dtype");
+    tk_vol v = new tk_vol_s();
+    v->ctx = ctx;
+    v->nx = nx; v->ny = ny; v->nz = nz;
+    v->hx = hx; v->hy = hy; v->hz = hz;
+    v->dtype = dtype;
+    // image3d storage is value-initialised (std::vector::resize, image3d_imp.hxx:40)
+    if (hipMalloc(&v->d, v->bytes()) != hipSuccess) {
+        delete v;
+        set_error("tk_vol_alloc: hipMalloc of %zu bytes failed", (size_t)nx * ny * nz * dtype_size(dtype));
+        return TK_ERR_NOMEM;
+    }
+    if (hipMemsetAsync(v->d, 0, v->bytes(), ctx->stream) != hipSuccess) {
+        (void)hipFree(v->d);
+        delete v;
+        set_error("tk_vol_alloc: memset failed");
+        return TK_ERR_HIP;
+    }
+    *out = v;
+    return TK_OK;
+}
+
+int tk_vol_wrap(tk_ctx ctx, void* p, int nx, int ny, int nz, double hx, double hy, double hz,
+                tk_dtype dtype, tk_vol* out)
+{
+    TK_CHECK(ctx && out && p, TK_ERR_ARG, "tk_vol_wrap: null argument");
+    TK_CHECK(nx > 0 && ny > 0 && nz > 0, TK_ERR_ARG, "tk_vol_wrap: bad dims");
+    tk_vol v = new tk_vol_s();
+    v->ctx = ctx;
+    v->nx = nx; v->ny = ny; v->nz = nz;
+    v->hx = hx; v->hy = hy; v->hz = hz;
+    v->dtype = dtype;
+    v->d = p;
+    v->owned = false;
+    *out = v;
+    return TK_OK;
+}
+
+int tk_vol_free(tk_vol v)
+{
+    if (!v) return TK_OK;
+    if (v->owned && v->d) {
+        (void)hipStreamSynchronize(v->ctx->stream);
+        (void)hipFree(v->d);
+    }
+    delete v;
+    return TK_OK;
+}
+
+int tk_vol_info(tk_vol v, int dims[3], double h[3], tk_dtype* dtype)
+{
+    CHECK_VOL(v);
+    if (dims) { dims[0] = v->nx; dims[1] = v->ny; dims[2] = v->nz; }
+    if (h) { h[0] = v->hx; h[1] = v->hy; h[2] = v->hz; }
+    if (dtype) *dtype = v->dtype;
+    return TK_OK;
+}
+
+void* tk_vol_device_ptr(tk_vol v) { return v ? v->d : nullptr; }
+
+static int convert_on_device(tk_ctx ctx, void* dst, tk_dtype dt, const void* src, tk_dtype st,
+                             long n)
+{
+    unsigned nb = blocks_for(n, 256, 8192);
+    if (dt == TK_F64 && st == TK_F32)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)dst, (const float*)src, n);
+    else if (dt == TK_F32 && st == TK_F64)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (float*)dst, (const double*)src, n);
+    else if (dt == TK_F64 && st == TK_U8)
+        hipLaunchKernelGGL((k_convert<double, uint8_t>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)dst, (const uint8_t*)src, n);
+    else if (dt == TK_F32 && st == TK_U8)
+        hipLaunchKernelGGL((k_convert<float, uint8_t>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (float*)dst, (const uint8_t*)src, n);
+    else if (dt == TK_U8 && st == TK_F64)
+        hipLaunchKernelGGL((k_convert<uint8_t, double>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (uint8_t*)dst, (const double*)src, n);
+    else if (dt == TK_U8 && st == TK_F32)
+        hipLaunchKernelGGL((k_convert<uint8_t, float>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (uint8_t*)dst, (const float*)src, n);
+    else
+        TK_CHECK(false, TK_ERR_DTYPE, "convert: unsupported pair %d <- %d", (int)dt, (int)st);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_vol_upload(tk_vol v, const void* host, tk_dtype hd)
+{
+    CHECK_VOL(v);
+    TK_CHECK(host, TK_ERR_ARG, "tk_vol_upload: null host pointer");
+    tk_ctx ctx = v->ctx;
+    long n = v->n();
+    if (hd == v->dtype) {
+        TK_HIP(hipMemcpyAsync(v->d, host, v->bytes(), hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        void* stage;
+        TK_TRY(ctx_work(ctx, 3, (size_t)n * dtype_size(hd), &stage));
+        TK_HIP(hipMemcpyAsync(stage, host, (size_t)n * dtype_size(hd), hipMemcpyHostToDevice,
+                              ctx->stream));
+        TK_TRY(convert_on_device(ctx, v->d, v->dtype, stage, hd, n));
+    }
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
+int tk_vol_download(tk_vol v, void* host, tk_dtype hd)
+{
+    CHECK_VOL(v);
+    TK_CHECK(host, TK_ERR_ARG, "tk_vol_download: null host pointer");
+    tk_ctx ctx = v->ctx;
+    long n = v->n();
+    if (hd == v->dtype) {
+        TK_HIP(hipMemcpyAsync(host, v->d, v->bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        void* stage;
+        TK_TRY(ctx_work(ctx, 3, (size_t)n * dtype_size(hd), &stage));
+        TK_TRY(convert_on_device(ctx, stage, hd, v->d, v->dtype, n));
+        TK_HIP(hipMemcpyAsync(host, stage, (size_t)n * dtype_size(hd), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
+int tk_vol_fill(tk_vol v, double value)
+{
+    CHECK_VOL(v);
+    long n = v->n();
+    unsigned nb = blocks_for(n, 256, 8192);
+    hipStream_t s = v->ctx->stream;
+    if (v->dtype == TK_F64)
+        hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, (double*)v->d, value, n);
+    else if (v->dtype == TK_F32)
+        hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, s, (float*)v->d, (float)value, n);
+    else
+        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(nb), dim3(256), 0, s, (uint8_t*)v->d,
+                           (uint8_t)value, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_vol_copy(tk_vol dst, tk_vol src)
+{
+    CHECK_VOL(dst);
+    CHECK_VOL(src);
+    TK_CHECK(same_dims(dst, src), TK_ERR_SHAPE, "tk_vol_copy: dims differ");
+    if (dst->dtype == src->dtype) {
+        TK_HIP(hipMemcpyAsync(dst->d, src->d, dst->bytes(), hipMemcpyDeviceToDevice,
+                              dst->ctx->stream));
+        return TK_OK;
+    }
+    return convert_on_device(dst->ctx, dst->d, dst->dtype, src->d, src->dtype, dst->n());
+}
+
+int tk_norm_l2(tk_ctx ctx, tk_vol a, double* out)
+{
+    CHECK_VOL(a);
+    CHECK_FP(a);
+    TK_CHECK(out, TK_ERR_ARG, "tk_norm_l2: null out");
+    double s;
+    TK_TRY(reduce_sumsq(ctx, a, nullptr, &s, nullptr));
+    *out = std::sqrt(a->hx * a->hy * a->hz * s);
+    return TK_OK;
+}
+
+int tk_norm_l2_diff(tk_ctx ctx, tk_vol a, tk_vol b, double* out)
+{
+    CHECK_VOL(a);
+    CHECK_VOL(b);
+    CHECK_FP(a);
+    TK_CHECK(same_dims(a, b), TK_ERR_SHAPE, "tk_norm_l2_diff: dims differ");
+    TK_CHECK(a->dtype == b->dtype, TK_ERR_DTYPE, "tk_norm_l2_diff: dtypes differ");
+    double s1, s2;
+    TK_TRY(reduce_sumsq(ctx, a, b, &s1, &s2));
+    *out = std::sqrt(a->hx * a->hy * a->hz * s1);
+    return TK_OK;
+}
+
+int tk_minmax(tk_ctx ctx, tk_vol a, double* mn, double* mx)
+{
+    CHECK_VOL(a);
+    return minmax(ctx, a, mn, mx);
+}
+
+int tk_change_range_of_intensity(tk_ctx ctx, tk_vol a, double max, double min)
+{
+    CHECK_VOL(a);
+    CHECK_FP(a);
+    double mn, mx;
+    TK_TRY(minmax(ctx, a, &mn, &mx));
+    long n = a->n();
+    unsigned nb = blocks_for(n, kRedThreads, kRedBlocks);
+    // max() of the shifted image, evaluated in T like the reference
+    double* part = ctx->acc + 64;
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL(k_max_shifted_partial<double>, dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const double*)a->d, n, mn, part);
+    else
+        hipLaunchKernelGGL(k_max_shifted_partial<float>, dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const float*)a->d, n, (float)mn, part);
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, ctx->stream, part, (int)nb, ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    double smax = ctx->h_acc[1];
+    unsigned nb2 = blocks_for(n, 256, 8192);
+    if (a->dtype == TK_F64) {
+        double div = smax / (max - min);
+        hipLaunchKernelGGL(k_change_range<double>, dim3(nb2), dim3(256), 0, ctx->stream,
+                           (double*)a->d, n, mn, div, min);
+    } else {
+        float div = (float)smax / ((float)max - (float)min);
+        hipLaunchKernelGGL(k_change_range<float>, dim3(nb2), dim3(256), 0, ctx->stream,
+                           (float*)a->d, n, (float)mn, div, (float)min);
+    }
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_cut(tk_ctx ctx, tk_vol phi, double a0, double b0)
+{
+    CHECK_VOL(phi);
+    CHECK_FP(phi);
+    return launch_cut(ctx, phi, a0, b0);
+}
+
+int tk_heaviside(tk_ctx ctx, tk_vol phi, double a0, double b0, double eps, tk_vol H)
+{
+    CHECK_VOL(phi);
+    CHECK_VOL(H);
+    CHECK_FP(phi);
+    TK_CHECK(same_dims(phi, H) && phi->dtype == H->dtype, TK_ERR_SHAPE,
+             "tk_heaviside: operands differ");
+    long n = phi->n();
+    unsigned nb = blocks_for(n, 256, 8192);
+    if (phi->dtype == TK_F64)
+        hipLaunchKernelGGL(k_heaviside<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)phi->d, (double*)H->d, n, (a0 + b0) / 2., eps);
+    else
+        hipLaunchKernelGGL(k_heaviside_f32, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)phi->d, (float*)H->d, n, ((float)a0 + (float)b0) / 2.f,
+                           (float)eps);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_shrink(tk_ctx ctx, tk_vol b, tk_vol g, tk_vol d)
+{
+    CHECK_VOL(b);
+    CHECK_VOL(g);
+    CHECK_VOL(d);
+    CHECK_FP(b);
+    TK_CHECK(same_dims(b, g) && same_dims(b, d), TK_ERR_SHAPE, "tk_shrink: dims differ");
+    TK_CHECK(b->dtype == g->dtype && b->dtype == d->dtype, TK_ERR_DTYPE, "tk_shrink: dtypes");
+    long n = b->n();
+    unsigned nb = blocks_for(n, 256, 8192);
+    if (b->dtype == TK_F64)
+        hipLaunchKernelGGL(k_shrink<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)b->d, (const double*)g->d, (double*)d->d, n);
+    else
+        hipLaunchKernelGGL(k_shrink<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)b->d, (const float*)g->d, (float*)d->d, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_im_to_black_and_white(tk_ctx ctx, tk_vol img, double threshold, int negative, tk_vol out)
+{
+    CHECK_VOL(img);
+    CHECK_VOL(out);
+    CHECK_FP(img);
+    TK_CHECK(out->dtype == TK_U8 && same_dims(img, out), TK_ERR_ARG,
+             "tk_im_to_black_and_white: out must be U8 of the same dims");
+    if (threshold > 1 || threshold < 0) threshold = 0.5;
+    double mn, mx;
+    TK_TRY(minmax(ctx, img, &mn, &mx));
+    long n = img->n();
+    unsigned nb = blocks_for(n, 256, 8192);
+    if (img->dtype == TK_F64) {
+        double pivot = mn + (mx - mn) * threshold;
+        hipLaunchKernelGGL(k_bw<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)img->d, (uint8_t*)out->d, n, pivot, negative ? 1 : 0);
+    } else {
+        // T pivot(min + (max - min) * threshold): the product is formed in double
+        float pivot = (float)((float)mn + (double)((float)mx - (float)mn) * threshold);
+        hipLaunchKernelGGL(k_bw<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)img->d, (uint8_t*)out->d, n, pivot, negative ? 1 : 0);
+    }
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+}  // extern "C"

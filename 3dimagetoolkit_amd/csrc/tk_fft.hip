@@ -1,0 +1,229 @@
+// tk_fft.hip -- Gaussian filter spectra (R4) and cyclic convolution (R5).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "tk_fft_kernel.hpp"
+
+namespace tk {
+
+// radix plan for the Stockham FFT: 4s first, then 2, 3, 5; 0 if L has another factor
+int fft_plan(int L, int* radix, int* nstage)
+{
+    int n = 0, m = L;
+    while (m % 4 == 0) { radix[n++] = 4; m /= 4; }
+    while (m % 2 == 0) { radix[n++] = 2; m /= 2; }
+    while (m % 3 == 0) { radix[n++] = 3; m /= 3; }
+    while (m % 5 == 0) { radix[n++] = 5; m /= 5; }
+    if (m != 1 || n > kMaxStages) return 0;
+    *nstage = n;
+    return 1;
+}
+
+// Nh of convolution_imp.hxx:34-60 along one axis
+static int taps(int dim_is_z_of_2d, double radius, double sigma, double h)
+{
+    if (dim_is_z_of_2d) return 1;
+    int N = (int)std::ceil(2 * radius * sigma / h);
+    if (N % 2 == 0) N++;
+    return N;
+}
+
+// real spectrum of the 1-D normalised Gaussian wrapped onto length L:
+// H(k) = sum_{n=-s..s} w(n) cos(2 pi k n / L),  w = g / sum g,  g(n) = exp(-n^2 h^2 / (2 sigma^2))
+static void axis_spectrum(int L, int Nh, double sigma, double h, std::vector<double>& H)
+{
+    const int s = Nh / 2;
+    std::vector<long double> w(Nh);
+    long double sum = 0;
+    for (int n = -s; n <= s; ++n) {
+        long double e = -(long double)n * n * h * h / (2.0L * sigma * sigma);
+        w[n + s] = expl(e);
+        sum += w[n + s];
+    }
+    H.assign(L, 0.0);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < L; ++k) {
+        long double acc = 0;
+        for (int n = -s; n <= s; ++n) {
+            long m = ((long)k * n) % L;
+            if (m < 0) m += L;
+            acc += w[n + s] * cosl(two_pi * m / L);
+        }
+        H[k] = (double)(acc / sum);
+    }
+}
+
+template <class T>
+static int upload_axis(tk_filter_s* f, int a)
+{
+    const int L = f->dims[a];
+    tk::AxisDev& ax = f->ax[a];
+    ax.L = L;
+    ax.nh = f->nh[a];
+    if (L == 1) return TK_OK;
+    if (!fft_plan(L, ax.radix, &ax.nstage)) {
+        // not 2/3/5-smooth: direct circular convolution with the normalised taps
+        ax.nstage = 0;
+        const int nh = f->nh[a], s = nh / 2;
+        const double hh = f->pixel ? 1. : f->h[a];
+        std::vector<long double> w(nh);
+        long double sum = 0;
+        for (int n = -s; n <= s; ++n) {
+            w[n + s] = expl(-(long double)n * n * hh * hh / (2.0L * f->sigma * f->sigma));
+            sum += w[n + s];
+        }
+        std::vector<T> taps(nh);
+        for (int n = 0; n < nh; ++n) taps[n] = (T)(w[n] / sum);
+        TK_HIP(hipMalloc(&f->spec_dev[a], sizeof(T) * nh));
+        TK_HIP(hipMemcpy(f->spec_dev[a], taps.data(), sizeof(T) * nh, hipMemcpyHostToDevice));
+        ax.spec = f->spec_dev[a];
+        ax.tw = nullptr;
+        return TK_OK;
+    }
+    std::vector<cplx<T>> tw(L);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int t = 0; t < L; ++t) {
+        long double ang = two_pi * t / L;
+        tw[t].x = (T)cosl(ang);
+        tw[t].y = (T)(-sinl(ang));
+    }
+    std::vector<T> sp(L);
+    for (int k = 0; k < L; ++k) sp[k] = (T)((long double)f->spec_host[a][k] / L);
+    TK_HIP(hipMalloc(&f->tw_dev[a], sizeof(cplx<T>) * L));
+    TK_HIP(hipMalloc(&f->spec_dev[a], sizeof(T) * L));
+    TK_HIP(hipMemcpy(f->tw_dev[a], tw.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
+    TK_HIP(hipMemcpy(f->spec_dev[a], sp.data(), sizeof(T) * L, hipMemcpyHostToDevice));
+    ax.tw = f->tw_dev[a];
+    ax.spec = f->spec_dev[a];
+    return TK_OK;
+}
+
+// run the convolution passes over the axes of length > 1.  The first pass loads with
+// `ld`, the last stores with `st`; intermediate passes go through the complex buffer.
+template <class T, class Ld, class St>
+static int conv_passes(tk_ctx ctx, const tk_filter_s* f, Ld ld, St st)
+{
+    int axes[3], na = 0;
+    for (int a = 0; a < 3; ++a)
+        if (f->dims[a] > 1) axes[na++] = a;
+    TK_CHECK(na > 0, TK_ERR_UNSUPPORTED, "conv: 1x1x1 volume");
+    const long n = (long)f->dims[0] * f->dims[1] * f->dims[2];
+    void* wk;
+    TK_TRY(ctx_work(ctx, 0, sizeof(cplx<T>) * n, &wk));
+    cplx<T>* C = (cplx<T>*)wk;
+    for (int q = 0; q < na; ++q) {
+        const bool first = q == 0, last = q == na - 1;
+        if (first && last) TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, st)));
+        else if (first)
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, StComplex<T>{C})));
+        else if (last)
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, LdComplex<T>{C}, MdNone{}, st)));
+        else
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, LdComplex<T>{C}, MdNone{},
+                                        StComplex<T>{C})));
+    }
+    return TK_OK;
+}
+
+template <class T>
+static int conv_pair(tk_ctx ctx, const tk_filter_s* f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib)
+{
+    LdPair<T> ld{ia ? (const T*)ia->d : nullptr, ib ? (const T*)ib->d : nullptr, (T)0, (T)0};
+    StPair<T> st{oa ? (T*)oa->d : nullptr, ob ? (T*)ob->d : nullptr};
+    return conv_passes<T>(ctx, f, ld, st);
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" {
+
+int tk_filter_gaussian(tk_ctx ctx, int nx, int ny, int nz, double hx, double hy, double hz,
+                       double sigma, double radius, int pixel_approach, tk_dtype dtype,
+                       tk_filter* out)
+{
+    TK_CHECK(ctx && out, TK_ERR_ARG, "tk_filter_gaussian: null argument");
+    TK_CHECK(nx > 0 && ny > 0 && nz > 0, TK_ERR_ARG, "tk_filter_gaussian: bad dims");
+    TK_CHECK(sigma > 0 && radius > 0, TK_ERR_ARG, "tk_filter_gaussian: sigma/radius <= 0");
+    TK_CHECK(dtype == TK_F32 || dtype == TK_F64, TK_ERR_DTYPE, "tk_filter_gaussian: dtype");
+    tk_filter_s* f = new tk_filter_s();
+    f->ctx = ctx;
+    f->dims[0] = nx; f->dims[1] = ny; f->dims[2] = nz;
+    f->h[0] = hx; f->h[1] = hy; f->h[2] = hz;
+    f->sigma = sigma;
+    f->radius = radius;
+    f->pixel = pixel_approach;
+    f->dtype = dtype;
+    const double hh[3] = {pixel_approach ? 1. : hx, pixel_approach ? 1. : hy,
+                          pixel_approach ? 1. : hz};
+    for (int a = 0; a < 3; ++a) {
+        f->nh[a] = taps(a == 2 && nz == 1, radius, sigma, hh[a]);
+        axis_spectrum(f->dims[a], f->nh[a], sigma, hh[a], f->spec_host[a]);
+        int r = dtype == TK_F64 ? upload_axis<double>(f, a) : upload_axis<float>(f, a);
+        if (r != TK_OK) {
+            tk_filter_free(f);
+            return r;
+        }
+    }
+    *out = f;
+    return TK_OK;
+}
+
+int tk_filter_free(tk_filter f)
+{
+    if (!f) return TK_OK;
+    for (int a = 0; a < 3; ++a) {
+        if (f->tw_dev[a]) (void)hipFree(f->tw_dev[a]);
+        if (f->spec_dev[a]) (void)hipFree(f->spec_dev[a]);
+    }
+    delete f;
+    return TK_OK;
+}
+
+int tk_filter_taps(tk_filter f, int nh[3])
+{
+    TK_CHECK(f && nh, TK_ERR_ARG, "tk_filter_taps: null argument");
+    for (int a = 0; a < 3; ++a) nh[a] = f->nh[a];
+    return TK_OK;
+}
+
+int tk_filter_axis_spectrum(tk_filter f, int axis, double* out)
+{
+    TK_CHECK(f && out && axis >= 0 && axis < 3, TK_ERR_ARG, "tk_filter_axis_spectrum: argument");
+    for (int k = 0; k < f->dims[axis]; ++k) out[k] = f->spec_host[axis][k];
+    return TK_OK;
+}
+
+static int check_conv(tk_filter f, tk_vol v)
+{
+    TK_CHECK(v && v->d, TK_ERR_ARG, "conv: null volume");
+    TK_CHECK(v->nx == f->dims[0] && v->ny == f->dims[1] && v->nz == f->dims[2], TK_ERR_SHAPE,
+             "conv: volume %dx%dx%d vs filter %dx%dx%d", v->nx, v->ny, v->nz, f->dims[0],
+             f->dims[1], f->dims[2]);
+    TK_CHECK(v->dtype == f->dtype, TK_ERR_DTYPE, "conv: dtype differs from the filter's");
+    return TK_OK;
+}
+
+int tk_conv_cyclic(tk_ctx ctx, tk_filter f, tk_vol out, tk_vol in)
+{
+    TK_CHECK(ctx && f, TK_ERR_ARG, "tk_conv_cyclic: null argument");
+    TK_TRY(check_conv(f, out));
+    TK_TRY(check_conv(f, in));
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, out, nullptr, in, nullptr);
+    return conv_pair<float>(ctx, f, out, nullptr, in, nullptr);
+}
+
+int tk_conv_cyclic_pair(tk_ctx ctx, tk_filter f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib)
+{
+    TK_CHECK(ctx && f, TK_ERR_ARG, "tk_conv_cyclic_pair: null argument");
+    TK_TRY(check_conv(f, oa));
+    TK_TRY(check_conv(f, ob));
+    TK_TRY(check_conv(f, ia));
+    TK_TRY(check_conv(f, ib));
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, oa, ob, ia, ib);
+    return conv_pair<float>(ctx, f, oa, ob, ia, ib);
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// tk_gs.hip -- Dirichlet Gauss-Seidel sweeps for (1/dt - lap_h) phi = rhs + phi/dt
+// (lapl::DirGaussSeidel, poisson_imp.hxx:340-429) and the sweep loop of
+// rsfe_splitbregman::apply (rsfe_splitbregman_imp.hxx:1762-1797).
+//
+// TK_GS_LEX -- bit-exact lexicographic order on a GPU.  In the reference's i-j-k loop
+// every update reads new values at the minus neighbours and old values at the plus
+// neighbours.  The interior is cut into 8x8xTZ tiles; tile (I,J,K) performs sweep s
+// at pipeline step tau = I+J+K+2s.  At that step its minus-neighbour tiles finished
+// sweep s (step tau-1), its plus-neighbour tiles finished sweep s-1 (step tau-1) and
+// have not started sweep s (step tau+1), and no two face-adjacent tiles are active
+// together, so each tile sees exactly the values the serial loop would.  Inside a
+// tile, one 64-lane wave walks the skewed schedule k = t - a - b (lane (a,b) owns
+// column (a,b)), which orders the updates the same way.  C sweeps take
+// (nTx+nTy+nTz-3) + 2C - 1 steps: about two launches per sweep, each over half the
+// tiles.  Expression order and -ffp-contract=off match the reference bit for bit.
+//
+// The relative-update stop rule needs ||phi_s - phi_{s-1}|| for every sweep: each
+// tile-step belongs to one sweep, so its wave reduces (new-old)^2 and old^2 in FP64
+// and adds them to that sweep's slot.  Sweeps run in chunks; when the rule fires
+// inside a chunk the chunk is replayed from a saved copy up to the stopping sweep
+// (deterministic, hence identical).
+//
+// TK_GS_REDBLACK -- the FP32 fast mode: red (i+j+k even) then black, each pass
+// in place; FP64 per-sweep update norms; the black pass's last workgroup evaluates
+// the stop rule on the device and raises a flag that makes later sweeps return at once.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "tk_device.hpp"
+#include "tk_internal.hpp"
+
+namespace tk {
+
+struct GsCoef {
+    double hx2, hy2, hz2, dt, htilde;
+};
+
+inline GsCoef gs_coef(const tk_vol phi, double dt)
+{
+    GsCoef c;
+    c.hx2 = phi->hx * phi->hx;
+    c.hy2 = phi->hy * phi->hy;
+    c.hz2 = phi->hz * phi->hz;
+    c.dt = dt;
+    // poisson_imp.hxx:414
+    c.htilde = 1. / (1. / dt + 2. / (phi->hx * phi->hx) + 2. / (phi->hy * phi->hy) +
+                     2. / (phi->hz * phi->hz));
+    return c;
+}
+
+// poisson_imp.hxx:420-424, same operand order; UNIT skips the exact divisions by 1
+template <class T, bool UNIT>
+__device__ __forceinline__ T gs_update(T xp, T xm, T yp, T ym, T zp, T zm, T b, T self,
+                                       const GsCoef& c)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        if constexpr (UNIT) return ((xp + xm) + (yp + ym) + (zp + zm) + b + self) * c.htilde;
+        else
+            return ((xp + xm) / c.hx2 + (yp + ym) / c.hy2 + (zp + zm) / c.hz2 + b + self / c.dt) *
+                   c.htilde;
+    } else {
+        const float ht = (float)c.htilde;
+        if constexpr (UNIT) return ((xp + xm) + (yp + ym) + (zp + zm) + b + self) * ht;
+        else
+            return ((xp + xm) / (float)c.hx2 + (yp + ym) / (float)c.hy2 + (zp + zm) / (float)c.hz2 +
+                    b + self / (float)c.dt) *
+                   ht;
+    }
+}
+
+// ---- boundary: all six faces := bc, with the sweep-0 update norm of the faces -----
+template <class T>
+__global__ void k_set_boundary(T* __restrict__ phi, int X, int Y, int Z, T bc, double* __restrict__ acc)
+{
+    // enumerate the voxels of the boundary shell once each
+    const long nxy = (long)X * Y;
+    double d2 = 0, o2 = 0;
+    const long total = 2 * nxy + 2L * X * (Z - 2 > 0 ? Z - 2 : 0) +
+                       2L * (Y - 2 > 0 ? Y - 2 : 0) * (Z - 2 > 0 ? Z - 2 : 0);
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        long r = q;
+        if (r < 2 * nxy) {  // k = 0 and k = Z-1 planes
+            k = (r < nxy) ? 0 : Z - 1;
+            r %= nxy;
+            i = (int)(r / Y);
+            j = (int)(r % Y);
+        } else {
+            r -= 2 * nxy;
+            const int zi = Z - 2;
+            const long nxz = (long)X * zi;
+            if (r < 2 * nxz) {  // j = 0 and j = Y-1, k interior
+                j = (r < nxz) ? 0 : Y - 1;
+                r %= nxz;
+                i = (int)(r / zi);
+                k = 1 + (int)(r % zi);
+            } else {            // i = 0 and i = X-1, j and k interior
+                r -= 2 * nxz;
+                const long nyz = (long)(Y - 2) * zi;
+                i = (r < nyz) ? 0 : X - 1;
+                r %= nyz;
+                j = 1 + (int)(r / zi);
+                k = 1 + (int)(r % zi);
+            }
+        }
+        const long v = ((long)i * Y + j) * Z + k;
+        const double old = (double)phi[v];
+        const double d = (double)bc - old;
+        d2 += d * d;
+        o2 += old * old;
+        phi[v] = bc;
+    }
+    if (acc) {
+        block_sum2(d2, o2);
+        if (threadIdx.x == 0) {
+            atomicAdd(&acc[0], d2);
+            atomicAdd(&acc[1], o2);
+        }
+    }
+}
+
+inline long boundary_count(int X, int Y, int Z)
+{
+    const long zi = Z - 2 > 0 ? Z - 2 : 0, yi = Y - 2 > 0 ? Y - 2 : 0;
+    return 2L * X * Y + 2L * X * zi + 2L * yi * zi;
+}
+
+// ---- lexicographic pipelined tile wavefront --------------------------------------
+constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
+constexpr int kLexTZ = 32;  // tile depth in z
+
+template <class T, bool UNIT>
+__global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __restrict__ rhs,
+                                               int X, int Y, int Z, int nTx, int nTy, int nTz,
+                                               int tau, int C, GsCoef c, double* __restrict__ acc)
+{
+    constexpr int TX = kLexT, TY = kLexT, TZ = kLexTZ;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    __shared__ T ph[HX * HY * HZ];
+    __shared__ T rb[TX * TY * TZ];
+    const int tile = blockIdx.x;
+    const int K = tile % nTz, J = (tile / nTz) % nTy, I = tile / (nTz * nTy);
+    const int s2 = tau - (I + J + K);
+    if (s2 < 0 || (s2 & 1) || (s2 >> 1) >= C) return;
+    const int s = s2 >> 1;
+    const int i0 = 1 + I * TX, j0 = 1 + J * TY, k0 = 1 + K * TZ;
+    const int tx = min(TX, X - 1 - i0), ty = min(TY, Y - 1 - j0), tz = min(TZ, Z - 1 - k0);
+    const long sY = Z, sX = (long)Y * Z;
+    const int lane = threadIdx.x;
+    // load phi tile + halo and the rhs tile (rows along z are contiguous)
+    const int hz = tz + 2;
+    for (int e = lane; e < (tx + 2) * (ty + 2) * hz; e += 64) {
+        const int kk = e % hz, r = e / hz, jj = r % (ty + 2), ii = r / (ty + 2);
+        ph[(ii * HY + jj) * HZ + kk] = phi[(i0 - 1 + ii) * sX + (j0 - 1 + jj) * sY + (k0 - 1 + kk)];
+    }
+    for (int e = lane; e < tx * ty * tz; e += 64) {
+        const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
+        rb[(ii * TY + jj) * TZ + kk] = rhs[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)];
+    }
+    __syncthreads();
+    const int a = lane / TY, b = lane % TY;
+    double d2 = 0, o2 = 0;
+    const int steps = tz + (tx - 1) + (ty - 1);
+    for (int t = 0; t < steps; ++t) {
+        const int k = t - a - b;
+        if (a < tx && b < ty && k >= 0 && k < tz) {
+            T* p = &ph[((a + 1) * HY + (b + 1)) * HZ + (k + 1)];
+            const T old = p[0];
+            const T v = gs_update<T, UNIT>(p[HY * HZ], p[-HY * HZ], p[HZ], p[-HZ], p[1], p[-1],
+                                           rb[(a * TY + b) * TZ + k], old, c);
+            p[0] = v;
+            const double d = (double)v - (double)old;
+            d2 += d * d;
+            o2 += (double)old * (double)old;
+        }
+        __syncthreads();
+    }
+    for (int e = lane; e < tx * ty * tz; e += 64) {
+        const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
+        phi[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)] =
+            ph[((ii + 1) * HY + (jj + 1)) * HZ + (kk + 1)];
+    }
+    if (acc) {
+        d2 = wave_sum(d2);
+        o2 = wave_sum(o2);
+        if (lane == 0) {
+            atomicAdd(&acc[2 * s], d2);
+            atomicAdd(&acc[2 * s + 1], o2);
+        }
+    }
+}
+
+template <class T>
+static int lex_chunk(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int C, double* acc)
+{
+    const int X = phi->nx, Y = phi->ny, Z = phi->nz;
+    const int nTx = (X - 2 + kLexT - 1) / kLexT, nTy = (Y - 2 + kLexT - 1) / kLexT,
+              nTz = (Z - 2 + kLexTZ - 1) / kLexTZ;
+    const int nT = nTx * nTy * nTz;
+    const int last = (nTx - 1) + (nTy - 1) + (nTz - 1) + 2 * (C - 1);
+    const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+    for (int tau = 0; tau <= last; ++tau) {
+        TimedLaunch tl(ctx, TK_K_GS);
+        if (unit)
+            hipLaunchKernelGGL((k_gs_lex<T, true>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,
+                               (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, acc);
+        else
+            hipLaunchKernelGGL((k_gs_lex<T, false>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,
+                               (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, acc);
+    }
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+// ---- red-black -----------------------------------------------------------------------
+// Device stop protocol (slots in ctx->cnt): cnt[0] = 1 + stopping sweep (0: running);
+// cnt[1 + slot] = arrivals of the black pass of the sweep using acc slot `slot`.
+struct RbGate {
+    double* acc;        // pair per sweep slot: sum (new-old)^2, sum old^2 (interior)
+    double* err;        // per slot: relative update error
+    unsigned* stop;     // cnt[0]
+    unsigned* arrive;   // cnt + 1
+    int ls_steps;
+    double ls_tol;
+    double h3;          // hx*hy*hz
+    double bnd_o2;      // boundary contribution to sum old^2 for sweeps >= 1
+};
+
+template <class T, bool UNIT>
+__global__ __launch_bounds__(256) void k_gs_rb(T* __restrict__ phi, const T* __restrict__ rhs,
+                                               int X, int Y, int Z, int color, GsCoef c, int s,
+                                               int slot, int gated, RbGate g)
+{
+    if (gated && *(volatile unsigned*)g.stop) return;
+    const int nk = (Z - 2 + 1) / 2;  // candidates per row
+    const long rows = (long)(X - 2) * (Y - 2);
+    const long total = rows * nk;
+    const long sY = Z, sX = (long)Y * Z;
+    double d2 = 0, o2 = 0;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        const int kk = (int)(q % nk);
+        const long r = q / nk;
+        const int j = 1 + (int)(r % (Y - 2)), i = 1 + (int)(r / (Y - 2));
+        const int k = 1 + 2 * kk + ((i + j + 1 + color) & 1);
+        if (k > Z - 2) continue;
+        T* p = phi + i * sX + j * sY + k;
+        const T old = p[0];
+        const T v = gs_update<T, UNIT>(p[sX], p[-sX], p[sY], p[-sY], p[1], p[-1],
+                                       rhs[i * sX + j * sY + k], old, c);
+        p[0] = v;
+        if (gated) {
+            const double d = (double)v - (double)old;
+            d2 += d * d;
+            o2 += (double)old * (double)old;
+        }
+    }
+    if (!gated) return;
+    block_sum2(d2, o2);
+    if (threadIdx.x == 0) {
+        atomicAdd(&g.acc[2 * slot], d2);
+        atomicAdd(&g.acc[2 * slot + 1], o2);
+        if (color == 1) {
+            __threadfence();
+            const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
+            if (ticket == gridDim.x - 1) {
+                const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
+                double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
+                if (s > 0) so += g.bnd_o2;
+                const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
+                g.err[slot] = err;
+                g.arrive[slot] = 0;
+                if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
+            }
+        }
+    }
+}
+
+__global__ void k_zero_d(double* p, int n)
+{
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0;
+}
+__global__ void k_zero_u(unsigned* p, int n)
+{
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0;
+}
+
+template <class T>
+static int rb_sweep(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int s, int slot,
+                    int gated, const RbGate& g)
+{
+    const int X = phi->nx, Y = phi->ny, Z = phi->nz;
+    const long total = (long)(X - 2) * (Y - 2) * ((Z - 1) / 2);
+    const unsigned nb = blocks_for(total, 256, (long)ctx->num_cu * 8);
+    const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+    for (int color = 0; color < 2; ++color) {
+        TimedLaunch tl(ctx, TK_K_GS);
+        if (unit)
+            hipLaunchKernelGGL((k_gs_rb<T, true>), dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d,
+                               (const T*)rhs->d, X, Y, Z, color, c, s, slot, gated, g);
+        else
+            hipLaunchKernelGGL((k_gs_rb<T, false>), dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d,
+                               (const T*)rhs->d, X, Y, Z, color, c, s, slot, gated, g);
+    }
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+template <class T>
+static int set_boundary(tk_ctx ctx, tk_vol phi, double bc, double* acc)
+{
+    const long nb_vox = boundary_count(phi->nx, phi->ny, phi->nz);
+    const unsigned nb = blocks_for(nb_vox, 256, 1024);
+    hipLaunchKernelGGL(k_set_boundary<T>, dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d, phi->nx,
+                       phi->ny, phi->nz, (T)bc, acc);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+// sweep loop of rsfe_splitbregman_imp.hxx:1762-1797
+template <class T>
+static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order, int ls_steps,
+                 double ls_tol, int max_sweeps, int* sweeps_out, double* err_out)
+{
+    const GsCoef c = gs_coef(phi, dt);
+    const int X = phi->nx, Y = phi->ny, Z = phi->nz;
+    const double h3 = phi->hx * phi->hy * phi->hz;
+    const double bnd_o2 = (double)boundary_count(X, Y, Z) * (double)(T)bc * (double)(T)bc;
+    if (ls_steps < 1) ls_steps = 1;
+    if (ls_tol == 0) {
+        // exactly ls_steps sweeps, no norms (the reference computes none either)
+        TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+        if (order == TK_GS_LEX) {
+            TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr));
+        } else {
+            RbGate g{};
+            for (int s = 0; s < ls_steps; ++s) TK_TRY(rb_sweep<T>(ctx, phi, rhs, c, s, 0, 0, g));
+        }
+        if (sweeps_out) *sweeps_out = ls_steps;
+        if (err_out) *err_out = -1;
+        return TK_OK;
+    }
+    if (max_sweeps <= 0) max_sweeps = 1 << 30;
+    double* acc = ctx->acc + 4096;  // per-sweep pairs (up to 2048 sweeps per chunk)
+    const int kMaxChunk = 2048;
+
+    if (order == TK_GS_REDBLACK) {
+        // batches of B sweeps; the device stops itself, the host polls between batches
+        const int B = 256;
+        RbGate g;
+        g.acc = acc;
+        g.err = ctx->acc + 64 + 2 * 1024;  // B error slots
+        g.stop = ctx->cnt;
+        g.arrive = ctx->cnt + 1;
+        g.ls_steps = ls_steps;
+        g.ls_tol = ls_tol;
+        g.h3 = h3;
+        g.bnd_o2 = bnd_o2;
+        hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(256), 0, ctx->stream, ctx->cnt, B + 1);
+        int done = 0;
+        while (done < max_sweeps) {
+            const int nb = std::min(B, max_sweeps - done);
+            hipLaunchKernelGGL(k_zero_d, dim3(4), dim3(256), 0, ctx->stream, acc, 2 * B);
+            if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, acc));
+            for (int q = 0; q < nb; ++q) TK_TRY(rb_sweep<T>(ctx, phi, rhs, c, done + q, q, 1, g));
+            unsigned stop = 0;
+            TK_HIP(hipMemcpyAsync(ctx->h_acc, g.err, sizeof(double) * B, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            TK_HIP(hipMemcpyAsync(&ctx->h_acc[B], ctx->cnt, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            TK_HIP(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&stop, &ctx->h_acc[B], sizeof(unsigned));
+            if (stop) {
+                const int sdone = (int)stop;
+                if (sweeps_out) *sweeps_out = sdone;
+                if (err_out) *err_out = ctx->h_acc[sdone - 1 - done];
+                return TK_OK;
+            }
+            done += nb;
+        }
+        if (sweeps_out) *sweeps_out = done;
+        if (err_out) *err_out = ctx->h_acc[(done - 1) % B];
+        set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", done);
+        return TK_ERR_SWEEPS;
+    }
+
+    // lexicographic: chunks with replay
+    void* save_p;
+    TK_TRY(ctx_work(ctx, 2, phi->bytes(), &save_p));
+    int done = 0;
+    int C = std::max(ls_steps, 16);
+    while (done < max_sweeps) {
+        C = std::min(C, std::min(kMaxChunk, max_sweeps - done));
+        TK_HIP(hipMemcpyAsync(save_p, phi->d, phi->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_zero_d, dim3(16), dim3(256), 0, ctx->stream, acc, 2 * C);
+        if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, acc));
+        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, C, acc));
+        TK_HIP(hipMemcpyAsync(ctx->h_acc, acc, sizeof(double) * 2 * C, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+        int stop_at = -1;
+        double err = -1;
+        for (int q = 0; q < C; ++q) {
+            const int s = done + q;
+            double so = ctx->h_acc[2 * q + 1];
+            if (s > 0) so += bnd_o2;
+            err = std::sqrt(h3 * ctx->h_acc[2 * q]) / std::sqrt(h3 * so);
+            if (s >= ls_steps - 1 && !(err > ls_tol)) {
+                stop_at = q;
+                break;
+            }
+        }
+        if (stop_at >= 0) {
+            if (stop_at < C - 1) {
+                // replay the chunk up to the stopping sweep from the saved state
+                TK_HIP(hipMemcpyAsync(phi->d, save_p, phi->bytes(), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+                if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+                TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, stop_at + 1, nullptr));
+            }
+            if (sweeps_out) *sweeps_out = done + stop_at + 1;
+            if (err_out) *err_out = err;
+            return TK_OK;
+        }
+        done += C;
+        if (err_out) *err_out = err;
+        C *= 2;
+    }
+    if (sweeps_out) *sweeps_out = done;
+    set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", done);
+    return TK_ERR_SWEEPS;
+}
+
+// residual r = rhs + lap_h phi on the interior; sums r^2 and rhs^2
+template <class T>
+__global__ __launch_bounds__(256) void k_residual(const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                  int X, int Y, int Z, double ihx2, double ihy2,
+                                                  double ihz2, double* __restrict__ part)
+{
+    const long total = (long)(X - 2) * (Y - 2) * (Z - 2);
+    const long sY = Z, sX = (long)Y * Z;
+    double r2 = 0, b2 = 0;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
+         q += (long)gridDim.x * blockDim.x) {
+        const int k = 1 + (int)(q % (Z - 2));
+        const long rr = q / (Z - 2);
+        const int j = 1 + (int)(rr % (Y - 2)), i = 1 + (int)(rr / (Y - 2));
+        const long v = i * sX + j * sY + k;
+        const double c = phi[v];
+        const double lap = ((double)phi[v + sX] + (double)phi[v - sX] - 2 * c) * ihx2 +
+                           ((double)phi[v + sY] + (double)phi[v - sY] - 2 * c) * ihy2 +
+                           ((double)phi[v + 1] + (double)phi[v - 1] - 2 * c) * ihz2;
+        const double b = rhs[v];
+        const double r = b + lap;
+        r2 += r * r;
+        b2 += b * b;
+    }
+    block_sum2(r2, b2);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = r2;
+        part[2 * blockIdx.x + 1] = b2;
+    }
+}
+
+__global__ void k_sum_pairs_gs(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        s1 += part[2 * i];
+        s2 += part[2 * i + 1];
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+static int check_gs(tk_vol phi, tk_vol rhs)
+{
+    TK_CHECK(phi && rhs && phi->d && rhs->d, TK_ERR_ARG, "poisson: null volume");
+    TK_CHECK(same_dims(phi, rhs), TK_ERR_SHAPE, "poisson: phi and rhs dims differ");
+    TK_CHECK(phi->dtype == rhs->dtype, TK_ERR_DTYPE, "poisson: dtypes differ");
+    TK_CHECK(phi->dtype == TK_F32 || phi->dtype == TK_F64, TK_ERR_DTYPE, "poisson: not FP");
+    TK_CHECK(phi->nx >= 3 && phi->ny >= 3 && phi->nz >= 3, TK_ERR_UNSUPPORTED,
+             "poisson: 3-D volumes with every edge >= 3 only");
+    return TK_OK;
+}
+
+extern "C" {
+
+int tk_poisson_dir_sweeps(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int n_sweeps)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_dir_sweeps: null ctx");
+    TK_TRY(check_gs(phi, rhs));
+    TK_CHECK(order == TK_GS_LEX || order == TK_GS_REDBLACK, TK_ERR_ARG, "bad order");
+    TK_CHECK(dt > 0, TK_ERR_ARG, "tk_poisson_dir_sweeps: dt <= 0");
+    if (n_sweeps <= 0) return TK_OK;
+    if (phi->dtype == TK_F64)
+        return solve<double>(ctx, phi, rhs, dt, bc, order, n_sweeps, 0, 0, nullptr, nullptr);
+    return solve<float>(ctx, phi, rhs, dt, bc, order, n_sweeps, 0, 0, nullptr, nullptr);
+}
+
+int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                         int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
+                         double* err_out)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_dir_solve: null ctx");
+    TK_TRY(check_gs(phi, rhs));
+    TK_CHECK(order == TK_GS_LEX || order == TK_GS_REDBLACK, TK_ERR_ARG, "bad order");
+    TK_CHECK(dt > 0 && ls_tol >= 0, TK_ERR_ARG, "tk_poisson_dir_solve: dt <= 0 or ls_tol < 0");
+    if (phi->dtype == TK_F64)
+        return solve<double>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
+                             err_out);
+    return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
+                        err_out);
+}
+
+int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel)
+{
+    TK_CHECK(ctx && rel, TK_ERR_ARG, "tk_poisson_residual: null argument");
+    TK_TRY(check_gs(phi, rhs));
+    const long total = (long)(phi->nx - 2) * (phi->ny - 2) * (phi->nz - 2);
+    const int nb = (int)blocks_for(total, 256, 1024);
+    double* part = ctx->acc + 64;
+    const double ihx2 = 1. / (phi->hx * phi->hx), ihy2 = 1. / (phi->hy * phi->hy),
+                 ihz2 = 1. / (phi->hz * phi->hz);
+    if (phi->dtype == TK_F64)
+        hipLaunchKernelGGL(k_residual<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)phi->d, (const double*)rhs->d, phi->nx, phi->ny, phi->nz,
+                           ihx2, ihy2, ihz2, part);
+    else
+        hipLaunchKernelGGL(k_residual<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)phi->d, (const float*)rhs->d, phi->nx, phi->ny, phi->nz,
+                           ihx2, ihy2, ihz2, part);
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_pairs_gs, dim3(1), dim3(1024), 0, ctx->stream, part, nb, ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    *rel = std::sqrt(ctx->h_acc[0]) / std::sqrt(ctx->h_acc[1]);
+    return TK_OK;
+}
+
+}  // extern "C"

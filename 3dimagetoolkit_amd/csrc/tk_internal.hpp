@@ -1,0 +1,125 @@
+// tk_internal.hpp -- shared definitions of libtk.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tk.h"
+
+namespace tk {
+
+void set_error(const char* fmt, ...);
+
+#define TK_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::tk::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr,                \
+                            hipGetErrorString(e_));                                     \
+            return TK_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+#define TK_CHECK(cond, code, ...)                                                       \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::tk::set_error(__VA_ARGS__);                                               \
+            return (code);                                                              \
+        }                                                                               \
+    } while (0)
+
+#define TK_TRY(expr)                                                                    \
+    do {                                                                                \
+        int r_ = (expr);                                                                \
+        if (r_ != TK_OK) return r_;                                                     \
+    } while (0)
+
+// after a kernel launch
+#define TK_LAUNCHED() TK_HIP(hipGetLastError())
+
+inline size_t dtype_size(tk_dtype t) { return t == TK_F64 ? 8 : (t == TK_F32 ? 4 : 1); }
+
+// Device scratch owned by a context: FP64 accumulators, counters and flags used by
+// the reductions and the device-side Poisson stop test.  Sizes are fixed.
+constexpr int kAccSlots = 8192;  // doubles
+constexpr int kCntSlots = 4096;  // unsigned
+
+}  // namespace tk
+
+struct tk_ctx_s {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    double* acc = nullptr;        // device, kAccSlots
+    unsigned* cnt = nullptr;      // device, kCntSlots
+    double* h_acc = nullptr;      // pinned host mirror, kAccSlots
+    int num_cu = 256;
+    // cached complex work buffers (bytes); reused across calls of the same size
+    void* work[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t work_bytes[4] = {0, 0, 0, 0};
+    // per-kernel event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending {
+        hipEvent_t a, b;
+        int tag;
+    };
+    std::vector<Pending> pending;
+    double tag_ms[TK_K_COUNT] = {0};
+    long tag_n[TK_K_COUNT] = {0};
+};
+
+struct tk_vol_s {
+    tk_ctx ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    double hx = 1, hy = 1, hz = 1;
+    tk_dtype dtype = TK_F64;
+    void* d = nullptr;
+    bool owned = true;
+    long n() const { return (long)nx * ny * nz; }
+    size_t bytes() const { return (size_t)n() * tk::dtype_size(dtype); }
+};
+
+namespace tk {
+
+// get (grow if needed) context work buffer `slot` of at least `bytes`
+int ctx_work(tk_ctx ctx, int slot, size_t bytes, void** out);
+
+inline bool same_dims(const tk_vol a, const tk_vol b)
+{
+    return a->nx == b->nx && a->ny == b->ny && a->nz == b->nz;
+}
+
+// 1-D grid helper
+inline unsigned blocks_for(long n, int threads, long cap = 1 << 20)
+{
+    long b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// FP64 sums of two quantities over a volume; results copied to host.
+// kinds implemented in tk_core.hip
+int reduce_sumsq(tk_ctx ctx, tk_vol a, tk_vol b_or_null, double* sum_d2, double* sum_b2);
+
+// internal stage launchers shared between translation units
+int launch_cut(tk_ctx ctx, tk_vol phi, double a0, double b0);
+
+// event timing around one launch: begin returns a slot (or -1 when timing is off)
+int timing_begin(tk_ctx ctx, int tag);
+void timing_end(tk_ctx ctx, int slot);
+int timing_flush(tk_ctx ctx);
+
+struct TimedLaunch {
+    tk_ctx ctx;
+    int slot;
+    TimedLaunch(tk_ctx c, int tag) : ctx(c), slot(timing_begin(c, tag)) {}
+    ~TimedLaunch() { timing_end(ctx, slot); }
+};
+
+}  // namespace tk

@@ -1,0 +1,359 @@
+// tk_rsfe.hip -- the RSFE Split-Bregman outer loop on the device
+// (segm::rsfe_splitbregman<T>::apply, rsfe_splitbregman_imp.hxx:1547-1995).
+//
+// One outer iteration = 5 convolution passes + the Poisson sweeps + 2 stencil passes:
+//   x1  load phi,u0 -> (H*u0, H)         [Heaviside prologue, phiold copy]    imp:1667,1700
+//   y1  complex -> complex
+//   z   conv z of pair 1, then region fit f1,f2 -> (l1 f1 - l2 f2, l1 f1^2 - l2 f2^2),
+//       then conv z of pair 2, in LDS                                          imp:1703-1731
+//   y2  complex -> complex
+//   x2  -> forcing term rhs                [epilogue]                         imp:1750-1759
+//   Poisson sweeps (lexicographic or red-black)                              imp:1762-1797
+//   sb  cut_phi + ||phi-phiold|| + grad/shrink/Bregman update                imp:1800-1906
+//   div divdb = div(d - b)                                                   imp:1932
+// Convolution passes commute (the Gaussian is separable), which is what lets pair 2
+// start on the axis where pair 1 ends.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "tk_device.hpp"
+#include "tk_fft_kernel.hpp"
+
+namespace tk {
+
+template <class T>
+int launch_div(tk_ctx, tk_vol, tk_vol, tk_vol, tk_vol);
+template <class T>
+int launch_edge(tk_ctx, tk_vol, double, double, double, tk_vol);
+template <class T>
+int launch_sb_update(tk_ctx, tk_vol, tk_vol, tk_vol*, tk_vol, tk_vol*, double, double, int*,
+                     double*);
+
+template <class T>
+__device__ __forceinline__ T heaviside(T p, T center, T eps)
+{
+    if constexpr (std::is_same<T, double>::value)
+        return 0.5 * (1. + 2. / M_PI * atan((p - center) / eps));
+    else
+        return 0.5f * (1.f + (float)(2. / M_PI) * atanf((p - center) / eps));
+}
+
+template <class T>
+struct LdHeaviside {
+    const T* phi;
+    const T* u0;
+    T* phiold;
+    T center, eps;
+    __device__ cplx<T> operator()(long v) const
+    {
+        const T p = phi[v];
+        phiold[v] = p;
+        const T H = heaviside(p, center, eps);
+        return {H * u0[v], H};
+    }
+};
+
+// imp:1709-1731 on (num, den) -> (kr2 factor, kr1 factor)
+template <class T>
+struct MdRegionFit {
+    const T* kmyim;
+    const T* kones;
+    T l1, l2;
+    __device__ cplx<T> operator()(long v, cplx<T> c) const
+    {
+        const T num = c.x, den = c.y;
+        T f1 = num / den;
+        T nm = num, dn = den;
+        nm -= kmyim[v];
+        dn -= kones[v];
+        nm /= dn;
+        T f2 = nm;
+        const T a = f1 * l1 - f2 * l2;
+        f1 *= f1;
+        f1 *= l1;
+        f2 *= f2;
+        f2 *= l2;
+        f1 -= f2;
+        return {a, f1};
+    }
+};
+
+// imp:1750-1759 on (kr2, kr1)
+template <class T>
+struct StRhs {
+    const T* u0;
+    const T* kones;
+    const T* divdb;
+    T* rhs;
+    T l12, lam;
+    __device__ void operator()(long v, cplx<T> c) const
+    {
+        const T u = u0[v];
+        T r = c.x;
+        r *= u;
+        r *= (T)2;
+        T aux = l12;
+        aux *= u;
+        aux *= u;
+        aux *= kones[v];
+        r -= aux;
+        r -= c.y;
+        r /= lam;
+        r -= divdb[v];
+        rhs[v] = r;
+    }
+};
+
+__global__ void k_sum_pairs_rsfe(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        s1 += part[2 * i];
+        s2 += part[2 * i + 1];
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+struct tk_rsfe_s {
+    tk_ctx ctx = nullptr;
+    tk_rsfe_params p{};
+    tk_dtype dtype = TK_F64;
+    int X = 0, Y = 0, Z = 0;
+    double h[3] = {1, 1, 1};
+    tk_filter flt = nullptr;
+    tk_vol u0 = nullptr, phi = nullptr, phiold = nullptr, rhs = nullptr, kones = nullptr,
+           kmyim = nullptr, g = nullptr, divdb = nullptr;
+    tk_vol b[3] = {nullptr, nullptr, nullptr}, q[3] = {nullptr, nullptr, nullptr};
+    void* C = nullptr;
+    int iter = 0;
+    double tollerance = 1000;
+};
+
+namespace {
+
+template <class T>
+int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st)
+{
+    tk_ctx ctx = s->ctx;
+    const tk_rsfe_params& p = s->p;
+    cplx<T>* C = (cplx<T>*)s->C;
+    const tk_filter_s* f = s->flt;
+    // conv pair 1: x, y, then z with the region fit in the middle
+    LdHeaviside<T> ld{(const T*)s->phi->d, (const T*)s->u0->d, (T*)s->phiold->d,
+                      (T)((p.a0 + p.b0) / 2.), (T)p.epsilon};
+    {
+        TimedLaunch tl(ctx, TK_K_CONV_X1);
+        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C})));
+    }
+    {
+        TimedLaunch tl(ctx, TK_K_CONV_Y);
+        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+    }
+    MdRegionFit<T> md{(const T*)s->kmyim->d, (const T*)s->kones->d, (T)p.lamda1, (T)p.lamda2};
+    {
+        TimedLaunch tl(ctx, TK_K_CONV_Z2);
+        TK_TRY((launch_conv_axis<T>(ctx, f, 2, 2, LdComplex<T>{C}, md, StComplex<T>{C})));
+    }
+    {
+        TimedLaunch tl(ctx, TK_K_CONV_Y);
+        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+    }
+    StRhs<T> sr{(const T*)s->u0->d, (const T*)s->kones->d, (const T*)s->divdb->d, (T*)s->rhs->d,
+                (T)(p.lamda1 - p.lamda2), (T)p.lamda};
+    {
+        TimedLaunch tl(ctx, TK_K_CONV_X2);
+        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, LdComplex<T>{C}, MdNone{}, sr)));
+    }
+    // Poisson loop
+    const double chosen_bc = p.bc_dir ? p.a0 : 0.;
+    int sweeps = 0;
+    double ls_err = -1;
+    int r = tk_poisson_dir_solve(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
+                                 p.ls_tol, p.max_sweeps, &sweeps, &ls_err);
+    if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
+    // cut + norms + grad/shrink/Bregman, then div
+    int nb = 0;
+    double* part = ctx->acc + 64;
+    {
+        TimedLaunch tl(ctx, TK_K_SB);
+        TK_TRY(launch_sb_update<T>(ctx, s->phi, s->phiold, s->b, s->g, s->q, p.a0, p.b0, &nb,
+                                   part));
+    }
+    hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb, ctx->acc);
+    TK_LAUNCHED();
+    {
+        TimedLaunch tl(ctx, TK_K_DIV);
+        TK_TRY(launch_div<T>(ctx, s->q[0], s->q[1], s->q[2], s->divdb));
+    }
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    TK_TRY(timing_flush(ctx));
+    const double h3 = s->h[0] * s->h[1] * s->h[2];
+    s->tollerance = std::sqrt(h3 * ctx->h_acc[0]) / std::sqrt(h3 * ctx->h_acc[1]);
+    if (st) {
+        st->sweeps = sweeps;
+        st->ls_err = ls_err;
+        st->tollerance = s->tollerance;
+        st->guess_used = 0;
+    }
+    s->iter++;
+    return TK_OK;
+}
+
+// kones = K*1 and kmyim = K*u0 as one complex transform
+template <class T>
+int conv_passes_pair_ones(tk_ctx ctx, const tk_filter_s* f, tk_vol kones, tk_vol kmyim, tk_vol u0,
+                          void* work)
+{
+    cplx<T>* C = (cplx<T>*)work;
+    LdPair<T> ld{nullptr, (const T*)u0->d, (T)1, (T)0};
+    TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C})));
+    TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+    StPair<T> st{(T*)kones->d, (T*)kmyim->d};
+    TK_TRY((launch_conv_axis<T>(ctx, f, 2, 1, LdComplex<T>{C}, MdNone{}, st)));
+    return TK_OK;
+}
+
+void free_state(tk_rsfe s)
+{
+    if (!s) return;
+    tk_vol* vols[] = {&s->u0, &s->phi, &s->phiold, &s->rhs, &s->kones, &s->kmyim, &s->g, &s->divdb,
+                      &s->b[0], &s->b[1], &s->b[2], &s->q[0], &s->q[1], &s->q[2]};
+    for (tk_vol* v : vols) {
+        tk_vol_free(*v);
+        *v = nullptr;
+    }
+    if (s->C) (void)hipFree(s->C);
+    if (s->flt) tk_filter_free(s->flt);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tk_rsfe_default_params(tk_rsfe_params* p)
+{
+    TK_CHECK(p, TK_ERR_ARG, "tk_rsfe_default_params: null");
+    // rsfe_splitbregman_imp.hxx:25-41
+    p->sigma = 3;
+    p->lamda = 5.e-2;
+    p->lamda1 = 1.e-5;
+    p->lamda2 = 1.e-5;
+    p->beta = 100.;
+    p->dt = 1;
+    p->epsilon = 1.e-3;
+    p->a0 = 0.;
+    p->b0 = 1.;
+    p->nu = 1.;
+    p->tol = 1.e-5;
+    p->ls_tol = 0;
+    p->radius = 3;
+    p->maxiter = 100;
+    p->ls_steps = 5;
+    p->bc_dir = 0;
+    p->gaussian_pixel_approach = 0;
+    p->gs_order = TK_GS_LEX;
+    p->max_sweeps = 0;
+    p->warm_start = 0;
+    return TK_OK;
+}
+
+int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, tk_rsfe* out)
+{
+    TK_CHECK(ctx && u0 && init && p && out, TK_ERR_ARG, "tk_rsfe_create: null argument");
+    TK_CHECK(u0->dtype == TK_F32 || u0->dtype == TK_F64, TK_ERR_DTYPE, "tk_rsfe_create: u0 dtype");
+    TK_CHECK(same_dims(u0, init), TK_ERR_SHAPE,
+             "tk_rsfe_create: init must have the dims of the image");
+    TK_CHECK(u0->nx >= 3 && u0->ny >= 3 && u0->nz >= 3, TK_ERR_UNSUPPORTED,
+             "tk_rsfe_create: 3-D volumes with every edge >= 3 only");
+    TK_CHECK(p->warm_start == 0, TK_ERR_UNSUPPORTED,
+             "tk_rsfe_create: warm_start needs a trained network (tk_unet_*)");
+    TK_CHECK(p->gs_order == TK_GS_LEX || p->gs_order == TK_GS_REDBLACK, TK_ERR_ARG,
+             "tk_rsfe_create: gs_order");
+    tk_rsfe s = new tk_rsfe_s();
+    s->ctx = ctx;
+    s->p = *p;
+    s->dtype = u0->dtype;
+    s->X = u0->nx; s->Y = u0->ny; s->Z = u0->nz;
+    s->h[0] = u0->hx; s->h[1] = u0->hy; s->h[2] = u0->hz;
+    const tk_dtype dt = s->dtype;
+    int r = TK_OK;
+    auto alloc = [&](tk_vol* v) {
+        if (r == TK_OK)
+            r = tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], dt, v);
+    };
+    tk_vol* vols[] = {&s->u0, &s->phi, &s->phiold, &s->rhs, &s->kones, &s->kmyim, &s->g, &s->divdb,
+                      &s->b[0], &s->b[1], &s->b[2], &s->q[0], &s->q[1], &s->q[2]};
+    for (tk_vol* v : vols) alloc(v);
+    if (r == TK_OK && hipMalloc(&s->C, 2 * dtype_size(dt) * (size_t)u0->n()) != hipSuccess) {
+        set_error("tk_rsfe_create: complex buffer allocation failed");
+        r = TK_ERR_NOMEM;
+    }
+    if (r == TK_OK) r = tk_vol_copy(s->u0, u0);
+    // initialize_phi_with_init, same dims: phi = init; change_range_of_intensity(b0, a0)
+    if (r == TK_OK) r = tk_vol_copy(s->phi, init);
+    if (r == TK_OK) r = tk_change_range_of_intensity(ctx, s->phi, p->b0, p->a0);
+    // build_gaussian_filter(phi, sigma, 3, gpa) (imp:1641-1644)
+    if (r == TK_OK)
+        r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], p->sigma,
+                               p->radius, p->gaussian_pixel_approach, dt, &s->flt);
+    // kones = cv(1), kmyim = cv(myim) in one complex transform (imp:1648-1651)
+    if (r == TK_OK) {
+        if (dt == TK_F64)
+            r = conv_passes_pair_ones<double>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C);
+        else
+            r = conv_passes_pair_ones<float>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C);
+    }
+    // gforshrink = edge_detect(myim) (imp:1655)
+    if (r == TK_OK)
+        r = dt == TK_F64 ? launch_edge<double>(ctx, s->u0, p->beta, p->lamda, p->nu, s->g)
+                         : launch_edge<float>(ctx, s->u0, p->beta, p->lamda, p->nu, s->g);
+    if (r == TK_OK) r = tk_ctx_sync(ctx);
+    if (r != TK_OK) {
+        free_state(s);
+        return r;
+    }
+    *out = s;
+    return TK_OK;
+}
+
+int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done)
+{
+    TK_CHECK(s, TK_ERR_ARG, "tk_rsfe_iterate: null state");
+    int done = 0;
+    // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
+    while (done < n_iters && s->tollerance > s->p.tol && s->iter < s->p.maxiter) {
+        int r = s->dtype == TK_F64 ? rsfe_iteration<double>(s, stats ? &stats[done] : nullptr)
+                                   : rsfe_iteration<float>(s, stats ? &stats[done] : nullptr);
+        if (r != TK_OK) {
+            if (n_done) *n_done = done;
+            return r;
+        }
+        ++done;
+    }
+    if (n_done) *n_done = done;
+    return TK_OK;
+}
+
+tk_vol tk_rsfe_phi(tk_rsfe s) { return s ? s->phi : nullptr; }
+
+int tk_rsfe_destroy(tk_rsfe s)
+{
+    if (!s) return TK_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    free_state(s);
+    return TK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""Benchmark: RSFE Split-Bregman voxel-iterations/s at 256^3 on one MI355X.
+
+Workload (a "step" = one outer Split-Bregman iteration of rsfe_splitbregman::apply):
+256^3 volume from the seeded two-ellipsoid generator scaled x4 (background N(0.2,0.05),
+ellipsoids 0.8, x bias 0.7 -> 1.3, noise 0.05), rescaled to [0,255] as the reference's
+driver does, Gaussian sigma 6 (37 taps) with circular boundaries, phi0 a sphere of radius
+80, reference default lambda/lambda1/lambda2/beta/nu/epsilon, Dirichlet Gauss-Seidel
+with the reference default Poisson setting (ls_steps 5, ls_tol 0: S = 5 sweeps per
+iteration).  FP32 arithmetic with FP64 reductions and red-black sweeps (default), or
+--dtype f64 for the FP64 plain mode with lexicographic sweeps.
+
+value = N * K / t_loop, inputs resident in HBM; set-up (filter spectrum, K*1, K*u0,
+edge detector) is outside the timed region.  The roofline object describes the kernel
+with the largest total time, from HIP events recorded around every launch on the
+context's stream in a second pass over K steps.  cpu_baseline times the CPU FP64
+oracle (OpenMP) on the same workload for a bounded number of iterations.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20240602
+
+# algorithmic bytes per voxel per launch, in element sizes w (model of DESIGN.md)
+BYTES_PER_VOXEL_W = {
+    "conv_x1": 5,   # read phi, u0; write complex (2); write phiold
+    "conv_y": 4,    # read complex (2), write complex (2)
+    "conv_z2": 6,   # read complex (2), kmyim, kones; write complex (2)
+    "conv_x2": 6,   # read complex (2), u0, kones, divdb; write rhs
+    "gs": 1.5,      # one colour pass: half of read phi + rhs, write phi (3w per sweep)
+    "sb": 13,       # read phi, phiold, b(3), g; write phi, b(3), q(3)
+    "div": 4,       # read q(3), write divdb
+}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--ls-steps", type=int, default=5)
+    ap.add_argument("--ls-tol", type=float, default=0.0)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
+                    help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every box here has "
+              "one GPU, and this path has no multi-GPU sharding built)", file=sys.stderr)
+        sys.exit(2)
+
+    tk = importlib.import_module("3dimagetoolkit_amd")
+    n = a.size
+    shape = (n, n, n)
+    N = n ** 3
+    scale = max(1, n // 64)
+    sigma = 6.0 * n / 256.0 if n != 64 else 3.0
+    dt = np.float32 if a.dtype == "f32" else np.float64
+    w = 4 if a.dtype == "f32" else 8
+    order = tk.TK_GS_REDBLACK if a.dtype == "f32" else tk.TK_GS_LEX
+    if a.gs != "default":
+        order = tk.TK_GS_REDBLACK if a.gs == "rb" else tk.TK_GS_LEX
+
+    u0, phi0 = tk.gen_standard(shape, SEED, scale, 2)
+    ctx = tk.Context(0)
+    vu = ctx.from_numpy(u0.astype(dt))
+    tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+    vi = ctx.from_numpy(phi0.astype(dt))
+    p = tk.default_params(sigma=sigma, maxiter=1 << 30, tol=0.0, ls_steps=a.ls_steps,
+                          ls_tol=a.ls_tol, gs_order=order)
+    t0 = time.perf_counter()
+    run = tk.Rsfe(ctx, vu, vi, p)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    if a.warmup:
+        run.iterate(a.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    stats = run.iterate(a.steps)
+    ctx.sync()
+    t_loop = time.perf_counter() - t0
+    sweeps = [s["sweeps"] for s in stats]
+    value = N * a.steps / t_loop
+
+    roof = None
+    kernels = {}
+    if not a.no_timing_pass:
+        ctx.set_timing(True)
+        ctx.kernel_times(reset=True)
+        run.iterate(a.steps)
+        ctx.sync()
+        kt = ctx.kernel_times(reset=True)
+        ctx.set_timing(False)
+        for tag, (ms, cnt) in kt.items():
+            if cnt and tag in BYTES_PER_VOXEL_W:
+                avg = ms / cnt
+                gbs = BYTES_PER_VOXEL_W[tag] * w * N / (avg * 1e-3) / 1e9
+                kernels[tag] = {"total_ms": round(ms, 3), "launches": cnt,
+                                "avg_us": round(avg * 1e3, 2), "GBps": round(gbs, 1)}
+        if kernels:
+            dom = max(kernels, key=lambda t: kernels[t]["total_ms"])
+            k = kernels[dom]
+            roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": BYTES_PER_VOXEL_W[dom] * w * N}
+    # whole-loop model: conv 5 passes (25w) + S sweeps (3w) + sb (13w) + div (4w)
+    S = float(np.mean(sweeps))
+    model_bytes = N * (25 * w + 3 * w * S + 13 * w + 4 * w)
+    loop_gbs = model_bytes * a.steps / t_loop / 1e9
+    run.close()
+
+    cpu = None
+    if not a.no_cpu and a.cpu_iters > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        import ctypes as C
+        st = orc.OrStats()
+        po = orc.default_params(sigma=sigma, ls_steps=a.ls_steps, ls_tol=0.0, tol=0.0)
+        u255 = orc.change_range(u0, 255.0, 0.0)
+        phi = phi0.copy()
+        orc.lib().or_rsfe_time_iters(orc.dp(u255), C.c_int(n), C.c_int(n), C.c_int(n),
+                                     C.byref(po), C.c_int(a.cpu_iters), orc.dp(phi), C.byref(st))
+        cpu = {"value": N * st.iters / st.t_loop, "unit": "voxel-iters/s",
+               "cores": orc.lib().or_num_threads(), "kind": "port",
+               "sample": f"{n}^3 FP64 CPU oracle (OpenMP), {st.iters} outer iterations at "
+                         f"S={a.ls_steps}, set-up excluded; convolution share "
+                         f"{st.t_fft / st.t_loop:.2f} of the loop",
+               "t_loop_s": round(st.t_loop, 3)}
+
+    out = {
+        "metric": "Split-Bregman voxel-iters/s @256^3",
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_loop / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-reductions" if a.dtype == "f32" else "f64",
+        "data": "synthetic (seeded ellipsoid phantom, rescaled to [0,255])",
+        "config": {"workload": f"rsfe_splitbregman {n}^3 {a.dtype}, sigma {sigma:g} cyclic, "
+                               f"Gauss-Seidel {'red-black' if order else 'lexicographic'} "
+                               f"ls_steps {a.ls_steps} ls_tol {a.ls_tol:g}",
+                   "voxels": N, "sweeps_per_iter": sweeps[:4] + (["..."] if len(sweeps) > 4 else []),
+                   "t_setup_s": round(t_setup, 4),
+                   "loop_model_GBps": round(loop_gbs, 1),
+                   "loop_model_frac": round(loop_gbs / HBM_PEAK_GBS, 4),
+                   "kernels": kernels},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,228 @@
+/*
+ * tk.h -- C ABI of the MI355X RSFE Split-Bregman segmentation library (libtk.so).
+ *
+ * Plain C: opaque handles, pointers and sizes; every function returns an int status
+ * (TK_OK == 0, negative on error) and never aborts.  The C++ class layer in
+ * include/tk/ (im3d::image3d, conv::filtering, lapl::unsteady_poisson_functor,
+ * segm::rsfe_splitbregman) is built on these entry points, and a C, ctypes, cgo or
+ * JNI caller can bind them directly.  Reference citations are to
+ * marco-fedele/3dimagetoolkit (header/...).
+ *
+ * Volumes: nx*ny*nz voxels, index (i*ny + j)*nz + k (z fastest), exactly the
+ * reference's image3d order (image3d_imp.hxx:141-152), so uploads and downloads
+ * are plain copies.  Element type TK_F64 (the reference's double instantiation)
+ * or TK_F32 (FP32 arithmetic with FP64 reductions); masks are TK_U8.
+ */
+#ifndef TK_H
+#define TK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ---------------------------------------------------------------- */
+enum tk_status {
+    TK_OK = 0,
+    TK_ERR_ARG = -1,         /* bad argument (null handle, bad size, bad enum) */
+    TK_ERR_HIP = -2,         /* HIP runtime error; tk_last_error() has the text */
+    TK_ERR_NOMEM = -3,       /* device or host allocation failed */
+    TK_ERR_SHAPE = -4,       /* dimensions of the operands do not agree */
+    TK_ERR_DTYPE = -5,       /* element types of the operands do not agree */
+    TK_ERR_UNSUPPORTED = -6, /* valid request outside what this build implements */
+    TK_ERR_SWEEPS = -7       /* Poisson solve hit max_sweeps before ls_tol */
+};
+
+typedef enum tk_dtype { TK_F32 = 1, TK_F64 = 2, TK_U8 = 3 } tk_dtype;
+
+typedef struct tk_ctx_s* tk_ctx;       /* one device + one HIP stream + scratch */
+typedef struct tk_vol_s* tk_vol;       /* a volume resident in device memory */
+typedef struct tk_filter_s* tk_filter; /* a Gaussian filter's cyclic spectrum */
+typedef struct tk_rsfe_s* tk_rsfe;     /* state of one Split-Bregman run */
+
+/* message of the last error on the calling thread ("" if none) */
+const char* tk_last_error(void);
+const char* tk_version(void);
+
+/* ---- context ----------------------------------------------------------------
+ * No reference counterpart (the reference is a host-only library).  One context
+ * per host thread; all work of a context is ordered on its stream. */
+int tk_ctx_create(int device, tk_ctx* out);
+int tk_ctx_destroy(tk_ctx ctx);
+int tk_ctx_sync(tk_ctx ctx);
+/* run on a caller-owned hipStream_t (e.g. PyTorch's current stream); NULL restores
+   the context's own stream */
+int tk_ctx_set_stream(tk_ctx ctx, void* hip_stream);
+void* tk_ctx_stream(tk_ctx ctx);
+
+/* per-kernel timing with HIP events recorded on the context's stream around every
+   launch of the Split-Bregman path (off by default; no reference counterpart) */
+enum tk_kernel_tag {
+    TK_K_CONV_X1 = 0, /* x pass of pair 1 with the Heaviside prologue */
+    TK_K_CONV_Y = 1,  /* y passes (complex -> complex) */
+    TK_K_CONV_Z2 = 2, /* z pass: pair 1, region fit, pair 2 */
+    TK_K_CONV_X2 = 3, /* x pass of pair 2 with the forcing-term epilogue */
+    TK_K_GS = 4,      /* one Gauss-Seidel launch */
+    TK_K_SB = 5,      /* cut + norms + grad/shrink/Bregman update */
+    TK_K_DIV = 6,     /* divergence */
+    TK_K_OTHER = 7,
+    TK_K_COUNT = 8
+};
+int tk_ctx_set_timing(tk_ctx ctx, int enable);
+/* totals since the last call with reset != 0: milliseconds and launches per tag */
+int tk_ctx_kernel_times(tk_ctx ctx, double ms[TK_K_COUNT], long launches[TK_K_COUNT], int reset);
+
+/* ---- volumes: im3d::image3d<T> storage ----------------------------------------
+ * image3d(x,y,z,hx,hy,hz) ctor (image3d_imp.hxx:35-41), setdim/seth (93-135),
+ * operator=(scalar) (158-167), operator=(image3d) (170-184). */
+int tk_vol_alloc(tk_ctx ctx, int nx, int ny, int nz, double hx, double hy, double hz,
+                 tk_dtype dtype, tk_vol* out);
+/* wrap caller-owned device memory (not freed by tk_vol_free) */
+int tk_vol_wrap(tk_ctx ctx, void* device_ptr, int nx, int ny, int nz, double hx, double hy,
+                double hz, tk_dtype dtype, tk_vol* out);
+int tk_vol_free(tk_vol vol);
+int tk_vol_info(tk_vol vol, int dims[3], double h[3], tk_dtype* dtype);
+void* tk_vol_device_ptr(tk_vol vol);
+/* host <-> device; host_dtype may differ from the volume's (converted on device) */
+int tk_vol_upload(tk_vol vol, const void* host, tk_dtype host_dtype);
+int tk_vol_download(tk_vol vol, void* host, tk_dtype host_dtype);
+int tk_vol_fill(tk_vol vol, double value);
+int tk_vol_copy(tk_vol dst, tk_vol src); /* converts F32 <-> F64 */
+
+/* ---- R4: Gaussian filter and its cyclic spectrum ---------------------------------
+ * conv::filtering<T>::build_gaussian_filter (convolution_imp.hxx:22-91) and the
+ * cyclic part of build_freq_filters (218-264, 404-531).  The spectrum of the
+ * separable normalised kernel is kept per axis (the 3-D real spectrum is their
+ * product); Nh > dim wraps the kernel as the reference's multiple-of-dims padding. */
+int tk_filter_gaussian(tk_ctx ctx, int nx, int ny, int nz, double hx, double hy, double hz,
+                       double sigma, double radius, int pixel_approach, tk_dtype dtype,
+                       tk_filter* out);
+int tk_filter_free(tk_filter flt);
+int tk_filter_taps(tk_filter flt, int nh[3]);
+/* real spectrum along one axis (length dims[axis]), including nothing else */
+int tk_filter_axis_spectrum(tk_filter flt, int axis, double* out);
+
+/* ---- R5: cyclic (periodic) convolution ------------------------------------------
+ * conv::cyclic_fftw_convolution (convolution_imp.hxx:657-757), the function the
+ * functor conv::filtering<T>::operator() (convolution.hxx:104-107) dispatches to
+ * after set_cv(filtering<T>(cyclic_fftw_convolution)).  out may alias in. */
+int tk_conv_cyclic(tk_ctx ctx, tk_filter flt, tk_vol out, tk_vol in);
+/* two real fields through one complex transform (outputs may alias inputs) */
+int tk_conv_cyclic_pair(tk_ctx ctx, tk_filter flt, tk_vol out_a, tk_vol out_b, tk_vol in_a,
+                        tk_vol in_b);
+
+/* ---- R8 / R9: Dirichlet Gauss-Seidel Poisson sweeps -------------------------------
+ * lapl::DirGaussSeidel (poisson_imp.hxx:340-429) behind
+ * lapl::unsteady_poisson_functor<T>::operator() (poisson.hxx:90-96), and the
+ * sweep loop of rsfe_splitbregman::apply (rsfe_splitbregman_imp.hxx:1762-1797).
+ * TK_GS_LEX reproduces the reference's lexicographic order bit for bit (pipelined
+ * tile wavefront); TK_GS_REDBLACK is the red-black ordering (FP32 fast mode). */
+enum tk_gs_order { TK_GS_LEX = 0, TK_GS_REDBLACK = 1 };
+int tk_poisson_dir_sweeps(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int n_sweeps);
+/* runs at least ls_steps sweeps, then until the relative update
+   ||phi_s - phi_{s-1}||_L2 / ||phi_{s-1}||_L2 <= ls_tol (reference rule); ls_tol == 0
+   runs exactly ls_steps.  max_sweeps caps the loop (TK_ERR_SWEEPS if reached). */
+int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                         int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
+                         double* ls_err_out);
+/* build-defined residual (no reference counterpart):
+   ||rhs - (phi/dt - lap_h phi) + phi/dt||_2 over the interior / ||rhs||_2, i.e.
+   ||rhs + lap_h phi|| / ||rhs|| with FP64 accumulation */
+int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel_res);
+
+/* ---- R6, R10-R13: pointwise and stencil operators ----------------------------- */
+/* rsfe_splitbregman::set_heaviside (rsfe_splitbregman_imp.hxx:174-192) */
+int tk_heaviside(tk_ctx ctx, tk_vol phi, double a0, double b0, double epsilon, tk_vol H);
+/* image3d::grad (image3d_imp.hxx:743-888) */
+int tk_grad(tk_ctx ctx, tk_vol f, tk_vol g0, tk_vol g1, tk_vol g2);
+/* im3d::div (image3d_imp.hxx:947-1383), including its two boundary quirks */
+int tk_div(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol out);
+/* rsfe_splitbregman::edge_detect, GRAD branch, edge_detector_sigma == 0 (imp:196-316) */
+int tk_edge_detect(tk_ctx ctx, tk_vol u0, double beta, double lamda, double nu, tk_vol g);
+/* rsfe_splitbregman::shrink (imp:343-365): d = sign(b) max(|b| - g, 0) */
+int tk_shrink(tk_ctx ctx, tk_vol b, tk_vol g, tk_vol d);
+/* image3d::normL2 (image3d_imp.hxx:1419-1423) with FP64 accumulation */
+int tk_norm_l2(tk_ctx ctx, tk_vol a, double* out);
+/* ||a - b||_L2 without a temporary (rsfe_splitbregman_imp.hxx:1783,1832) */
+int tk_norm_l2_diff(tk_ctx ctx, tk_vol a, tk_vol b, double* out);
+/* image3d::min / max (image3d_imp.hxx:1456-1509) */
+int tk_minmax(tk_ctx ctx, tk_vol a, double* mn, double* mx);
+/* image3d::change_range_of_intensity (image3d_imp.hxx:1627-1635) */
+int tk_change_range_of_intensity(tk_ctx ctx, tk_vol a, double max, double min);
+/* rsfe_splitbregman::cut_phi (imp:320-339) */
+int tk_cut(tk_ctx ctx, tk_vol phi, double a0, double b0);
+
+/* ---- R14: threshold and seeded connected component ------------------------------
+ * image3d::im_to_black_and_white (image3d_imp.hxx:1591-1623) and
+ * image3d::connected_component(res, i, j, k, threshold, full, binary=true)
+ * (1694-1757 + sweep core 1821-2118).  out is TK_U8 (0/1).  full: 26- instead of
+ * 6-connectivity.  passes_out (may be NULL) receives the number of device passes. */
+int tk_im_to_black_and_white(tk_ctx ctx, tk_vol img, double threshold, int negative,
+                             tk_vol out);
+int tk_cc_seeded(tk_ctx ctx, tk_vol img, int i, int j, int k, double threshold, int full,
+                 tk_vol out, int* passes_out);
+/* full 6-connected component labelling of a U8 mask (labels are int32, 0 = background,
+   each component labelled by 1 + its smallest linear voxel index); no reference
+   counterpart beyond the seeded extraction above */
+int tk_cc_label(tk_ctx ctx, tk_vol mask_u8, int32_t* labels_device, int* iters_out);
+
+/* ---- R2 / R3 / R15: the Split-Bregman solver -------------------------------------
+ * segm::rsfe_splitbregman<T> (rsfe_splitbregman.hxx:99-894): constructor defaults
+ * (rsfe_splitbregman_imp.hxx:25-41) and apply(myim, init) (1547-1995) with
+ * cv = cyclic convolution and onestep_poisson = DirGaussSeidel. */
+typedef struct tk_rsfe_params {
+    double sigma, lamda, lamda1, lamda2, beta, dt, epsilon, a0, b0, nu, tol, ls_tol;
+    double radius;            /* Gaussian radius argument (3, imp:1641-1644) */
+    int maxiter, ls_steps;
+    int bc_dir;               /* 0: bc flag NEU -> boundary value 0; 1: DIR -> a0 (imp:1735-1743) */
+    int gaussian_pixel_approach;
+    int gs_order;             /* TK_GS_LEX / TK_GS_REDBLACK */
+    int max_sweeps;           /* cap per Poisson solve when ls_tol > 0 */
+    int warm_start;           /* 0: previous iterate (reference); 1: learned guess (R15) */
+} tk_rsfe_params;
+
+typedef struct tk_rsfe_iter_stats {
+    int sweeps;               /* S_t: Gauss-Seidel sweeps in this outer iteration */
+    double ls_err;            /* last relative update of the Poisson loop (-1 if not computed) */
+    double tollerance;        /* ||phi - phiold||_L2 / ||phiold||_L2 (imp:1832) */
+    int guess_used;           /* R15: 1 if the learned guess was accepted */
+} tk_rsfe_iter_stats;
+
+int tk_rsfe_default_params(tk_rsfe_params* p);
+/* set-up: filter spectrum, kones = K*1, kmyim = K*u0, g = edge_detect(u0) (imp:1641-1655);
+   phi := init rescaled to [a0,b0] (initialize_phi_with_init, imp:99-170, same dims) */
+int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, tk_rsfe* out);
+/* run up to n_iters outer iterations (stops early on tollerance <= tol);
+   stats may be NULL, else has room for n_iters entries */
+int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done);
+tk_vol tk_rsfe_phi(tk_rsfe s);
+int tk_rsfe_destroy(tk_rsfe s);
+
+/* ---- batched uneven volumes (many independent runs, one launch per stage) ------- */
+typedef struct tk_batch_s* tk_batch;
+int tk_batch_create(tk_ctx ctx, int n_vol, const int* dims3, tk_dtype dtype,
+                    const tk_rsfe_params* p, tk_batch* out);
+/* volume v's u0 and init are host arrays in the given dtype */
+int tk_batch_set_volume(tk_batch b, int v, const void* u0, const void* init, tk_dtype host_dtype);
+int tk_batch_iterate(tk_batch b, int n_iters, int use_graph);
+int tk_batch_get_phi(tk_batch b, int v, void* host, tk_dtype host_dtype);
+int tk_batch_destroy(tk_batch b);
+
+/* ---- synthetic volumes (host functions; no reference counterpart: the reference
+ * reads images from disk through VTK, interface_imp.hxx:33-88) -------------------- */
+int tk_gen_ellipsoids(int nx, int ny, int nz, uint64_t seed, int n_ell, const double* semi_axes,
+                      int margin, double sphere_radius, double* img, double* phi0);
+/* the standard phantom: n_ell ellipsoids with semi-axes cycling (12,9,7),(8,8,10)
+   times scale, margin 15*scale, phi0 sphere of radius 20*scale (<= 0.3125 min edge) */
+int tk_gen_standard(int nx, int ny, int nz, uint64_t seed, int scale, int n_ell, double* img,
+                    double* phi0);
+int tk_gen_batch_dims(uint64_t seed, int n_vol, int* dims3, int* n_ell);
+int tk_gen_ct(int nx, int ny, int nz, uint64_t seed, int n_vessels, double* img, int* seed_ijk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TK_H */

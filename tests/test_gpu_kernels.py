@@ -1,0 +1,288 @@
+"""GPU parity of each kernel of libtk.so against the CPU FP64 oracle (and, for the FP32
+red-black sweep, against a numpy float32 restatement).  Integer/index work and the
+FP64 lexicographic sweep are compared bit for bit; FP64 FFT work to 1e-13 relative;
+FP32 work within the tolerances written in each test."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ---- volumes ------------------------------------------------------------------------
+def test_volume_roundtrip_and_convert(tk, ctx, rng):
+    a = rng.standard_normal((7, 9, 11))
+    v = ctx.from_numpy(a)
+    assert np.array_equal(v.numpy(), a)
+    assert np.array_equal(v.numpy(np.float32), a.astype(np.float32))
+    w = ctx.volume(a.shape, np.float32)
+    w.upload(a)  # F64 host -> F32 device
+    assert np.array_equal(w.numpy(), a.astype(np.float32))
+    v.fill(2.5)
+    assert np.all(v.numpy() == 2.5)
+
+
+# ---- R4 / R5 ----------------------------------------------------------------------
+CONV_CASES = [((16, 16, 16), 1.5, (1.0, 1.0, 1.0)), ((48, 64, 80), 3.0, (1.0, 1.0, 1.0)),
+              ((32, 40, 24), 2.0, (0.5, 1.0, 2.0)), ((20, 30, 25), 1.0, (1.0, 1.0, 1.0)),
+              ((8, 9, 10), 2.0, (1.0, 1.0, 1.0)), ((64, 64, 64), 3.0, (1.0, 1.0, 1.0)),
+              # lengths that are not 2/3/5-smooth take the direct path; Nh > L wraps
+              ((28, 21, 26), 1.5, (1.0, 1.0, 1.0)), ((14, 11, 13), 3.0, (1.0, 1.0, 1.0))]
+
+
+@pytest.mark.parametrize("shape,sigma,h", CONV_CASES)
+def test_filter_spectrum_matches_oracle(tk, orc, ctx, shape, sigma, h):
+    say("spectrum", shape, sigma, h)
+    f = tk.Filter(ctx, shape, sigma, h=h)
+    tf, nh, spec, nc = orc.gaussian(shape, sigma, 3.0, False, h)
+    assert tuple(nh) == f.taps()
+    H = [f.axis_spectrum(a) for a in range(3)]
+    if tuple(nc) == shape:
+        prod = np.einsum("i,j,k->ijk", *H) / (h[0] * h[1] * h[2])
+        assert np.max(np.abs(prod - spec)) < 1e-13 / (h[0] * h[1] * h[2])
+
+
+@pytest.mark.parametrize("shape,sigma,h", CONV_CASES)
+def test_conv_cyclic_f64(tk, orc, ctx, shape, sigma, h, rng):
+    say("conv f64", shape, sigma, h)
+    x = rng.random(shape)
+    tf, nh, spec, nc = orc.gaussian(shape, sigma, 3.0, False, h)
+    ref = orc.cyclic_conv(spec, nc, x, h)
+    f = tk.Filter(ctx, shape, sigma, h=h)
+    vx = ctx.from_numpy(x, h=h)
+    vy = ctx.volume(shape, h=h)
+    f.conv(vy, vx)
+    got = vy.numpy()
+    assert np.max(np.abs(got - ref)) < 2e-15 * np.max(np.abs(x)) * np.log2(x.size)
+    # pair: two fields through one complex transform, outputs aliasing inputs
+    y2 = rng.standard_normal(shape)
+    va, vb = ctx.from_numpy(x, h=h), ctx.from_numpy(y2, h=h)
+    f.conv_pair(va, vb, va, vb)
+    ref2 = orc.cyclic_conv(spec, nc, y2, h)
+    assert np.max(np.abs(va.numpy() - ref)) < 4e-15 * np.log2(x.size) * np.max(np.abs(y2))
+    assert np.max(np.abs(vb.numpy() - ref2)) < 4e-15 * np.log2(x.size) * np.max(np.abs(y2))
+
+
+@pytest.mark.parametrize("shape,sigma", [((64, 64, 64), 3.0), ((48, 80, 96), 3.0),
+                                         ((128, 64, 160), 4.0)])
+def test_conv_cyclic_f32(tk, orc, ctx, shape, sigma, rng):
+    say("conv f32", shape, sigma)
+    x = rng.random(shape)
+    tf, nh, spec, nc = orc.gaussian(shape, sigma)
+    ref = orc.cyclic_conv(spec, nc, x)
+    f = tk.Filter(ctx, shape, sigma, dtype=np.float32)
+    vx = ctx.from_numpy(x.astype(np.float32))
+    vy = ctx.volume(shape, np.float32)
+    f.conv(vy, vx)
+    # FP32 FFT error grows like eps * log2(N): bound 1e-6 of the input scale
+    assert np.max(np.abs(vy.numpy().astype(np.float64) - ref)) < 1e-6
+
+
+# ---- R9 / R8: Gauss-Seidel -------------------------------------------------------
+GS_SHAPES = [(20, 18, 40), (11, 37, 70), (64, 64, 64), (5, 6, 7)]
+
+
+@pytest.mark.parametrize("shape", GS_SHAPES)
+@pytest.mark.parametrize("nsw", [1, 3])
+def test_gs_lex_bitwise(tk, orc, ctx, shape, nsw, rng):
+    say("gs lex", shape, nsw)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    ref = phi
+    for _ in range(nsw):
+        ref = orc.gs_sweep(ref, rhs, 1.0, 0.0)
+    vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    tk.poisson_sweeps(ctx, vp, vr, nsw, 1.0, 0.0, tk.TK_GS_LEX)
+    assert np.array_equal(vp.numpy(), ref)
+
+
+def test_gs_lex_bitwise_general_spacing(tk, orc, ctx, rng):
+    shape = (19, 23, 45)
+    h = (0.5, 1.0, 2.0)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    ref = orc.gs_sweep(orc.gs_sweep(phi, rhs, 0.7, 0.3, h), rhs, 0.7, 0.3, h)
+    vp, vr = ctx.from_numpy(phi, h=h), ctx.from_numpy(rhs, h=h)
+    tk.poisson_sweeps(ctx, vp, vr, 2, 0.7, 0.3, tk.TK_GS_LEX)
+    assert np.array_equal(vp.numpy(), ref)
+
+
+@pytest.mark.parametrize("ls_steps,ls_tol", [(5, 1e-6), (3, 1e-9), (40, 1e-3)])
+def test_gs_solve_lex_matches_oracle_loop(tk, orc, ctx, ls_steps, ls_tol, rng):
+    say("gs solve lex", ls_steps, ls_tol)
+    shape = (24, 22, 26)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    ref, s_ref, e_ref = orc.poisson_loop(phi, rhs, ls_steps, ls_tol)
+    vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    s, e = tk.poisson_solve(ctx, vp, vr, ls_steps, ls_tol, order=tk.TK_GS_LEX)
+    assert s == s_ref
+    assert np.array_equal(vp.numpy(), ref)
+    assert abs(e - e_ref) <= 1e-10 * e_ref
+
+
+def rb_sweep_f32(phi, rhs):
+    """red (i+j+k even) then black, float32 throughout, operand order of the kernel."""
+    X, Y, Z = phi.shape
+    p = phi.astype(np.float32).copy()
+    b = rhs.astype(np.float32)
+    ht = np.float32(1.0 / 7.0)
+    i, j, k = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+    par = (i + j + k) % 2
+    p[0] = 0; p[-1] = 0; p[:, 0] = 0; p[:, -1] = 0; p[:, :, 0] = 0; p[:, :, -1] = 0
+    for c in (0, 1):
+        s = slice(1, -1)
+        v = (((p[2:, s, s] + p[:-2, s, s]) + (p[s, 2:, s] + p[s, :-2, s])) +
+             (p[s, s, 2:] + p[s, s, :-2]) + b[s, s, s] + p[s, s, s]) * ht
+        m = par[s, s, s] == c
+        inner = p[s, s, s]
+        inner[m] = v[m]
+        p[s, s, s] = inner
+    return p
+
+
+@pytest.mark.parametrize("shape", [(16, 18, 20), (33, 17, 9)])
+def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rng):
+    say("gs rb", shape)
+    phi = rng.random(shape).astype(np.float32)
+    rhs = rng.standard_normal(shape).astype(np.float32)
+    ref = phi
+    for _ in range(3):
+        ref = rb_sweep_f32(ref, rhs)
+    vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    tk.poisson_sweeps(ctx, vp, vr, 3, 1.0, 0.0, tk.TK_GS_REDBLACK)
+    assert np.array_equal(vp.numpy(), ref)
+
+
+def test_gs_redblack_gate_matches_replay(tk, ctx, rng):
+    shape = (20, 22, 24)
+    phi = rng.random(shape).astype(np.float32)
+    rhs = rng.standard_normal(shape).astype(np.float32)
+    tol = 1e-4
+    q = phi
+    s = 0
+    while True:
+        prev = q.astype(np.float64)  # the whole volume before the sweep, faces included
+        q = rb_sweep_f32(q, rhs)
+        s += 1
+        d = q.astype(np.float64) - prev
+        err = np.sqrt((d * d).sum()) / np.sqrt((prev * prev).sum())
+        if s >= 5 and not err > tol:
+            break
+    vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    sg, eg = tk.poisson_solve(ctx, vp, vr, 5, tol, order=tk.TK_GS_REDBLACK)
+    assert sg == s
+    assert np.array_equal(vp.numpy(), q)
+    assert abs(eg - err) < 1e-6 * err
+
+
+def test_poisson_residual(tk, ctx, rng):
+    shape = (12, 13, 14)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    s = slice(1, -1)
+    lap = ((phi[2:, s, s] + phi[:-2, s, s] - 2 * phi[s, s, s]) +
+           (phi[s, 2:, s] + phi[s, :-2, s] - 2 * phi[s, s, s]) +
+           (phi[s, s, 2:] + phi[s, s, :-2] - 2 * phi[s, s, s]))
+    ref = np.linalg.norm(rhs[s, s, s] + lap) / np.linalg.norm(rhs[s, s, s])
+    got = tk.poisson_residual(ctx, ctx.from_numpy(phi), ctx.from_numpy(rhs))
+    assert abs(got - ref) < 1e-12 * ref
+
+
+# ---- R6, R10-R13 --------------------------------------------------------------------
+def test_pointwise_and_stencils_f64(tk, orc, ctx, rng):
+    lib = tk.lib
+    shape = (13, 17, 19)
+    h = (0.5, 1.0, 2.0)
+    phi = rng.random(shape) * 1.2 - 0.1
+    v = ctx.from_numpy(phi, h=h)
+    out = ctx.volume(shape, h=h)
+    tk._check(lib.tk_heaviside(ctx.h, v.h, 0.0, 1.0, 1e-3, out.h))
+    assert np.allclose(out.numpy(), orc.heaviside(phi), rtol=4e-16, atol=4e-16)
+    g = [ctx.volume(shape, h=h) for _ in range(3)]
+    tk._check(lib.tk_grad(ctx.h, v.h, g[0].h, g[1].h, g[2].h))
+    rg = orc.grad(phi, h)
+    for a in range(3):
+        assert np.array_equal(g[a].numpy(), rg[a])
+    tk._check(lib.tk_div(ctx.h, g[0].h, g[1].h, g[2].h, out.h))
+    assert np.array_equal(out.numpy(), orc.div(*rg, h))
+    # fields that expose the two div quirks
+    i, j, k = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    f = [ctx.from_numpy(i * 1.0 + 0.1 * k, h=h), ctx.from_numpy(j * 2.0, h=h),
+         ctx.from_numpy(k * 3.0 + i * 0.5, h=h)]
+    tk._check(lib.tk_div(ctx.h, f[0].h, f[1].h, f[2].h, out.h))
+    assert np.array_equal(out.numpy(), orc.div(i * 1.0 + 0.1 * k, j * 2.0, k * 3.0 + i * 0.5, h))
+    tk._check(lib.tk_edge_detect(ctx.h, v.h, 100.0, 5e-2, 1.0, out.h))
+    assert np.array_equal(out.numpy(), orc.edge_detect(phi, 100.0, 5e-2, 1.0, h))
+    b = rng.standard_normal(shape)
+    gg = rng.random(shape)
+    vb, vg = ctx.from_numpy(b, h=h), ctx.from_numpy(gg, h=h)
+    tk._check(lib.tk_shrink(ctx.h, vb.h, vg.h, out.h))
+    assert np.array_equal(out.numpy(), orc.shrink(b, gg))
+
+
+def test_norms_minmax_range_cut(tk, orc, ctx, rng):
+    import ctypes as C
+    lib = tk.lib
+    a = rng.standard_normal((21, 22, 23))
+    b = rng.standard_normal(a.shape)
+    va, vb = ctx.from_numpy(a), ctx.from_numpy(b)
+    r = C.c_double()
+    tk._check(lib.tk_norm_l2(ctx.h, va.h, C.byref(r)))
+    assert abs(r.value - orc.norm_l2(a)) < 1e-14 * r.value
+    tk._check(lib.tk_norm_l2_diff(ctx.h, va.h, vb.h, C.byref(r)))
+    assert abs(r.value - orc.norm_l2(a - b)) < 1e-14 * r.value
+    mn, mx = C.c_double(), C.c_double()
+    tk._check(lib.tk_minmax(ctx.h, va.h, C.byref(mn), C.byref(mx)))
+    assert mn.value == a.min() and mx.value == a.max()
+    tk._check(lib.tk_change_range_of_intensity(ctx.h, va.h, 255.0, 0.0))
+    assert np.array_equal(va.numpy(), orc.change_range(a, 255.0, 0.0))
+    tk._check(lib.tk_cut(ctx.h, vb.h, 0.0, 1.0))
+    assert np.array_equal(vb.numpy(), np.clip(b, 0.0, 1.0))
+
+
+# ---- R14 ---------------------------------------------------------------------------
+def enumerate_img(X, Y, Z):
+    i, j, k = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
+    return (i + j * X + k * X * Y).astype(np.float64)
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_cc_seeded_bit_exact(tk, orc, ctx, full, rng):
+    cases = [(enumerate_img(4, 4, 4), (3, 3, 3)),
+             ((enumerate_img(3, 3, 3) % 2 == 0).astype(float), (3, 3, 3))]
+    for q in range(3):
+        img = rng.random((17, 19, 23))
+        img = (img > 0.42 + 0.04 * q).astype(np.float64) + rng.random(img.shape) * 0.05
+        cases.append((img, (8, 9, 10)))
+    tube = np.zeros((40, 30, 50))
+    tube[5:35, 14:17, 20:23] = 1.0
+    tube[33:36, 14:17, 5:45] = 1.0
+    cases.append((tube, (6, 15, 21)))
+    for img, seed in cases:
+        say("cc", img.shape, seed, full)
+        ref, _ = orc.connected_component(img, seed, 0.5, full)
+        vi = ctx.from_numpy(img)
+        out = ctx.volume(img.shape, np.uint8)
+        tk._check(tk.lib.tk_cc_seeded(ctx.h, vi.h, *seed, 0.5, int(full), out.h, None))
+        assert np.array_equal(out.numpy(), ref.astype(np.uint8))
+
+
+def test_cc_label_matches_scipy(tk, ctx, rng):
+    from scipy import ndimage
+    import ctypes as C
+    m = (rng.random((30, 31, 32)) > 0.6).astype(np.uint8)
+    vm = ctx.from_numpy(m)
+    lab_dev = ctx.volume(m.shape, np.float32)  # 4-byte storage for int32 labels
+    tk._check(tk.lib.tk_cc_label(ctx.h, vm.h, tk.lib.tk_vol_device_ptr(lab_dev.h), None))
+    raw = lab_dev.numpy().view(np.int32)
+    ref, n = ndimage.label(m)
+    flat = np.arange(m.size).reshape(m.shape)
+    expect = np.zeros(m.shape, np.int32)
+    for c in range(1, n + 1):
+        sel = ref == c
+        expect[sel] = flat[sel].min() + 1
+    assert np.array_equal(raw, expect)
