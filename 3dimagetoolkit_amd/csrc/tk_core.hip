@@ -389,7 +389,7 @@ int tk_ctx_destroy(tk_ctx c)
     if (!c) return TK_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 6; ++i)
         if (c->work[i]) (void)hipFree(c->work[i]);
     if (c->acc) (void)hipFree(c->acc);
     if (c->cnt) (void)hipFree(c->cnt);

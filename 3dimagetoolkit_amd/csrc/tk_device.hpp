@@ -22,23 +22,108 @@ __device__ inline float wave_min(float v)
     return v;
 }
 
-// block sum of two doubles; result valid in thread 0.  blockDim.x <= 1024.
+__device__ inline int flat_tid()
+{
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+
+// block sum of two doubles; result valid in the thread with flat_tid() == 0.
+// At most 1024 threads per block.
 __device__ inline void block_sum2(double& a, double& b)
 {
     __shared__ double sa[16], sb[16];
     a = wave_sum(a);
     b = wave_sum(b);
-    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-    const int nw = (blockDim.x + kWave - 1) / kWave;
+    const int t = flat_tid();
+    const int w = t / kWave, l = t % kWave;
+    const int nw = (blockDim.x * blockDim.y * blockDim.z + kWave - 1) / kWave;
     __syncthreads();
     if (l == 0) { sa[w] = a; sb[w] = b; }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         double x = 0, y = 0;
         for (int i = 0; i < nw; ++i) { x += sa[i]; y += sb[i]; }
         a = x;
         b = y;
     }
+}
+
+// Visit the voxels of an X*Y*Z volume with a 2-D block: threads along k (contiguous),
+// block rows over the X*Y lines.  One 32-bit division per line, none per voxel.
+#define TK_FOR_VOXELS(X, Y, Z, i, j, k, v)                                                   \
+    for (int r_ = blockIdx.y * blockDim.y + threadIdx.y; r_ < (X) * (Y);                  \
+         r_ += gridDim.y * blockDim.y)                                                       \
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x, i = r_ / (Y), j = r_ - i * (Y);  \
+             k < (Z); k += gridDim.x * blockDim.x)                                           \
+            for (long v = (long)r_ * (Z) + k, once_ = 1; once_; once_ = 0)
+
+inline dim3 vox_block(int Z) { return Z >= 64 ? dim3(64, 4) : dim3(32, 8); }
+// grid with at most max_blocks workgroups (the row loop covers the rest)
+inline dim3 vox_grid(int X, int Y, int Z, int max_blocks)
+{
+    const dim3 b = vox_block(Z);
+    const int gx = (Z + b.x - 1) / b.x;
+    long gy = ((long)X * Y + b.y - 1) / b.y;
+    long cap = max_blocks / gx;
+    if (cap < 1) cap = 1;
+    if (gy > cap) gy = cap;
+    return dim3(gx, (unsigned)gy);
+}
+
+// 16-byte rows: V = 4 floats or 2 doubles per lane (global_load_dwordx4)
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int n = 2;
+};
+template <class T>
+struct Row {
+    T v[Vec16<T>::n];
+};
+template <class T>
+__device__ __forceinline__ Row<T> ldrow(const T* __restrict__ p)
+{
+    typedef typename Vec16<T>::type VT;
+    const VT x = *(const VT*)p;
+    Row<T> r;
+    __builtin_memcpy(r.v, &x, 16);
+    return r;
+}
+template <class T>
+__device__ __forceinline__ void strow(T* __restrict__ p, const Row<T>& r)
+{
+    typedef typename Vec16<T>::type VT;
+    VT x;
+    __builtin_memcpy(&x, r.v, 16);
+    *(VT*)p = x;
+}
+
+// Visit an X*Y*Z volume in 16-byte rows (Z % V == 0): threads along k in steps of V.
+#define TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)                                                \
+    for (int r_ = blockIdx.y * blockDim.y + threadIdx.y; r_ < (X) * (Y);                  \
+         r_ += gridDim.y * blockDim.y)                                                       \
+        for (int k0 = (blockIdx.x * blockDim.x + threadIdx.x) * (V), i = r_ / (Y),           \
+                 j = r_ - i * (Y);                                                           \
+             k0 < (Z); k0 += gridDim.x * blockDim.x * (V))                                   \
+            for (long v0 = (long)r_ * (Z) + k0, once_ = 1; once_; once_ = 0)
+
+inline dim3 row_block(int Zv) { return Zv >= 64 ? dim3(64, 4) : (Zv >= 32 ? dim3(32, 8) : dim3(16, 16)); }
+inline dim3 row_grid(int X, int Y, int Zv, int max_blocks)
+{
+    const dim3 b = row_block(Zv);
+    const int gx = (Zv + b.x - 1) / b.x;
+    long gy = ((long)X * Y + b.y - 1) / b.y;
+    long cap = max_blocks / gx;
+    if (cap < 1) cap = 1;
+    if (gy > cap) gy = cap;
+    return dim3(gx, (unsigned)gy);
 }
 
 template <class T>

@@ -64,7 +64,8 @@ struct AxisGeom {
     int nx, ny, nz;
     long nlines;   // number of pencils
     long stride;   // element stride along the axis
-    int P;         // pencils per workgroup
+    int P;         // pencils per workgroup (a power of two)
+    int logP;
 };
 
 // base voxel index of pencil c

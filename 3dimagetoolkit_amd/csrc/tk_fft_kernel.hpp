@@ -72,15 +72,27 @@ __device__ __forceinline__ void dftR(cplx<T>* a, int s)
     else dft5(a, s);
 }
 
-// one Stockham stage over P pencils of length L held in LDS as buf[pos*PP + p]
+// exact quotient of small non-negative integers (n < 2^13, d <= 2^12) through a float
+// reciprocal: the distance of (n + 1/2)/d to an integer is >= 1/(2d) >> the rounding
+// error of the product, so the truncation is exact
+__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
+// LDS layout of a group of P pencils of length L: element (pos, p) at pos*SP + p*SQ.
+// x/y passes use SP = P+1, SQ = 1 (pencils interleaved); z passes use SP = 1,
+// SQ = L+1 (pencils as padded rows), so the contiguous global rows map to LDS rows.
+struct Lay {
+    int L, P, lp, SP, SQ;
+};
+
+// one Stockham stage; butterfly b handles pencil p = b mod P at index j = b / P
 template <class T, int R, int EPT, int NT>
 __device__ __forceinline__ void fft_stage(cplx<T>* __restrict__ buf, const cplx<T>* __restrict__ tw,
-                                          int L, int P, int PP, int Ns, int sign)
+                                          const Lay& y, int Ns, float inv_ns, int sign)
 {
     constexpr int MAXB = (EPT + R - 1) / R;
-    const int LR = L / R;
-    const int nbf = P * LR;
-    const int stw = L / (Ns * R);
+    const int LR = y.L / R;
+    const int nbf = y.P * LR;
+    const int stw = y.L / (Ns * R);
     cplx<T> v[MAXB][R];
     int dst[MAXB];
 #pragma unroll
@@ -88,10 +100,12 @@ __device__ __forceinline__ void fft_stage(cplx<T>* __restrict__ buf, const cplx<
         const int b = threadIdx.x + q * NT;
         dst[q] = -1;
         if (b < nbf) {
-            const int p = b % P, j = b / P;
-            const int k = j % Ns;
+            const int p = b & (y.P - 1), j = b >> y.lp;
+            const int jq = qdiv(j, inv_ns);
+            const int k = j - jq * Ns;
+            const int src = j * y.SP + p * y.SQ;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = buf[(j + r * LR) * PP + p];
+            for (int r = 0; r < R; ++r) v[q][r] = buf[src + r * LR * y.SP];
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 cplx<T> w = tw[r * k * stw];
@@ -99,7 +113,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* __restrict__ buf, const cplx<
                 v[q][r] = cmul(v[q][r], w);
             }
             dftR<T, R>(v[q], sign);
-            dst[q] = ((j / Ns) * Ns * R + k) * PP + p;
+            dst[q] = (jq * Ns * R + k) * y.SP + p * y.SQ;
         }
     }
     __syncthreads();
@@ -107,23 +121,24 @@ __device__ __forceinline__ void fft_stage(cplx<T>* __restrict__ buf, const cplx<
     for (int q = 0; q < MAXB; ++q) {
         if (dst[q] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns * PP] = v[q][r];
+            for (int r = 0; r < R; ++r) buf[dst[q] + r * Ns * y.SP] = v[q][r];
         }
     }
     __syncthreads();
 }
 
 template <class T, int EPT, int NT>
-__device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, const AxisDev& ax, int P,
-                                        int PP, int sign)
+__device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, const AxisDev& ax,
+                                        const Lay& y, int sign)
 {
     int Ns = 1;
     for (int s = 0; s < ax.nstage; ++s) {
         const int R = ax.radix[s];
-        if (R == 4) fft_stage<T, 4, EPT, NT>(buf, tw, ax.L, P, PP, Ns, sign);
-        else if (R == 2) fft_stage<T, 2, EPT, NT>(buf, tw, ax.L, P, PP, Ns, sign);
-        else if (R == 3) fft_stage<T, 3, EPT, NT>(buf, tw, ax.L, P, PP, Ns, sign);
-        else fft_stage<T, 5, EPT, NT>(buf, tw, ax.L, P, PP, Ns, sign);
+        const float inv = 1.0f / (float)Ns;
+        if (R == 4) fft_stage<T, 4, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        else if (R == 2) fft_stage<T, 2, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        else if (R == 3) fft_stage<T, 3, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        else fft_stage<T, 5, EPT, NT>(buf, tw, y, Ns, inv, sign);
         Ns *= R;
     }
 }
@@ -131,23 +146,23 @@ __device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, const A
 // direct circular convolution along the pencil with nh taps (lengths that are not
 // 2/3/5-smooth): y(i) = sum_n w(n) x((i - n) mod L), n in [-s, s]
 template <class T, int EPT, int NT>
-__device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisDev& ax, int P,
-                                           int PP)
+__device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisDev& ax,
+                                           const Lay& y)
 {
-    const int L = ax.L, s = ax.nh / 2;
-    const int E = L * P;
+    const int L = y.L, s = ax.nh / 2;
+    const int E = L * y.P;
     cplx<T> out[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * NT;
         out[q] = {0, 0};
         if (e < E) {
-            const int pos = e / P, p = e % P;
+            const int p = e & (y.P - 1), pos = e >> y.lp;
             T ax_ = 0, ay = 0;
             for (int n = -s; n <= s; ++n) {
                 int m = (pos - n) % L;
                 if (m < 0) m += L;
-                const cplx<T> v = buf[m * PP + p];
+                const cplx<T> v = buf[m * y.SP + p * y.SQ];
                 ax_ += w[n + s] * v.x;
                 ay += w[n + s] * v.y;
             }
@@ -158,7 +173,7 @@ __device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisD
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * NT;
-        if (e < E) buf[(e / P) * PP + e % P] = out[q];
+        if (e < E) buf[(e >> y.lp) * y.SP + (e & (y.P - 1)) * y.SQ] = out[q];
     }
     __syncthreads();
 }
@@ -166,29 +181,85 @@ __device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisD
 // forward FFT, multiply by the (real, 1/L-scaled) spectrum, inverse FFT
 template <class T, int EPT, int NT>
 __device__ __forceinline__ void conv_lds(cplx<T>* buf, const cplx<T>* tw, const T* spec,
-                                         const AxisDev& ax, int P, int PP)
+                                         const AxisDev& ax, const Lay& y)
 {
     if (ax.nstage == 0) {
-        direct_lds<T, EPT, NT>(buf, spec, ax, P, PP);
+        direct_lds<T, EPT, NT>(buf, spec, ax, y);
         return;
     }
-    fft_lds<T, EPT, NT>(buf, tw, ax, P, PP, -1);
-    const int E = ax.L * P;
+    fft_lds<T, EPT, NT>(buf, tw, ax, y, -1);
+    const int E = y.L * y.P;
     for (int e = threadIdx.x; e < E; e += NT) {
-        const int pos = e / P, p = e % P;
-        cplx<T> v = buf[pos * PP + p];
+        const int p = e & (y.P - 1), pos = e >> y.lp;
+        cplx<T>& v = buf[pos * y.SP + p * y.SQ];
         const T s = spec[pos];
-        buf[pos * PP + p] = {v.x * s, v.y * s};
+        v = {v.x * s, v.y * s};
     }
     __syncthreads();
-    fft_lds<T, EPT, NT>(buf, tw, ax, P, PP, +1);
+    fft_lds<T, EPT, NT>(buf, tw, ax, y, +1);
+}
+
+// ---- quads: 4 consecutive voxels moved with 16-byte accesses ------------------------
+template <class T>
+struct Quad {
+    T v[4];
+};
+template <class T>
+__device__ __forceinline__ Quad<T> ldq(const T* __restrict__ p)
+{
+    Quad<T> q;
+    if constexpr (sizeof(T) == 4) {
+        const float4 x = *(const float4*)p;
+        __builtin_memcpy(q.v, &x, 16);
+    } else {
+        const double2 x = ((const double2*)p)[0], y = ((const double2*)p)[1];
+        __builtin_memcpy(q.v, &x, 16);
+        __builtin_memcpy(q.v + 2, &y, 16);
+    }
+    return q;
+}
+template <class T>
+__device__ __forceinline__ void stq(T* __restrict__ p, const Quad<T>& q)
+{
+    if constexpr (sizeof(T) == 4) {
+        float4 x;
+        __builtin_memcpy(&x, q.v, 16);
+        *(float4*)p = x;
+    } else {
+        double2 x, y;
+        __builtin_memcpy(&x, q.v, 16);
+        __builtin_memcpy(&y, q.v + 2, 16);
+        ((double2*)p)[0] = x;
+        ((double2*)p)[1] = y;
+    }
+}
+// four consecutive complex values (32 or 64 bytes)
+template <class T>
+__device__ __forceinline__ void ldc4(const cplx<T>* __restrict__ p, cplx<T>* c)
+{
+    const T* s = (const T*)p;
+    Quad<T> a = ldq(s), b = ldq(s + 4);
+    c[0] = {a.v[0], a.v[1]};
+    c[1] = {a.v[2], a.v[3]};
+    c[2] = {b.v[0], b.v[1]};
+    c[3] = {b.v[2], b.v[3]};
+}
+template <class T>
+__device__ __forceinline__ void stc4(cplx<T>* __restrict__ p, const cplx<T>* c)
+{
+    T* s = (T*)p;
+    stq(s, Quad<T>{{c[0].x, c[0].y, c[1].x, c[1].y}});
+    stq(s + 4, Quad<T>{{c[2].x, c[2].y, c[3].x, c[3].y}});
 }
 
 // ---- functors ---------------------------------------------------------------------
+// Each functor works on one voxel (operator()) or on a quad of four consecutive
+// voxels starting at v (ld4 / md4 / st4), with identical arithmetic.
 template <class T>
 struct LdComplex {
     const cplx<T>* src;
     __device__ cplx<T> operator()(long v) const { return src[v]; }
+    __device__ void ld4(long v, cplx<T>* c) const { ldc4(src + v, c); }
 };
 // (a, b) packed as one complex field; a null pointer reads as `fill`
 template <class T>
@@ -200,6 +271,13 @@ struct LdPair {
     {
         return {a ? a[v] : fill_a, b ? b[v] : fill_b};
     }
+    __device__ void ld4(long v, cplx<T>* c) const
+    {
+        const Quad<T> qa = a ? ldq(a + v) : Quad<T>{{fill_a, fill_a, fill_a, fill_a}};
+        const Quad<T> qb = b ? ldq(b + v) : Quad<T>{{fill_b, fill_b, fill_b, fill_b}};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = {qa.v[e], qb.v[e]};
+    }
 };
 struct MdNone {
 };
@@ -207,6 +285,7 @@ template <class T>
 struct StComplex {
     cplx<T>* dst;
     __device__ void operator()(long v, cplx<T> c) const { dst[v] = c; }
+    __device__ void st4(long v, const cplx<T>* c) const { stc4(dst + v, c); }
 };
 template <class T>
 struct StPair {
@@ -217,57 +296,150 @@ struct StPair {
         if (a) a[v] = c.x;
         if (b) b[v] = c.y;
     }
+    __device__ void st4(long v, const cplx<T>* c) const
+    {
+        if (a) stq(a + v, Quad<T>{{c[0].x, c[1].x, c[2].x, c[3].x}});
+        if (b) stq(b + v, Quad<T>{{c[0].y, c[1].y, c[2].y, c[3].y}});
+    }
 };
 
-template <class T, int EPT, int NT, class Ld, class Md, class St>
+// Global <-> LDS element maps, chosen for coalescing.  x/y passes: thread t owns
+// pencil p = t mod P for all its elements (one base address per thread), positions
+// advance by NT/P.  z passes: the P pencils are P contiguous rows, element e of the
+// group is voxel c0*L + e.
+template <class T, class F>
+__device__ __forceinline__ void for_group(const AxisGeom& g, const Lay& y, long c0, int NT, F f)
+{
+    const int E = y.L * y.P;
+    if (g.axis == 2) {
+        const float invL = 1.0f / (float)y.L;
+        const long v0 = c0 * y.L;
+        const long vend = g.nlines * (long)y.L;
+        for (int e = threadIdx.x; e < E; e += NT) {
+            const int p = qdiv(e, invL), pos = e - p * y.L;
+            if (v0 + e < vend) f(v0 + e, pos * y.SP + p * y.SQ);
+        }
+    } else {
+        const int p = threadIdx.x & (y.P - 1);
+        const long c = c0 + p;
+        if (c >= g.nlines) return;
+        long base;
+        if (g.axis == 1) {
+            const int cc = (int)c;
+            const int i = cc / g.nz;
+            base = (long)i * g.ny * g.nz + (cc - i * g.nz);
+        } else {
+            base = c;
+        }
+        const int step = NT >> y.lp;
+        for (int pos = threadIdx.x >> y.lp; pos < y.L; pos += step)
+            f(base + (long)pos * g.stride, pos * y.SP + p);
+    }
+}
+
+// Quad traversal (every axis length and nz a multiple of 4, P >= 4): x/y passes give a
+// thread 4 adjacent pencils at one position (4 adjacent voxels in memory); z passes give
+// it 4 adjacent positions of one pencil.  f(v, l, lstep): quad at voxel v, LDS elements
+// l + q*lstep.
+template <class T, class F>
+__device__ __forceinline__ void for_quads(const AxisGeom& g, const Lay& y, long c0, int NT, F f)
+{
+    if (g.axis == 2) {
+        const int L4 = y.L >> 2;
+        const float inv = 1.0f / (float)L4;
+        const int nq = y.P * L4;
+        for (int e4 = threadIdx.x; e4 < nq; e4 += NT) {
+            const int p = qdiv(e4, inv), pos0 = (e4 - p * L4) << 2;
+            if (c0 + p < g.nlines) f((c0 + p) * y.L + pos0, pos0 + p * y.SQ, 1);
+        }
+    } else {
+        const int PQ = y.P >> 2;
+        const int p0 = (threadIdx.x & (PQ - 1)) << 2;
+        const long c = c0 + p0;
+        if (c >= g.nlines) return;
+        long base;
+        if (g.axis == 1) {
+            const int cc = (int)c;
+            const int i = cc / g.nz;
+            base = (long)i * g.ny * g.nz + (cc - i * g.nz);
+        } else {
+            base = c;
+        }
+        const int lpq = y.lp - 2;
+        const int step = NT >> lpq;
+        for (int pos = threadIdx.x >> lpq; pos < y.L; pos += step)
+            f(base + (long)pos * g.stride, pos * y.SP + p0, 1);
+    }
+}
+
+template <class T, int EPT, int NT, bool QUAD, class Ld, class Md, class St>
 __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int rounds, Ld ld,
                                                   Md md, St st)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int L = ax.L, P = g.P, PP = g.P + 1;
+    Lay y;
+    y.L = ax.L;
+    y.P = g.P;
+    y.lp = g.logP;
+    y.SP = g.axis == 2 ? 1 : g.P + 1;
+    y.SQ = g.axis == 2 ? ax.L + 1 : 1;
+    const int nbuf = g.axis == 2 ? g.P * (ax.L + 1) : ax.L * (g.P + 1);
     cplx<T>* buf = (cplx<T>*)smem;
-    cplx<T>* tw = buf + L * PP;
-    T* spec = (T*)(tw + L);
+    cplx<T>* tw = buf + nbuf;
+    T* spec = (T*)(tw + ax.L);
     if (ax.nstage > 0) {
-        for (int t = threadIdx.x; t < L; t += NT) {
+        for (int t = threadIdx.x; t < ax.L; t += NT) {
             tw[t] = ((const cplx<T>*)ax.tw)[t];
             spec[t] = ((const T*)ax.spec)[t];
         }
     } else {
         for (int t = threadIdx.x; t < ax.nh; t += NT) spec[t] = ((const T*)ax.spec)[t];
     }
-    const long c0 = (long)blockIdx.x * P;
-    const int E = L * P;
-    // load: coalesced along memory; z pencils are contiguous rows, x/y pencils are
-    // P adjacent columns of a plane
-    for (int e = threadIdx.x; e < E; e += NT) {
-        int pos, p;
-        if (g.axis == 2) { p = e / L; pos = e % L; }
-        else { pos = e / P; p = e % P; }
-        cplx<T> v = {0, 0};
-        if (c0 + p < g.nlines) v = ld(pencil_base(g, c0 + p) + pos * g.stride);
-        buf[pos * PP + p] = v;
+    const long c0 = (long)blockIdx.x * g.P;
+    // pencils past the end of the volume stay zero
+    if ((c0 + g.P) > g.nlines) {
+        for (int e = threadIdx.x; e < nbuf; e += NT) buf[e] = {0, 0};
+        __syncthreads();
+    }
+    if constexpr (QUAD) {
+        for_quads<T>(g, y, c0, NT, [&](long v, int l, int ls) {
+            cplx<T> c[4];
+            ld.ld4(v, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) buf[l + q * ls] = c[q];
+        });
+    } else {
+        for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = ld(v); });
     }
     __syncthreads();
-    conv_lds<T, EPT, NT>(buf, tw, spec, ax, P, PP);
+    conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
     if constexpr (!std::is_same<Md, MdNone>::value) {
         if (rounds > 1) {
-            for (int e = threadIdx.x; e < E; e += NT) {
-                const int pos = e / P, p = e % P;
-                if (c0 + p < g.nlines) {
-                    const long v = pencil_base(g, c0 + p) + pos * g.stride;
-                    buf[pos * PP + p] = md(v, buf[pos * PP + p]);
-                }
+            if constexpr (QUAD) {
+                for_quads<T>(g, y, c0, NT, [&](long v, int l, int ls) {
+                    cplx<T> c[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q] = buf[l + q * ls];
+                    md.md4(v, c);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) buf[l + q * ls] = c[q];
+                });
+            } else {
+                for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = md(v, buf[l]); });
             }
             __syncthreads();
-            conv_lds<T, EPT, NT>(buf, tw, spec, ax, P, PP);
+            conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
         }
     }
-    for (int e = threadIdx.x; e < E; e += NT) {
-        int pos, p;
-        if (g.axis == 2) { p = e / L; pos = e % L; }
-        else { pos = e / P; p = e % P; }
-        if (c0 + p < g.nlines) st(pencil_base(g, c0 + p) + pos * g.stride, buf[pos * PP + p]);
+    if constexpr (QUAD) {
+        for_quads<T>(g, y, c0, NT, [&](long v, int l, int ls) {
+            cplx<T> c[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q] = buf[l + q * ls];
+            st.st4(v, c);
+        });
+    } else {
+        for_group<T>(g, y, c0, NT, [&](long v, int l) { st(v, buf[l]); });
     }
 }
 
@@ -285,9 +457,13 @@ inline AxisGeom make_geom(int axis, int nx, int ny, int nz, int L, int elems)
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.nlines = (long)nx * ny * nz / L;
     g.stride = axis == 2 ? 1 : (axis == 1 ? nz : (long)ny * nz);
-    int P = 1;
-    while (P * 2 * L <= elems) P *= 2;
+    int P = 1, lp = 0;
+    while (P * 2 * L <= elems && P < 256) {
+        P *= 2;
+        ++lp;
+    }
     g.P = P;
+    g.logP = lp;
     return g;
 }
 
@@ -298,13 +474,20 @@ int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld 
     constexpr int EPT = conv_ept<T>();
     const AxisDev& ax = f->ax[axis];
     AxisGeom g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, EPT * NT);
-    TK_CHECK(g.P >= 1 && ax.L * g.P <= EPT * NT, TK_ERR_UNSUPPORTED,
-             "conv: axis length %d exceeds %d", ax.L, EPT * NT);
-    size_t lds = (size_t)ax.L * (g.P + 1) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
+    TK_CHECK(ax.L * g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
+             ax.L, EPT * NT);
+    const size_t nbuf = axis == 2 ? (size_t)g.P * (ax.L + 1) : (size_t)ax.L * (g.P + 1);
+    size_t lds = nbuf * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
                  (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
     unsigned nb = (unsigned)((g.nlines + g.P - 1) / g.P);
-    hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, Ld, Md, St>), dim3(nb), dim3(NT), lds,
-                       ctx->stream, ax, g, rounds, ld, md, st);
+    const bool quad = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0 &&
+                      g.P >= 4;
+    if (quad)
+        hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, true, Ld, Md, St>), dim3(nb), dim3(NT), lds,
+                           ctx->stream, ax, g, rounds, ld, md, st);
+    else
+        hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, false, Ld, Md, St>), dim3(nb), dim3(NT), lds,
+                           ctx->stream, ax, g, rounds, ld, md, st);
     TK_LAUNCHED();
     return TK_OK;
 }

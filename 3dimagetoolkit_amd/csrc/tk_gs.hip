@@ -228,52 +228,269 @@ struct RbGate {
     double bnd_o2;      // boundary contribution to sum old^2 for sweeps >= 1
 };
 
+// Fused red-black sweep, one launch per sweep, dst = sweep(src) (ping-pong buffers).
+// A workgroup owns a TJ x TK tile of (j, k) and marches over CI planes in i.  For each
+// plane p it forms R(p), the state after the red half-sweep, on the tile plus a one-voxel
+// ring (from old values on a two-voxel ring), then the black half-sweep of the tile from
+// R(p-1), R(p), R(p+1).  Neighbouring tiles recompute the same ring values from the same
+// old values, so they agree bit for bit and no workgroup waits for another.  The result
+// equals the in-place red pass followed by the black pass.
+constexpr int kRbTJ = 8, kRbTK = 64, kRbCI = 16;
+
 template <class T, bool UNIT>
-__global__ __launch_bounds__(256) void k_gs_rb(T* __restrict__ phi, const T* __restrict__ rhs,
-                                               int X, int Y, int Z, int color, GsCoef c, int s,
-                                               int slot, int gated, RbGate g)
+__global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* __restrict__ dst,
+                                                  const T* __restrict__ rhs, int X, int Y, int Z,
+                                                  GsCoef c, int s, int slot, int gated, RbGate g)
 {
     if (gated && *(volatile unsigned*)g.stop) return;
-    const int nk = (Z - 2 + 1) / 2;  // candidates per row
-    const long rows = (long)(X - 2) * (Y - 2);
-    const long total = rows * nk;
-    const long sY = Z, sX = (long)Y * Z;
-    double d2 = 0, o2 = 0;
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
-         q += (long)gridDim.x * blockDim.x) {
-        const int kk = (int)(q % nk);
-        const long r = q / nk;
-        const int j = 1 + (int)(r % (Y - 2)), i = 1 + (int)(r / (Y - 2));
-        const int k = 1 + 2 * kk + ((i + j + 1 + color) & 1);
-        if (k > Z - 2) continue;
-        T* p = phi + i * sX + j * sY + k;
-        const T old = p[0];
-        const T v = gs_update<T, UNIT>(p[sX], p[-sX], p[sY], p[-sY], p[1], p[-1],
-                                       rhs[i * sX + j * sY + k], old, c);
-        p[0] = v;
-        if (gated) {
-            const double d = (double)v - (double)old;
-            d2 += d * d;
-            o2 += (double)old * (double)old;
+    constexpr int TJ = kRbTJ, TK = kRbTK;
+    constexpr int OJ = TJ + 4, OK = TK + 4;  // old planes, two-voxel ring
+    constexpr int RJ = TJ + 2, RK = TK + 2;  // red planes, one-voxel ring
+    __shared__ T O[4][OJ * OK];
+    __shared__ T Rp[3][RJ * RK];
+    const int k0 = 1 + blockIdx.x * TK, j0 = 1 + blockIdx.y * TJ;
+    const int ia = 1 + blockIdx.z * kRbCI;
+    const int ib = min(ia + kRbCI, X - 1);
+    const long sX = (long)Y * Z;
+    const int tid = threadIdx.x;
+
+    auto load_O = [&](int p) {
+        T* o = O[p & 3];
+        const T* plane = src + (long)p * sX;
+        for (int e = tid; e < OJ * OK; e += 256) {
+            const int jj = e / OK, kk = e - jj * OK;
+            const int j = j0 - 2 + jj, k = k0 - 2 + kk;
+            o[e] = (j >= 0 && j < Y && k >= 0 && k < Z) ? plane[(long)j * Z + k] : (T)0;
         }
+    };
+    auto red = [&](int p) {
+        T* r = Rp[p % 3];
+        const T* o = O[p & 3];
+        const T* om = O[(p + 3) & 3];
+        const T* op = O[(p + 1) & 3];
+        const bool inner_p = p >= 1 && p <= X - 2;
+        for (int e = tid; e < RJ * RK; e += 256) {
+            const int jj = e / RK, kk = e - jj * RK;
+            const int j = j0 - 1 + jj, k = k0 - 1 + kk;
+            const int oi = (jj + 1) * OK + (kk + 1);
+            T v = o[oi];
+            if (inner_p && j >= 1 && j <= Y - 2 && k >= 1 && k <= Z - 2 && ((p + j + k) & 1) == 0)
+                v = gs_update<T, UNIT>(op[oi], om[oi], o[oi + OK], o[oi - OK], o[oi + 1], o[oi - 1],
+                                       rhs[(long)p * sX + (long)j * Z + k], v, c);
+            r[e] = v;
+        }
+    };
+    double d2 = 0, o2 = 0;
+    auto black = [&](int p) {
+        const T* r = Rp[p % 3];
+        const T* rm = Rp[(p + 2) % 3];
+        const T* rp = Rp[(p + 1) % 3];
+        const T* o = O[p & 3];
+        for (int e = tid; e < TJ * TK; e += 256) {
+            const int jj = e / TK, kk = e - jj * TK;
+            const int j = j0 + jj, k = k0 + kk;
+            if (j > Y - 2 || k > Z - 2) continue;
+            const int ri = (jj + 1) * RK + (kk + 1);
+            T v = r[ri];
+            if (((p + j + k) & 1) == 1)
+                v = gs_update<T, UNIT>(rp[ri], rm[ri], r[ri + RK], r[ri - RK], r[ri + 1], r[ri - 1],
+                                       rhs[(long)p * sX + (long)j * Z + k], v, c);
+            dst[(long)p * sX + (long)j * Z + k] = v;
+            if (gated) {
+                const double old = (double)o[(jj + 2) * OK + (kk + 2)];
+                const double d = (double)v - old;
+                d2 += d * d;
+                o2 += old * old;
+            }
+        }
+    };
+    if (ia - 2 >= 0) load_O(ia - 2);
+    load_O(ia - 1);
+    load_O(ia);
+    if (ia + 1 <= X - 1) load_O(ia + 1);
+    __syncthreads();
+    red(ia - 1);
+    red(ia);
+    // the first load of the loop reuses the slot of O(ia - 2), read by red(ia - 1)
+    __syncthreads();
+    for (int p = ia; p < ib; ++p) {
+        if (p + 2 <= X - 1) load_O(p + 2);
+        __syncthreads();
+        red(p + 1);
+        __syncthreads();
+        black(p);
     }
     if (!gated) return;
     block_sum2(d2, o2);
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         atomicAdd(&g.acc[2 * slot], d2);
         atomicAdd(&g.acc[2 * slot + 1], o2);
-        if (color == 1) {
-            __threadfence();
-            const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
-            if (ticket == gridDim.x - 1) {
-                const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
-                double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
-                if (s > 0) so += g.bnd_o2;
-                const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
-                g.err[slot] = err;
-                g.arrive[slot] = 0;
-                if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
+        __threadfence();
+        const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+        const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
+        if (ticket == nblk - 1) {
+            const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
+            double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
+            if (s > 0) so += g.bnd_o2;
+            const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
+            g.err[slot] = err;
+            g.arrive[slot] = 0;
+            if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
+        }
+    }
+}
+
+// The same sweep for rows of 16 bytes (nz a multiple of V): the tile's k columns are
+// aligned to multiples of 64, so every plane row is moved with 16-byte accesses; the rhs
+// rows are staged in LDS with the old planes, and the next plane is fetched into
+// registers while the current one is computed.
+constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
+constexpr int kRbOJ = kRbTJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
+constexpr int kRbRJ = kRbTJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
+
+template <class T, bool UNIT>
+__global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
+                                                  const T* __restrict__ rhs, int X, int Y, int Z,
+                                                  int CI, GsCoef c, int s, int slot, int gated,
+                                                  RbGate g)
+{
+    if (gated && *(volatile unsigned*)g.stop) return;
+    constexpr int V = Vec16<T>::n;
+    constexpr int W = kRbW, WV = W / V, TJ = kRbTJ, OJ = kRbOJ, RJ = kRbRJ;
+    constexpr int NO = (OJ * WV + 255) / 256, NB = (RJ * WV + 255) / 256;
+    __shared__ T O[4][OJ * W];
+    __shared__ T Rp[3][RJ * W];
+    __shared__ T Bp[3][RJ * W];
+    const int kb = blockIdx.x * kRbTK, kbase = kb - 4;
+    const int j0 = 1 + blockIdx.y * TJ;
+    const int ia = 1 + blockIdx.z * CI;
+    const int ib = min(ia + CI, X - 1);
+    const long sX = (long)Y * Z;
+    const int tid = threadIdx.x;
+    Row<T> po[NO], pb[NB];
+    auto fetch = [&](const T* base, int rows, int row0, Row<T>* out, int n) {
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            const int e = tid + q * 256;
+            Row<T> v = {};
+            if (e < rows * WV) {
+                const int r = e / WV, c4 = e - r * WV;
+                const int j = row0 + r, k = kbase + c4 * V;
+                if (j >= 0 && j < Y && k >= 0 && k < Z) v = ldrow(base + (long)j * Z + k);
             }
+            out[q] = v;
+        }
+    };
+    auto deposit = [&](T* lds, int rows, const Row<T>* in, int n) {
+#pragma unroll
+        for (int q = 0; q < n; ++q) {
+            const int e = tid + q * 256;
+            if (e < rows * WV) {
+#pragma unroll
+                for (int u = 0; u < V; ++u) lds[e * V + u] = in[q].v[u];
+            }
+        }
+    };
+    auto red = [&](int p) {
+        T* r = Rp[p % 3];
+        const T* o = O[p & 3];
+        const T* om = O[(p + 3) & 3];
+        const T* op = O[(p + 1) & 3];
+        const T* b = Bp[p % 3];
+        const bool inner_p = p >= 1 && p <= X - 2;
+        for (int e = tid; e < RJ * WV; e += 256) {
+            const int rr = e / WV, c4 = e - rr * WV;
+            const int j = j0 - 1 + rr;
+            const bool jin = inner_p && j >= 1 && j <= Y - 2;
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                const int col = c4 * V + u, k = kbase + col;
+                const int oi = (rr + 1) * W + col;
+                T v = o[oi];
+                if (jin && col >= 3 && col <= kRbTK + 4 && k >= 1 && k <= Z - 2 &&
+                    ((p + j + k) & 1) == 0)
+                    v = gs_update<T, UNIT>(op[oi], om[oi], o[oi + W], o[oi - W], o[oi + 1], o[oi - 1],
+                                           b[rr * W + col], v, c);
+                r[rr * W + col] = v;
+            }
+        }
+    };
+    double d2 = 0, o2 = 0;
+    auto black = [&](int p) {
+        const T* r = Rp[p % 3];
+        const T* rm = Rp[(p + 2) % 3];
+        const T* rp = Rp[(p + 1) % 3];
+        const T* o = O[p & 3];
+        const T* b = Bp[p % 3];
+        constexpr int TV = kRbTK / V;
+        for (int e = tid; e < TJ * TV; e += 256) {
+            const int jj = e / TV, c4 = e - jj * TV;
+            const int j = j0 + jj, k0 = kb + c4 * V;
+            if (j > Y - 2 || k0 >= Z) continue;
+            Row<T> out;
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                const int col = 4 + c4 * V + u, k = k0 + u;
+                const int ri = (jj + 1) * W + col;
+                T v = r[ri];
+                const bool kin = k >= 1 && k <= Z - 2;
+                if (kin && ((p + j + k) & 1) == 1)
+                    v = gs_update<T, UNIT>(rp[ri], rm[ri], r[ri + W], r[ri - W], r[ri + 1], r[ri - 1],
+                                           b[ri], v, c);
+                out.v[u] = v;
+                if (gated && kin) {
+                    const double old = (double)o[(jj + 2) * W + col];
+                    const double d = (double)v - old;
+                    d2 += d * d;
+                    o2 += old * old;
+                }
+            }
+            strow(dst + (long)p * sX + (long)j * Z + k0, out);
+        }
+    };
+    // prologue: old planes ia-2 .. ia+1 and rhs planes ia-1, ia into LDS
+    for (int p = ia - 2; p <= ia + 1; ++p)
+        if (p >= 0 && p <= X - 1) {
+            fetch(src + (long)p * sX, OJ, j0 - 2, po, NO);
+            deposit(O[p & 3], OJ, po, NO);
+        }
+    for (int p = ia - 1; p <= ia; ++p) {
+        fetch(rhs + (long)p * sX, RJ, j0 - 1, pb, NB);
+        deposit(Bp[p % 3], RJ, pb, NB);
+    }
+    if (ia + 2 <= X - 1) fetch(src + (long)(ia + 2) * sX, OJ, j0 - 2, po, NO);
+    fetch(rhs + (long)(ia + 1) * sX, RJ, j0 - 1, pb, NB);
+    __syncthreads();
+    red(ia - 1);
+    red(ia);
+    __syncthreads();
+    for (int p = ia; p < ib; ++p) {
+        // slots refilled here were last read two planes ago (separated by barriers)
+        if (p + 2 <= X - 1) deposit(O[(p + 2) & 3], OJ, po, NO);
+        deposit(Bp[(p + 1) % 3], RJ, pb, NB);
+        if (p + 3 <= X - 1) fetch(src + (long)(p + 3) * sX, OJ, j0 - 2, po, NO);
+        if (p + 2 <= X - 1) fetch(rhs + (long)(p + 2) * sX, RJ, j0 - 1, pb, NB);
+        __syncthreads();
+        red(p + 1);
+        __syncthreads();
+        black(p);
+    }
+    if (!gated) return;
+    block_sum2(d2, o2);
+    if (tid == 0) {
+        atomicAdd(&g.acc[2 * slot], d2);
+        atomicAdd(&g.acc[2 * slot + 1], o2);
+        __threadfence();
+        const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+        const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
+        if (ticket == nblk - 1) {
+            const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
+            double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
+            if (s > 0) so += g.bnd_o2;
+            const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
+            g.err[slot] = err;
+            g.arrive[slot] = 0;
+            if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
         }
     }
 }
@@ -288,22 +505,40 @@ __global__ void k_zero_u(unsigned* p, int n)
 }
 
 template <class T>
-static int rb_sweep(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int s, int slot,
-                    int gated, const RbGate& g)
+static int rb_sweep(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
+                    int slot, int gated, const RbGate& g)
 {
-    const int X = phi->nx, Y = phi->ny, Z = phi->nz;
-    const long total = (long)(X - 2) * (Y - 2) * ((Z - 1) / 2);
-    const unsigned nb = blocks_for(total, 256, (long)ctx->num_cu * 8);
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
-    for (int color = 0; color < 2; ++color) {
-        TimedLaunch tl(ctx, TK_K_GS);
+    TimedLaunch tl(ctx, TK_K_GS);
+    constexpr int V = Vec16<T>::n;
+    if (Z % V == 0) {
+        // chunk length in x chosen so the grid is about five workgroups per CU (one round)
+        const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + kRbTJ - 1) / kRbTJ;
+        long chunks = ((long)ctx->num_cu * 5 + (long)kt * jt / 2) / ((long)kt * jt);
+        if (chunks < 1) chunks = 1;
+        if (chunks > X - 2) chunks = X - 2;
+        const int CI = (int)((X - 2 + chunks - 1) / chunks);
+        const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
         if (unit)
-            hipLaunchKernelGGL((k_gs_rb<T, true>), dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d,
-                               (const T*)rhs->d, X, Y, Z, color, c, s, slot, gated, g);
+            hipLaunchKernelGGL((k_gs_rb_pv<T, true>), grid, dim3(256), 0, ctx->stream,
+                               (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                               gated, g);
         else
-            hipLaunchKernelGGL((k_gs_rb<T, false>), dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d,
-                               (const T*)rhs->d, X, Y, Z, color, c, s, slot, gated, g);
+            hipLaunchKernelGGL((k_gs_rb_pv<T, false>), grid, dim3(256), 0, ctx->stream,
+                               (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                               gated, g);
+        TK_LAUNCHED();
+        return TK_OK;
     }
+    const dim3 grid((Z - 2 + kRbTK - 1) / kRbTK, (Y - 2 + kRbTJ - 1) / kRbTJ,
+                    (X - 2 + kRbCI - 1) / kRbCI);
+    if (unit)
+        hipLaunchKernelGGL((k_gs_rb_pp<T, true>), grid, dim3(256), 0, ctx->stream, (const T*)src,
+                           (T*)dst, (const T*)rhs->d, X, Y, Z, c, s, slot, gated, g);
+    else
+        hipLaunchKernelGGL((k_gs_rb_pp<T, false>), grid, dim3(256), 0, ctx->stream, (const T*)src,
+                           (T*)dst, (const T*)rhs->d, X, Y, Z, c, s, slot, gated, g);
     TK_LAUNCHED();
     return TK_OK;
 }
@@ -329,62 +564,94 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
     const double h3 = phi->hx * phi->hy * phi->hz;
     const double bnd_o2 = (double)boundary_count(X, Y, Z) * (double)(T)bc * (double)(T)bc;
     if (ls_steps < 1) ls_steps = 1;
+    if (order == TK_GS_REDBLACK) {
+        // ping-pong between phi and a scratch volume; both carry the boundary value
+        void* other;
+        TK_TRY(ctx_work(ctx, 4, phi->bytes(), &other));
+        tk_vol_s tmp = *phi;
+        tmp.d = other;
+        tmp.owned = false;
+        TK_TRY(set_boundary<T>(ctx, &tmp, bc, nullptr));
+        void* buf[2] = {phi->d, other};
+        int sweeps = 0;
+        double err = -1;
+        int r = TK_OK;
+        if (ls_tol == 0) {
+            TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+            RbGate g{};
+            for (int q = 0; q < ls_steps; ++q)
+                TK_TRY(rb_sweep<T>(ctx, buf[q & 1], buf[(q + 1) & 1], rhs, c, q, 0, 0, g));
+            sweeps = ls_steps;
+        } else {
+            if (max_sweeps <= 0) max_sweeps = 1 << 30;
+            // batches of B sweeps; the device stops itself, the host polls between batches
+            const int B = 256;
+            RbGate g;
+            g.acc = ctx->acc + kAccSweep;
+            g.err = ctx->acc + kAccErr;  // B error slots
+            g.stop = ctx->cnt;
+            g.arrive = ctx->cnt + 1;
+            g.ls_steps = ls_steps;
+            g.ls_tol = ls_tol;
+            g.h3 = h3;
+            g.bnd_o2 = bnd_o2;
+            hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(256), 0, ctx->stream, ctx->cnt, B + 1);
+            int done = 0;
+            r = TK_ERR_SWEEPS;
+            while (done < max_sweeps) {
+                const int nb = std::min(B, max_sweeps - done);
+                hipLaunchKernelGGL(k_zero_d, dim3(4), dim3(256), 0, ctx->stream, g.acc, 2 * B);
+                if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, g.acc));
+                for (int q = 0; q < nb; ++q) {
+                    const int sw = done + q;
+                    TK_TRY(rb_sweep<T>(ctx, buf[sw & 1], buf[(sw + 1) & 1], rhs, c, sw, q, 1, g));
+                }
+                unsigned stop = 0;
+                TK_HIP(hipMemcpyAsync(ctx->h_acc, g.err, sizeof(double) * B, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                TK_HIP(hipMemcpyAsync(&ctx->h_acc[B], ctx->cnt, sizeof(unsigned),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+                TK_HIP(hipStreamSynchronize(ctx->stream));
+                std::memcpy(&stop, &ctx->h_acc[B], sizeof(unsigned));
+                if (stop) {
+                    sweeps = (int)stop;
+                    err = ctx->h_acc[sweeps - 1 - done];
+                    r = TK_OK;
+                    break;
+                }
+                done += nb;
+                sweeps = done;
+                err = ctx->h_acc[nb - 1];
+            }
+            if (r != TK_OK)
+                set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", sweeps);
+        }
+        if (sweeps & 1) {
+            // the result is in the scratch volume: hand it to phi
+            if (phi->owned) {
+                ctx->work[4] = phi->d;
+                ctx->work_bytes[4] = phi->bytes();
+                phi->d = other;
+            } else {
+                TK_HIP(hipMemcpyAsync(phi->d, other, phi->bytes(), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            }
+        }
+        if (sweeps_out) *sweeps_out = sweeps;
+        if (err_out) *err_out = err;
+        return r;
+    }
     if (ls_tol == 0) {
         // exactly ls_steps sweeps, no norms (the reference computes none either)
         TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
-        if (order == TK_GS_LEX) {
-            TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr));
-        } else {
-            RbGate g{};
-            for (int s = 0; s < ls_steps; ++s) TK_TRY(rb_sweep<T>(ctx, phi, rhs, c, s, 0, 0, g));
-        }
+        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr));
         if (sweeps_out) *sweeps_out = ls_steps;
         if (err_out) *err_out = -1;
         return TK_OK;
     }
     if (max_sweeps <= 0) max_sweeps = 1 << 30;
-    double* acc = ctx->acc + 4096;  // per-sweep pairs (up to 2048 sweeps per chunk)
+    double* acc = ctx->acc + kAccSweep;  // per-sweep pairs (up to 2048 sweeps per chunk)
     const int kMaxChunk = 2048;
-
-    if (order == TK_GS_REDBLACK) {
-        // batches of B sweeps; the device stops itself, the host polls between batches
-        const int B = 256;
-        RbGate g;
-        g.acc = acc;
-        g.err = ctx->acc + 64 + 2 * 1024;  // B error slots
-        g.stop = ctx->cnt;
-        g.arrive = ctx->cnt + 1;
-        g.ls_steps = ls_steps;
-        g.ls_tol = ls_tol;
-        g.h3 = h3;
-        g.bnd_o2 = bnd_o2;
-        hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(256), 0, ctx->stream, ctx->cnt, B + 1);
-        int done = 0;
-        while (done < max_sweeps) {
-            const int nb = std::min(B, max_sweeps - done);
-            hipLaunchKernelGGL(k_zero_d, dim3(4), dim3(256), 0, ctx->stream, acc, 2 * B);
-            if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, acc));
-            for (int q = 0; q < nb; ++q) TK_TRY(rb_sweep<T>(ctx, phi, rhs, c, done + q, q, 1, g));
-            unsigned stop = 0;
-            TK_HIP(hipMemcpyAsync(ctx->h_acc, g.err, sizeof(double) * B, hipMemcpyDeviceToHost,
-                                  ctx->stream));
-            TK_HIP(hipMemcpyAsync(&ctx->h_acc[B], ctx->cnt, sizeof(unsigned), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-            TK_HIP(hipStreamSynchronize(ctx->stream));
-            std::memcpy(&stop, &ctx->h_acc[B], sizeof(unsigned));
-            if (stop) {
-                const int sdone = (int)stop;
-                if (sweeps_out) *sweeps_out = sdone;
-                if (err_out) *err_out = ctx->h_acc[sdone - 1 - done];
-                return TK_OK;
-            }
-            done += nb;
-        }
-        if (sweeps_out) *sweeps_out = done;
-        if (err_out) *err_out = ctx->h_acc[(done - 1) % B];
-        set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", done);
-        return TK_ERR_SWEEPS;
-    }
 
     // lexicographic: chunks with replay
     void* save_p;

@@ -45,8 +45,15 @@ inline size_t dtype_size(tk_dtype t) { return t == TK_F64 ? 8 : (t == TK_F32 ? 4
 
 // Device scratch owned by a context: FP64 accumulators, counters and flags used by
 // the reductions and the device-side Poisson stop test.  Sizes are fixed.
-constexpr int kAccSlots = 8192;  // doubles
-constexpr int kCntSlots = 4096;  // unsigned
+constexpr int kAccSlots = 32768;  // doubles
+constexpr int kCntSlots = 4096;   // unsigned
+// layout of ctx->acc: [0, 64) results; [kAccPart, +2*kMaxPart) per-block partial pairs;
+// [kAccErr, +512) per-sweep relative errors; [kAccSweep, +4096) per-sweep sum pairs
+constexpr int kAccPart = 64;
+constexpr int kMaxPart = 8192;
+constexpr int kAccErr = kAccPart + 2 * kMaxPart;
+constexpr int kAccSweep = kAccErr + 512;
+static_assert(kAccSweep + 4096 <= kAccSlots, "acc layout");
 
 }  // namespace tk
 
@@ -59,8 +66,8 @@ struct tk_ctx_s {
     double* h_acc = nullptr;      // pinned host mirror, kAccSlots
     int num_cu = 256;
     // cached complex work buffers (bytes); reused across calls of the same size
-    void* work[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t work_bytes[4] = {0, 0, 0, 0};
+    void* work[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t work_bytes[6] = {0, 0, 0, 0, 0, 0};
     // per-kernel event timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

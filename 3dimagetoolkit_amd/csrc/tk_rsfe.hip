@@ -52,6 +52,16 @@ struct LdHeaviside {
         const T H = heaviside(p, center, eps);
         return {H * u0[v], H};
     }
+    __device__ void ld4(long v, cplx<T>* c) const
+    {
+        const Quad<T> p = ldq(phi + v), u = ldq(u0 + v);
+        stq(phiold + v, p);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const T H = heaviside(p.v[e], center, eps);
+            c[e] = {H * u.v[e], H};
+        }
+    }
 };
 
 // imp:1709-1731 on (num, den) -> (kr2 factor, kr1 factor)
@@ -60,13 +70,20 @@ struct MdRegionFit {
     const T* kmyim;
     const T* kones;
     T l1, l2;
-    __device__ cplx<T> operator()(long v, cplx<T> c) const
+    __device__ cplx<T> operator()(long v, cplx<T> c) const { return fit(c, kmyim[v], kones[v]); }
+    __device__ void md4(long v, cplx<T>* c) const
+    {
+        const Quad<T> km = ldq(kmyim + v), ko = ldq(kones + v);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] = fit(c[e], km.v[e], ko.v[e]);
+    }
+    __device__ cplx<T> fit(cplx<T> c, T km, T ko) const
     {
         const T num = c.x, den = c.y;
         T f1 = num / den;
         T nm = num, dn = den;
-        nm -= kmyim[v];
-        dn -= kones[v];
+        nm -= km;
+        dn -= ko;
         nm /= dn;
         T f2 = nm;
         const T a = f1 * l1 - f2 * l2;
@@ -89,19 +106,30 @@ struct StRhs {
     T l12, lam;
     __device__ void operator()(long v, cplx<T> c) const
     {
-        const T u = u0[v];
+        rhs[v] = forcing(c, u0[v], kones[v], divdb[v]);
+    }
+    __device__ void st4(long v, const cplx<T>* c) const
+    {
+        const Quad<T> u = ldq(u0 + v), ko = ldq(kones + v), dd = ldq(divdb + v);
+        Quad<T> r;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r.v[e] = forcing(c[e], u.v[e], ko.v[e], dd.v[e]);
+        stq(rhs + v, r);
+    }
+    __device__ T forcing(cplx<T> c, T u, T ko, T dd) const
+    {
         T r = c.x;
         r *= u;
         r *= (T)2;
         T aux = l12;
         aux *= u;
         aux *= u;
-        aux *= kones[v];
+        aux *= ko;
         r -= aux;
         r -= c.y;
         r /= lam;
-        r -= divdb[v];
-        rhs[v] = r;
+        r -= dd;
+        return r;
     }
 };
 

@@ -12,10 +12,9 @@ template <class T>
 __global__ void k_grad(const T* __restrict__ f, T* __restrict__ g0, T* __restrict__ g1,
                        T* __restrict__ g2, int X, int Y, int Z, double hx, double hy, double hz)
 {
-    const long n = (long)X * Y * Z, sX = (long)Y * Z, sY = Z;
-    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n;
-         v += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(v % Z), j = (int)((v / Z) % Y), i = (int)(v / sX);
+    const long sX = (long)Y * Z, sY = Z;
+    TK_FOR_VOXELS(X, Y, Z, i, j, k, v)
+    {
         g0[v] = dterm(f, v, sX, plane_class(i, X), X, i, hx);
         g1[v] = dterm(f, v, sY, plane_class(j, Y), Y, j, hy);
         g2[v] = dterm(f, v, 1, plane_class(k, Z), Z, k, hz);
@@ -26,10 +25,8 @@ template <class T>
 __global__ void k_div(const T* __restrict__ f0, const T* __restrict__ f1, const T* __restrict__ f2,
                       T* __restrict__ out, int X, int Y, int Z, double hx, double hy, double hz)
 {
-    const long n = (long)X * Y * Z, sX = (long)Y * Z;
-    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n;
-         v += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(v % Z), j = (int)((v / Z) % Y), i = (int)(v / sX);
+    TK_FOR_VOXELS(X, Y, Z, i, j, k, v)
+    {
         out[v] = div_at(f0, f1, f2, i, j, k, X, Y, Z, hx, hy, hz);
     }
 }
@@ -39,10 +36,9 @@ template <class T>
 __global__ void k_edge(const T* __restrict__ f, T* __restrict__ g, int X, int Y, int Z, double hx,
                        double hy, double hz, T beta, T lamda, T nu)
 {
-    const long n = (long)X * Y * Z, sX = (long)Y * Z, sY = Z;
-    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n;
-         v += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(v % Z), j = (int)((v / Z) % Y), i = (int)(v / sX);
+    const long sX = (long)Y * Z, sY = Z;
+    TK_FOR_VOXELS(X, Y, Z, i, j, k, v)
+    {
         const T a0 = dterm(f, v, sX, plane_class(i, X), X, i, hx);
         const T a1 = dterm(f, v, sY, plane_class(j, Y), Y, j, hy);
         const T a2 = dterm(f, v, 1, plane_class(k, Z), Z, k, hz);
@@ -81,11 +77,10 @@ __global__ __launch_bounds__(256) void k_sb_update(T* __restrict__ phi, const T*
                                                    double hx, double hy, double hz, T lo, T hi,
                                                    double* __restrict__ part)
 {
-    const long n = (long)X * Y * Z, sX = (long)Y * Z, sY = Z;
+    const long sX = (long)Y * Z, sY = Z;
     double s1 = 0, s2 = 0;
-    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n;
-         v += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(v % Z), j = (int)((v / Z) % Y), i = (int)(v / sX);
+    TK_FOR_VOXELS(X, Y, Z, i, j, k, v)
+    {
         // clamped phi along the three lines through v
         auto cp = [&](long w) { return clampv(phi[w], lo, hi); };
         T d[3];
@@ -145,17 +140,173 @@ __global__ __launch_bounds__(256) void k_sb_update(T* __restrict__ phi, const T*
         phi[v] = pc;
     }
     block_sum2(s1, s2);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s1;
-        part[2 * blockIdx.x + 1] = s2;
+    if (flat_tid() == 0) {
+        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        part[2 * b] = s1;
+        part[2 * b + 1] = s2;
+    }
+}
+
+// ---- 16-byte row versions (nz a multiple of V): each lane owns V consecutive voxels
+// of a line; the x and y neighbours are whole rows, the z neighbours a (V+2)-window.
+// Arithmetic and operand order are those of the per-voxel kernels above.
+
+// derivative along z of element e of a window c[0..V+1] (c[1+e] = f(k0+e))
+template <class T, int V>
+__device__ __forceinline__ T dz_win(const T* c, int e, int k, int Z, double h)
+{
+    if (k == 0) return (4 * c[2] - 3 * c[1] - c[3]) / (T)(2. * h);
+    if (k == Z - 1) return (3 * c[e + 1] - 4 * c[e] + c[e - 1]) / (T)(2. * h);
+    return (c[e + 2] - c[e]) / (T)(2. * h);
+}
+
+// rows of f at the three lines needed for d/dx (or d/dy) of the row at index `idx`
+// along an axis of length n with line stride st: central -> (+1, -1); first -> (1, 0, 2);
+// last -> (n-1, n-2, n-3)
+template <class T, class Fn>
+__device__ __forceinline__ void drow(const T* __restrict__ f, long v0, long st, int idx, int n,
+                                     double h, Fn clampf, T* out)
+{
+    constexpr int V = Vec16<T>::n;
+    const int c = plane_class(idx, n);
+    const long base = v0 - (long)idx * st;
+    if (c == 1) {
+        Row<T> p = ldrow(f + v0 + st), m = ldrow(f + v0 - st);
+#pragma unroll
+        for (int e = 0; e < V; ++e) out[e] = (clampf(p.v[e]) - clampf(m.v[e])) / (T)(2. * h);
+    } else if (c == 0) {
+        Row<T> a0 = ldrow(f + base), a1 = ldrow(f + base + st), a2 = ldrow(f + base + 2 * st);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            out[e] = (4 * clampf(a1.v[e]) - 3 * clampf(a0.v[e]) - clampf(a2.v[e])) / (T)(2. * h);
+    } else {
+        Row<T> a1 = ldrow(f + base + (long)(n - 1) * st), a2 = ldrow(f + base + (long)(n - 2) * st),
+               a3 = ldrow(f + base + (long)(n - 3) * st);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+            out[e] = (3 * clampf(a1.v[e]) - 4 * clampf(a2.v[e]) + clampf(a3.v[e])) / (T)(2. * h);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const T* __restrict__ phiold,
+                                                     T* __restrict__ b0, T* __restrict__ b1,
+                                                     T* __restrict__ b2, const T* __restrict__ g,
+                                                     T* __restrict__ q0, T* __restrict__ q1,
+                                                     T* __restrict__ q2, int X, int Y, int Z,
+                                                     double hx, double hy, double hz, T lo, T hi,
+                                                     double* __restrict__ part)
+{
+    constexpr int V = Vec16<T>::n;
+    const long sX = (long)Y * Z, sY = Z;
+    auto cl = [&](T x) { return clampv(x, lo, hi); };
+    double s1 = 0, s2 = 0;
+    TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)
+    {
+        T d[3][V];
+        drow(phi, v0, sX, i, X, hx, cl, d[0]);
+        drow(phi, v0, sY, j, Y, hy, cl, d[1]);
+        T c[V + 2];
+        const Row<T> cc = ldrow(phi + v0);
+        c[0] = k0 > 0 ? cl(phi[v0 - 1]) : (T)0;
+        c[V + 1] = k0 + V < Z ? cl(phi[v0 + V]) : (T)0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) c[e + 1] = cl(cc.v[e]);
+#pragma unroll
+        for (int e = 0; e < V; ++e) d[2][e] = dz_win<T, V>(c, e, k0 + e, Z, hz);
+        const Row<T> po = ldrow(phiold + v0);
+        const Row<T> gv = ldrow(g + v0);
+        Row<T> pc;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            pc.v[e] = c[e + 1];
+            const double dd = (double)pc.v[e] - (double)po.v[e];
+            s1 += dd * dd;
+            s2 += (double)po.v[e] * (double)po.v[e];
+        }
+        T* bp[3] = {b0, b1, b2};
+        T* qp[3] = {q0, q1, q2};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            Row<T> bb = ldrow(bp[a] + v0), qq;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                T b = bb.v[e];
+                b += d[a][e];
+                const T m = fabs(b) - gv.v[e];
+                const T z = 0;
+                T dn = (T)((int)(b > 0) - (int)(b < 0)) * ((m < z) ? z : m);
+                b -= dn;
+                dn -= b;
+                bb.v[e] = b;
+                qq.v[e] = dn;
+            }
+            strow(bp[a] + v0, bb);
+            strow(qp[a] + v0, qq);
+        }
+        strow(phi + v0, pc);
+    }
+    block_sum2(s1, s2);
+    if (flat_tid() == 0) {
+        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        part[2 * b] = s1;
+        part[2 * b + 1] = s2;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_div_v(const T* __restrict__ f0, const T* __restrict__ f1,
+                                               const T* __restrict__ f2, T* __restrict__ out, int X,
+                                               int Y, int Z, double hx, double hy, double hz)
+{
+    constexpr int V = Vec16<T>::n;
+    const long sX = (long)Y * Z, sY = Z;
+    auto id = [](T x) { return x; };
+    TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)
+    {
+        const int ci = plane_class(i, X), cj = plane_class(j, Y);
+        T tx[V], ty[V];
+        drow(f0, v0, sX, i, X, hx, id, tx);
+        if (ci == 2 && cj == 1) {
+            // face i = X-1 (j, k inside): "/ 2. * (hx)" (image3d_imp.hxx:1092-1095)
+            const long base = v0 - (long)i * sX;
+            const Row<T> a1 = ldrow(f0 + base + (long)(X - 1) * sX),
+                         a2 = ldrow(f0 + base + (long)(X - 2) * sX),
+                         a3 = ldrow(f0 + base + (long)(X - 3) * sX);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const int k = k0 + e;
+                if (k >= 1 && k <= Z - 2) tx[e] = (3 * a1.v[e] - 4 * a2.v[e] + a3.v[e]) / (T)2. * (T)hx;
+            }
+        }
+        drow(f1, v0, sY, j, Y, hy, id, ty);
+        T c[V + 2];
+        const Row<T> cc = ldrow(f2 + v0);
+        c[0] = k0 > 0 ? f2[v0 - 1] : (T)0;
+        c[V + 1] = k0 + V < Z ? f2[v0 + V] : (T)0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) c[e + 1] = cc.v[e];
+        Row<T> o;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const int k = k0 + e;
+            T tz;
+            if (ci == 1 && cj == 0 && k == Z - 1)
+                // edge (i inside, 0, Z-1): Z-2 used twice (image3d_imp.hxx:1192-1194)
+                tz = (3 * c[e + 1] - 4 * c[e] + c[e]) / (T)(2. * hz);
+            else
+                tz = dz_win<T, V>(c, e, k, Z, hz);
+            o.v[e] = tx[e] + ty[e] + tz;
+        }
+        strow(out + v0, o);
     }
 }
 
 template <class T>
 int launch_grad(tk_ctx ctx, tk_vol f, tk_vol g0, tk_vol g1, tk_vol g2)
 {
-    const long n = f->n();
-    hipLaunchKernelGGL(k_grad<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_grad<T>, vox_grid(f->nx, f->ny, f->nz, 8192), vox_block(f->nz), 0,
+                       ctx->stream,
                        (const T*)f->d, (T*)g0->d, (T*)g1->d, (T*)g2->d, f->nx, f->ny, f->nz, f->hx,
                        f->hy, f->hz);
     TK_LAUNCHED();
@@ -165,8 +316,17 @@ int launch_grad(tk_ctx ctx, tk_vol f, tk_vol g0, tk_vol g1, tk_vol g2)
 template <class T>
 int launch_div(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol out)
 {
-    const long n = out->n();
-    hipLaunchKernelGGL(k_div<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream,
+    constexpr int V = Vec16<T>::n;
+    if (out->nz % V == 0 && out->nz > V) {
+        const int zv = out->nz / V;
+        hipLaunchKernelGGL(k_div_v<T>, row_grid(out->nx, out->ny, zv, 8192), row_block(zv), 0,
+                           ctx->stream, (const T*)f0->d, (const T*)f1->d, (const T*)f2->d,
+                           (T*)out->d, out->nx, out->ny, out->nz, out->hx, out->hy, out->hz);
+        TK_LAUNCHED();
+        return TK_OK;
+    }
+    hipLaunchKernelGGL(k_div<T>, vox_grid(out->nx, out->ny, out->nz, 8192), vox_block(out->nz),
+                       0, ctx->stream,
                        (const T*)f0->d, (const T*)f1->d, (const T*)f2->d, (T*)out->d, out->nx,
                        out->ny, out->nz, out->hx, out->hy, out->hz);
     TK_LAUNCHED();
@@ -176,8 +336,8 @@ int launch_div(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol out)
 template <class T>
 int launch_edge(tk_ctx ctx, tk_vol u0, double beta, double lamda, double nu, tk_vol g)
 {
-    const long n = u0->n();
-    hipLaunchKernelGGL(k_edge<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_edge<T>, vox_grid(u0->nx, u0->ny, u0->nz, 8192), vox_block(u0->nz), 0,
+                       ctx->stream,
                        (const T*)u0->d, (T*)g->d, u0->nx, u0->ny, u0->nz, u0->hx, u0->hy, u0->hz,
                        (T)beta, (T)lamda, (T)nu);
     TK_LAUNCHED();
@@ -188,9 +348,21 @@ template <class T>
 int launch_sb_update(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol* b, tk_vol g, tk_vol* q,
                      double a0, double b0, int* nblocks, double* part)
 {
-    const long n = phi->n();
-    const unsigned nb = blocks_for(n, 256, 1024);
-    hipLaunchKernelGGL(k_sb_update<T>, dim3(nb), dim3(256), 0, ctx->stream, (T*)phi->d,
+    constexpr int V = Vec16<T>::n;
+    if (phi->nz % V == 0 && phi->nz > V) {
+        const int zv = phi->nz / V;
+        const dim3 grid = row_grid(phi->nx, phi->ny, zv, kMaxPart);
+        hipLaunchKernelGGL(k_sb_update_v<T>, grid, row_block(zv), 0, ctx->stream, (T*)phi->d,
+                           (const T*)phiold->d, (T*)b[0]->d, (T*)b[1]->d, (T*)b[2]->d,
+                           (const T*)g->d, (T*)q[0]->d, (T*)q[1]->d, (T*)q[2]->d, phi->nx, phi->ny,
+                           phi->nz, phi->hx, phi->hy, phi->hz, (T)a0, (T)b0, part);
+        TK_LAUNCHED();
+        *nblocks = (int)(grid.x * grid.y);
+        return TK_OK;
+    }
+    const dim3 grid = vox_grid(phi->nx, phi->ny, phi->nz, 1024);
+    const unsigned nb = grid.x * grid.y;
+    hipLaunchKernelGGL(k_sb_update<T>, grid, vox_block(phi->nz), 0, ctx->stream, (T*)phi->d,
                        (const T*)phiold->d, (T*)b[0]->d, (T*)b[1]->d, (T*)b[2]->d, (const T*)g->d,
                        (T*)q[0]->d, (T*)q[1]->d, (T*)q[2]->d, phi->nx, phi->ny, phi->nz, phi->hx,
                        phi->hy, phi->hz, (T)a0, (T)b0, part);

@@ -36,7 +36,7 @@ BYTES_PER_VOXEL_W = {
     "conv_y": 4,    # read complex (2), write complex (2)
     "conv_z2": 6,   # read complex (2), kmyim, kones; write complex (2)
     "conv_x2": 6,   # read complex (2), u0, kones, divdb; write rhs
-    "gs": 1.5,      # one colour pass: half of read phi + rhs, write phi (3w per sweep)
+    "gs": 3,        # one red-black sweep: read phi, rhs; write phi
     "sb": 13,       # read phi, phiold, b(3), g; write phi, b(3), q(3)
     "div": 4,       # read q(3), write divdb
 }
@@ -107,15 +107,21 @@ def main():
         for tag, (ms, cnt) in kt.items():
             if cnt and tag in BYTES_PER_VOXEL_W:
                 avg = ms / cnt
-                gbs = BYTES_PER_VOXEL_W[tag] * w * N / (avg * 1e-3) / 1e9
+                per_launch = BYTES_PER_VOXEL_W[tag] * w * N
+                if tag == "gs" and order == tk.TK_GS_LEX:
+                    # a lexicographic launch is one pipeline step over part of the tiles:
+                    # count the sweeps' bytes over all steps
+                    per_launch = 3 * w * N * a.steps * a.ls_steps / cnt
+                gbs = per_launch / (avg * 1e-3) / 1e9
                 kernels[tag] = {"total_ms": round(ms, 3), "launches": cnt,
-                                "avg_us": round(avg * 1e3, 2), "GBps": round(gbs, 1)}
+                                "avg_us": round(avg * 1e3, 2), "GBps": round(gbs, 1),
+                                "bytes_per_launch": int(per_launch)}
         if kernels:
             dom = max(kernels, key=lambda t: kernels[t]["total_ms"])
             k = kernels[dom]
             roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
-                    "bytes_per_launch": BYTES_PER_VOXEL_W[dom] * w * N}
+                    "bytes_per_launch": kernels[dom]["bytes_per_launch"]}
     # whole-loop model: conv 5 passes (25w) + S sweeps (3w) + sb (13w) + div (4w)
     S = float(np.mean(sweeps))
     model_bytes = N * (25 * w + 3 * w * S + 13 * w + 4 * w)

@@ -193,9 +193,9 @@ def test_poisson_residual(tk, ctx, rng):
 
 
 # ---- R6, R10-R13 --------------------------------------------------------------------
-def test_pointwise_and_stencils_f64(tk, orc, ctx, rng):
+@pytest.mark.parametrize("shape", [(13, 17, 19), (12, 13, 16)])  # per-voxel and 16-byte-row paths
+def test_pointwise_and_stencils_f64(tk, orc, ctx, shape, rng):
     lib = tk.lib
-    shape = (13, 17, 19)
     h = (0.5, 1.0, 2.0)
     phi = rng.random(shape) * 1.2 - 0.1
     v = ctx.from_numpy(phi, h=h)
@@ -222,6 +222,19 @@ def test_pointwise_and_stencils_f64(tk, orc, ctx, rng):
     vb, vg = ctx.from_numpy(b, h=h), ctx.from_numpy(gg, h=h)
     tk._check(lib.tk_shrink(ctx.h, vb.h, vg.h, out.h))
     assert np.array_equal(out.numpy(), orc.shrink(b, gg))
+
+
+def test_div_f32_rows(tk, orc, ctx, rng):
+    """FP32 divergence on the 16-byte-row path against the FP64 oracle, including the
+    two quirk locations."""
+    shape = (10, 11, 24)
+    h = (0.5, 1.0, 2.0)
+    f = [rng.standard_normal(shape) for _ in range(3)]
+    vf = [ctx.from_numpy(a.astype(np.float32), h=h) for a in f]
+    out = ctx.volume(shape, np.float32, h=h)
+    tk._check(tk.lib.tk_div(ctx.h, vf[0].h, vf[1].h, vf[2].h, out.h))
+    ref = orc.div(*[a.astype(np.float32).astype(np.float64) for a in f], h)
+    assert np.max(np.abs(out.numpy() - ref)) < 2e-5 * np.max(np.abs(ref))
 
 
 def test_norms_minmax_range_cut(tk, orc, ctx, rng):
