@@ -79,6 +79,10 @@ def _load():
         "tk_filter_free": (i, [vp]),
         "tk_filter_taps": (i, [vp, ip]),
         "tk_filter_axis_spectrum": (i, [vp, i, dp]),
+        "tk_filter_get_method": (i, [vp, i, ip]),
+        "tk_filter_set_method": (i, [vp, i, i]),
+        "tk_filter_autotune": (i, [vp, vp, dp]),
+        "tk_rsfe_filter": (vp, [vp]),
         "tk_conv_cyclic": (i, [vp, vp, vp, vp]),
         "tk_conv_cyclic_pair": (i, [vp, vp, vp, vp, vp, vp]),
         "tk_poisson_dir_sweeps": (i, [vp, vp, vp, d, d, i, i]),
@@ -279,6 +283,23 @@ class Filter:
         out = np.empty(self.shape[axis], np.float64)
         _check(lib.tk_filter_axis_spectrum(self.h, axis, _dptr(out)))
         return out
+
+    def methods(self):
+        """per-axis method: 0 FFT pass, 1 direct circular convolution"""
+        out = []
+        for a in range(3):
+            m = C.c_int()
+            _check(lib.tk_filter_get_method(self.h, a, C.byref(m)))
+            out.append(m.value)
+        return out
+
+    def set_method(self, axis, method):
+        _check(lib.tk_filter_set_method(self.h, axis, method))
+
+    def autotune(self):
+        ms = (C.c_double * 6)()
+        _check(lib.tk_filter_autotune(self.ctx.h, self.h, ms))
+        return [(ms[2 * a], ms[2 * a + 1]) for a in range(3)]
 
     def conv(self, out, inp):
         _check(lib.tk_conv_cyclic(self.ctx.h, self.h, out.h, inp.h))

@@ -7,10 +7,16 @@
 
 namespace tk {
 
-// radix plan for the Stockham FFT: 4s first, then 2, 3, 5; 0 if L has another factor
-int fft_plan(int L, int* radix, int* nstage)
+// radix plan for the Stockham FFT: the largest power-of-two radices first (16 for FP32,
+// 8 for FP64, where a thread holds 16 or 8 elements), then 2, 3, 5; 0 if L has another
+// prime factor
+int fft_plan(int L, int* radix, int* nstage, int max_radix)
 {
     int n = 0, m = L;
+    if (max_radix >= 16)
+        while (m % 16 == 0) { radix[n++] = 16; m /= 16; }
+    if (max_radix >= 8)
+        while (m % 8 == 0) { radix[n++] = 8; m /= 8; }
     while (m % 4 == 0) { radix[n++] = 4; m /= 4; }
     while (m % 2 == 0) { radix[n++] = 2; m /= 2; }
     while (m % 3 == 0) { radix[n++] = 3; m /= 3; }
@@ -62,7 +68,29 @@ static int upload_axis(tk_filter_s* f, int a)
     ax.L = L;
     ax.nh = f->nh[a];
     if (L == 1) return TK_OK;
-    if (!fft_plan(L, ax.radix, &ax.nstage)) {
+    {
+        // symmetric taps w(0..s) of the normalised 1-D Gaussian for the direct path
+        const int nh = f->nh[a], s = nh / 2;
+        const double hh = f->pixel ? 1. : f->h[a];
+        std::vector<long double> w(s + 1);
+        long double sum = 0;
+        for (int n = 0; n <= s; ++n) {
+            w[n] = expl(-(long double)n * n * hh * hh / (2.0L * f->sigma * f->sigma));
+            sum += n ? 2 * w[n] : w[n];
+        }
+        std::vector<T> half(s + 1);
+        for (int n = 0; n <= s; ++n) half[n] = (T)(w[n] / sum);
+        TK_HIP(hipMalloc(&f->half_dev[a], sizeof(T) * (s + 1)));
+        TK_HIP(hipMemcpy(f->half_dev[a], half.data(), sizeof(T) * (s + 1), hipMemcpyHostToDevice));
+        ax.half = f->half_dev[a];
+        // FP32: the direct path costs ~2(2s+1) fused operations per complex element against
+        // ~140 for the FFT pass, so it is the default where a register window is compiled;
+        // tk_rsfe_create times both on the device and keeps the faster
+        ax.direct = (sizeof(T) == 4 && direct_width_supported(nh)) ? 1 : 0;
+    }
+    // FP64 (the parity mode) keeps the radix-4 family: the loop amplifies last-bit
+    // differences of the convolution, and radix 8 moved the 64^3 result by ~1e-9
+    if (!fft_plan(L, ax.radix, &ax.nstage, sizeof(T) == 4 ? 16 : 4)) {
         // not 2/3/5-smooth: direct circular convolution with the normalised taps
         ax.nstage = 0;
         const int nh = f->nh[a], s = nh / 2;
@@ -177,6 +205,7 @@ int tk_filter_free(tk_filter f)
     for (int a = 0; a < 3; ++a) {
         if (f->tw_dev[a]) (void)hipFree(f->tw_dev[a]);
         if (f->spec_dev[a]) (void)hipFree(f->spec_dev[a]);
+        if (f->half_dev[a]) (void)hipFree(f->half_dev[a]);
     }
     delete f;
     return TK_OK;
@@ -194,6 +223,80 @@ int tk_filter_axis_spectrum(tk_filter f, int axis, double* out)
     TK_CHECK(f && out && axis >= 0 && axis < 3, TK_ERR_ARG, "tk_filter_axis_spectrum: argument");
     for (int k = 0; k < f->dims[axis]; ++k) out[k] = f->spec_host[axis][k];
     return TK_OK;
+}
+
+int tk_filter_get_method(tk_filter f, int axis, int* method)
+{
+    TK_CHECK(f && method && axis >= 0 && axis < 3, TK_ERR_ARG, "tk_filter_get_method: argument");
+    *method = f->ax[axis].direct;
+    return TK_OK;
+}
+
+int tk_filter_set_method(tk_filter f, int axis, int method)
+{
+    TK_CHECK(f && axis >= 0 && axis < 3 && (method == 0 || method == 1), TK_ERR_ARG,
+             "tk_filter_set_method: argument");
+    const AxisDev& ax = f->ax[axis];
+    if (method == 1)
+        TK_CHECK(direct_width_supported(ax.nh) && f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 &&
+                     f->dims[2] % 4 == 0,
+                 TK_ERR_UNSUPPORTED, "tk_filter_set_method: no direct path for %d taps", ax.nh);
+    if (method == 0)
+        TK_CHECK(ax.nstage > 0 || ax.L == 1, TK_ERR_UNSUPPORTED,
+                 "tk_filter_set_method: length %d has no FFT plan", ax.L);
+    f->ax[axis].direct = method;
+    return TK_OK;
+}
+
+}  // extern "C"
+
+template <class T>
+static int autotune(tk_ctx ctx, tk_filter_s* f, double* ms)
+{
+    const long n = (long)f->dims[0] * f->dims[1] * f->dims[2];
+    void* wk;
+    TK_TRY(ctx_work(ctx, 0, sizeof(cplx<T>) * n, &wk));
+    cplx<T>* C = (cplx<T>*)wk;
+    hipEvent_t e0, e1;
+    TK_HIP(hipEventCreate(&e0));
+    TK_HIP(hipEventCreate(&e1));
+    int r = TK_OK;
+    for (int a = 0; a < 3 && r == TK_OK; ++a) {
+        double t[2] = {-1, -1};
+        const int keep = f->ax[a].direct;
+        for (int m = 0; m < 2 && r == TK_OK; ++m) {
+            if (f->ax[a].L == 1 || tk_filter_set_method(f, a, m) != TK_OK) continue;
+            // one warm-up pass, then three timed passes, complex -> complex in place
+            for (int rep = 0; rep < 4 && r == TK_OK; ++rep) {
+                if (rep == 1) (void)hipEventRecord(e0, ctx->stream);
+                r = launch_conv_axis<T>(ctx, f, a, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C});
+            }
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            float el = 0;
+            (void)hipEventElapsedTime(&el, e0, e1);
+            t[m] = el / 3.0;
+        }
+        f->ax[a].direct = keep;
+        if (t[0] >= 0 && t[1] >= 0) f->ax[a].direct = t[1] < t[0] ? 1 : 0;
+        else if (t[1] >= 0) f->ax[a].direct = 1;
+        else if (t[0] >= 0) f->ax[a].direct = 0;
+        if (ms) {
+            ms[2 * a] = t[0];
+            ms[2 * a + 1] = t[1];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+}
+
+extern "C" {
+
+int tk_filter_autotune(tk_ctx ctx, tk_filter f, double* ms)
+{
+    TK_CHECK(ctx && f, TK_ERR_ARG, "tk_filter_autotune: null argument");
+    return f->dtype == TK_F64 ? autotune<double>(ctx, f, ms) : autotune<float>(ctx, f, ms);
 }
 
 static int check_conv(tk_filter f, tk_vol v)

@@ -54,8 +54,10 @@ struct AxisDev {
     int nstage = 0;              // 0 with L > 1: length not 2/3/5-smooth, direct convolution
     int radix[kMaxStages] = {0};
     int nh = 1;                  // taps of the direct path (odd)
+    int direct = 0;              // 1: symmetric direct circular convolution (halo-padded LDS)
     const void* tw = nullptr;    // cplx<T>[L], exp(-2 pi i t / L)
     const void* spec = nullptr;  // T[L], real spectrum / L (FFT path) or T[nh] taps (direct)
+    const void* half = nullptr;  // T[nh/2 + 1]: w(0), w(1), .. w(s) of the direct path
 };
 
 // how the pencils of one axis sit in memory
@@ -76,7 +78,7 @@ __device__ __forceinline__ long pencil_base(const AxisGeom& g, long c)
     return c;
 }
 
-int fft_plan(int L, int* radix, int* nstage);
+int fft_plan(int L, int* radix, int* nstage, int max_radix);
 
 }  // namespace tk
 
@@ -91,5 +93,6 @@ struct tk_filter_s {
     std::vector<double> spec_host[3];  // real spectrum along each axis (not divided by L)
     void* spec_dev[3] = {nullptr, nullptr, nullptr};
     void* tw_dev[3] = {nullptr, nullptr, nullptr};
+    void* half_dev[3] = {nullptr, nullptr, nullptr};
     tk::AxisDev ax[3];
 };

@@ -63,13 +63,82 @@ __device__ __forceinline__ void dft5(cplx<T>* a, int s)
     a[3] = csub(r2, q2);
 }
 
+// e^{s 2 pi i m / 16}
+template <class T>
+__device__ __forceinline__ cplx<T> w16(int m, int s)
+{
+    const T c1 = (T)0.92387953251128675612818318939678829;  // cos(pi/8)
+    const T s1 = (T)0.38268343236508977172845998403039887;  // sin(pi/8)
+    const T c2 = (T)0.70710678118654752440084436210484904;  // cos(pi/4)
+    T re, im;
+    switch (m & 15) {
+        case 0: re = 1; im = 0; break;
+        case 1: re = c1; im = s1; break;
+        case 2: re = c2; im = c2; break;
+        case 3: re = s1; im = c1; break;
+        case 4: re = 0; im = 1; break;
+        case 5: re = -s1; im = c1; break;
+        case 6: re = -c2; im = c2; break;
+        case 7: re = -c1; im = s1; break;
+        case 8: re = -1; im = 0; break;
+        case 9: re = -c1; im = -s1; break;
+        case 10: re = -c2; im = -c2; break;
+        case 11: re = -s1; im = -c1; break;
+        case 12: re = 0; im = -1; break;
+        case 13: re = s1; im = -c1; break;
+        case 14: re = c2; im = -c2; break;
+        default: re = c1; im = -s1; break;
+    }
+    return {re, s > 0 ? im : -im};
+}
+
+// 8 = 2 x 4 and 16 = 4 x 4 Cooley-Tukey in registers: column DFTs, internal twiddles
+// W_R^{n2 k1}, row DFTs; output in natural order
+template <class T>
+__device__ __forceinline__ void dft8(cplx<T>* a, int s)
+{
+    cplx<T> e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+    dft4(e, s);
+    dft4(o, s);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const cplx<T> t = k == 0 ? o[0] : cmul(o[k], w16<T>(2 * k, s));
+        a[k] = cadd(e[k], t);
+        a[k + 4] = csub(e[k], t);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void dft16(cplx<T>* a, int s)
+{
+    cplx<T> b[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) b[n2][n1] = a[4 * n1 + n2];
+        dft4(b[n2], s);
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1)
+            if (n2 > 0) b[n2][k1] = cmul(b[n2][k1], w16<T>(n2 * k1, s));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cplx<T> c[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+        dft4(c, s);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = c[k2];
+    }
+}
+
 template <class T, int R>
 __device__ __forceinline__ void dftR(cplx<T>* a, int s)
 {
     if constexpr (R == 2) dft2(a, s);
     else if constexpr (R == 3) dft3(a, s);
     else if constexpr (R == 4) dft4(a, s);
-    else dft5(a, s);
+    else if constexpr (R == 5) dft5(a, s);
+    else if constexpr (R == 8) dft8(a, s);
+    else dft16(a, s);
 }
 
 // exact quotient of small non-negative integers (n < 2^13, d <= 2^12) through a float
@@ -82,6 +151,7 @@ __device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n
 // SQ = L+1 (pencils as padded rows), so the contiguous global rows map to LDS rows.
 struct Lay {
     int L, P, lp, SP, SQ;
+    int off;  // LDS index of position 0 (the direct path keeps a wrapped halo before it)
 };
 
 // one Stockham stage; butterfly b handles pencil p = b mod P at index j = b / P
@@ -135,7 +205,10 @@ __device__ __forceinline__ void fft_lds(cplx<T>* buf, const cplx<T>* tw, const A
     for (int s = 0; s < ax.nstage; ++s) {
         const int R = ax.radix[s];
         const float inv = 1.0f / (float)Ns;
-        if (R == 4) fft_stage<T, 4, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        if (R == 16) {
+            if constexpr (EPT >= 16) fft_stage<T, 16, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        } else if (R == 8) fft_stage<T, 8, EPT, NT>(buf, tw, y, Ns, inv, sign);
+        else if (R == 4) fft_stage<T, 4, EPT, NT>(buf, tw, y, Ns, inv, sign);
         else if (R == 2) fft_stage<T, 2, EPT, NT>(buf, tw, y, Ns, inv, sign);
         else if (R == 3) fft_stage<T, 3, EPT, NT>(buf, tw, y, Ns, inv, sign);
         else fft_stage<T, 5, EPT, NT>(buf, tw, y, Ns, inv, sign);
@@ -175,6 +248,71 @@ __device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisD
         const int e = threadIdx.x + q * NT;
         if (e < E) buf[(e >> y.lp) * y.SP + (e & (y.P - 1)) * y.SQ] = out[q];
     }
+    __syncthreads();
+}
+
+template <class T>
+__device__ __forceinline__ T fmat(T a, T b, T c)
+{
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// Direct circular convolution with the symmetric taps w(0..S), S compile-time:
+//   y(i) = w(0) x(i) + sum_{n=1..S} w(n) (x(i-n) + x(i+n))
+// The pencils sit in LDS with S wrapped halo positions on each side (filled here), a
+// thread computes B consecutive outputs of one pencil from a register window of B+2S
+// inputs; results stay in registers until every window has been read.
+template <class T, int EPT, int NT, int S>
+__device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay& y)
+{
+    constexpr int B = 4, W = B + 2 * S, MAXI = (EPT + B - 1) / B;
+    const int L = y.L;
+    // halo: positions -S..-1 and L..L+S-1 hold x((pos mod L))
+    for (int e = threadIdx.x; e < 2 * S * y.P; e += NT) {
+        const int p = e & (y.P - 1), t = e >> y.lp;
+        const int pos = t < S ? t - S : L + (t - S);
+        int src = pos % L;
+        if (src < 0) src += L;
+        buf[y.off + pos * y.SP + p * y.SQ] = buf[y.off + src * y.SP + p * y.SQ];
+    }
+    __syncthreads();
+    T w[S + 1];
+#pragma unroll
+    for (int n = 0; n <= S; ++n) w[n] = wh[n];
+    const int items = y.P * (L / B);
+    cplx<T> out[MAXI][B];
+    int dst[MAXI];
+#pragma unroll
+    for (int q = 0; q < MAXI; ++q) {
+        const int it = threadIdx.x + q * NT;
+        dst[q] = -1;
+        if (it < items) {
+            const int p = it & (y.P - 1), b = it >> y.lp;
+            const int base = y.off + (b * B - S) * y.SP + p * y.SQ;
+            cplx<T> x[W];
+#pragma unroll
+            for (int t = 0; t < W; ++t) x[t] = buf[base + t * y.SP];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                cplx<T> a = {w[0] * x[u + S].x, w[0] * x[u + S].y};
+#pragma unroll
+                for (int n = 1; n <= S; ++n) {
+                    a.x = fmat(w[n], x[u + S - n].x + x[u + S + n].x, a.x);
+                    a.y = fmat(w[n], x[u + S - n].y + x[u + S + n].y, a.y);
+                }
+                out[q][u] = a;
+            }
+            dst[q] = y.off + b * B * y.SP + p * y.SQ;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXI; ++q)
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int u = 0; u < B; ++u) buf[dst[q] + u * y.SP] = out[q][u];
+        }
     __syncthreads();
 }
 
@@ -317,7 +455,7 @@ __device__ __forceinline__ void for_group(const AxisGeom& g, const Lay& y, long 
         const long vend = g.nlines * (long)y.L;
         for (int e = threadIdx.x; e < E; e += NT) {
             const int p = qdiv(e, invL), pos = e - p * y.L;
-            if (v0 + e < vend) f(v0 + e, pos * y.SP + p * y.SQ);
+            if (v0 + e < vend) f(v0 + e, y.off + pos * y.SP + p * y.SQ);
         }
     } else {
         const int p = threadIdx.x & (y.P - 1);
@@ -333,7 +471,7 @@ __device__ __forceinline__ void for_group(const AxisGeom& g, const Lay& y, long 
         }
         const int step = NT >> y.lp;
         for (int pos = threadIdx.x >> y.lp; pos < y.L; pos += step)
-            f(base + (long)pos * g.stride, pos * y.SP + p);
+            f(base + (long)pos * g.stride, y.off + pos * y.SP + p);
     }
 }
 
@@ -350,7 +488,7 @@ __device__ __forceinline__ void for_quads(const AxisGeom& g, const Lay& y, long 
         const int nq = y.P * L4;
         for (int e4 = threadIdx.x; e4 < nq; e4 += NT) {
             const int p = qdiv(e4, inv), pos0 = (e4 - p * L4) << 2;
-            if (c0 + p < g.nlines) f((c0 + p) * y.L + pos0, pos0 + p * y.SQ, 1);
+            if (c0 + p < g.nlines) f((c0 + p) * y.L + pos0, y.off + pos0 + p * y.SQ, 1);
         }
     } else {
         const int PQ = y.P >> 2;
@@ -368,11 +506,19 @@ __device__ __forceinline__ void for_quads(const AxisGeom& g, const Lay& y, long 
         const int lpq = y.lp - 2;
         const int step = NT >> lpq;
         for (int pos = threadIdx.x >> lpq; pos < y.L; pos += step)
-            f(base + (long)pos * g.stride, pos * y.SP + p0, 1);
+            f(base + (long)pos * g.stride, y.off + pos * y.SP + p0, 1);
     }
 }
 
-template <class T, int EPT, int NT, bool QUAD, class Ld, class Md, class St>
+// LDS elements of a pencil group; the direct path (S > 0) adds S halo positions per side
+inline __host__ __device__ int conv_nbuf(int axis, int L, int P, int S)
+{
+    return axis == 2 ? P * (L + 2 * S + 1) : (L + 2 * S) * (P + 1);
+}
+
+// per-axis convolution pass: S == 0 FFT (or the generic direct fallback), S > 0 the
+// register-window direct path with half-width S
+template <class T, int EPT, int NT, bool QUAD, int S, class Ld, class Md, class St>
 __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int rounds, Ld ld,
                                                   Md md, St st)
 {
@@ -382,12 +528,15 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     y.P = g.P;
     y.lp = g.logP;
     y.SP = g.axis == 2 ? 1 : g.P + 1;
-    y.SQ = g.axis == 2 ? ax.L + 1 : 1;
-    const int nbuf = g.axis == 2 ? g.P * (ax.L + 1) : ax.L * (g.P + 1);
+    y.SQ = g.axis == 2 ? ax.L + 2 * S + 1 : 1;
+    y.off = S * y.SP;
+    const int nbuf = conv_nbuf(g.axis, ax.L, g.P, S);
     cplx<T>* buf = (cplx<T>*)smem;
     cplx<T>* tw = buf + nbuf;
     T* spec = (T*)(tw + ax.L);
-    if (ax.nstage > 0) {
+    if constexpr (S > 0) {
+        for (int t = threadIdx.x; t <= S; t += NT) spec[t] = ((const T*)ax.half)[t];
+    } else if (ax.nstage > 0) {
         for (int t = threadIdx.x; t < ax.L; t += NT) {
             tw[t] = ((const cplx<T>*)ax.tw)[t];
             spec[t] = ((const T*)ax.spec)[t];
@@ -412,7 +561,8 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
         for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = ld(v); });
     }
     __syncthreads();
-    conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
+    if constexpr (S > 0) direct_win<T, EPT, NT, S>(buf, spec, y);
+    else conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
     if constexpr (!std::is_same<Md, MdNone>::value) {
         if (rounds > 1) {
             if constexpr (QUAD) {
@@ -428,7 +578,8 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
                 for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = md(v, buf[l]); });
             }
             __syncthreads();
-            conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
+            if constexpr (S > 0) direct_win<T, EPT, NT, S>(buf, spec, y);
+            else conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
         }
     }
     if constexpr (QUAD) {
@@ -476,20 +627,29 @@ int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld 
     AxisGeom g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, EPT * NT);
     TK_CHECK(ax.L * g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
              ax.L, EPT * NT);
-    const size_t nbuf = axis == 2 ? (size_t)g.P * (ax.L + 1) : (size_t)ax.L * (g.P + 1);
-    size_t lds = nbuf * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
-                 (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
     unsigned nb = (unsigned)((g.nlines + g.P - 1) / g.P);
     const bool quad = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0 &&
                       g.P >= 4;
-    if (quad)
-        hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, true, Ld, Md, St>), dim3(nb), dim3(NT), lds,
-                           ctx->stream, ax, g, rounds, ld, md, st);
-    else
-        hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, false, Ld, Md, St>), dim3(nb), dim3(NT), lds,
-                           ctx->stream, ax, g, rounds, ld, md, st);
+    // register-window direct path for the instantiated half-widths (sigma 3, 4, 6 at h=1)
+    const int S = ax.direct && quad ? ax.nh / 2 : 0;
+    auto lds_for = [&](int s) {
+        return (size_t)conv_nbuf(axis, ax.L, g.P, s) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
+               (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
+    };
+#define TK_CONV_LAUNCH(Q, SS)                                                                  \
+    hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, Q, SS, Ld, Md, St>), dim3(nb), dim3(NT),        \
+                       lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
+    if (S == 9) TK_CONV_LAUNCH(true, 9);
+    else if (S == 12) TK_CONV_LAUNCH(true, 12);
+    else if (S == 18) TK_CONV_LAUNCH(true, 18);
+    else if (quad) TK_CONV_LAUNCH(true, 0);
+    else TK_CONV_LAUNCH(false, 0);
+#undef TK_CONV_LAUNCH
     TK_LAUNCHED();
     return TK_OK;
 }
+
+// half-widths with a compiled register-window direct path
+inline bool direct_width_supported(int nh) { return nh == 19 || nh == 25 || nh == 37; }
 
 }  // namespace tk

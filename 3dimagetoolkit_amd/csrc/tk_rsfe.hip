@@ -336,6 +336,9 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     if (r == TK_OK)
         r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], p->sigma,
                                p->radius, p->gaussian_pixel_approach, dt, &s->flt);
+    // FP32: pick FFT or direct per axis by timing both on this volume's sizes (FP64, the
+    // parity mode, keeps the FFT passes)
+    if (r == TK_OK && dt == TK_F32) r = tk_filter_autotune(ctx, s->flt, nullptr);
     // kones = cv(1), kmyim = cv(myim) in one complex transform (imp:1648-1651)
     if (r == TK_OK) {
         if (dt == TK_F64)
@@ -375,6 +378,7 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
 }
 
 tk_vol tk_rsfe_phi(tk_rsfe s) { return s ? s->phi : nullptr; }
+tk_filter tk_rsfe_filter(tk_rsfe s) { return s ? s->flt : nullptr; }
 
 int tk_rsfe_destroy(tk_rsfe s)
 {

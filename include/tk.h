@@ -103,6 +103,15 @@ int tk_filter_free(tk_filter flt);
 int tk_filter_taps(tk_filter flt, int nh[3]);
 /* real spectrum along one axis (length dims[axis]), including nothing else */
 int tk_filter_axis_spectrum(tk_filter flt, int axis, double* out);
+/* per-axis method: 0 = FFT pass (forward, spectrum multiply, inverse in LDS),
+   1 = direct circular convolution with the symmetric taps (same result up to rounding).
+   No reference counterpart: the reference always uses FFTW. */
+int tk_filter_get_method(tk_filter flt, int axis, int* method);
+int tk_filter_set_method(tk_filter flt, int axis, int method);
+/* autotuner: time both methods of every axis on the device (on the filter's own
+   sizes) and keep the faster; ms (may be NULL) receives {fft, direct} per axis, -1
+   where a method is unavailable */
+int tk_filter_autotune(tk_ctx ctx, tk_filter flt, double ms[6]);
 
 /* ---- R5: cyclic (periodic) convolution ------------------------------------------
  * conv::cyclic_fftw_convolution (convolution_imp.hxx:657-757), the function the
@@ -199,6 +208,8 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
    stats may be NULL, else has room for n_iters entries */
 int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done);
 tk_vol tk_rsfe_phi(tk_rsfe s);
+/* the run's filter (FP32 runs autotune its per-axis methods at creation) */
+tk_filter tk_rsfe_filter(tk_rsfe s);
 int tk_rsfe_destroy(tk_rsfe s);
 
 /* ---- batched uneven volumes (many independent runs, one launch per stage) ------- */

@@ -66,19 +66,37 @@ def test_conv_cyclic_f64(tk, orc, ctx, shape, sigma, h, rng):
     assert np.max(np.abs(vb.numpy() - ref2)) < 4e-15 * np.log2(x.size) * np.max(np.abs(y2))
 
 
+@pytest.mark.parametrize("method", [0, 1])  # FFT pass, direct circular convolution
 @pytest.mark.parametrize("shape,sigma", [((64, 64, 64), 3.0), ((48, 80, 96), 3.0),
-                                         ((128, 64, 160), 4.0)])
-def test_conv_cyclic_f32(tk, orc, ctx, shape, sigma, rng):
-    say("conv f32", shape, sigma)
+                                         ((128, 64, 160), 4.0), ((32, 48, 64), 6.0)])
+def test_conv_cyclic_f32(tk, orc, ctx, shape, sigma, method, rng):
+    say("conv f32", shape, sigma, method)
     x = rng.random(shape)
     tf, nh, spec, nc = orc.gaussian(shape, sigma)
     ref = orc.cyclic_conv(spec, nc, x)
     f = tk.Filter(ctx, shape, sigma, dtype=np.float32)
+    for a in range(3):
+        f.set_method(a, method)
+    assert f.methods() == [method] * 3
     vx = ctx.from_numpy(x.astype(np.float32))
     vy = ctx.volume(shape, np.float32)
     f.conv(vy, vx)
     # FP32 FFT error grows like eps * log2(N): bound 1e-6 of the input scale
     assert np.max(np.abs(vy.numpy().astype(np.float64) - ref)) < 1e-6
+
+
+def test_filter_autotune_keeps_results(tk, orc, ctx, rng):
+    shape = (64, 48, 80)
+    f = tk.Filter(ctx, shape, 3.0, dtype=np.float32)
+    ms = f.autotune()
+    say("autotune ms (fft, direct) per axis:", ms, "->", f.methods())
+    assert all(t[0] > 0 and t[1] > 0 for t in ms)
+    x = rng.random(shape)
+    tf, nh, spec, nc = orc.gaussian(shape, 3.0)
+    vx = ctx.from_numpy(x.astype(np.float32))
+    vy = ctx.volume(shape, np.float32)
+    f.conv(vy, vx)
+    assert np.max(np.abs(vy.numpy() - orc.cyclic_conv(spec, nc, x))) < 1e-6
 
 
 # ---- R9 / R8: Gauss-Seidel -------------------------------------------------------
