@@ -22,6 +22,30 @@ __device__ inline float wave_min(float v)
     return v;
 }
 
+// Workgroups are dispatched round-robin over the XCDs, each with its own L2.  Renumber
+// them so that XCD x runs the contiguous range of logical ids starting at x*q + min(x, r)
+// (q, r = G div/mod nxcd): neighbouring tiles, which share cache lines and halos, then run
+// on the same XCD.  A bijection on [0, G) for any G.
+constexpr unsigned kNumXcd = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned w, unsigned G)
+{
+    const unsigned q = G / kNumXcd, r = G % kNumXcd, x = w % kNumXcd;
+    return x * q + (x < r ? x : r) + w / kNumXcd;
+}
+
+// logical (x, y, z) block index of a 3-D grid after xcd_remap of the x-fastest flat id
+__device__ __forceinline__ uint3 xcd_block3()
+{
+    unsigned w = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    w = xcd_remap(w, gridDim.x * gridDim.y * gridDim.z);
+    uint3 b;
+    b.x = w % gridDim.x;
+    w /= gridDim.x;
+    b.y = w % gridDim.y;
+    b.z = w / gridDim.y;
+    return b;
+}
+
 __device__ inline int flat_tid()
 {
     return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -106,10 +130,15 @@ __device__ __forceinline__ void strow(T* __restrict__ p, const Row<T>& r)
 }
 
 // Visit an X*Y*Z volume in 16-byte rows (Z % V == 0): threads along k in steps of V.
+// Blocks are renumbered with xcd_remap so that rows j-1, j+1 of a block's edge rows are
+// read by blocks of the same XCD.
 #define TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)                                                \
-    for (int r_ = blockIdx.y * blockDim.y + threadIdx.y; r_ < (X) * (Y);                  \
+    for (unsigned w_ = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y), \
+                  bx_ = w_ % gridDim.x, by_ = w_ / gridDim.x, o0_ = 1;                       \
+         o0_; o0_ = 0)                                                                       \
+    for (int r_ = (int)(by_ * blockDim.y + threadIdx.y); r_ < (X) * (Y);                   \
          r_ += gridDim.y * blockDim.y)                                                       \
-        for (int k0 = (blockIdx.x * blockDim.x + threadIdx.x) * (V), i = r_ / (Y),           \
+        for (int k0 = (int)(bx_ * blockDim.x + threadIdx.x) * (V), i = r_ / (Y),             \
                  j = r_ - i * (Y);                                                           \
              k0 < (Z); k0 += gridDim.x * blockDim.x * (V))                                   \
             for (long v0 = (long)r_ * (Z) + k0, once_ = 1; once_; once_ = 0)

@@ -5,6 +5,7 @@
 
 #include <type_traits>
 
+#include "tk_device.hpp"
 #include "tk_fft.hpp"
 
 namespace tk {
@@ -544,7 +545,8 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     } else {
         for (int t = threadIdx.x; t < ax.nh; t += NT) spec[t] = ((const T*)ax.spec)[t];
     }
-    const long c0 = (long)blockIdx.x * g.P;
+    // consecutive groups share 128-byte lines along x and y: keep them on one XCD
+    const long c0 = (long)xcd_remap(blockIdx.x, gridDim.x) * g.P;
     // pencils past the end of the volume stay zero
     if ((c0 + g.P) > g.nlines) {
         for (int e = threadIdx.x; e < nbuf; e += NT) buf[e] = {0, 0};

@@ -248,8 +248,9 @@ __global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* 
     constexpr int RJ = TJ + 2, RK = TK + 2;  // red planes, one-voxel ring
     __shared__ T O[4][OJ * OK];
     __shared__ T Rp[3][RJ * RK];
-    const int k0 = 1 + blockIdx.x * TK, j0 = 1 + blockIdx.y * TJ;
-    const int ia = 1 + blockIdx.z * kRbCI;
+    const uint3 bl = xcd_block3();  // tiles sharing halo rows run on one XCD
+    const int k0 = 1 + bl.x * TK, j0 = 1 + bl.y * TJ;
+    const int ia = 1 + bl.z * kRbCI;
     const int ib = min(ia + kRbCI, X - 1);
     const long sX = (long)Y * Z;
     const int tid = threadIdx.x;
@@ -361,9 +362,10 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     __shared__ T O[4][OJ * W];
     __shared__ T Rp[3][RJ * W];
     __shared__ T Bp[3][RJ * W];
-    const int kb = blockIdx.x * kRbTK, kbase = kb - 4;
-    const int j0 = 1 + blockIdx.y * TJ;
-    const int ia = 1 + blockIdx.z * CI;
+    const uint3 bl = xcd_block3();  // tiles sharing halo rows run on one XCD
+    const int kb = bl.x * kRbTK, kbase = kb - 4;
+    const int j0 = 1 + bl.y * TJ;
+    const int ia = 1 + bl.z * CI;
     const int ib = min(ia + CI, X - 1);
     const long sX = (long)Y * Z;
     const int tid = threadIdx.x;
