@@ -88,6 +88,8 @@ def _load():
         "tk_poisson_dir_sweeps": (i, [vp, vp, vp, d, d, i, i]),
         "tk_poisson_dir_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
+        "tk_poisson_rb_autotune": (i, [vp, vp, ip, dp]),
+        "tk_poisson_rb_set_tile": (i, [vp, vp, i]),
         "tk_heaviside": (i, [vp, vp, d, d, d, vp]),
         "tk_grad": (i, [vp, vp, vp, vp, vp]),
         "tk_div": (i, [vp, vp, vp, vp, vp]),
@@ -339,6 +341,18 @@ def poisson_residual(ctx, phi, rhs):
     r = C.c_double()
     _check(lib.tk_poisson_residual(ctx.h, phi.h, rhs.h, C.byref(r)))
     return r.value
+
+
+def poisson_rb_autotune(ctx, rhs):
+    """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16))."""
+    rows = C.c_int()
+    ms = (C.c_double * 2)()
+    _check(lib.tk_poisson_rb_autotune(ctx.h, rhs.h, C.byref(rows), ms))
+    return rows.value, (ms[0], ms[1])
+
+
+def poisson_rb_set_tile(ctx, rhs, rows):
+    _check(lib.tk_poisson_rb_set_tile(ctx.h, rhs.h, int(rows)))
 
 
 class Rsfe:

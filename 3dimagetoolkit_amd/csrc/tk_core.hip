@@ -56,6 +56,14 @@ __global__ void k_fill(T* __restrict__ a, T v, long n)
         a[i] = v;
 }
 
+// 16-byte vector copy (the kernel of a same-type volume copy)
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src,
+                                                float4* __restrict__ dst, long n16)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n16; i += (long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
@@ -597,8 +605,16 @@ int tk_vol_copy(tk_vol dst, tk_vol src)
     CHECK_VOL(src);
     TK_CHECK(same_dims(dst, src), TK_ERR_SHAPE, "tk_vol_copy: dims differ");
     if (dst->dtype == src->dtype) {
-        TK_HIP(hipMemcpyAsync(dst->d, src->d, dst->bytes(), hipMemcpyDeviceToDevice,
-                              dst->ctx->stream));
+        const size_t bytes = dst->bytes();
+        if (bytes % 16 == 0 && (uintptr_t)dst->d % 16 == 0 && (uintptr_t)src->d % 16 == 0) {
+            const long n16 = (long)(bytes / 16);
+            hipLaunchKernelGGL(k_copy16, dim3((unsigned)std::min<long>((n16 + 255) / 256, 65536)),
+                               dim3(256), 0, dst->ctx->stream, (const float4*)src->d,
+                               (float4*)dst->d, n16);
+            TK_LAUNCHED();
+            return TK_OK;
+        }
+        TK_HIP(hipMemcpyAsync(dst->d, src->d, bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
         return TK_OK;
     }
     return convert_on_device(dst->ctx, dst->d, dst->dtype, src->d, src->dtype, dst->n());

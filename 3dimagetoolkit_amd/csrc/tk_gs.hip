@@ -25,6 +25,7 @@
 // the stop rule on the device and raises a flag that makes later sweeps return at once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "tk_device.hpp"
@@ -343,13 +344,12 @@ __global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* 
 
 // The same sweep for rows of 16 bytes (nz a multiple of V): the tile's k columns are
 // aligned to multiples of 64, so every plane row is moved with 16-byte accesses; the rhs
-// rows are staged in LDS with the old planes, and the next plane is fetched into
-// registers while the current one is computed.
+// rows are staged in LDS with the old planes, and the planes two steps ahead are fetched
+// into registers (two alternating sets) while the current one is computed.  TJ (rows per
+// tile) is 8 or 16, picked per volume shape by timing both.
 constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
-constexpr int kRbOJ = kRbTJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
-constexpr int kRbRJ = kRbTJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
 
-template <class T, bool UNIT>
+template <class T, bool UNIT, int TJ>
 __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
                                                   const T* __restrict__ rhs, int X, int Y, int Z,
                                                   int CI, GsCoef c, int s, int slot, int gated,
@@ -357,7 +357,9 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
 {
     if (gated && *(volatile unsigned*)g.stop) return;
     constexpr int V = Vec16<T>::n;
-    constexpr int W = kRbW, WV = W / V, TJ = kRbTJ, OJ = kRbOJ, RJ = kRbRJ;
+    constexpr int W = kRbW, WV = W / V;
+    constexpr int OJ = TJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
+    constexpr int RJ = TJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
     constexpr int NO = (OJ * WV + 255) / 256, NB = (RJ * WV + 255) / 256;
     __shared__ T O[4][OJ * W];
     __shared__ T Rp[3][RJ * W];
@@ -369,7 +371,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     const int ib = min(ia + CI, X - 1);
     const long sX = (long)Y * Z;
     const int tid = threadIdx.x;
-    Row<T> po[NO], pb[NB];
+    Row<T> poA[NO], pbA[NB], poB[NO], pbB[NB];
     auto fetch = [&](const T* base, int rows, int row0, Row<T>* out, int n) {
 #pragma unroll
         for (int q = 0; q < n; ++q) {
@@ -453,29 +455,38 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     // prologue: old planes ia-2 .. ia+1 and rhs planes ia-1, ia into LDS
     for (int p = ia - 2; p <= ia + 1; ++p)
         if (p >= 0 && p <= X - 1) {
-            fetch(src + (long)p * sX, OJ, j0 - 2, po, NO);
-            deposit(O[p & 3], OJ, po, NO);
+            fetch(src + (long)p * sX, OJ, j0 - 2, poA, NO);
+            deposit(O[p & 3], OJ, poA, NO);
         }
     for (int p = ia - 1; p <= ia; ++p) {
-        fetch(rhs + (long)p * sX, RJ, j0 - 1, pb, NB);
-        deposit(Bp[p % 3], RJ, pb, NB);
+        fetch(rhs + (long)p * sX, RJ, j0 - 1, pbA, NB);
+        deposit(Bp[p % 3], RJ, pbA, NB);
     }
-    if (ia + 2 <= X - 1) fetch(src + (long)(ia + 2) * sX, OJ, j0 - 2, po, NO);
-    fetch(rhs + (long)(ia + 1) * sX, RJ, j0 - 1, pb, NB);
+    // set A holds old plane ia+2 and rhs plane ia+1, set B old ia+3 and rhs ia+2
+    if (ia + 2 <= X - 1) fetch(src + (long)(ia + 2) * sX, OJ, j0 - 2, poA, NO);
+    fetch(rhs + (long)(ia + 1) * sX, RJ, j0 - 1, pbA, NB);
+    if (ia + 3 <= X - 1) fetch(src + (long)(ia + 3) * sX, OJ, j0 - 2, poB, NO);
+    if (ia + 2 <= X - 1) fetch(rhs + (long)(ia + 2) * sX, RJ, j0 - 1, pbB, NB);
     __syncthreads();
     red(ia - 1);
     red(ia);
     __syncthreads();
-    for (int p = ia; p < ib; ++p) {
+    // step p deposits old plane p+2 and rhs plane p+1 (fetched two steps earlier) and
+    // refills the same register set with old plane p+4 and rhs plane p+3
+    auto step = [&](int p, Row<T>* po, Row<T>* pb) {
         // slots refilled here were last read two planes ago (separated by barriers)
         if (p + 2 <= X - 1) deposit(O[(p + 2) & 3], OJ, po, NO);
         deposit(Bp[(p + 1) % 3], RJ, pb, NB);
-        if (p + 3 <= X - 1) fetch(src + (long)(p + 3) * sX, OJ, j0 - 2, po, NO);
-        if (p + 2 <= X - 1) fetch(rhs + (long)(p + 2) * sX, RJ, j0 - 1, pb, NB);
+        if (p + 4 <= X - 1) fetch(src + (long)(p + 4) * sX, OJ, j0 - 2, po, NO);
+        if (p + 3 <= X - 1) fetch(rhs + (long)(p + 3) * sX, RJ, j0 - 1, pb, NB);
         __syncthreads();
         red(p + 1);
         __syncthreads();
         black(p);
+    };
+    for (int p = ia; p < ib; p += 2) {
+        step(p, poA, pbA);
+        if (p + 1 < ib) step(p + 1, poB, pbB);
     }
     if (!gated) return;
     block_sum2(d2, o2);
@@ -506,6 +517,39 @@ __global__ void k_zero_u(unsigned* p, int n)
     for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0;
 }
 
+inline long long rb_key(int X, int Y, int Z, int w)
+{
+    return (((long long)X * 65536 + Y) * 65536 + Z) * 16 + w;
+}
+
+// aligned red-black sweep with TJ-row tiles
+template <class T, int TJ>
+static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
+                        int s, int slot, int gated, const RbGate& g, bool unit)
+{
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    // chunk length in x chosen so the grid is one round of as many workgroups per CU as
+    // the LDS holds (160 KB per CU)
+    const long lds = (long)sizeof(T) * (4L * (TJ + 4) + 6L * (TJ + 2)) * kRbW + 256;
+    const long per_cu = std::max(1L, 163840L / lds);
+    const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + TJ - 1) / TJ;
+    long chunks = ((long)ctx->num_cu * per_cu + (long)kt * jt / 2) / ((long)kt * jt);
+    if (chunks < 1) chunks = 1;
+    if (chunks > X - 2) chunks = X - 2;
+    const int CI = (int)((X - 2 + chunks - 1) / chunks);
+    const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
+    if (unit)
+        hipLaunchKernelGGL((k_gs_rb_pv<T, true, TJ>), grid, dim3(256), 0, ctx->stream,
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                           gated, g);
+    else
+        hipLaunchKernelGGL((k_gs_rb_pv<T, false, TJ>), grid, dim3(256), 0, ctx->stream,
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                           gated, g);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
 template <class T>
 static int rb_sweep(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
                     int slot, int gated, const RbGate& g)
@@ -515,23 +559,10 @@ static int rb_sweep(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const Gs
     TimedLaunch tl(ctx, TK_K_GS);
     constexpr int V = Vec16<T>::n;
     if (Z % V == 0) {
-        // chunk length in x chosen so the grid is about five workgroups per CU (one round)
-        const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + kRbTJ - 1) / kRbTJ;
-        long chunks = ((long)ctx->num_cu * 5 + (long)kt * jt / 2) / ((long)kt * jt);
-        if (chunks < 1) chunks = 1;
-        if (chunks > X - 2) chunks = X - 2;
-        const int CI = (int)((X - 2 + chunks - 1) / chunks);
-        const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
-        if (unit)
-            hipLaunchKernelGGL((k_gs_rb_pv<T, true>), grid, dim3(256), 0, ctx->stream,
-                               (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                               gated, g);
-        else
-            hipLaunchKernelGGL((k_gs_rb_pv<T, false>), grid, dim3(256), 0, ctx->stream,
-                               (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                               gated, g);
-        TK_LAUNCHED();
-        return TK_OK;
+        const auto it = ctx->rb_rows.find(rb_key(X, Y, Z, (int)sizeof(T)));
+        if (it != ctx->rb_rows.end() && it->second == 16)
+            return launch_rb_pv<T, 16>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
+        return launch_rb_pv<T, 8>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
     }
     const dim3 grid((Z - 2 + kRbTK - 1) / kRbTK, (Y - 2 + kRbTJ - 1) / kRbTJ,
                     (X - 2 + kRbCI - 1) / kRbCI);
@@ -542,6 +573,61 @@ static int rb_sweep(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const Gs
         hipLaunchKernelGGL((k_gs_rb_pp<T, false>), grid, dim3(256), 0, ctx->stream, (const T*)src,
                            (T*)dst, (const T*)rhs->d, X, Y, Z, c, s, slot, gated, g);
     TK_LAUNCHED();
+    return TK_OK;
+}
+
+// time the 8- and 16-row tile variants of the aligned sweep on rhs's shape (ungated sweeps
+// of a zero field into a scratch volume); keep the faster for this context
+template <class T>
+static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
+{
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    if (Z % Vec16<T>::n != 0) {
+        if (rows_out) *rows_out = 0;  // unaligned rows: single scalar variant
+        return TK_OK;
+    }
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double ms[2] = {0, 0};
+    int rc = TK_OK;
+    const GsCoef c = gs_coef(rhs, 1.0);
+    const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1.;
+    RbGate g = {};
+    auto body = [&]() -> int {
+        TK_HIP(hipMalloc(&a, rhs->bytes()));
+        TK_HIP(hipMalloc(&b, rhs->bytes()));
+        TK_HIP(hipMemsetAsync(a, 0, rhs->bytes(), ctx->stream));
+        TK_HIP(hipEventCreate(&e0));
+        TK_HIP(hipEventCreate(&e1));
+        for (int v = 0; v < 2; ++v) {
+            auto launch = [&]() {
+                return v == 0 ? launch_rb_pv<T, 8>(ctx, a, b, rhs, c, 0, 0, 0, g, unit)
+                              : launch_rb_pv<T, 16>(ctx, a, b, rhs, c, 0, 0, 0, g, unit);
+            };
+            TK_TRY(launch());
+            TK_HIP(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 3; ++r) TK_TRY(launch());
+            TK_HIP(hipEventRecord(e1, ctx->stream));
+            TK_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            TK_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms[v] = t / 3.0;
+        }
+        return TK_OK;
+    };
+    rc = body();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (rc != TK_OK) return rc;
+    const int rows = ms[1] < ms[0] ? 16 : 8;
+    ctx->rb_rows[rb_key(X, Y, Z, (int)sizeof(T))] = rows;
+    if (rows_out) *rows_out = rows;
+    if (ms_out) {
+        ms_out[0] = ms[0];
+        ms_out[1] = ms[1];
+    }
     return TK_OK;
 }
 
@@ -790,6 +876,24 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
                              err_out);
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
                         err_out);
+}
+
+int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_rb_autotune: null context");
+    TK_TRY(check_gs(rhs, rhs));
+    return rhs->dtype == TK_F64 ? rb_autotune<double>(ctx, rhs, tile_rows, ms)
+                                : rb_autotune<float>(ctx, rhs, tile_rows, ms);
+}
+
+int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_rb_set_tile: null context");
+    TK_TRY(check_gs(rhs, rhs));
+    TK_CHECK(tile_rows == 8 || tile_rows == 16, TK_ERR_ARG,
+             "tk_poisson_rb_set_tile: tile_rows must be 8 or 16");
+    ctx->rb_rows[rb_key(rhs->nx, rhs->ny, rhs->nz, (int)dtype_size(rhs->dtype))] = tile_rows;
+    return TK_OK;
 }
 
 int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel)

@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,8 @@ struct tk_ctx_s {
     std::vector<Pending> pending;
     double tag_ms[TK_K_COUNT] = {0};
     long tag_n[TK_K_COUNT] = {0};
+    // red-black sweep tile rows (8 or 16) picked by timing, per volume shape and type
+    std::map<long long, int> rb_rows;
 };
 
 struct tk_vol_s {

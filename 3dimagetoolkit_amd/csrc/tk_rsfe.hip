@@ -339,6 +339,9 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     // FP32: pick FFT or direct per axis by timing both on this volume's sizes (FP64, the
     // parity mode, keeps the FFT passes)
     if (r == TK_OK && dt == TK_F32) r = tk_filter_autotune(ctx, s->flt, nullptr);
+    // red-black runs: pick the sweep tile by timing (the result does not depend on it)
+    if (r == TK_OK && p->gs_order == TK_GS_REDBLACK)
+        r = tk_poisson_rb_autotune(ctx, s->u0, nullptr, nullptr);
     // kones = cv(1), kmyim = cv(myim) in one complex transform (imp:1648-1651)
     if (r == TK_OK) {
         if (dt == TK_F64)

@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 SEED = 20240602
 
-# algorithmic bytes per voxel per launch, in element sizes w (model of DESIGN.md)
+# algorithmic bytes per voxel per launch, in element sizes w
 BYTES_PER_VOXEL_W = {
     "conv_x1": 5,   # read phi, u0; write complex (2); write phiold
     "conv_y": 4,    # read complex (2), write complex (2)
@@ -40,6 +40,34 @@ BYTES_PER_VOXEL_W = {
     "sb": 13,       # read phi, phiold, b(3), g; write phi, b(3), q(3)
     "div": 4,       # read q(3), write divdb
 }
+
+# HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this bench's
+# default workload, each counter scaled by what it reads for k_copy16 over 512 MiB (FETCH x2.0,
+# WRITE x1.0); keyed by (size, dtype, sweep order, kernel tag).
+TRAFFIC_SOURCE = "profiles/rsfe256_f32_traffic.md"
+MEASURED_TRAFFIC = {}
+
+
+def copy_probe(tk, steps):
+    """512^3 FP32 (512 MiB) volume copy, larger than the last-level cache: the plain
+    copy of known size (512 MiB read + 512 MiB written per launch)."""
+    ctx = tk.Context(0)
+    shape = (512, 512, 512)
+    a = ctx.volume(shape, np.float32)
+    b = ctx.volume(shape, np.float32)
+    tk._check(tk.lib.tk_vol_fill(a.h, 1.0))
+    for _ in range(2):
+        tk._check(tk.lib.tk_vol_copy(b.h, a.h))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tk._check(tk.lib.tk_vol_copy(b.h, a.h))
+    ctx.sync()
+    t = (time.perf_counter() - t0) / steps
+    nbytes = 2 * 512 ** 3 * 4
+    print(json.dumps({"probe": "k_copy16 512 MiB", "bytes_per_launch": nbytes,
+                      "us_per_launch": round(t * 1e6, 1), "GBps": round(nbytes / t / 1e9, 1)}),
+          flush=True)
 
 
 def main():
@@ -56,6 +84,9 @@ def main():
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time a 512 MiB volume copy (k_copy16): the known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counts")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every box here has "
@@ -63,6 +94,9 @@ def main():
         sys.exit(2)
 
     tk = importlib.import_module("3dimagetoolkit_amd")
+    if a.copy_probe:
+        copy_probe(tk, a.steps)
+        return
     n = a.size
     shape = (n, n, n)
     N = n ** 3
@@ -119,13 +153,22 @@ def main():
         if kernels:
             dom = max(kernels, key=lambda t: kernels[t]["total_ms"])
             k = kernels[dom]
+            key = (n, a.dtype, "rb" if order == tk.TK_GS_REDBLACK else "lex", dom)
+            traffic = MEASURED_TRAFFIC.get(key)
             roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": traffic,
                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]}
-    # whole-loop model: conv 5 passes (25w) + S sweeps (3w) + sb (13w) + div (4w)
+            if traffic is not None:
+                roof["traffic_source"] = TRAFFIC_SOURCE
+    # whole-loop models.  Fused passes of this implementation: conv 5 passes (25w) + S sweeps
+    # (3w each) + sb (13w) + div (4w) = N (42w + 3wS).  Reference data-flow model of the
+    # metric's definition: N (58w + 3wS) (every temporary of rsfe_splitbregman_imp.hxx
+    # :1547-1995 read and written once).
     S = float(np.mean(sweeps))
-    model_bytes = N * (25 * w + 3 * w * S + 13 * w + 4 * w)
+    model_bytes = N * (42 * w + 3 * w * S)
     loop_gbs = model_bytes * a.steps / t_loop / 1e9
+    ref_model_gbs = N * (58 * w + 3 * w * S) * a.steps / t_loop / 1e9
     run.close()
 
     cpu = None
@@ -166,6 +209,8 @@ def main():
                    "t_setup_s": round(t_setup, 4),
                    "loop_model_GBps": round(loop_gbs, 1),
                    "loop_model_frac": round(loop_gbs / HBM_PEAK_GBS, 4),
+                   "loop_ref_model_GBps": round(ref_model_gbs, 1),
+                   "loop_ref_model_frac": round(ref_model_gbs / HBM_PEAK_GBS, 4),
                    "kernels": kernels},
         "roofline": roof,
         "cpu_baseline": cpu,

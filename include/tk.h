@@ -141,6 +141,14 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
    ||rhs - (phi/dt - lap_h phi) + phi/dt||_2 over the interior / ||rhs||_2, i.e.
    ||rhs + lap_h phi|| / ||rhs|| with FP64 accumulation */
 int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel_res);
+/* autotuner of the red-black sweep (no reference counterpart: the reference has one fixed
+   lexicographic loop): times the 8-row and 16-row tile variants on rhs's shape and type,
+   keeps the faster for every later red-black sweep of that shape on ctx, returns the rows
+   in *tile_rows and the times in ms[0..1] (either may be null).  Both variants give
+   identical results.  Runs synchronously; tk_rsfe_create calls it for FP32 runs. */
+int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms);
+/* set the red-black tile rows (8 or 16) for rhs's shape and type on ctx */
+int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows);
 
 /* ---- R6, R10-R13: pointwise and stencil operators ----------------------------- */
 /* rsfe_splitbregman::set_heaviside (rsfe_splitbregman_imp.hxx:174-192) */

@@ -162,20 +162,36 @@ def rb_sweep_f32(phi, rhs):
     return p
 
 
-@pytest.mark.parametrize("shape", [(16, 18, 20), (33, 17, 9)])
-def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rng):
-    say("gs rb", shape)
+@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("shape", [(16, 18, 20), (33, 17, 9), (40, 37, 136)])
+def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rows, rng):
+    say("gs rb", shape, "tile rows", rows)
     phi = rng.random(shape).astype(np.float32)
     rhs = rng.standard_normal(shape).astype(np.float32)
     ref = phi
     for _ in range(3):
         ref = rb_sweep_f32(ref, rhs)
     vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    tk.poisson_rb_set_tile(ctx, vr, rows)
     tk.poisson_sweeps(ctx, vp, vr, 3, 1.0, 0.0, tk.TK_GS_REDBLACK)
     assert np.array_equal(vp.numpy(), ref)
 
 
-def test_gs_redblack_gate_matches_replay(tk, ctx, rng):
+def test_gs_redblack_autotune(tk, ctx, rng):
+    shape = (64, 66, 128)
+    rhs = ctx.from_numpy(rng.standard_normal(shape).astype(np.float32))
+    rows, ms = tk.poisson_rb_autotune(ctx, rhs)
+    say("rb autotune ms (8 rows, 16 rows):", ms, "->", rows)
+    assert rows in (8, 16) and min(ms) > 0
+    assert rows == (16 if ms[1] < ms[0] else 8)
+    phi = rng.random(shape).astype(np.float32)
+    vp = ctx.from_numpy(phi)
+    tk.poisson_sweeps(ctx, vp, rhs, 2, 1.0, 0.0, tk.TK_GS_REDBLACK)
+    assert np.array_equal(vp.numpy(), rb_sweep_f32(rb_sweep_f32(phi, rhs.numpy()), rhs.numpy()))
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+def test_gs_redblack_gate_matches_replay(tk, ctx, rows, rng):
     shape = (20, 22, 24)
     phi = rng.random(shape).astype(np.float32)
     rhs = rng.standard_normal(shape).astype(np.float32)
@@ -191,6 +207,7 @@ def test_gs_redblack_gate_matches_replay(tk, ctx, rng):
         if s >= 5 and not err > tol:
             break
     vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    tk.poisson_rb_set_tile(ctx, vr, rows)
     sg, eg = tk.poisson_solve(ctx, vp, vr, 5, tol, order=tk.TK_GS_REDBLACK)
     assert sg == s
     assert np.array_equal(vp.numpy(), q)
