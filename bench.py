@@ -45,7 +45,16 @@ BYTES_PER_VOXEL_W = {
 # default workload, each counter scaled by what it reads for k_copy16 over 512 MiB (FETCH x2.0,
 # WRITE x1.0); keyed by (size, dtype, sweep order, kernel tag).
 TRAFFIC_SOURCE = "profiles/rsfe256_f32_traffic.md"
-MEASURED_TRAFFIC = {}
+_TRAFFIC_256_F32_RB = {
+    "conv_x1": 335834595,
+    "conv_z2": 402997128,
+    "conv_x2": 404011419,
+    "conv_y": 268616707,
+    "gs": 225215412,
+    "sb": 942136823,
+    "div": 322915812,
+}
+MEASURED_TRAFFIC = {(256, "f32", "rb", k): v for k, v in _TRAFFIC_256_F32_RB.items()}
 
 
 def copy_probe(tk, steps):
