@@ -70,6 +70,7 @@ def _load():
         "tk_vol_wrap": (i, [vp, vp, i, i, i, d, d, d, i, C.POINTER(vp)]),
         "tk_vol_free": (i, [vp]),
         "tk_vol_info": (i, [vp, ip, dp, ip]),
+        "tk_vol_set_spacing": (i, [vp, d, d, d]),
         "tk_vol_device_ptr": (vp, [vp]),
         "tk_vol_upload": (i, [vp, vp, i]),
         "tk_vol_download": (i, [vp, vp, i]),
@@ -359,8 +360,10 @@ class Rsfe:
     """tk_rsfe: one segm::rsfe_splitbregman run resident on the device."""
 
     def __init__(self, ctx, u0, init, params):
+        """init: a Volume (initialize_phi_with_init) or None (initialize_phi_with_cube)."""
         hv = C.c_void_p()
-        _check(lib.tk_rsfe_create(ctx.h, u0.h, init.h, C.byref(params), C.byref(hv)))
+        _check(lib.tk_rsfe_create(ctx.h, u0.h, init.h if init is not None else None,
+                                  C.byref(params), C.byref(hv)))
         self.h = hv
         self.ctx = ctx
         self.shape = u0.shape

@@ -71,6 +71,20 @@ def build_oracle():
     return os.path.join(odir, "liboracle.so")
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_class_api.cpp")
+CPP_TEST_LIB = os.path.join(ROOT, "tests", "cpp", "libtk_cpptest.so")
+
+
+def build_cpp_tests():
+    """The C++ class-layer test library: include/tk/*.hxx over libtk.so, checked against the
+    oracle library (test infrastructure; loaded by tests/test_gpu_cpp_api.py)."""
+    odir = os.path.join(ROOT, "oracle")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-o", CPP_TEST_LIB, CPP_TEST_SRC,
+          "-L", PKG, "-ltk", "-L", odir, "-loracle", "-fopenmp",
+          "-Wl,-rpath,$ORIGIN/../../3dimagetoolkit_amd", "-Wl,-rpath,$ORIGIN/../../oracle"])
+    return CPP_TEST_LIB
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--no-oracle", action="store_true")
@@ -79,6 +93,7 @@ def main():
     print(build_lib(a.j))
     if not a.no_oracle and shutil.which("make"):
         print(build_oracle())
+        print(build_cpp_tests())
 
 
 if __name__ == "__main__":

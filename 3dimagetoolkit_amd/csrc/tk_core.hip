@@ -505,6 +505,16 @@ int tk_vol_free(tk_vol v)
     return TK_OK;
 }
 
+int tk_vol_set_spacing(tk_vol v, double hx, double hy, double hz)
+{
+    TK_CHECK(v, TK_ERR_ARG, "tk_vol_set_spacing: null volume");
+    TK_CHECK(hx > 0 && hy > 0 && hz > 0, TK_ERR_ARG, "tk_vol_set_spacing: spacing must be > 0");
+    v->hx = hx;
+    v->hy = hy;
+    v->hz = hz;
+    return TK_OK;
+}
+
 int tk_vol_info(tk_vol v, int dims[3], double h[3], tk_dtype* dtype)
 {
     CHECK_VOL(v);

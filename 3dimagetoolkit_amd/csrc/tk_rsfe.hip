@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 
 #include "tk_device.hpp"
 #include "tk_fft_kernel.hpp"
@@ -132,6 +133,39 @@ struct StRhs {
         return r;
     }
 };
+
+// initialize_phi_with_init, 2nd case (rsfe_splitbregman_imp.hxx:117-160): init is one voxel
+// short along some axes; the reference copies init, then the last-but-one face onto the last
+// face, axis by axis, which equals reading init at the clamped index
+template <class T, class S>
+__global__ void k_init_extend(T* __restrict__ phi, const S* __restrict__ init, int X, int Y,
+                              int Z, int ix, int iy, int iz)
+{
+    const long n = (long)X * Y * Z;
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        const int k = (int)(v % Z);
+        const long r = v / Z;
+        const int j = (int)(r % Y), i = (int)(r / Y);
+        const int si = i < ix ? i : ix - 1, sj = j < iy ? j : iy - 1, sk = k < iz ? k : iz - 1;
+        phi[v] = (T)init[((long)si * iy + sj) * iz + sk];
+    }
+}
+
+// initialize_phi_with_cube (rsfe_splitbregman_imp.hxx:48-95): a0 everywhere, b0 on
+// [ix,fx) x [iy,fy) x [iz,fz)
+template <class T>
+__global__ void k_init_cube(T* __restrict__ phi, int X, int Y, int Z, int ix, int fx, int iy,
+                            int fy, int iz, int fz, T a0, T b0)
+{
+    const long n = (long)X * Y * Z;
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        const int k = (int)(v % Z);
+        const long r = v / Z;
+        const int j = (int)(r % Y), i = (int)(r / Y);
+        const bool in = i >= ix && i < fx && j >= iy && j < fy && k >= iz && k < fz;
+        phi[v] = in ? b0 : a0;
+    }
+}
 
 __global__ void k_sum_pairs_rsfe(const double* __restrict__ part, int nb, double* __restrict__ out)
 {
@@ -266,6 +300,62 @@ void free_state(tk_rsfe s)
     delete s;
 }
 
+// phi := init as initialize_phi_with_init (rsfe_splitbregman_imp.hxx:99-170); no init, as
+// apply(myim), or init of other dims: initialize_phi_with_cube (48-95)
+int init_phi(tk_rsfe s, tk_vol init)
+{
+    tk_ctx ctx = s->ctx;
+    const int X = s->X, Y = s->Y, Z = s->Z;
+    const unsigned nb = blocks_for((long)X * Y * Z, 256, 8192);
+    if (init && same_dims(init, s->u0)) {
+        // 1st case: phi = init; change_range_of_intensity(b0, a0)
+        TK_TRY(tk_vol_copy(s->phi, init));
+        return tk_change_range_of_intensity(ctx, s->phi, s->p.b0, s->p.a0);
+    }
+    auto near = [](int a, int d) { return a == d || a == d - 1; };
+    if (init && near(init->nx, X) && near(init->ny, Y) && near(init->nz, Z)) {
+        // 2nd case: copy, extend the last faces, then change_range_of_intensity(b0, a0)
+        const int ix = init->nx, iy = init->ny, iz = init->nz;
+        if (s->dtype == TK_F64) {
+            if (init->dtype == TK_F64)
+                hipLaunchKernelGGL((k_init_extend<double, double>), dim3(nb), dim3(256), 0,
+                                   ctx->stream, (double*)s->phi->d, (const double*)init->d, X, Y,
+                                   Z, ix, iy, iz);
+            else
+                hipLaunchKernelGGL((k_init_extend<double, float>), dim3(nb), dim3(256), 0,
+                                   ctx->stream, (double*)s->phi->d, (const float*)init->d, X, Y,
+                                   Z, ix, iy, iz);
+        } else {
+            if (init->dtype == TK_F64)
+                hipLaunchKernelGGL((k_init_extend<float, double>), dim3(nb), dim3(256), 0,
+                                   ctx->stream, (float*)s->phi->d, (const double*)init->d, X, Y,
+                                   Z, ix, iy, iz);
+            else
+                hipLaunchKernelGGL((k_init_extend<float, float>), dim3(nb), dim3(256), 0,
+                                   ctx->stream, (float*)s->phi->d, (const float*)init->d, X, Y,
+                                   Z, ix, iy, iz);
+        }
+        TK_LAUNCHED();
+        return tk_change_range_of_intensity(ctx, s->phi, s->p.b0, s->p.a0);
+    }
+    if (init)
+        fprintf(stderr, "WARNING::apply: wrong dimensions of init, cube initial contour is used "
+                        "instead.\n");
+    // 3rd case / no init: the cube [0.33 d, 0.67 d) per axis (imp:53-60)
+    const double frac = 0.33;
+    const int ix = (int)(X * frac), iy = (int)(Y * frac), iz = (int)(Z * frac);
+    const int fx = (int)(X * (1 - frac)), fy = (int)(Y * (1 - frac)), fz = (int)(Z * (1 - frac));
+    if (s->dtype == TK_F64)
+        hipLaunchKernelGGL(k_init_cube<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)s->phi->d, X, Y, Z, ix, fx, iy, fy, iz, fz, s->p.a0, s->p.b0);
+    else
+        hipLaunchKernelGGL(k_init_cube<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (float*)s->phi->d, X, Y, Z, ix, fx, iy, fy, iz, fz, (float)s->p.a0,
+                           (float)s->p.b0);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -299,10 +389,10 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
 
 int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, tk_rsfe* out)
 {
-    TK_CHECK(ctx && u0 && init && p && out, TK_ERR_ARG, "tk_rsfe_create: null argument");
+    TK_CHECK(ctx && u0 && p && out, TK_ERR_ARG, "tk_rsfe_create: null argument");
     TK_CHECK(u0->dtype == TK_F32 || u0->dtype == TK_F64, TK_ERR_DTYPE, "tk_rsfe_create: u0 dtype");
-    TK_CHECK(same_dims(u0, init), TK_ERR_SHAPE,
-             "tk_rsfe_create: init must have the dims of the image");
+    TK_CHECK(!init || init->dtype == TK_F32 || init->dtype == TK_F64, TK_ERR_DTYPE,
+             "tk_rsfe_create: init dtype");
     TK_CHECK(u0->nx >= 3 && u0->ny >= 3 && u0->nz >= 3, TK_ERR_UNSUPPORTED,
              "tk_rsfe_create: 3-D volumes with every edge >= 3 only");
     TK_CHECK(p->warm_start == 0, TK_ERR_UNSUPPORTED,
@@ -329,9 +419,7 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
         r = TK_ERR_NOMEM;
     }
     if (r == TK_OK) r = tk_vol_copy(s->u0, u0);
-    // initialize_phi_with_init, same dims: phi = init; change_range_of_intensity(b0, a0)
-    if (r == TK_OK) r = tk_vol_copy(s->phi, init);
-    if (r == TK_OK) r = tk_change_range_of_intensity(ctx, s->phi, p->b0, p->a0);
+    if (r == TK_OK) r = init_phi(s, init);
     // build_gaussian_filter(phi, sigma, 3, gpa) (imp:1641-1644)
     if (r == TK_OK)
         r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], p->sigma,

@@ -84,6 +84,8 @@ int tk_vol_wrap(tk_ctx ctx, void* device_ptr, int nx, int ny, int nz, double hx,
                 double hz, tk_dtype dtype, tk_vol* out);
 int tk_vol_free(tk_vol vol);
 int tk_vol_info(tk_vol vol, int dims[3], double h[3], tk_dtype* dtype);
+/* image3d::seth (image3d.hxx:238): spacing only, voxels unchanged */
+int tk_vol_set_spacing(tk_vol vol, double hx, double hy, double hz);
 void* tk_vol_device_ptr(tk_vol vol);
 /* host <-> device; host_dtype may differ from the volume's (converted on device) */
 int tk_vol_upload(tk_vol vol, const void* host, tk_dtype host_dtype);
@@ -210,7 +212,10 @@ typedef struct tk_rsfe_iter_stats {
 
 int tk_rsfe_default_params(tk_rsfe_params* p);
 /* set-up: filter spectrum, kones = K*1, kmyim = K*u0, g = edge_detect(u0) (imp:1641-1655);
-   phi := init rescaled to [a0,b0] (initialize_phi_with_init, imp:99-170, same dims) */
+   phi := initialize_phi_with_init(init) (imp:99-170): init of the image's dims is rescaled
+   to [a0,b0]; init one voxel short along some axes has its last faces extended, then is
+   rescaled; init NULL (apply(myim)) or of other dims (with the reference's WARNING on
+   stderr) gives initialize_phi_with_cube (imp:48-95).  init may be F32 or F64. */
 int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, tk_rsfe* out);
 /* run up to n_iters outer iterations (stops early on tollerance <= tol);
    stats may be NULL, else has room for n_iters entries */

@@ -1,0 +1,210 @@
+/*
+ * tk/rsfe_splitbregman.hxx -- segm::segmentation<T> and segm::rsfe_splitbregman<T> over the
+ * device Split-Bregman solver of libtk (tk_rsfe_*).
+ *
+ * Mirrors segm::segmentation<T> (header/segmentation.hxx:50-130: apply(myim) and
+ * apply(myim, init) pure virtual, getoutput() returning phi by value) and
+ * segm::rsfe_splitbregman<T> (header/rsfe_splitbregman.hxx:99-894): the constructor
+ * defaults (rsfe_splitbregman_imp.hxx:25-41), the setters of the parameters the loop reads
+ * with the reference's validation and WARNING-and-keep behaviour (imp:990-1330), set_cv and
+ * set_onestep_poisson (imp:836-852), set_bc (imp:1294-1322), and apply (imp:1547-1995).
+ *
+ * What this layer runs: cv = conv::cyclic_fftw_convolution (the reference's default cv is
+ * the acyclic FFT convolution, not built here: set_cv with any other function keeps the
+ * cyclic one and prints a WARNING), onestep_poisson = lapl::DirGaussSeidel (bit-exact
+ * lexicographic sweeps) or lapl::DirRedBlackGaussSeidel (red-black), edge detector GRAD
+ * with edge_detector_sigma = 0.  set_bc(NEU) would switch the reference to NeuGaussSeidel,
+ * which is not built: it prints a WARNING and keeps the Dirichlet sweep (the reference's
+ * default state is the NEU flag, i.e. boundary value 0, with DirGaussSeidel).
+ */
+#ifndef TK_RSFE_SPLITBREGMAN_HXX
+#define TK_RSFE_SPLITBREGMAN_HXX
+
+#include <vector>
+
+#include "convolution.hxx"
+#include "poisson.hxx"
+
+namespace segm {
+
+template <typename T>
+class segmentation {
+public:
+    virtual ~segmentation() {}
+    virtual void apply(im3d::image3d<T> const& myim, im3d::image3d<T> const& init) = 0;
+    virtual void apply(im3d::image3d<T> const& myim) = 0;
+    im3d::image3d<T> getoutput() const { return phi; }
+    virtual void set_alpha(T const& alpha) { this->alpha = alpha; }
+    T get_alpha() const { return alpha; }
+
+protected:
+    im3d::image3d<T> phi;
+    T alpha = 0.5;
+};
+
+template <typename T>
+class rsfe_splitbregman : public segmentation<T> {
+public:
+    enum bc_type { NEU = 1, DIR };
+    enum ed_type { GRAD = 1, GRAD_THRESHOLD, LBP };
+
+    // constructor defaults (rsfe_splitbregman_imp.hxx:25-41)
+    rsfe_splitbregman() : cv_(conv::cyclic_fftw_convolution<T>), onestep_(lapl::DirGaussSeidel<T>)
+    {
+        tk_rsfe_default_params(&p_);
+        this->alpha = (T)0.5;
+    }
+
+    // ---- setters (rsfe_splitbregman_imp.hxx:990-1330) ------------------------------------
+    void set_verbosity(bool const v) { verbosity_ = v; }
+    void set_maxiter(unsigned const& maxiter) { p_.maxiter = (int)maxiter; }
+    void set_tol(T const& tol)
+    {
+        if (tol >= 0) p_.tol = (double)tol;
+        else tk::warning("set_tol: tol must be non negative");
+    }
+    void set_lamda(T const& v) { positive(v, p_.lamda, "set_lamda: lamda must be greater than zero"); }
+    void set_lamda1(T const& v) { positive(v, p_.lamda1, "set_lamda1: lamda1 must be greater than zero"); }
+    void set_lamda2(T const& v) { positive(v, p_.lamda2, "set_lamda2: lamda2 must be greater than zero"); }
+    void set_sigma(T const& v) { positive(v, p_.sigma, "set_sigma: sigma must be greater than zero"); }
+    void set_dt(T const& v) { positive(v, p_.dt, "set_dt: dt must be greater than zero"); }
+    void set_epsilon(T const& v) { positive(v, p_.epsilon, "set_epsilon: epsilon must be greater than zero"); }
+    void set_nu(T const& v)
+    {
+        if (v >= 0) p_.nu = (double)v;
+        else tk::warning("set_nu: nu must be non negative");
+    }
+    void set_beta(T const& v)
+    {
+        if (v >= 0) p_.beta = (double)v;
+        else tk::warning("set_beta: beta must be non negative");
+    }
+    // a0 / b0 keep b0 > a0 by moving the other bound (imp:1220-1280)
+    void set_a0(T const& a0)
+    {
+        p_.a0 = (double)a0;
+        this->alpha = (T)((p_.a0 + p_.b0) / 2.);
+        if (p_.b0 < p_.a0) {
+            p_.b0 = p_.a0 + 1;
+            this->alpha = (T)((p_.a0 + p_.b0) / 2.);
+            tk::warning("set_a0: b0 has to be greater than a0, hence its value is automatically "
+                        "modified");
+        }
+    }
+    void set_b0(T const& b0)
+    {
+        p_.b0 = (double)b0;
+        this->alpha = (T)((p_.a0 + p_.b0) / 2.);
+        if (p_.b0 < p_.a0) {
+            p_.a0 = p_.b0 - 1;
+            this->alpha = (T)((p_.a0 + p_.b0) / 2.);
+            tk::warning("set_b0: a0 has to be lower than b0, hence its value is automatically "
+                        "modified");
+        }
+    }
+    void set_ls_tol(T const& tol)
+    {
+        if (tol >= 0) p_.ls_tol = (double)tol;
+        else tk::warning("set_ls_tol: ls_tol must be non negative");
+    }
+    void set_ls_steps(unsigned const steps) { p_.ls_steps = (int)steps; }
+    void set_gaussian_pixel_approach(bool const gpa) { p_.gaussian_pixel_approach = gpa ? 1 : 0; }
+    void set_edge_detector(ed_type const ed)
+    {
+        if (ed == GRAD) return;
+        tk::warning("set_edge_detector: only the GRAD edge detector is built");
+    }
+    void set_edge_detector_sigma(T const& eds)
+    {
+        if (eds == 0) return;
+        tk::warning("set_edge_detector_sigma: only edge_detector_sigma = 0 is built");
+    }
+    // bc flag: NEU -> boundary value 0 with the Dirichlet sweep kept; DIR -> value a0
+    // (imp:1294-1322, 1735-1743)
+    void set_bc(bc_type const bc)
+    {
+        if (bc == DIR) {
+            p_.bc_dir = 1;
+            onestep_.changef(lapl::DirGaussSeidel<T>);
+        } else if (bc == NEU) {
+            tk::warning("set_bc: NeuGaussSeidel is not built; the Dirichlet sweep is kept");
+        } else {
+            tk::warning("set_bc: specify a proper bc_type condition");
+        }
+    }
+    void set_cv(conv::filtering<T> const cv)
+    {
+        if (cv.getf() == &conv::cyclic_fftw_convolution<T>) cv_ = cv;
+        else tk::warning("set_cv: only conv::cyclic_fftw_convolution is built; kept");
+    }
+    void set_onestep_poisson(lapl::unsteady_poisson_functor<T> const osl)
+    {
+        if (osl.getf() == &lapl::DirGaussSeidel<T> || osl.getf() == &lapl::DirRedBlackGaussSeidel<T>)
+            onestep_ = osl;
+        else tk::warning("set_onestep_poisson: only DirGaussSeidel and DirRedBlackGaussSeidel "
+                         "are built; kept");
+    }
+    // cap on the sweeps of one Poisson solve when ls_tol > 0 (0: none)
+    void set_max_sweeps(int const m) { p_.max_sweeps = m; }
+
+    // ---- apply (rsfe_splitbregman_imp.hxx:1547-1995) --------------------------------------
+    void apply(im3d::image3d<T> const& myim, im3d::image3d<T> const& init) override
+    {
+        run(myim, init.handle());
+    }
+    // initial contour: the cube (initialize_phi_with_cube, imp:48-95)
+    void apply(im3d::image3d<T> const& myim) override { run(myim, nullptr); }
+
+    // per outer iteration: Gauss-Seidel sweeps, tollerance (imp:1832)
+    std::vector<int> const& get_sweeps() const { return sweeps_; }
+    std::vector<double> const& get_tollerances() const { return tols_; }
+    unsigned get_iterations() const { return (unsigned)sweeps_.size(); }
+    tk_rsfe_params const& params() const { return p_; }
+
+private:
+    void positive(T const& v, double& slot, const char* msg)
+    {
+        if (v > 0) slot = (double)v;
+        else tk::warning(msg);
+    }
+
+    void run(im3d::image3d<T> const& myim, tk_vol init)
+    {
+        tk_rsfe_params p = p_;
+        p.gs_order = onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ? TK_GS_REDBLACK
+                                                                           : TK_GS_LEX;
+        tk_rsfe s = nullptr;
+        tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
+        sweeps_.clear();
+        tols_.clear();
+        const int n = p.maxiter > 0 ? p.maxiter : 0;
+        std::vector<tk_rsfe_iter_stats> st((size_t)(n > 0 ? n : 1));
+        int done = 0;
+        int r = tk_rsfe_iterate(s, n, st.data(), &done);
+        for (int t = 0; t < done; ++t) {
+            sweeps_.push_back(st[t].sweeps);
+            tols_.push_back(st[t].tollerance);
+            if (verbosity_)
+                std::clog << "--- iteration " << t + 1 << ": tollerance = " << st[t].tollerance
+                          << ", Gauss-Seidel sweeps = " << st[t].sweeps << std::endl;
+        }
+        if (r == TK_OK) {
+            this->phi = im3d::image3d<T>(myim.getdimx(), myim.getdimy(), myim.getdimz(),
+                                         myim.gethx(), myim.gethy(), myim.gethz());
+            r = tk_vol_copy(this->phi.handle(), tk_rsfe_phi(s));
+        }
+        tk_rsfe_destroy(s);
+        tk::check(r, "apply");
+    }
+
+    tk_rsfe_params p_;
+    bool verbosity_ = true;
+    conv::filtering<T> cv_;
+    lapl::unsteady_poisson_functor<T> onestep_;
+    std::vector<int> sweeps_;
+    std::vector<double> tols_;
+};
+
+}  // namespace segm
+
+#endif

@@ -756,6 +756,48 @@ void or_rsfe_forcing(const double* myim, const double* phi, const double* divdb,
     forcing(myim, phi, kones.data(), kmyim.data(), divdb, n, p, cv, rhs);
 }
 
+/* ---- R3: initialize_phi_with_init / initialize_phi_with_cube ---------------------- */
+int or_init_phi(const double* init, int ix, int iy, int iz, int X, int Y, int Z, double a0,
+                double b0, double* phi)
+{
+    const long n = (long)X * Y * Z;
+    auto at = [&](int i, int j, int k) -> double& { return phi[((long)i * Y + j) * Z + k]; };
+    if (init && ix == X && iy == Y && iz == Z) {
+        /* 1st case (imp:103-110) */
+        for (long v = 0; v < n; ++v) phi[v] = init[v];
+        or_change_range_of_intensity(phi, n, b0, a0);
+        return 1;
+    }
+    auto near = [](int a, int d) { return a == d || a == d - 1; };
+    if (init && near(ix, X) && near(iy, Y) && near(iz, Z)) {
+        /* 2nd case (imp:114-160): setdim, copy, then the last faces axis by axis */
+        for (long v = 0; v < n; ++v) phi[v] = 0.;
+        for (int i = 0; i < ix; ++i)
+            for (int j = 0; j < iy; ++j)
+                for (int k = 0; k < iz; ++k) at(i, j, k) = init[((long)i * iy + j) * iz + k];
+        if (X != ix)
+            for (int j = 0; j < Y; ++j)
+                for (int k = 0; k < Z; ++k) at(ix, j, k) = at(ix - 1, j, k);
+        if (Y != iy)
+            for (int i = 0; i < X; ++i)
+                for (int k = 0; k < Z; ++k) at(i, iy, k) = at(i, iy - 1, k);
+        if (Z != iz)
+            for (int i = 0; i < X; ++i)
+                for (int j = 0; j < Y; ++j) at(i, j, iz) = at(i, j, iz - 1);
+        or_change_range_of_intensity(phi, n, b0, a0);
+        return 1;
+    }
+    /* 3rd case and apply(myim): the cube (imp:48-95), 3-D branch */
+    const double frac = 0.33;
+    const int cx = (int)(X * frac), cy = (int)(Y * frac), cz = (int)(Z * frac);
+    const int fx = (int)(X * (1 - frac)), fy = (int)(Y * (1 - frac)), fz = (int)(Z * (1 - frac));
+    for (long v = 0; v < n; ++v) phi[v] = a0;
+    for (int i = cx; i < fx; ++i)
+        for (int j = cy; j < fy; ++j)
+            for (int k = cz; k < fz; ++k) at(i, j, k) = b0;
+    return 0;
+}
+
 /* ---- R2: apply(myim, init), rsfe_splitbregman_imp.hxx:1547-1995 ------------------
    cv = cyclic_fftw_convolution, onestep_poisson = DirGaussSeidel, edge detector GRAD
    with edge_detector_sigma == 0, no display / dump / on-the-go / auto CC. */

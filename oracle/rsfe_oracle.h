@@ -95,6 +95,11 @@ typedef struct or_stats {
 } or_stats;
 
 void or_default_params(or_params* p);
+/* R3: initialize_phi_with_init (rsfe_splitbregman_imp.hxx:99-170) for an init of dims
+   (ix,iy,iz) (NULL: initialize_phi_with_cube, 48-95, as apply(myim)); phi has the image's
+   dims (X,Y,Z).  Returns 1 for the init cases, 0 when the cube was used. */
+int or_init_phi(const double* init, int ix, int iy, int iz, int X, int Y, int Z, double a0,
+                double b0, double* phi);
 /* R6 + R7: forcing term rhs of one outer iteration for given phi and divdb */
 void or_rsfe_forcing(const double* myim, const double* phi, const double* divdb, int nx, int ny,
                      int nz, double hx, double hy, double hz, const or_params* p, double* rhs);

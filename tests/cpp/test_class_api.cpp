@@ -1,0 +1,210 @@
+// C++ class layer (include/tk/*.hxx) used the way the reference's examples use
+// im3d::image3d, conv::filtering, lapl::unsteady_poisson_functor and
+// segm::rsfe_splitbregman (examples/rsfe_splitbregman.cxx:79-113, test.cxx), each result
+// checked against the CPU oracle (oracle/liboracle.so, linked as the checker).
+// Built as a shared library and called in-process by tests/test_gpu_cpp_api.py:
+// tk_cpp_class_api_test(log, cap) writes one "ok <case> ..." or "FAIL <case> ..." line per
+// case into log and returns 0 when every case passed.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/tk/rsfe_splitbregman.hxx"
+#include "../../oracle/rsfe_oracle.h"
+
+static int g_fail = 0;
+static std::string g_log;
+
+static void report(const char* name, double err, double tol)
+{
+    const bool ok = err <= tol;
+    char line[256];
+    std::snprintf(line, sizeof line, "%s %s err=%.3e tol=%.1e\n", ok ? "ok" : "FAIL", name, err,
+                  tol);
+    g_log += line;
+    if (!ok) g_fail = 1;
+}
+
+static double maxabs(const std::vector<double>& a, const std::vector<double>& b)
+{
+    double m = 0;
+    for (size_t i = 0; i < a.size(); ++i) m = std::fmax(m, std::fabs(a[i] - b[i]));
+    return a.size() == b.size() ? m : INFINITY;
+}
+
+static void run_cases()
+{
+    std::mt19937_64 gen(12345);
+    std::uniform_real_distribution<double> U(0., 1.);
+    const unsigned X = 20, Y = 18, Z = 24;
+    const long n = (long)X * Y * Z;
+    std::vector<double> raw(n);
+    for (auto& x : raw) x = U(gen);
+
+    // ---- image3d: storage, copies, norms, range ----------------------------------------
+    im3d::image3d<double> a(X, Y, Z, 1., 2., 0.5);
+    a.upload(raw);
+    report("image3d upload/download", maxabs(a.download(), raw), 0);
+    im3d::image3d<double> b(a);
+    report("image3d deep copy", maxabs(b.download(), raw), 0);
+    b = 3.0;
+    report("image3d operator=(scalar) leaves the copy source", maxabs(a.download(), raw), 0);
+    report("image3d element read", std::fabs(a(3, 4, 5) - raw[(3 * Y + 4) * Z + 5]), 0);
+    report("image3d normL2", std::fabs(a.normL2() - or_norm_l2(raw.data(), n, 1., 2., .5)),
+           1e-12 * a.normL2());
+    report("image3d max/min", std::fabs(a.max() - or_max(raw.data(), n)) +
+                                  std::fabs(a.min() - or_min(raw.data(), n)), 0);
+    {
+        im3d::image3d<double> c(a);
+        c.change_range_of_intensity(255., 0.);
+        std::vector<double> ref(raw);
+        or_change_range_of_intensity(ref.data(), n, 255., 0.);
+        report("image3d change_range_of_intensity", maxabs(c.download(), ref), 1e-12);
+    }
+
+    // ---- grad and div ----------------------------------------------------------------
+    {
+        std::vector<im3d::image3d<double> > g;
+        a.grad(g);
+        std::vector<double> g0(n), g1(n), g2(n), dv(n);
+        or_grad(raw.data(), X, Y, Z, 1., 2., .5, g0.data(), g1.data(), g2.data());
+        report("image3d grad", std::fmax(maxabs(g[0].download(), g0),
+                                         std::fmax(maxabs(g[1].download(), g1),
+                                                   maxabs(g[2].download(), g2))), 1e-12);
+        im3d::image3d<double> d;
+        im3d::div(d, g);
+        or_div(g0.data(), g1.data(), g2.data(), X, Y, Z, 1., 2., .5, dv.data());
+        report("im3d::div", maxabs(d.download(), dv), 1e-11);
+    }
+
+    // ---- conv::filtering + cyclic_fftw_convolution --------------------------------------
+    {
+        im3d::image3d<double> f(X, Y, Z);
+        f.upload(raw);
+        conv::filtering<double> cv(conv::cyclic_fftw_convolution<double>);
+        conv::filtering<double>::build_gaussian_filter(f, 2.0);
+        im3d::image3d<double> out;
+        cv(out, f);
+        int nh[3], nc[3];
+        or_gaussian_dims(Z, 1., 1., 1., 2.0, 3., 0, nh);
+        std::vector<double> tf((long)nh[0] * nh[1] * nh[2]);
+        or_gaussian_time_filter(Z, 1., 1., 1., 2.0, 3., 0, tf.data());
+        or_cyclic_dims(X, Y, Z, nh, nc);
+        std::vector<double> spec((long)nc[0] * nc[1] * nc[2]), ref(n);
+        or_cyclic_spectrum(tf.data(), nh, nc, spec.data());
+        or_cyclic_conv(spec.data(), nc, raw.data(), X, Y, Z, 1., 1., 1., ref.data());
+        report("conv::filtering cyclic_fftw_convolution", maxabs(out.download(), ref), 1e-13);
+        report("convolution result dims", (out.getdimx() == X && out.getdimy() == Y &&
+                                           out.getdimz() == Z) ? 0 : 1, 0);
+    }
+
+    // ---- lapl::unsteady_poisson_functor(DirGaussSeidel): bitwise --------------------------
+    {
+        std::vector<double> rhs(n), p0(n);
+        for (auto& x : rhs) x = U(gen) - 0.5;
+        for (auto& x : p0) x = U(gen);
+        im3d::image3d<double> phi(X, Y, Z, 1., 2., .5), r(X, Y, Z, 1., 2., .5);
+        phi.upload(p0);
+        r.upload(rhs);
+        lapl::unsteady_poisson_functor<double> osl(lapl::DirGaussSeidel<double>);
+        for (int s = 0; s < 3; ++s) {
+            osl(phi, r, 1., 0.25);
+            or_dir_gauss_seidel(p0.data(), rhs.data(), X, Y, Z, 1., 2., .5, 1., 0.25);
+        }
+        report("unsteady_poisson_functor DirGaussSeidel x3 (bitwise)", maxabs(phi.download(), p0),
+               0);
+    }
+
+    // ---- connected component ----------------------------------------------------------
+    {
+        im3d::image3d<double> img(X, Y, Z);
+        img.upload(raw);
+        im3d::image3d<unsigned char> cc;
+        img.connected_component(cc, 10, 9, 12, 0.5, false);
+        std::vector<double> ref(n);
+        or_connected_component(raw.data(), X, Y, Z, 10, 9, 12, 0.5, 0, ref.data());
+        std::vector<unsigned char> got = cc.download();
+        double bad = 0;
+        for (long v = 0; v < n; ++v) bad += (double)got[v] != ref[v];
+        report("image3d connected_component (voxels differing)", bad, 0);
+    }
+
+    // ---- segm::rsfe_splitbregman: apply(myim, init) -------------------------------------
+    {
+        const int SX = 24, SY = 20, SZ = 28;
+        const long m = (long)SX * SY * SZ;
+        std::vector<double> u0(m), init(m);
+        tk_gen_standard(SX, SY, SZ, 5, 1, 2, u0.data(), init.data());
+        im3d::image3d<double> myim(SX, SY, SZ), phi0(SX, SY, SZ);
+        myim.upload(u0);
+        phi0.upload(init);
+        myim.change_range_of_intensity(255.);  // examples/rsfe_splitbregman.cxx:86
+        segm::rsfe_splitbregman<double> rsfe;
+        rsfe.set_verbosity(false);
+        rsfe.set_cv(conv::filtering<double>(conv::cyclic_fftw_convolution<double>));
+        rsfe.set_sigma(2.0);
+        rsfe.set_maxiter(6);
+        rsfe.set_tol(0.);
+        rsfe.apply(myim, phi0);
+        std::vector<double> got = rsfe.getoutput().download();
+
+        or_params p;
+        or_default_params(&p);
+        p.sigma = 2.0;
+        p.maxiter = 6;
+        p.tol = 0.;
+        std::vector<double> u255(u0), ref(m);
+        or_change_range_of_intensity(u255.data(), m, 255., 0.);
+        std::vector<int> sw(6);
+        or_rsfe_apply(u255.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p, ref.data(), sw.data(),
+                      nullptr, nullptr);
+        report("rsfe_splitbregman apply FP64", maxabs(got, ref), 1e-9);
+        int sweeps_equal = rsfe.get_iterations() == 6;
+        for (unsigned t = 0; t < rsfe.get_iterations() && t < 6; ++t)
+            sweeps_equal &= rsfe.get_sweeps()[t] == sw[t];
+        report("rsfe_splitbregman sweep counts", sweeps_equal ? 0 : 1, 0);
+
+        // apply(myim): cube initial contour
+        segm::rsfe_splitbregman<double> cube;
+        cube.set_verbosity(false);
+        cube.set_sigma(2.0);
+        cube.set_maxiter(3);
+        cube.set_tol(0.);
+        cube.apply(myim);
+        std::vector<double> cinit(m), cref(m);
+        or_init_phi(nullptr, 0, 0, 0, SX, SY, SZ, 0., 1., cinit.data());
+        p.maxiter = 3;
+        or_rsfe_apply(u255.data(), cinit.data(), SX, SY, SZ, 1., 1., 1., &p, cref.data(),
+                      nullptr, nullptr, nullptr);
+        report("rsfe_splitbregman apply(myim) cube FP64", maxabs(cube.getoutput().download(), cref),
+               1e-9);
+
+        // setter validation keeps the old value, as the reference
+        segm::rsfe_splitbregman<double> v;
+        v.set_sigma(-1.);
+        v.set_b0(-2.);  // a0 moves to b0 - 1
+        report("setter validation", std::fabs(v.params().sigma - 3.) +
+                                        std::fabs(v.params().a0 + 3.), 0);
+    }
+}
+
+extern "C" int tk_cpp_class_api_test(char* log, int cap)
+{
+    g_fail = 0;
+    g_log.clear();
+    try {
+        run_cases();
+    } catch (const std::exception& e) {
+        g_log += std::string("FAIL exception: ") + e.what() + "\n";
+        g_fail = 1;
+    }
+    g_log += g_fail ? "FAILED\n" : "ALL OK\n";
+    if (log && cap > 0) {
+        std::strncpy(log, g_log.c_str(), (size_t)cap - 1);
+        log[cap - 1] = 0;
+    }
+    return g_fail;
+}

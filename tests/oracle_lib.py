@@ -200,6 +200,19 @@ def connected_component(img, seed, thr=0.5, full=False):
     return res, passes
 
 
+def init_phi(init, shape, a0=0.0, b0=1.0):
+    """initialize_phi_with_init / _with_cube; init None gives the cube.  Returns (phi, case)."""
+    X, Y, Z = shape
+    phi = np.empty(shape)
+    if init is None:
+        case = lib().or_init_phi(None, 0, 0, 0, X, Y, Z, C.c_double(a0), C.c_double(b0), dp(phi))
+    else:
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        case = lib().or_init_phi(dp(init), *init.shape, X, Y, Z, C.c_double(a0), C.c_double(b0),
+                                 dp(phi))
+    return phi, case
+
+
 def default_params(**kw):
     p = OrParams()
     lib().or_default_params(C.byref(p))

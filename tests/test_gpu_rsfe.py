@@ -52,6 +52,29 @@ def test_loop_f64_matches_oracle_small(tk, orc, ctx, shape, ls_tol):
     assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("init_shape", ["same", (15, 16, 11), (16, 15, 12), None, (10, 16, 12)])
+def test_init_phi_cases_vs_oracle(tk, orc, ctx, rng, init_shape, dtype):
+    """initialize_phi_with_init / _with_cube (rsfe_splitbregman_imp.hxx:48-170) through
+    tk_rsfe_create: same dims, one voxel short (faces extended), no init, other dims."""
+    shape = (16, 16, 12)
+    say("init phi", init_shape, np.dtype(dtype).name)
+    u0 = rng.random(shape)
+    init = None
+    if init_shape is not None:
+        init = rng.random(shape if init_shape == "same" else init_shape)
+    vu = ctx.from_numpy(u0.astype(dtype))
+    vi = ctx.from_numpy(init.astype(dtype)) if init is not None else None
+    r = tk.Rsfe(ctx, vu, vi, tk.default_params(a0=0.0, b0=1.0))
+    phi = r.phi()
+    r.close()
+    src = init.astype(dtype).astype(np.float64) if init is not None else None
+    ref, case = orc.init_phi(src, shape, a0=0.0, b0=1.0)
+    tol = 1e-15 if dtype == np.float64 else 5e-7
+    assert np.max(np.abs(phi.astype(np.float64) - ref)) <= tol
+    assert case == (0 if init_shape in (None, (10, 16, 12)) else 1)
+
+
 def load_golden(name):
     d = np.load(os.path.join(GOLDEN, name))
     return d["phi"], d["sweeps"], d["tols"], json.loads(str(d["meta"]))

@@ -442,3 +442,29 @@ def test_apply_matches_composition_of_pinned_pieces(orc, tk):
             d[a] = d[a] - b[a]
         divdb = orc.div(*d)
     assert np.array_equal(phi, phi_or)
+
+
+def test_init_phi_cases(orc):
+    """initialize_phi_with_init / _with_cube (rsfe_splitbregman_imp.hxx:48-170), pinned by
+    closed forms: same dims = min-max rescale; one voxel short = edge padding (numpy) then
+    rescale; other dims and no init = the [0.33 d, 0.67 d) cube of a0/b0."""
+    rng = np.random.default_rng(7)
+    shape = (12, 10, 9)
+    init = rng.random(shape)
+    phi, case = orc.init_phi(init, shape, a0=0.0, b0=1.0)
+    assert case == 1
+    assert np.allclose(phi, (init - init.min()) / (init.max() - init.min()), atol=1e-15)
+    small = rng.random((11, 10, 8))
+    phi, case = orc.init_phi(small, shape, a0=-1.0, b0=2.0)
+    ext = np.pad(small, ((0, 1), (0, 0), (0, 1)), mode="edge")
+    assert case == 1
+    assert np.allclose(phi, (ext - ext.min()) / (ext.max() - ext.min()) * 3.0 - 1.0, atol=1e-14)
+    for bad in (None, rng.random((10, 10, 9))):
+        phi, case = orc.init_phi(bad, shape, a0=0.0, b0=1.0)
+        assert case == 0
+        # 12*0.33 = 3.96 -> 3, 12*0.67 = 8.04 -> 8; 10 -> [3, 6); 9 -> [2, 6)
+        cube = np.zeros(shape)
+        cube[3:8, 3:6, 2:6] = 1.0
+        assert np.array_equal(phi, cube)
+    phi, _ = orc.init_phi(None, (64, 64, 64))
+    assert phi.sum() == 21 ** 3  # [21, 42) per axis
