@@ -113,6 +113,7 @@ def _load():
         "tk_batch_set_volume": (i, [vp, i, vp, vp, i]),
         "tk_batch_iterate": (i, [vp, i, i]),
         "tk_batch_get_phi": (i, [vp, i, vp, i]),
+        "tk_batch_info": (i, [vp, ip, ip, ip]),
         "tk_batch_destroy": (i, [vp]),
         "tk_gen_ellipsoids": (i, [i, i, i, u64, i, dp, i, d, dp, dp]),
         "tk_gen_standard": (i, [i, i, i, u64, i, i, dp, dp]),
@@ -344,6 +345,16 @@ def poisson_residual(ctx, phi, rhs):
     return r.value
 
 
+def cc_seeded(ctx, img, seed, threshold, full, out):
+    """image3d::connected_component(res, i, j, k, threshold, full) into the U8 volume out;
+    returns the number of device passes."""
+    passes = C.c_int()
+    i, j, k = (int(x) for x in seed)
+    _check(lib.tk_cc_seeded(ctx.h, img.h, i, j, k, float(threshold), int(bool(full)), out.h,
+                            C.byref(passes)))
+    return passes.value
+
+
 def poisson_rb_autotune(ctx, rhs):
     """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16))."""
     rows = C.c_int()
@@ -391,3 +402,47 @@ class Rsfe:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """tk_batch: independent rsfe_splitbregman runs over volumes of uneven sizes."""
+
+    def __init__(self, ctx, dims, params, dtype=np.float32):
+        dims = np.ascontiguousarray(dims, np.int32).reshape(-1, 3)
+        self.dims = [tuple(int(x) for x in d) for d in dims]
+        self.dtype = np.dtype(dtype)
+        hv = C.c_void_p()
+        _check(lib.tk_batch_create(ctx.h, len(self.dims), dims.ctypes.data_as(C.POINTER(C.c_int)),
+                                   _NP2TK[self.dtype], C.byref(params), C.byref(hv)))
+        self.h = hv
+        self.ctx = ctx
+
+    def set_volume(self, v, u0, init=None):
+        u0 = np.ascontiguousarray(u0, np.float64)
+        assert u0.shape == self.dims[v]
+        ip = None
+        if init is not None:
+            init = np.ascontiguousarray(init, np.float64)
+            assert init.shape == self.dims[v]
+            ip = init.ctypes.data_as(C.c_void_p)
+        _check(lib.tk_batch_set_volume(self.h, v, u0.ctypes.data_as(C.c_void_p), ip,
+                                       _NP2TK[np.dtype(np.float64)]))
+
+    def iterate(self, n, use_graph=True):
+        _check(lib.tk_batch_iterate(self.h, int(n), int(bool(use_graph))))
+
+    def phi(self, v):
+        out = np.empty(self.dims[v], np.float64)
+        _check(lib.tk_batch_get_phi(self.h, v, out.ctypes.data_as(C.c_void_p),
+                                    _NP2TK[np.dtype(np.float64)]))
+        return out
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.tk_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(lanes=a.value, graph_nodes=b.value, iterations=c.value)
+
+    def close(self):
+        if self.h:
+            _check(lib.tk_batch_destroy(self.h))
+            self.h = None

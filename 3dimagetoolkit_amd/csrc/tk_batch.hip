@@ -1,14 +1,29 @@
 // tk_batch.hip -- many independent Split-Bregman runs over volumes of uneven sizes.
 //
-// Each volume keeps its own filter spectrum and state (the reference's filter state
-// is static and class-wide, convolution.hxx:307-314, so it cannot serve two sizes at
-// once).  This first version advances the volumes one after another on the context's
-// stream; every stage of a volume is already a single launch.
+// Each volume keeps its own filter spectrum, state, red-black ping-pong buffer and
+// tolerance slot (the reference's filter state is static and class-wide,
+// convolution.hxx:307-314, so it could not serve two sizes at once).  The volumes are spread
+// over up to four lanes; a lane is a context of its own (one HIP stream, its own scratch and
+// reduction slots), so lanes run concurrently and share nothing.  Volumes are assigned to
+// lanes by longest-processing-time on their voxel counts.
+//
+// With a fixed sweep count (ls_tol == 0) and no early stop (tol == 0) an outer iteration
+// enqueues kernels only, so:
+//  - eager mode enqueues iteration after iteration of every volume without a host wait;
+//  - graph mode captures two outer iterations of all volumes (forked over the lanes and
+//    joined back) into one hipGraph and replays it.  Two, because an odd sweep count swaps
+//    phi with its ping-pong partner every iteration: after two the buffers are back where
+//    the capture found them.  The graph holds kernel nodes only (no memset nodes).
+// Otherwise (ls_tol > 0 gates the Poisson loop on the host, tol > 0 stops a run early) each
+// volume runs the reference's loop with its host checks, lane after lane.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "tk_internal.hpp"
+
+using namespace tk;
 
 struct tk_batch_s {
     tk_ctx ctx = nullptr;
@@ -16,7 +31,90 @@ struct tk_batch_s {
     tk_rsfe_params p{};
     std::vector<int> dims;
     std::vector<tk_rsfe> runs;
+    std::vector<int> lane;       // lane of each volume
+    std::vector<tk_ctx> lanes;   // lane contexts
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_nodes = 0;
 };
+
+namespace {
+
+void drop_graph(tk_batch b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// volumes in an order that alternates lanes, so consecutive launches go to different streams
+std::vector<int> lane_order(tk_batch b)
+{
+    std::vector<std::vector<int>> per(b->lanes.size());
+    for (size_t v = 0; v < b->runs.size(); ++v) per[b->lane[v]].push_back((int)v);
+    std::vector<int> order;
+    for (size_t r = 0;; ++r) {
+        bool any = false;
+        for (auto& l : per)
+            if (r < l.size()) {
+                order.push_back(l[r]);
+                any = true;
+            }
+        if (!any) break;
+    }
+    return order;
+}
+
+int step_all(tk_batch b, const std::vector<int>& order)
+{
+    for (int v : order) TK_TRY(rsfe_step_async(b->runs[v]));
+    return TK_OK;
+}
+
+int capture(tk_batch b, const std::vector<int>& order)
+{
+    drop_graph(b);
+    hipStream_t s0 = b->lanes[0]->stream;
+    std::vector<int> it0(b->runs.size());
+    for (size_t v = 0; v < b->runs.size(); ++v) it0[v] = rsfe_iterations(b->runs[v]);
+    TK_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    int r = TK_OK;
+    if (hipEventRecord(b->fork, s0) != hipSuccess) r = TK_ERR_HIP;
+    for (size_t k = 1; r == TK_OK && k < b->lanes.size(); ++k)
+        if (hipStreamWaitEvent(b->lanes[k]->stream, b->fork, 0) != hipSuccess) r = TK_ERR_HIP;
+    for (int rep = 0; r == TK_OK && rep < 2; ++rep) r = step_all(b, order);
+    for (size_t k = 1; r == TK_OK && k < b->lanes.size(); ++k) {
+        if (hipEventRecord(b->join[k], b->lanes[k]->stream) != hipSuccess ||
+            hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
+            r = TK_ERR_HIP;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s0, &g);
+    // the host side ran two iterations' bookkeeping; the device has run none
+    for (size_t v = 0; v < b->runs.size(); ++v) rsfe_set_iterations(b->runs[v], it0[v]);
+    if (r != TK_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        set_error("tk_batch_iterate: graph capture failed (%s)", hipGetErrorString(e));
+        return TK_ERR_HIP;
+    }
+    b->graph = g;
+    size_t nn = 0;
+    TK_HIP(hipGraphGetNodes(g, nullptr, &nn));
+    b->graph_nodes = (int)nn;
+    TK_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return TK_OK;
+}
+
+int sync_lanes(tk_batch b)
+{
+    for (tk_ctx c : b->lanes) TK_HIP(hipStreamSynchronize(c->stream));
+    return TK_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -25,30 +123,64 @@ int tk_batch_create(tk_ctx ctx, int n_vol, const int* dims3, tk_dtype dtype,
 {
     TK_CHECK(ctx && dims3 && p && out && n_vol > 0, TK_ERR_ARG, "tk_batch_create: argument");
     TK_CHECK(dtype == TK_F32 || dtype == TK_F64, TK_ERR_DTYPE, "tk_batch_create: dtype");
+    for (int v = 0; v < n_vol; ++v)
+        TK_CHECK(dims3[3 * v] >= 3 && dims3[3 * v + 1] >= 3 && dims3[3 * v + 2] >= 3,
+                 TK_ERR_SHAPE, "tk_batch_create: volume %d: every edge must be >= 3", v);
     tk_batch b = new tk_batch_s();
     b->ctx = ctx;
     b->dtype = dtype;
     b->p = *p;
     b->dims.assign(dims3, dims3 + 3 * (size_t)n_vol);
     b->runs.assign(n_vol, nullptr);
+    const int n_lanes = std::min(4, n_vol);
+    int r = TK_OK;
+    for (int k = 0; k < n_lanes && r == TK_OK; ++k) {
+        tk_ctx c = nullptr;
+        r = tk_ctx_create(ctx->device, &c);
+        if (r == TK_OK) b->lanes.push_back(c);
+    }
+    if (r == TK_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        r = TK_ERR_HIP;
+    b->join.assign(b->lanes.size(), nullptr);
+    for (size_t k = 1; r == TK_OK && k < b->lanes.size(); ++k)
+        if (hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess)
+            r = TK_ERR_HIP;
+    if (r != TK_OK) {
+        tk_batch_destroy(b);
+        return r;
+    }
+    // longest processing time first: biggest volume to the least loaded lane
+    std::vector<int> idx(n_vol);
+    for (int v = 0; v < n_vol; ++v) idx[v] = v;
+    auto vox = [&](int v) { return (long)dims3[3 * v] * dims3[3 * v + 1] * dims3[3 * v + 2]; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return vox(a) > vox(c); });
+    std::vector<long> load(b->lanes.size(), 0);
+    b->lane.assign(n_vol, 0);
+    for (int v : idx) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->lane[v] = k;
+        load[k] += vox(v);
+    }
     *out = b;
     return TK_OK;
 }
 
 int tk_batch_set_volume(tk_batch b, int v, const void* u0, const void* init, tk_dtype hd)
 {
-    TK_CHECK(b && u0 && init && v >= 0 && v < (int)b->runs.size(), TK_ERR_ARG,
+    TK_CHECK(b && u0 && v >= 0 && v < (int)b->runs.size(), TK_ERR_ARG,
              "tk_batch_set_volume: argument");
     const int* d = &b->dims[3 * (size_t)v];
+    tk_ctx c = b->lanes[b->lane[v]];
     tk_vol vu = nullptr, vi = nullptr;
-    int r = tk_vol_alloc(b->ctx, d[0], d[1], d[2], 1, 1, 1, b->dtype, &vu);
-    if (r == TK_OK) r = tk_vol_alloc(b->ctx, d[0], d[1], d[2], 1, 1, 1, b->dtype, &vi);
+    int r = tk_vol_alloc(c, d[0], d[1], d[2], 1, 1, 1, b->dtype, &vu);
+    if (r == TK_OK && init) r = tk_vol_alloc(c, d[0], d[1], d[2], 1, 1, 1, b->dtype, &vi);
     if (r == TK_OK) r = tk_vol_upload(vu, u0, hd);
-    if (r == TK_OK) r = tk_vol_upload(vi, init, hd);
+    if (r == TK_OK && init) r = tk_vol_upload(vi, init, hd);
     if (r == TK_OK) {
+        drop_graph(b);
         if (b->runs[v]) tk_rsfe_destroy(b->runs[v]);
         b->runs[v] = nullptr;
-        r = tk_rsfe_create(b->ctx, vu, vi, &b->p, &b->runs[v]);
+        r = tk_rsfe_create(c, vu, vi, &b->p, &b->runs[v]);
     }
     tk_vol_free(vu);
     tk_vol_free(vi);
@@ -57,13 +189,43 @@ int tk_batch_set_volume(tk_batch b, int v, const void* u0, const void* init, tk_
 
 int tk_batch_iterate(tk_batch b, int n_iters, int use_graph)
 {
-    TK_CHECK(b, TK_ERR_ARG, "tk_batch_iterate: null batch");
-    (void)use_graph;
-    for (size_t v = 0; v < b->runs.size(); ++v) {
+    TK_CHECK(b && n_iters >= 0, TK_ERR_ARG, "tk_batch_iterate: argument");
+    for (size_t v = 0; v < b->runs.size(); ++v)
         TK_CHECK(b->runs[v], TK_ERR_ARG, "tk_batch_iterate: volume %zu not set", v);
-        int done = 0;
-        TK_TRY(tk_rsfe_iterate(b->runs[v], n_iters, nullptr, &done));
+    if (b->p.ls_tol > 0 || b->p.tol > 0) {
+        // host-gated runs: the reference's loop, volume by volume
+        for (tk_rsfe run : b->runs) {
+            int done = 0;
+            TK_TRY(tk_rsfe_iterate(run, n_iters, nullptr, &done));
+        }
+        return TK_OK;
     }
+    // every run advances together up to maxiter
+    int it = rsfe_iterations(b->runs[0]);
+    for (tk_rsfe run : b->runs)
+        TK_CHECK(rsfe_iterations(run) == it, TK_ERR_ARG,
+                 "tk_batch_iterate: volumes are at different iterations");
+    int n = std::min(n_iters, std::max(0, b->p.maxiter - it));
+    const std::vector<int> order = lane_order(b);
+    if (use_graph && n >= 2) {
+        if (!b->exec) {
+            TK_TRY(sync_lanes(b));
+            TK_TRY(capture(b, order));
+        }
+        hipStream_t s0 = b->lanes[0]->stream;
+        // the graph's first nodes wait for nothing on the other lanes: drain them first
+        TK_TRY(sync_lanes(b));
+        for (int q = 0; q + 2 <= n; q += 2) TK_HIP(hipGraphLaunch(b->exec, s0));
+        for (tk_rsfe run : b->runs) rsfe_set_iterations(run, it + (n / 2) * 2);
+        if (n & 1) {
+            TK_HIP(hipStreamSynchronize(s0));
+            TK_TRY(step_all(b, order));
+        }
+    } else {
+        for (int q = 0; q < n; ++q) TK_TRY(step_all(b, order));
+    }
+    TK_TRY(sync_lanes(b));
+    for (tk_rsfe run : b->runs) TK_TRY(rsfe_read_tollerance(run, nullptr));
     return TK_OK;
 }
 
@@ -74,11 +236,26 @@ int tk_batch_get_phi(tk_batch b, int v, void* host, tk_dtype hd)
     return tk_vol_download(tk_rsfe_phi(b->runs[v]), host, hd);
 }
 
+int tk_batch_info(tk_batch b, int* n_lanes, int* graph_nodes, int* iterations)
+{
+    TK_CHECK(b, TK_ERR_ARG, "tk_batch_info: null batch");
+    if (n_lanes) *n_lanes = (int)b->lanes.size();
+    if (graph_nodes) *graph_nodes = b->graph_nodes;
+    if (iterations) *iterations = b->runs.empty() || !b->runs[0] ? 0 : rsfe_iterations(b->runs[0]);
+    return TK_OK;
+}
+
 int tk_batch_destroy(tk_batch b)
 {
     if (!b) return TK_OK;
+    for (tk_ctx c : b->lanes) (void)hipStreamSynchronize(c->stream);
+    drop_graph(b);
     for (tk_rsfe r : b->runs)
         if (r) tk_rsfe_destroy(r);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    for (tk_ctx c : b->lanes) tk_ctx_destroy(c);
     delete b;
     return TK_OK;
 }

@@ -83,15 +83,30 @@ static int upload_axis(tk_filter_s* f, int a)
         TK_HIP(hipMalloc(&f->half_dev[a], sizeof(T) * (s + 1)));
         TK_HIP(hipMemcpy(f->half_dev[a], half.data(), sizeof(T) * (s + 1), hipMemcpyHostToDevice));
         ax.half = f->half_dev[a];
-        // FP32: the direct path costs ~2(2s+1) fused operations per complex element against
-        // ~140 for the FFT pass, so it is the default where a register window is compiled;
-        // tk_rsfe_create times both on the device and keeps the faster
-        ax.direct = (sizeof(T) == 4 && direct_width_supported(nh)) ? 1 : 0;
+        // FP32 computes every convolution directly.  An FFT pass has rounding errors
+        // relative to the largest value along the line, while the region fit divides
+        // small local quantities (f2 = (num - kmyim) / (den - kones) where H is near 0 or
+        // 1): in the Split-Bregman loop the FP32 FFT pass was 2.5e-2 away from FP64 after
+        // one outer iteration, the direct pass 3.2e-5.  The direct pass is also the faster
+        // one (~2(2s+1) fused operations per complex element against ~140).  The register
+        // window needs a compiled half-width and 16-byte quads of pencils (dims % 4 == 0,
+        // at least four pencils per group); otherwise the generic direct taps pass runs.
+        if (sizeof(T) == 4) {
+            const AxisGeom g = make_geom(a, f->dims[0], f->dims[1], f->dims[2], L,
+                                         conv_ept<T>() * 256);
+            const bool quad = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 &&
+                              f->dims[2] % 4 == 0 && g.P >= 4;
+            ax.direct = direct_width_supported(nh) && quad ? 1 : 0;
+        } else {
+            ax.direct = 0;
+        }
     }
+    const bool fp32_taps = sizeof(T) == 4 && !ax.direct;
     // FP64 (the parity mode) keeps the radix-4 family: the loop amplifies last-bit
     // differences of the convolution, and radix 8 moved the 64^3 result by ~1e-9
-    if (!fft_plan(L, ax.radix, &ax.nstage, sizeof(T) == 4 ? 16 : 4)) {
-        // not 2/3/5-smooth: direct circular convolution with the normalised taps
+    if (fp32_taps || !fft_plan(L, ax.radix, &ax.nstage, sizeof(T) == 4 ? 16 : 4)) {
+        // FP32 without a register window, or not 2/3/5-smooth: direct circular
+        // convolution with the normalised taps
         ax.nstage = 0;
         const int nh = f->nh[a], s = nh / 2;
         const double hh = f->pixel ? 1. : f->h[a];

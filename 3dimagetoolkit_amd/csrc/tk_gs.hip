@@ -645,7 +645,8 @@ static int set_boundary(tk_ctx ctx, tk_vol phi, double bc, double* acc)
 // sweep loop of rsfe_splitbregman_imp.hxx:1762-1797
 template <class T>
 static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order, int ls_steps,
-                 double ls_tol, int max_sweeps, int* sweeps_out, double* err_out)
+                 double ls_tol, int max_sweeps, int* sweeps_out, double* err_out,
+                 void** alt = nullptr)
 {
     const GsCoef c = gs_coef(phi, dt);
     const int X = phi->nx, Y = phi->ny, Z = phi->nz;
@@ -653,9 +654,11 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
     const double bnd_o2 = (double)boundary_count(X, Y, Z) * (double)(T)bc * (double)(T)bc;
     if (ls_steps < 1) ls_steps = 1;
     if (order == TK_GS_REDBLACK) {
-        // ping-pong between phi and a scratch volume; both carry the boundary value
+        // ping-pong between phi and a scratch volume (the caller's *alt, else the context's
+        // slot 4); both carry the boundary value
         void* other;
-        TK_TRY(ctx_work(ctx, 4, phi->bytes(), &other));
+        if (alt) other = *alt;
+        else TK_TRY(ctx_work(ctx, 4, phi->bytes(), &other));
         tk_vol_s tmp = *phi;
         tmp.d = other;
         tmp.owned = false;
@@ -716,7 +719,10 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
         }
         if (sweeps & 1) {
             // the result is in the scratch volume: hand it to phi
-            if (phi->owned) {
+            if (alt) {
+                *alt = phi->d;
+                phi->d = other;
+            } else if (phi->owned) {
                 ctx->work[4] = phi->d;
                 ctx->work_bytes[4] = phi->bytes();
                 phi->d = other;
@@ -847,6 +853,20 @@ static int check_gs(tk_vol phi, tk_vol rhs)
              "poisson: 3-D volumes with every edge >= 3 only");
     return TK_OK;
 }
+
+namespace tk {
+int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int ls_steps, double ls_tol, int max_sweeps, void** alt,
+                          int* sweeps_out, double* err_out)
+{
+    TK_TRY(check_gs(phi, rhs));
+    if (phi->dtype == TK_F64)
+        return solve<double>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps,
+                             sweeps_out, err_out, alt);
+    return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
+                        err_out, alt);
+}
+}  // namespace tk
 
 extern "C" {
 

@@ -99,6 +99,20 @@ namespace tk {
 // get (grow if needed) context work buffer `slot` of at least `bytes`
 int ctx_work(tk_ctx ctx, int slot, size_t bytes, void** out);
 
+// batch support: one outer iteration that enqueues kernels only (no host wait; capturable),
+// the tollerance of the last iteration read back (waits on the run's stream), the
+// iteration counter
+int rsfe_step_async(tk_rsfe s);
+int rsfe_read_tollerance(tk_rsfe s, double* tol);
+int rsfe_iterations(tk_rsfe s);
+void rsfe_set_iterations(tk_rsfe s, int iter);
+
+// tk_poisson_dir_solve with a caller-owned red-black ping-pong buffer *alt of phi's size
+// (swapped with phi->d when the sweep count is odd); alt == nullptr uses the context's slot
+int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int ls_steps, double ls_tol, int max_sweeps, void** alt,
+                          int* sweeps_out, double* err_out);
+
 inline bool same_dims(const tk_vol a, const tk_vol b)
 {
     return a->nx == b->nx && a->ny == b->ny && a->nz == b->nz;

@@ -196,14 +196,19 @@ struct tk_rsfe_s {
            kmyim = nullptr, g = nullptr, divdb = nullptr;
     tk_vol b[3] = {nullptr, nullptr, nullptr}, q[3] = {nullptr, nullptr, nullptr};
     void* C = nullptr;
+    void* alt = nullptr;       // red-black ping-pong partner of phi (this run's own)
+    double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
     int iter = 0;
     double tollerance = 1000;
 };
 
 namespace {
 
+// One outer iteration.  sync: read the tollerance back and wait (the reference's loop
+// condition needs it); without sync nothing waits on the host and nothing but kernels is
+// enqueued, so the iteration can be captured into a graph (the caller reads tol_dev later).
 template <class T>
-int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st)
+int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
 {
     tk_ctx ctx = s->ctx;
     const tk_rsfe_params& p = s->p;
@@ -239,8 +244,9 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st)
     const double chosen_bc = p.bc_dir ? p.a0 : 0.;
     int sweeps = 0;
     double ls_err = -1;
-    int r = tk_poisson_dir_solve(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
-                                 p.ls_tol, p.max_sweeps, &sweeps, &ls_err);
+    int r = poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
+                                  p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps,
+                                  &ls_err);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
     // cut + norms + grad/shrink/Bregman, then div
     int nb = 0;
@@ -250,13 +256,15 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st)
         TK_TRY(launch_sb_update<T>(ctx, s->phi, s->phiold, s->b, s->g, s->q, p.a0, p.b0, &nb,
                                    part));
     }
-    hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb, ctx->acc);
+    hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb, s->tol_dev);
     TK_LAUNCHED();
     {
         TimedLaunch tl(ctx, TK_K_DIV);
         TK_TRY(launch_div<T>(ctx, s->q[0], s->q[1], s->q[2], s->divdb));
     }
-    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+    s->iter++;
+    if (!sync) return TK_OK;
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, s->tol_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
     TK_HIP(hipStreamSynchronize(ctx->stream));
     TK_TRY(timing_flush(ctx));
@@ -268,7 +276,6 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st)
         st->tollerance = s->tollerance;
         st->guess_used = 0;
     }
-    s->iter++;
     return TK_OK;
 }
 
@@ -296,6 +303,8 @@ void free_state(tk_rsfe s)
         *v = nullptr;
     }
     if (s->C) (void)hipFree(s->C);
+    if (s->alt) (void)hipFree(s->alt);
+    if (s->tol_dev) (void)hipFree(s->tol_dev);
     if (s->flt) tk_filter_free(s->flt);
     delete s;
 }
@@ -418,15 +427,23 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
         set_error("tk_rsfe_create: complex buffer allocation failed");
         r = TK_ERR_NOMEM;
     }
+    if (r == TK_OK && p->gs_order == TK_GS_REDBLACK &&
+        hipMalloc(&s->alt, dtype_size(dt) * (size_t)u0->n()) != hipSuccess) {
+        set_error("tk_rsfe_create: red-black buffer allocation failed");
+        r = TK_ERR_NOMEM;
+    }
+    if (r == TK_OK && hipMalloc((void**)&s->tol_dev, 2 * sizeof(double)) != hipSuccess) {
+        set_error("tk_rsfe_create: allocation failed");
+        r = TK_ERR_NOMEM;
+    }
     if (r == TK_OK) r = tk_vol_copy(s->u0, u0);
     if (r == TK_OK) r = init_phi(s, init);
     // build_gaussian_filter(phi, sigma, 3, gpa) (imp:1641-1644)
     if (r == TK_OK)
         r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], p->sigma,
                                p->radius, p->gaussian_pixel_approach, dt, &s->flt);
-    // FP32: pick FFT or direct per axis by timing both on this volume's sizes (FP64, the
-    // parity mode, keeps the FFT passes)
-    if (r == TK_OK && dt == TK_F32) r = tk_filter_autotune(ctx, s->flt, nullptr);
+    // FP32 convolves directly on every axis (accuracy of the region fit; the filter picks
+    // the register-window or the generic taps pass), FP64 through the FFT passes
     // red-black runs: pick the sweep tile by timing (the result does not depend on it)
     if (r == TK_OK && p->gs_order == TK_GS_REDBLACK)
         r = tk_poisson_rb_autotune(ctx, s->u0, nullptr, nullptr);
@@ -467,6 +484,31 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
     if (n_done) *n_done = done;
     return TK_OK;
 }
+
+}  // extern "C"
+
+namespace tk {
+int rsfe_step_async(tk_rsfe s)
+{
+    if (s->iter >= s->p.maxiter) return TK_OK;
+    return s->dtype == TK_F64 ? rsfe_iteration<double>(s, nullptr, false)
+                              : rsfe_iteration<float>(s, nullptr, false);
+}
+int rsfe_read_tollerance(tk_rsfe s, double* tol)
+{
+    double h[2];
+    TK_HIP(hipMemcpyAsync(h, s->tol_dev, sizeof h, hipMemcpyDeviceToHost, s->ctx->stream));
+    TK_HIP(hipStreamSynchronize(s->ctx->stream));
+    const double h3 = s->h[0] * s->h[1] * s->h[2];
+    s->tollerance = std::sqrt(h3 * h[0]) / std::sqrt(h3 * h[1]);
+    if (tol) *tol = s->tollerance;
+    return TK_OK;
+}
+int rsfe_iterations(tk_rsfe s) { return s->iter; }
+void rsfe_set_iterations(tk_rsfe s, int iter) { s->iter = iter; }
+}  // namespace tk
+
+extern "C" {
 
 tk_vol tk_rsfe_phi(tk_rsfe s) { return s ? s->phi : nullptr; }
 tk_filter tk_rsfe_filter(tk_rsfe s) { return s ? s->flt : nullptr; }

@@ -79,6 +79,51 @@ def copy_probe(tk, steps):
           flush=True)
 
 
+BATCH_SEED = 20240604
+
+
+def bench_batch(tk, a):
+    """64 volumes, each edge from {48,64,80,96,128,160}, ellipsoid generator with 1-4
+    ellipsoids (seed BATCH_SEED + v), rescaled to [0,255], sigma 3, ls_steps 5, ls_tol 0,
+    FP32 red-black; a step is one outer iteration of every volume.  value = sum N_v * K / t."""
+    dims, nell = tk.gen_batch_dims(BATCH_SEED, 64)
+    ctx = tk.Context(0)
+    p = tk.default_params(sigma=3.0, maxiter=1 << 30, tol=0.0, ls_steps=5, ls_tol=0.0,
+                          gs_order=tk.TK_GS_REDBLACK)
+    b = tk.Batch(ctx, dims, p, np.float32)
+    vox = 0
+    t0 = time.perf_counter()
+    for v in range(64):
+        u0, phi0 = tk.gen_standard(tuple(int(x) for x in dims[v]), BATCH_SEED + v, 1, int(nell[v]))
+        mn, mx = u0.min(), u0.max()
+        b.set_volume(v, (u0 - mn) / ((mx - mn) / 255.0), phi0)
+        vox += int(np.prod(dims[v]))
+    t_setup = time.perf_counter() - t0
+    graph = not a.no_graph
+    steps = a.steps + (a.steps & 1 if graph else 0)
+    b.iterate(max(2, a.warmup + (a.warmup & 1)), use_graph=graph)
+    t0 = time.perf_counter()
+    b.iterate(steps, use_graph=graph)
+    t = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    value = vox * steps / t
+    ref_model = vox * (58 * 4 + 3 * 4 * 5) * steps / t / 1e9
+    out = {"metric": "Split-Bregman voxel-iters/s, 64 uneven volumes (aggregate)",
+           "value": value, "unit": "voxel-iters/s", "n_gpus": 1, "steps": steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * t / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64-reductions",
+           "data": "synthetic (seeded ellipsoid phantoms, rescaled to [0,255])",
+           "config": {"workload": "rsfe_splitbregman batch: 64 volumes, edges from "
+                                  "{48,64,80,96,128,160}, sigma 3, red-black ls_steps 5 ls_tol 0",
+                      "voxels": vox, "lanes": info["lanes"], "graph": graph,
+                      "graph_kernel_nodes": info["graph_nodes"], "t_setup_s": round(t_setup, 3),
+                      "loop_ref_model_GBps": round(ref_model, 1),
+                      "loop_ref_model_frac": round(ref_model / HBM_PEAK_GBS, 4)},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +138,9 @@ def main():
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
+    ap.add_argument("--workload", choices=["c1", "batch"], default="c1",
+                    help="c1: one 256^3 volume (default); batch: 64 uneven volumes")
+    ap.add_argument("--no-graph", action="store_true", help="batch: eager launches")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 512 MiB volume copy (k_copy16): the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counts")
@@ -105,6 +153,9 @@ def main():
     tk = importlib.import_module("3dimagetoolkit_amd")
     if a.copy_probe:
         copy_probe(tk, a.steps)
+        return
+    if a.workload == "batch":
+        bench_batch(tk, a)
         return
     n = a.size
     shape = (n, n, n)

@@ -225,14 +225,23 @@ tk_vol tk_rsfe_phi(tk_rsfe s);
 tk_filter tk_rsfe_filter(tk_rsfe s);
 int tk_rsfe_destroy(tk_rsfe s);
 
-/* ---- batched uneven volumes (many independent runs, one launch per stage) ------- */
+/* ---- batched uneven volumes (config: many independent runs of uneven sizes) --------
+ * No reference counterpart (the reference's static filter state serves one size at a time,
+ * convolution.hxx:307-314).  Each volume is one rsfe_splitbregman::apply run with its own
+ * state; volumes are spread over up to four lanes (a context and HIP stream each) by
+ * longest-processing-time.  With ls_tol == 0 and tol == 0, tk_batch_iterate enqueues the
+ * iterations of all volumes without host waits; use_graph != 0 captures two outer
+ * iterations of every volume into one hipGraph (forked over the lanes) and replays it.
+ * Otherwise each volume runs the reference's host-checked loop in turn. */
 typedef struct tk_batch_s* tk_batch;
 int tk_batch_create(tk_ctx ctx, int n_vol, const int* dims3, tk_dtype dtype,
                     const tk_rsfe_params* p, tk_batch* out);
-/* volume v's u0 and init are host arrays in the given dtype */
+/* volume v's u0 and init are host arrays in the given dtype; init NULL: cube contour */
 int tk_batch_set_volume(tk_batch b, int v, const void* u0, const void* init, tk_dtype host_dtype);
 int tk_batch_iterate(tk_batch b, int n_iters, int use_graph);
 int tk_batch_get_phi(tk_batch b, int v, void* host, tk_dtype host_dtype);
+/* lanes in use, kernel nodes of the captured graph (0 if none), outer iterations run */
+int tk_batch_info(tk_batch b, int* n_lanes, int* graph_nodes, int* iterations);
 int tk_batch_destroy(tk_batch b);
 
 /* ---- synthetic volumes (host functions; no reference counterpart: the reference

@@ -186,9 +186,11 @@ public:
     void im_to_black_and_white(image3d<unsigned char>& res, double threshold = 0.5,
                                bool negative = false) const
     {
-        if (threshold < 0 || threshold > 1) {
-            tk::warning("im_to_black_and_white: threshold must be between 0 and 1");
-            return;
+        if (threshold > 1 || threshold < 0) {
+            // image3d_imp.hxx:1597-1602
+            tk::warning("im_to_black_and_white: threshold must be between 0 and 1, otherwise "
+                        "default value 0.5 is applied");
+            threshold = 0.5;
         }
         res.shape_like(*this);
         tk::check(tk_im_to_black_and_white(ctx_->handle(), v_, threshold, negative ? 1 : 0,

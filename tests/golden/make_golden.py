@@ -9,8 +9,20 @@ tests/test_oracle_pins.py.  Inputs come from the project's seeded generator.
                  cyclic convolution, reference default lambda/mu/beta.
   s5_64.npz   -- the same input with the reference default Poisson setting
                  (ls_steps 5, ls_tol 0), 30 iterations.
+  c1_256.npz  -- 256^3, the same generator scaled x4 (semi-axes x4, margin 60, phi0 a
+                 sphere of radius 80), seed 20240602, sigma 6, 50 outer iterations, ls_steps
+                 5, ls_tol 0.  phi itself (128 MiB) is not stored: the fixture keeps the
+                 mask phi >= 0.5 (bit-packed, every voxel), phi at 65536 seeded sample
+                 voxels, the FP64 sum and sum of squares of phi, and the per-iteration
+                 tollerance.
 
-Usage: python tests/golden/make_golden.py [c0|s5|all]
+  c3_batch.npz -- 64 volumes, each edge drawn from {48,64,80,96,128,160} (tk_gen_batch_dims,
+                 seed 20240604), volume v from the ellipsoid generator with seed 20240604 + v
+                 and 1-4 ellipsoids, rescaled to [0,255], sigma 3, 30 outer iterations,
+                 ls_steps 5, ls_tol 0: per volume the bit-packed mask phi >= 0.5 and the
+                 FP64 sum of phi.
+
+Usage: python tests/golden/make_golden.py [c0|s5|c1|c3|all]
 """
 import importlib
 import json
@@ -30,6 +42,8 @@ import oracle_lib as orc  # noqa: E402
 tk = importlib.import_module("3dimagetoolkit_amd")
 
 C0_SEED = 20240601
+C1_SEED = 20240602
+C1_SAMPLES = 65536
 
 
 def run(name, ls_tol, maxiter=30):
@@ -48,9 +62,62 @@ def run(name, ls_tol, maxiter=30):
     print(name, meta, "sweeps", sweeps.tolist())
 
 
+def c1_sample_index(n_vox):
+    return np.sort(np.random.default_rng(1).choice(n_vox, C1_SAMPLES, replace=False))
+
+
+def run_c1(maxiter=50):
+    shape = (256, 256, 256)
+    u0, phi0 = tk.gen_standard(shape, C1_SEED, 4, 2)
+    u0 = orc.change_range(u0, 255.0, 0.0)
+    p = orc.default_params(sigma=6.0, maxiter=maxiter, tol=0.0, ls_steps=5, ls_tol=0.0)
+    t0 = time.time()
+    phi, sweeps, tols, st = orc.rsfe_apply(u0, phi0, p)
+    dt = time.time() - t0
+    flat = phi.reshape(-1)
+    idx = c1_sample_index(flat.size)
+    meta = dict(seed=C1_SEED, shape=list(shape), scale=4, sigma=6.0, maxiter=maxiter, ls_steps=5,
+                ls_tol=0.0, iters=int(st.iters), seconds=dt, threads=orc.lib().or_num_threads(),
+                samples=C1_SAMPLES, sample_rng="numpy default_rng(1).choice(N, 65536, replace=False), sorted")
+    np.savez_compressed(os.path.join(HERE, "c1_256.npz"),
+                        mask_bits=np.packbits(flat >= 0.5), sample_idx=idx.astype(np.int64),
+                        phi_samples=flat[idx], phi_sum=np.float64(flat.sum()),
+                        phi_sumsq=np.float64((flat * flat).sum()), sweeps=sweeps, tols=tols,
+                        meta=json.dumps(meta))
+    print("c1", meta, "tols", tols[-3:].tolist())
+
+
+C3_SEED = 20240604
+
+
+def c3_volume(v, dims, nell):
+    return tk.gen_standard(tuple(int(x) for x in dims), C3_SEED + v, 1, int(nell))
+
+
+def run_c3(maxiter=30):
+    dims, nell = tk.gen_batch_dims(C3_SEED, 64)
+    p = orc.default_params(sigma=3.0, maxiter=maxiter, tol=0.0, ls_steps=5, ls_tol=0.0)
+    out = {}
+    t0 = time.time()
+    for v in range(64):
+        u0, phi0 = c3_volume(v, dims[v], nell[v])
+        phi, sweeps, tols, st = orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0, p)
+        out[f"mask_{v}"] = np.packbits((phi >= 0.5).reshape(-1))
+        out[f"sum_{v}"] = np.float64(phi.sum())
+    meta = dict(seed=C3_SEED, n_vol=64, sigma=3.0, maxiter=maxiter, ls_steps=5, ls_tol=0.0,
+                seconds=time.time() - t0, threads=orc.lib().or_num_threads())
+    np.savez_compressed(os.path.join(HERE, "c3_batch.npz"), dims=dims, nell=nell,
+                        meta=json.dumps(meta), **out)
+    print("c3", meta)
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "all"
     if which in ("s5", "all"):
         run("s5", 0.0)
     if which in ("c0", "all"):
         run("c0", 1e-8)
+    if which in ("c1", "all"):
+        run_c1()
+    if which in ("c3", "all"):
+        run_c3()
