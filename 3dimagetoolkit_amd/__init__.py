@@ -11,6 +11,7 @@ There is no CPU fallback: if libtk.so is missing or cannot be loaded, importing
 the binding raises.
 """
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -42,12 +43,13 @@ class RsfeParams(C.Structure):
         "sigma", "lamda", "lamda1", "lamda2", "beta", "dt", "epsilon", "a0", "b0", "nu",
         "tol", "ls_tol", "radius")] + [(n, C.c_int) for n in (
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "gs_order",
-            "max_sweeps", "warm_start")]
+            "max_sweeps", "warm_start")] + [("ls_res_tol", C.c_double),
+                                            ("res_check_every", C.c_int)]
 
 
 class IterStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("ls_err", C.c_double), ("tollerance", C.c_double),
-                ("guess_used", C.c_int)]
+                ("guess_used", C.c_int), ("res_plain", C.c_double), ("res_guess", C.c_double)]
 
 
 def _load():
@@ -90,6 +92,9 @@ def _load():
         "tk_poisson_dir_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
         "tk_poisson_rb_autotune": (i, [vp, vp, ip, dp]),
+        "tk_poisson_solve_residual": (i, [vp, vp, vp, d, d, i, d, i, i, ip, dp]),
+        "tk_rsfe_rhs": (vp, [vp]),
+        "tk_rsfe_set_warm_start": (i, [vp, vp]),
         "tk_poisson_rb_set_tile": (i, [vp, vp, i]),
         "tk_heaviside": (i, [vp, vp, d, d, d, vp]),
         "tk_grad": (i, [vp, vp, vp, vp, vp]),
@@ -109,6 +114,27 @@ def _load():
         "tk_rsfe_iterate": (i, [vp, i, C.POINTER(IterStats), ip]),
         "tk_rsfe_phi": (vp, [vp]),
         "tk_rsfe_destroy": (i, [vp]),
+        "tk_nn_conv3d_fwd": (i, [vp, vp, i, i, i, vp, i, vp, vp, i, i, i, i, i, i, i, i]),
+        "tk_nn_conv3d_wgrad": (i, [vp, vp, i, i, i, vp, i, i, i, i, i, i, i, vp, i, vp, vp]),
+        "tk_nn_flip_weights": (i, [vp, vp, i, i, i, vp, i]),
+        "tk_nn_relu_bwd": (i, [vp, vp, i, i, vp, i, i, i, C.c_long]),
+        "tk_nn_avgpool2": (i, [vp, vp, i, i, i, i, i, i, vp, i, i]),
+        "tk_nn_avgpool2_bwd": (i, [vp, vp, i, i, i, i, i, i, vp, i, i, i]),
+        "tk_nn_upsample2": (i, [vp, vp, i, i, i, i, i, i, vp, i, i]),
+        "tk_nn_upsample2_bwd": (i, [vp, vp, i, i, i, i, i, i, vp, i, i]),
+        "tk_nn_add_slice": (i, [vp, vp, i, i, vp, i, i, i, C.c_long]),
+        "tk_nn_f32_to_bf16": (i, [vp, vp, C.c_long, vp]),
+        "tk_nn_bf16_to_f32": (i, [vp, vp, C.c_long, vp]),
+        "tk_unet_create": (i, [vp, i, i, i, u64, C.POINTER(vp)]),
+        "tk_unet_destroy": (i, [vp]),
+        "tk_unet_num_params": (i, [vp, C.POINTER(C.c_long)]),
+        "tk_unet_set_out_scale": (i, [vp, d]),
+        "tk_unet_get_out_scale": (i, [vp, dp]),
+        "tk_unet_predict": (i, [vp, vp, vp]),
+        "tk_unet_train_step": (i, [vp, vp, vp, d, dp]),
+        "tk_unet_get_params": (i, [vp, i, C.POINTER(C.c_float), C.c_long]),
+        "tk_unet_set_params": (i, [vp, C.POINTER(C.c_float), C.c_long]),
+        "tk_unet_get_output": (i, [vp, C.POINTER(C.c_float)]),
         "tk_batch_create": (i, [vp, i, ip, i, C.POINTER(RsfeParams), C.POINTER(vp)]),
         "tk_batch_set_volume": (i, [vp, i, vp, vp, i]),
         "tk_batch_iterate": (i, [vp, i, i]),
@@ -189,6 +215,11 @@ class Context:
         h = C.c_void_p()
         _check(lib.tk_ctx_create(device, C.byref(h)))
         self.h = h
+        # objects living on this context, released before the context itself
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def sync(self):
         _check(lib.tk_ctx_sync(self.h))
@@ -205,6 +236,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.free() if hasattr(ch, "free") else ch.close()
             lib.tk_ctx_destroy(self.h)
             self.h = None
 
@@ -230,6 +263,7 @@ class Volume:
 
     def __init__(self, ctx, shape, dtype=np.float64, h=(1.0, 1.0, 1.0)):
         self.ctx = ctx
+        ctx._adopt(self)
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         hv = C.c_void_p()
@@ -276,6 +310,7 @@ class Filter:
                                       C.byref(hv)))
         self.h = hv
         self.ctx = ctx
+        ctx._adopt(self)
         self.shape = tuple(shape)
 
     def taps(self):
@@ -377,6 +412,7 @@ class Rsfe:
                                   C.byref(params), C.byref(hv)))
         self.h = hv
         self.ctx = ctx
+        ctx._adopt(self)
         self.shape = u0.shape
         self.dtype = u0.dtype
 
@@ -385,7 +421,8 @@ class Rsfe:
         done = C.c_int()
         _check(lib.tk_rsfe_iterate(self.h, n, st, C.byref(done)))
         return [dict(sweeps=st[q].sweeps, ls_err=st[q].ls_err, tollerance=st[q].tollerance,
-                     guess_used=st[q].guess_used) for q in range(done.value)]
+                     guess_used=st[q].guess_used, res_plain=st[q].res_plain,
+                     res_guess=st[q].res_guess) for q in range(done.value)]
 
     def phi(self, dtype=np.float64):
         out = np.empty(self.shape, dtype)
@@ -416,6 +453,7 @@ class Batch:
                                    _NP2TK[self.dtype], C.byref(params), C.byref(hv)))
         self.h = hv
         self.ctx = ctx
+        ctx._adopt(self)
 
     def set_volume(self, v, u0, init=None):
         u0 = np.ascontiguousarray(u0, np.float64)
@@ -445,4 +483,59 @@ class Batch:
     def close(self):
         if self.h:
             _check(lib.tk_batch_destroy(self.h))
+            self.h = None
+
+
+class UNet:
+    """tk_unet: the learned Poisson warm start (3-level 3-D U-Net, BF16 on MFMA)."""
+
+    def __init__(self, ctx, dims, seed=1):
+        hv = C.c_void_p()
+        _check(lib.tk_unet_create(ctx.h, *[int(x) for x in dims], int(seed), C.byref(hv)))
+        self.h = hv
+        self.ctx = ctx
+        ctx._adopt(self)
+        self.dims = tuple(int(x) for x in dims)
+        c = C.c_long()
+        _check(lib.tk_unet_num_params(self.h, C.byref(c)))
+        self.n_params = c.value
+
+    def _fp(self, a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def params(self, grads=False):
+        out = np.empty(self.n_params, np.float32)
+        _check(lib.tk_unet_get_params(self.h, int(grads), self._fp(out), self.n_params))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib.tk_unet_set_params(self.h, self._fp(p), p.size))
+
+    def output(self):
+        out = np.empty(self.dims, np.float32)
+        _check(lib.tk_unet_get_output(self.h, self._fp(out)))
+        return out
+
+    @property
+    def out_scale(self):
+        s = C.c_double()
+        _check(lib.tk_unet_get_out_scale(self.h, C.byref(s)))
+        return s.value
+
+    @out_scale.setter
+    def out_scale(self, s):
+        _check(lib.tk_unet_set_out_scale(self.h, float(s)))
+
+    def predict(self, rhs, guess):
+        _check(lib.tk_unet_predict(self.h, rhs.h, guess.h))
+
+    def train_step(self, rhs, target, lr):
+        loss = C.c_double()
+        _check(lib.tk_unet_train_step(self.h, rhs.h, target.h, float(lr), C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            _check(lib.tk_unet_destroy(self.h))
             self.h = None

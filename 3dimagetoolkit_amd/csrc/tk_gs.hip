@@ -855,6 +855,44 @@ static int check_gs(tk_vol phi, tk_vol rhs)
 }
 
 namespace tk {
+int poisson_solve_residual_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,
+                               int order, double res_tol, int check_every, int max_sweeps,
+                               void** alt, int* sweeps_out, double* res_out)
+{
+    TK_TRY(check_gs(phi, rhs));
+    TK_CHECK(res_tol > 0 && check_every > 0, TK_ERR_ARG,
+             "poisson residual solve: res_tol and check_every must be > 0");
+    if (max_sweeps <= 0) max_sweeps = 1 << 30;
+    if (phi->dtype == TK_F64) TK_TRY(set_boundary<double>(ctx, phi, bc, nullptr));
+    else TK_TRY(set_boundary<float>(ctx, phi, bc, nullptr));
+    double res = 0;
+    TK_TRY(tk_poisson_residual(ctx, phi, rhs, &res));
+    int sweeps = 0;
+    while (!(res <= res_tol) && sweeps < max_sweeps) {
+        const int k = std::min(check_every, max_sweeps - sweeps);
+        if (phi->dtype == TK_F64)
+            TK_TRY(solve<double>(ctx, phi, rhs, dt, bc, order, k, 0, 0, nullptr, nullptr, alt));
+        else
+            TK_TRY(solve<float>(ctx, phi, rhs, dt, bc, order, k, 0, 0, nullptr, nullptr, alt));
+        sweeps += k;
+        TK_TRY(tk_poisson_residual(ctx, phi, rhs, &res));
+    }
+    if (sweeps_out) *sweeps_out = sweeps;
+    if (res_out) *res_out = res;
+    if (!(res <= res_tol)) {
+        set_error("poisson residual solve: %d sweeps, residual %.3e > %.3e", sweeps, res,
+                  res_tol);
+        return TK_ERR_SWEEPS;
+    }
+    return TK_OK;
+}
+
+int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc)
+{
+    if (phi->dtype == TK_F64) return set_boundary<double>(ctx, phi, bc, nullptr);
+    return set_boundary<float>(ctx, phi, bc, nullptr);
+}
+
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, void** alt,
                           int* sweeps_out, double* err_out)
@@ -896,6 +934,15 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
                              err_out);
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
                         err_out);
+}
+
+int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,
+                              int order, double res_tol, int check_every, int max_sweeps,
+                              int* sweeps_out, double* res_out)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_solve_residual: null context");
+    return poisson_solve_residual_alt(ctx, phi, rhs, dt, bc, order, res_tol, check_every,
+                                      max_sweeps, nullptr, sweeps_out, res_out);
 }
 
 int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms)

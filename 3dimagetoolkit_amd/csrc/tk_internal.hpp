@@ -107,6 +107,17 @@ int rsfe_read_tollerance(tk_rsfe s, double* tol);
 int rsfe_iterations(tk_rsfe s);
 void rsfe_set_iterations(tk_rsfe s, int iter);
 
+// tk_poisson_solve_residual with the caller-owned ping-pong buffer *alt (nullptr: slot)
+int poisson_solve_residual_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,
+                               int order, double res_tol, int check_every, int max_sweeps,
+                               void** alt, int* sweeps_out, double* res_out);
+// the context and dims a network was created for
+tk_ctx unet_ctx(tk_unet n);
+void unet_dims(tk_unet n, int d[3]);
+
+// set phi's six faces to bc (the first step of DirGaussSeidel, poisson_imp.hxx:370-390)
+int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc);
+
 // tk_poisson_dir_solve with a caller-owned red-black ping-pong buffer *alt of phi's size
 // (swapped with phi->d when the sweep count is odd); alt == nullptr uses the context's slot
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,

@@ -1,0 +1,471 @@
+// tk_nn.hip -- BF16 building blocks of the learned Poisson warm start (3-D U-Net).
+//
+// Activations are channels-last BF16 arrays: voxel v = (i*Y + j)*Z + k of an X*Y*Z level,
+// channel c at v*stride + offset + c, so a convolution can read or write a channel slice
+// of a wider (concatenated) buffer.  Weights of a 3x3x3 convolution are BF16 [co][kpad]
+// with k = tap*ci + c, tap = (dx+1)*9 + (dy+1)*3 + (dz+1), zero-padded from 27*ci up to kpad
+// (a multiple of 16).  Outside the volume the input is zero.
+//
+// The convolutions run on the matrix cores: v_mfma_f32_16x16x16_bf16, FP32 accumulation.
+// One wave computes a 16 (voxels along z) x 16 (output channels) tile of the forward pass,
+// or a 16 (output channels) x 16 (k) tile of the weight gradient summed over a chunk of
+// voxels.  Operand layout of the instruction for wave64 (lane l, element q = 0..3):
+// A[m = l % 16][k = 4*(l / 16) + q], B[k = 4*(l / 16) + q][n = l % 16],
+// D[m = 4*(l / 16) + q][n = l % 16].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "tk_device.hpp"
+#include "tk_internal.hpp"
+
+namespace tk {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+struct Tap {
+    int dx, dy, dz;
+};
+__device__ __forceinline__ Tap tap_of(int t) { return {t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1}; }
+
+// forward: y[v][yo + o] = act(b[o] + sum_k w[o][k] x[v + tap(k)][xo + c(k)])
+template <bool RELU, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ x, int xs, int xo,
+                                                   int ci, const uint16_t* __restrict__ w,
+                                                   int kpad, const float* __restrict__ bias,
+                                                   void* __restrict__ y, int ys, int yo, int co,
+                                                   int X, int Y, int Z)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nzt = (Z + 15) / 16, nct = (co + 15) / 16;
+    const long ntiles = (long)X * Y * nzt * nct;
+    const long t = (long)blockIdx.x * 4 + wave;
+    if (t >= ntiles) return;
+    const int ct = (int)(t % nct);
+    const long vt = t / nct;
+    const int zt = (int)(vt % nzt);
+    const long row = vt / nzt;
+    const int j = (int)(row % Y), i = (int)(row / Y);
+    const int z0 = zt * 16;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    const int za = z0 + m;           // A row: this lane's voxel
+    const int ob = ct * 16 + m;      // B column: this lane's output channel
+    const int K = 27 * ci;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < kpad; kc += 16) {
+        v4s a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kc + kq + q;
+            uint16_t av = 0;
+            if (k < K && za < Z) {
+                const int tp = k / ci, c = k - tp * ci;
+                const Tap d = tap_of(tp);
+                const int ii = i + d.dx, jj = j + d.dy, zz = za + d.dz;
+                if (ii >= 0 && ii < X && jj >= 0 && jj < Y && zz >= 0 && zz < Z)
+                    av = x[(((long)ii * Y + jj) * Z + zz) * xs + xo + c];
+            }
+            a[q] = (short)av;
+            b[q] = (short)(ob < co ? w[(long)ob * kpad + k] : (uint16_t)0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int od = ct * 16 + (lane & 15);
+    if (od >= co) return;
+    const float bo = bias ? bias[od] : 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int z = z0 + kq + q;
+        if (z >= Z) continue;
+        float v = acc[q] + bo;
+        if (RELU) v = fmaxf(v, 0.f);
+        const long e = (((long)i * Y + j) * Z + z) * ys + yo + od;
+        if (OUT_F32) ((float*)y)[e] = v;
+        else ((uint16_t*)y)[e] = f2bf(v);
+    }
+}
+
+// weight gradient of one chunk of voxels:
+// part[chunk][o][k] = sum_{v in chunk} dy[v][o] x[v + tap(k)][c(k)]
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t* __restrict__ x, int xs,
+                                                     int xo, int ci,
+                                                     const uint16_t* __restrict__ dy, int ds,
+                                                     int dof, int co, int X, int Y, int Z,
+                                                     int kpad, int vchunk, int nchunk,
+                                                     float* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nkt = kpad / 16, nct = (co + 15) / 16;
+    const long ntiles = (long)nchunk * nct * nkt;
+    const long t = (long)blockIdx.x * 4 + wave;
+    if (t >= ntiles) return;
+    const int kt = (int)(t % nkt);
+    const long r = t / nkt;
+    const int ct = (int)(r % nct), chunk = (int)(r / nct);
+    const long n = (long)X * Y * Z;
+    const long v0 = (long)chunk * vchunk;
+    const long v1 = v0 + vchunk < n ? v0 + vchunk : n;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    const int oa = ct * 16 + m;      // A row: output channel
+    const int kb = kt * 16 + m;      // B column: k index
+    const int K = 27 * ci;
+    int tp = 0, c = 0;
+    Tap d = {0, 0, 0};
+    if (kb < K) {
+        tp = kb / ci;
+        c = kb - tp * ci;
+        d = tap_of(tp);
+    }
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    for (long vb = v0; vb < v1; vb += 16) {
+        v4s a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long v = vb + kq + q;
+            uint16_t av = 0, bv = 0;
+            if (v < v1) {
+                if (oa < co) av = dy[v * ds + dof + oa];
+                if (kb < K) {
+                    const int z = (int)(v % Z);
+                    const long rr = v / Z;
+                    const int jj = (int)(rr % Y) + d.dy, ii = (int)(rr / Y) + d.dx, zz = z + d.dz;
+                    if (ii >= 0 && ii < X && jj >= 0 && jj < Y && zz >= 0 && zz < Z)
+                        bv = x[(((long)ii * Y + jj) * Z + zz) * xs + xo + c];
+                }
+            }
+            a[q] = (short)av;
+            b[q] = (short)bv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int kd = kt * 16 + (lane & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int o = ct * 16 + kq + q;
+        if (o < co) part[((long)chunk * co + o) * kpad + kd] = acc[q];
+    }
+}
+
+// dw[e] = sum over chunks of part[chunk][e] in chunk order (deterministic)
+__global__ void k_sum_chunks(const float* __restrict__ part, int nchunk, long len,
+                             float* __restrict__ dw)
+{
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < len; e += (long)gridDim.x * 256) {
+        float s = 0.f;
+        for (int q = 0; q < nchunk; ++q) s += part[(long)q * len + e];
+        dw[e] = s;
+    }
+}
+
+// bias gradient partials: bpart[chunk][o] = sum_{v in chunk} dy[v][o]
+__global__ void k_bias_grad(const uint16_t* __restrict__ dy, int ds, int dof, int co, long n,
+                            int vchunk, float* __restrict__ bpart)
+{
+    const int chunk = blockIdx.x;
+    const long v0 = (long)chunk * vchunk, v1 = v0 + vchunk < n ? v0 + vchunk : n;
+    for (int o = threadIdx.y; o < co; o += blockDim.y) {
+        float s = 0.f;
+        for (long v = v0 + threadIdx.x; v < v1; v += blockDim.x) s += bf2f(dy[v * ds + dof + o]);
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (threadIdx.x == 0) bpart[(long)chunk * co + o] = s;
+    }
+}
+
+// backward weights of the input gradient: wt[c][tap'][o] = w[o][26 - tap'][c] (the 3x3x3
+// kernel flipped, input and output channels exchanged), so dx = conv3_fwd(dy, wt)
+__global__ void k_flip_weights(const uint16_t* __restrict__ w, int ci, int co, int kpad,
+                               uint16_t* __restrict__ wt, int kpad_t)
+{
+    const long n = (long)ci * kpad_t;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const int c = (int)(e / kpad_t), kt = (int)(e % kpad_t);
+        uint16_t v = 0;
+        if (kt < 27 * co) {
+            const int tp = kt / co, o = kt - tp * co;
+            v = w[(long)o * kpad + (26 - tp) * ci + c];
+        }
+        wt[e] = v;
+    }
+}
+
+// g[v][c] *= (y[v][c] > 0): ReLU backward from the post-activation output
+__global__ void k_relu_bwd(uint16_t* __restrict__ g, int gs, int go, const uint16_t* __restrict__ y,
+                           int ysd, int yo, int C, long n)
+{
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n * C; e += (long)gridDim.x * 256) {
+        const long v = e / C;
+        const int c = (int)(e - v * C);
+        if (!(bf2f(y[v * ysd + yo + c]) > 0.f)) g[v * gs + go + c] = 0;
+    }
+}
+
+// 2x2x2 average pool (X, Y, Z even): out level X/2 x Y/2 x Z/2
+__global__ void k_avgpool2(const uint16_t* __restrict__ x, int xs, int xo, int C, int X, int Y,
+                           int Z, uint16_t* __restrict__ y, int ys, int yo)
+{
+    const int X2 = X / 2, Y2 = Y / 2, Z2 = Z / 2;
+    const long n = (long)X2 * Y2 * Z2 * C;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const long v = e / C;
+        const int k = (int)(v % Z2);
+        const long r = v / Z2;
+        const int j = (int)(r % Y2), i = (int)(r / Y2);
+        float s = 0.f;
+        for (int a = 0; a < 8; ++a) {
+            const int ii = 2 * i + (a >> 2), jj = 2 * j + ((a >> 1) & 1), kk = 2 * k + (a & 1);
+            s += bf2f(x[(((long)ii * Y + jj) * Z + kk) * xs + xo + c]);
+        }
+        y[v * ys + yo + c] = f2bf(s * 0.125f);
+    }
+}
+
+// its backward: dx[child][c] (+)= dy[parent][c] / 8
+__global__ void k_avgpool2_bwd(const uint16_t* __restrict__ dy, int ds, int dof, int C, int X,
+                               int Y, int Z, uint16_t* __restrict__ dx, int xs, int xo, int accum)
+{
+    const long n = (long)X * Y * Z * C;
+    const int Y2 = Y / 2, Z2 = Z / 2;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const long v = e / C;
+        const int k = (int)(v % Z);
+        const long r = v / Z;
+        const int j = (int)(r % Y), i = (int)(r / Y);
+        const long p = ((long)(i / 2) * Y2 + j / 2) * Z2 + k / 2;
+        float g = bf2f(dy[p * ds + dof + c]) * 0.125f;
+        if (accum) g += bf2f(dx[v * xs + xo + c]);
+        dx[v * xs + xo + c] = f2bf(g);
+    }
+}
+
+// nearest 2x upsample: y[child][c] = x[parent][c] (output level X x Y x Z, all even)
+__global__ void k_upsample2(const uint16_t* __restrict__ x, int xs, int xo, int C, int X, int Y,
+                            int Z, uint16_t* __restrict__ y, int ys, int yo)
+{
+    const long n = (long)X * Y * Z * C;
+    const int Y2 = Y / 2, Z2 = Z / 2;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const long v = e / C;
+        const int k = (int)(v % Z);
+        const long r = v / Z;
+        const int j = (int)(r % Y), i = (int)(r / Y);
+        const long p = ((long)(i / 2) * Y2 + j / 2) * Z2 + k / 2;
+        y[v * ys + yo + c] = x[p * xs + xo + c];
+    }
+}
+
+// its backward: dx[parent][c] = sum of the 8 children's dy (FP32 sum, one rounding)
+__global__ void k_upsample2_bwd(const uint16_t* __restrict__ dy, int ds, int dof, int C, int X,
+                                int Y, int Z, uint16_t* __restrict__ dx, int xs, int xo)
+{
+    const int X2 = X / 2, Y2 = Y / 2, Z2 = Z / 2;
+    const long n = (long)X2 * Y2 * Z2 * C;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const long v = e / C;
+        const int k = (int)(v % Z2);
+        const long r = v / Z2;
+        const int j = (int)(r % Y2), i = (int)(r / Y2);
+        float s = 0.f;
+        for (int a = 0; a < 8; ++a) {
+            const int ii = 2 * i + (a >> 2), jj = 2 * j + ((a >> 1) & 1), kk = 2 * k + (a & 1);
+            s += bf2f(dy[(((long)ii * Y + jj) * Z + kk) * ds + dof + c]);
+        }
+        dx[v * xs + xo + c] = f2bf(s);
+    }
+}
+
+// a[v][ao + c] = bf16(a + b[v][bo + c])
+__global__ void k_add_slice(uint16_t* __restrict__ a, int as, int ao,
+                            const uint16_t* __restrict__ b, int bs, int bo, int C, long n)
+{
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n * C; e += (long)gridDim.x * 256) {
+        const long v = e / C;
+        const int c = (int)(e - v * C);
+        a[v * as + ao + c] = f2bf(bf2f(a[v * as + ao + c]) + bf2f(b[v * bs + bo + c]));
+    }
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ s, long n, uint16_t* __restrict__ d)
+{
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256)
+        d[e] = f2bf(s[e]);
+}
+
+__global__ void k_bf16_to_f32(const uint16_t* __restrict__ s, long n, float* __restrict__ d)
+{
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256)
+        d[e] = bf2f(s[e]);
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+namespace {
+inline unsigned nblk(long n) { return (unsigned)std::min<long>((n + 255) / 256, 16384); }
+}  // namespace
+
+extern "C" {
+
+int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const void* w, int kpad,
+                     const float* bias, void* y, int ys, int yo, int co, int nx, int ny, int nz,
+                     int relu, int out_f32)
+{
+    TK_CHECK(ctx && x && w && y && ci > 0 && co > 0 && nx > 0 && ny > 0 && nz > 0, TK_ERR_ARG,
+             "tk_nn_conv3d_fwd: argument");
+    TK_CHECK(kpad % 16 == 0 && kpad >= 27 * ci, TK_ERR_ARG, "tk_nn_conv3d_fwd: kpad");
+    TK_CHECK(xo + ci <= xs && yo + co <= ys, TK_ERR_ARG, "tk_nn_conv3d_fwd: channel slice");
+    const long tiles = (long)nx * ny * ((nz + 15) / 16) * ((co + 15) / 16);
+    const unsigned nb = (unsigned)((tiles + 3) / 4);
+    const uint16_t* xp = (const uint16_t*)x;
+    const uint16_t* wp = (const uint16_t*)w;
+#define TK_FWD(R, F)                                                                           \
+    hipLaunchKernelGGL((k_conv3_fwd<R, F>), dim3(nb), dim3(256), 0, ctx->stream, xp, xs, xo, ci, \
+                       wp, kpad, bias, y, ys, yo, co, nx, ny, nz)
+    if (relu && out_f32) TK_FWD(true, true);
+    else if (relu) TK_FWD(true, false);
+    else if (out_f32) TK_FWD(false, true);
+    else TK_FWD(false, false);
+#undef TK_FWD
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_conv3d_wgrad(tk_ctx ctx, const void* x, int xs, int xo, int ci, const void* dy, int ds,
+                       int dof, int co, int nx, int ny, int nz, int kpad, float* part,
+                       int vchunk, float* dw, float* db)
+{
+    TK_CHECK(ctx && x && dy && part && dw && vchunk > 0 && vchunk % 16 == 0, TK_ERR_ARG,
+             "tk_nn_conv3d_wgrad: argument");
+    TK_CHECK(kpad % 16 == 0 && kpad >= 27 * ci, TK_ERR_ARG, "tk_nn_conv3d_wgrad: kpad");
+    const long n = (long)nx * ny * nz;
+    const int nchunk = (int)((n + vchunk - 1) / vchunk);
+    const long tiles = (long)nchunk * ((co + 15) / 16) * (kpad / 16);
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)x, xs, xo, ci, (const uint16_t*)dy, ds, dof, co, nx, ny,
+                       nz, kpad, vchunk, nchunk, part);
+    TK_LAUNCHED();
+    const long len = (long)co * kpad;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nblk(len)), dim3(256), 0, ctx->stream, part, nchunk, len,
+                       dw);
+    TK_LAUNCHED();
+    if (db) {
+        // bias partials reuse the head of part (the weight partials are consumed above)
+        hipLaunchKernelGGL(k_bias_grad, dim3(nchunk), dim3(64, 4), 0, ctx->stream,
+                           (const uint16_t*)dy, ds, dof, co, n, vchunk, part);
+        TK_LAUNCHED();
+        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, ctx->stream, part, nchunk,
+                           (long)co, db);
+        TK_LAUNCHED();
+    }
+    return TK_OK;
+}
+
+int tk_nn_flip_weights(tk_ctx ctx, const void* w, int ci, int co, int kpad, void* wt, int kpad_t)
+{
+    TK_CHECK(ctx && w && wt && kpad_t % 16 == 0 && kpad_t >= 27 * co, TK_ERR_ARG,
+             "tk_nn_flip_weights: argument");
+    hipLaunchKernelGGL(k_flip_weights, dim3(nblk((long)ci * kpad_t)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)w, ci, co, kpad, (uint16_t*)wt, kpad_t);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_relu_bwd(tk_ctx ctx, void* g, int gs, int go, const void* y, int ys, int yo, int C,
+                   long n)
+{
+    TK_CHECK(ctx && g && y, TK_ERR_ARG, "tk_nn_relu_bwd: argument");
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(n * C)), dim3(256), 0, ctx->stream, (uint16_t*)g, gs,
+                       go, (const uint16_t*)y, ys, yo, C, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_avgpool2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, int ny, int nz,
+                   void* y, int ys, int yo)
+{
+    TK_CHECK(ctx && x && y && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
+             "tk_nn_avgpool2: argument");
+    hipLaunchKernelGGL(k_avgpool2, dim3(nblk((long)nx * ny * nz / 8 * C)), dim3(256), 0,
+                       ctx->stream, (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y, ys,
+                       yo);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_avgpool2_bwd(tk_ctx ctx, const void* dy, int ds, int dof, int C, int nx, int ny, int nz,
+                       void* dx, int xs, int xo, int accum)
+{
+    TK_CHECK(ctx && dy && dx && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
+             "tk_nn_avgpool2_bwd: argument");
+    hipLaunchKernelGGL(k_avgpool2_bwd, dim3(nblk((long)nx * ny * nz * C)), dim3(256), 0,
+                       ctx->stream, (const uint16_t*)dy, ds, dof, C, nx, ny, nz, (uint16_t*)dx, xs,
+                       xo, accum);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_upsample2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, int ny, int nz,
+                    void* y, int ys, int yo)
+{
+    TK_CHECK(ctx && x && y && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
+             "tk_nn_upsample2: argument");
+    hipLaunchKernelGGL(k_upsample2, dim3(nblk((long)nx * ny * nz * C)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y, ys, yo);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_upsample2_bwd(tk_ctx ctx, const void* dy, int ds, int dof, int C, int nx, int ny,
+                        int nz, void* dx, int xs, int xo)
+{
+    TK_CHECK(ctx && dy && dx && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
+             "tk_nn_upsample2_bwd: argument");
+    hipLaunchKernelGGL(k_upsample2_bwd, dim3(nblk((long)nx * ny * nz / 8 * C)), dim3(256), 0,
+                       ctx->stream, (const uint16_t*)dy, ds, dof, C, nx, ny, nz, (uint16_t*)dx, xs,
+                       xo);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_add_slice(tk_ctx ctx, void* a, int as, int ao, const void* b, int bs, int bo, int C,
+                    long n)
+{
+    TK_CHECK(ctx && a && b, TK_ERR_ARG, "tk_nn_add_slice: argument");
+    hipLaunchKernelGGL(k_add_slice, dim3(nblk(n * C)), dim3(256), 0, ctx->stream, (uint16_t*)a, as,
+                       ao, (const uint16_t*)b, bs, bo, C, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_f32_to_bf16(tk_ctx ctx, const float* s, long n, void* d)
+{
+    TK_CHECK(ctx && s && d, TK_ERR_ARG, "tk_nn_f32_to_bf16: argument");
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(n)), dim3(256), 0, ctx->stream, s, n, (uint16_t*)d);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_nn_bf16_to_f32(tk_ctx ctx, const void* s, long n, float* d)
+{
+    TK_CHECK(ctx && s && d, TK_ERR_ARG, "tk_nn_bf16_to_f32: argument");
+    hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+                       (const uint16_t*)s, n, d);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+}  // extern "C"

@@ -197,6 +197,8 @@ struct tk_rsfe_s {
     tk_vol b[3] = {nullptr, nullptr, nullptr}, q[3] = {nullptr, nullptr, nullptr};
     void* C = nullptr;
     void* alt = nullptr;       // red-black ping-pong partner of phi (this run's own)
+    tk_unet net = nullptr;     // learned warm start (same context), or none
+    tk_vol guess = nullptr;    // its guess
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
     int iter = 0;
     double tollerance = 1000;
@@ -240,11 +242,37 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         TimedLaunch tl(ctx, TK_K_CONV_X2);
         TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, LdComplex<T>{C}, MdNone{}, sr)));
     }
-    // Poisson loop
+    // Poisson loop (rsfe_splitbregman_imp.hxx:1735-1797): bc value, then the sweeps from
+    // the previous iterate (or from the learned guess when its residual is lower)
     const double chosen_bc = p.bc_dir ? p.a0 : 0.;
-    int sweeps = 0;
-    double ls_err = -1;
-    int r = poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
+    int sweeps = 0, used = 0;
+    double ls_err = -1, res_plain = -1, res_guess = -1;
+    int judge = 0;
+    if (s->net) {
+        // The residual weights high frequencies (the Laplacian amplifies them), which a few
+        // sweeps remove, while the sweep count is set by the smooth error: both candidates
+        // get kJudge sweeps before their residuals are compared; all of them are counted.
+        constexpr int kJudge = 16;
+        TK_TRY(tk_unet_predict(s->net, s->rhs, s->guess));
+        TK_TRY(poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge, 0,
+                                     0, s->alt ? &s->alt : nullptr, nullptr, nullptr));
+        TK_TRY(poisson_dir_solve_alt(ctx, s->guess, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge,
+                                     0, 0, nullptr, nullptr, nullptr));
+        TK_TRY(tk_poisson_residual(ctx, s->phi, s->rhs, &res_plain));
+        TK_TRY(tk_poisson_residual(ctx, s->guess, s->rhs, &res_guess));
+        if (res_guess < res_plain) {
+            TK_TRY(tk_vol_copy(s->phi, s->guess));
+            used = 1;
+        }
+        judge = 2 * kJudge;
+    }
+    int r;
+    if (p.ls_res_tol > 0)
+        r = poisson_solve_residual_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order,
+                                       p.ls_res_tol, p.res_check_every > 0 ? p.res_check_every : 64,
+                                       p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps, &ls_err);
+    else
+        r = poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
                                   p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps,
                                   &ls_err);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
@@ -271,10 +299,12 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
     const double h3 = s->h[0] * s->h[1] * s->h[2];
     s->tollerance = std::sqrt(h3 * ctx->h_acc[0]) / std::sqrt(h3 * ctx->h_acc[1]);
     if (st) {
-        st->sweeps = sweeps;
+        st->sweeps = sweeps + judge;
         st->ls_err = ls_err;
         st->tollerance = s->tollerance;
-        st->guess_used = 0;
+        st->guess_used = used;
+        st->res_plain = res_plain;
+        st->res_guess = res_guess;
     }
     return TK_OK;
 }
@@ -304,6 +334,7 @@ void free_state(tk_rsfe s)
     }
     if (s->C) (void)hipFree(s->C);
     if (s->alt) (void)hipFree(s->alt);
+    if (s->guess) tk_vol_free(s->guess);
     if (s->tol_dev) (void)hipFree(s->tol_dev);
     if (s->flt) tk_filter_free(s->flt);
     delete s;
@@ -404,8 +435,7 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
              "tk_rsfe_create: init dtype");
     TK_CHECK(u0->nx >= 3 && u0->ny >= 3 && u0->nz >= 3, TK_ERR_UNSUPPORTED,
              "tk_rsfe_create: 3-D volumes with every edge >= 3 only");
-    TK_CHECK(p->warm_start == 0, TK_ERR_UNSUPPORTED,
-             "tk_rsfe_create: warm_start needs a trained network (tk_unet_*)");
+    TK_CHECK(p->warm_start == 0 || p->warm_start == 1, TK_ERR_ARG, "tk_rsfe_create: warm_start");
     TK_CHECK(p->gs_order == TK_GS_LEX || p->gs_order == TK_GS_REDBLACK, TK_ERR_ARG,
              "tk_rsfe_create: gs_order");
     tk_rsfe s = new tk_rsfe_s();
@@ -470,6 +500,8 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
 int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done)
 {
     TK_CHECK(s, TK_ERR_ARG, "tk_rsfe_iterate: null state");
+    TK_CHECK(s->p.warm_start == 0 || s->net, TK_ERR_ARG,
+             "tk_rsfe_iterate: warm_start set but no network attached (tk_rsfe_set_warm_start)");
     int done = 0;
     // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
     while (done < n_iters && s->tollerance > s->p.tol && s->iter < s->p.maxiter) {
@@ -511,6 +543,29 @@ void rsfe_set_iterations(tk_rsfe s, int iter) { s->iter = iter; }
 extern "C" {
 
 tk_vol tk_rsfe_phi(tk_rsfe s) { return s ? s->phi : nullptr; }
+tk_vol tk_rsfe_rhs(tk_rsfe s) { return s ? s->rhs : nullptr; }
+
+int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
+{
+    TK_CHECK(s, TK_ERR_ARG, "tk_rsfe_set_warm_start: null state");
+    if (!net) {
+        s->net = nullptr;
+        s->p.warm_start = 0;
+        return TK_OK;
+    }
+    TK_CHECK(unet_ctx(net) == s->ctx, TK_ERR_ARG,
+             "tk_rsfe_set_warm_start: the network must live on the run's context");
+    int d[3];
+    unet_dims(net, d);
+    TK_CHECK(d[0] == s->X && d[1] == s->Y && d[2] == s->Z, TK_ERR_SHAPE,
+             "tk_rsfe_set_warm_start: network dims differ from the run's");
+    if (!s->guess)
+        TK_TRY(tk_vol_alloc(s->ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], s->dtype,
+                            &s->guess));
+    s->net = net;
+    s->p.warm_start = 1;
+    return TK_OK;
+}
 tk_filter tk_rsfe_filter(tk_rsfe s) { return s ? s->flt : nullptr; }
 
 int tk_rsfe_destroy(tk_rsfe s)

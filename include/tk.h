@@ -41,6 +41,7 @@ typedef struct tk_ctx_s* tk_ctx;       /* one device + one HIP stream + scratch 
 typedef struct tk_vol_s* tk_vol;       /* a volume resident in device memory */
 typedef struct tk_filter_s* tk_filter; /* a Gaussian filter's cyclic spectrum */
 typedef struct tk_rsfe_s* tk_rsfe;     /* state of one Split-Bregman run */
+typedef struct tk_unet_s* tk_unet;     /* the learned Poisson warm start network */
 
 /* message of the last error on the calling thread ("" if none) */
 const char* tk_last_error(void);
@@ -50,6 +51,7 @@ const char* tk_version(void);
  * No reference counterpart (the reference is a host-only library).  One context
  * per host thread; all work of a context is ordered on its stream. */
 int tk_ctx_create(int device, tk_ctx* out);
+/* every volume, filter, run, network and batch created on ctx must be freed first */
 int tk_ctx_destroy(tk_ctx ctx);
 int tk_ctx_sync(tk_ctx ctx);
 /* run on a caller-owned hipStream_t (e.g. PyTorch's current stream); NULL restores
@@ -143,6 +145,11 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
    ||rhs - (phi/dt - lap_h phi) + phi/dt||_2 over the interior / ||rhs||_2, i.e.
    ||rhs + lap_h phi|| / ||rhs|| with FP64 accumulation */
 int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel_res);
+/* sweeps (in batches of check_every) until tk_poisson_residual <= res_tol, after setting
+   phi's faces to bc (no reference counterpart: the reference gates on the relative update) */
+int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,
+                              int order, double res_tol, int check_every, int max_sweeps,
+                              int* sweeps_out, double* res_out);
 /* autotuner of the red-black sweep (no reference counterpart: the reference has one fixed
    lexicographic loop): times the 8-row and 16-row tile variants on rhs's shape and type,
    keeps the faster for every later red-black sweep of that shape on ctx, returns the rows
@@ -200,7 +207,12 @@ typedef struct tk_rsfe_params {
     int gaussian_pixel_approach;
     int gs_order;             /* TK_GS_LEX / TK_GS_REDBLACK */
     int max_sweeps;           /* cap per Poisson solve when ls_tol > 0 */
-    int warm_start;           /* 0: previous iterate (reference); 1: learned guess (R15) */
+    int warm_start;           /* 0: previous iterate (reference); 1: learned guess (R15),
+                                 network attached with tk_rsfe_set_warm_start */
+    double ls_res_tol;        /* > 0: the Poisson step instead sweeps until the relative
+                                 residual ||rhs + lap phi|| / ||rhs|| <= ls_res_tol (no
+                                 reference counterpart; the learned-start evaluation) */
+    int res_check_every;      /* sweeps between residual checks (default 64) */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {
@@ -208,6 +220,8 @@ typedef struct tk_rsfe_iter_stats {
     double ls_err;            /* last relative update of the Poisson loop (-1 if not computed) */
     double tollerance;        /* ||phi - phiold||_L2 / ||phiold||_L2 (imp:1832) */
     int guess_used;           /* R15: 1 if the learned guess was accepted */
+    double res_plain;         /* R15: relative residual of the previous iterate after 16 sweeps */
+    double res_guess;         /* R15: relative residual of the guess after 16 sweeps (-1: none) */
 } tk_rsfe_iter_stats;
 
 int tk_rsfe_default_params(tk_rsfe_params* p);
@@ -221,9 +235,70 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
    stats may be NULL, else has room for n_iters entries */
 int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done);
 tk_vol tk_rsfe_phi(tk_rsfe s);
+/* the right-hand side of the last outer iteration's Poisson step (valid until the next) */
+tk_vol tk_rsfe_rhs(tk_rsfe s);
+/* R15: attach the learned warm start (created on the run's context for its dims).  Before
+   each Poisson step the network predicts a guess from rhs; the guess and the previous
+   iterate (the reference's start) each get 16 sweeps, and the one with the lower relative
+   residual continues (all 32 sweeps are counted in tk_rsfe_iter_stats.sweeps) */
+int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net);
 /* the run's filter (FP32 runs autotune its per-axis methods at creation) */
 tk_filter tk_rsfe_filter(tk_rsfe s);
 int tk_rsfe_destroy(tk_rsfe s);
+
+/* ---- R15 building blocks: BF16 3-D convolution network layers on the matrix cores ---
+ * No reference counterpart (the learned Poisson warm start is an addition).  Device
+ * pointers; activations are channels-last BF16 (uint16 bits): voxel v = (i*ny + j)*nz + k,
+ * channel c at v*stride + offset + c.  3x3x3 weights are BF16 [co][kpad], k = tap*ci + c,
+ * tap = (dx+1)*9 + (dy+1)*3 + (dz+1), zero-padded to kpad (multiple of 16, >= 27*ci).
+ * Zero padding outside the volume.  FP32 accumulation (v_mfma_f32_16x16x16_bf16). */
+int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const void* w, int kpad,
+                     const float* bias, void* y, int ys, int yo, int co, int nx, int ny, int nz,
+                     int relu, int out_f32);
+/* dw[co][kpad] = sum_v dy[v][o] x[v + tap(k)][c(k)] (FP32), db[co] = sum_v dy[v][o] (may be
+   NULL); part: FP32 scratch of ceil(nvox / vchunk) * co * kpad (vchunk multiple of 16);
+   deterministic (fixed-order chunk sums) */
+int tk_nn_conv3d_wgrad(tk_ctx ctx, const void* x, int xs, int xo, int ci, const void* dy, int ds,
+                       int dof, int co, int nx, int ny, int nz, int kpad, float* part,
+                       int vchunk, float* dw, float* db);
+/* weights of the input gradient: wt[c][kpad_t] = w flipped in space with ci and co exchanged,
+   so dx = tk_nn_conv3d_fwd(dy, wt) */
+int tk_nn_flip_weights(tk_ctx ctx, const void* w, int ci, int co, int kpad, void* wt, int kpad_t);
+int tk_nn_relu_bwd(tk_ctx ctx, void* g, int gs, int go, const void* y, int ys, int yo, int C,
+                   long n);
+int tk_nn_avgpool2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, int ny, int nz,
+                   void* y, int ys, int yo);
+int tk_nn_avgpool2_bwd(tk_ctx ctx, const void* dy, int ds, int dof, int C, int nx, int ny, int nz,
+                       void* dx, int xs, int xo, int accum);
+int tk_nn_upsample2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, int ny, int nz,
+                    void* y, int ys, int yo);
+int tk_nn_upsample2_bwd(tk_ctx ctx, const void* dy, int ds, int dof, int C, int nx, int ny,
+                        int nz, void* dx, int xs, int xo);
+int tk_nn_add_slice(tk_ctx ctx, void* a, int as, int ao, const void* b, int bs, int bo, int C,
+                    long n);
+int tk_nn_f32_to_bf16(tk_ctx ctx, const float* s, long n, void* d);
+int tk_nn_bf16_to_f32(tk_ctx ctx, const void* s, long n, float* d);
+
+/* ---- R15: the learned Poisson warm start (3-level 3-D U-Net, 16/32/64 channels) -------
+ * No reference counterpart.  Forward, backward and Adam steps on the device (tk_nn_*
+ * layers, BF16 weights and activations, FP32 accumulation and FP32 master weights).  The
+ * network maps rhs / rms(rhs) to phi* / (rms(rhs) * out_scale) for volumes of the dims it
+ * was created for (every edge a multiple of 4). */
+int tk_unet_create(tk_ctx ctx, int nx, int ny, int nz, uint64_t seed, tk_unet* out);
+int tk_unet_destroy(tk_unet net);
+int tk_unet_num_params(tk_unet net, long* count);
+int tk_unet_set_out_scale(tk_unet net, double s);
+int tk_unet_get_out_scale(tk_unet net, double* s);
+/* guess := network(rhs) in rhs's units (the Poisson initial guess) */
+int tk_unet_predict(tk_unet net, tk_vol rhs, tk_vol guess);
+/* one Adam step (lr > 0) on the pair (rhs, phi*); lr <= 0 computes the gradients only;
+   loss (may be NULL) = mean squared error in normalised units */
+int tk_unet_train_step(tk_unet net, tk_vol rhs, tk_vol target, double lr, double* loss);
+/* which 0: FP32 master parameters, 1: last gradients; per layer w[co][27*ci] then b[co] */
+int tk_unet_get_params(tk_unet net, int which, float* host, long count);
+int tk_unet_set_params(tk_unet net, const float* host, long count);
+/* the FP32 output of the last forward pass, normalised units */
+int tk_unet_get_output(tk_unet net, float* host);
 
 /* ---- batched uneven volumes (config: many independent runs of uneven sizes) --------
  * No reference counterpart (the reference's static filter state serves one size at a time,

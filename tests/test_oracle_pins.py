@@ -468,3 +468,34 @@ def test_init_phi_cases(orc):
         assert np.array_equal(phi, cube)
     phi, _ = orc.init_phi(None, (64, 64, 64))
     assert phi.sum() == 21 ** 3  # [21, 42) per axis
+
+
+def test_unet_oracle_gradient_by_finite_differences():
+    """the FP64 network oracle's backward pass against central differences of its loss"""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import unet_oracle as uo
+    rng = np.random.default_rng(3)
+    p = np.concatenate([np.concatenate([rng.standard_normal(co * 27 * ci) * np.sqrt(2 / (27 * ci)),
+                                        0.1 * rng.standard_normal(co)])
+                        for ci, co in uo.LAYERS])
+    assert p.size == uo.n_params()
+    inp = rng.standard_normal((8, 8, 8))
+    tgt = rng.standard_normal((8, 8, 8))
+    loss, g, _ = uo.loss_and_grad(p, inp, tgt)
+    # a few parameters of every layer, weights and biases
+    idx, o = [], 0
+    for ci, co in uo.LAYERS:
+        nw = co * 27 * ci
+        idx += [o + int(rng.integers(nw)), o + nw + int(rng.integers(co))]
+        o += nw + co
+    eps = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = eps
+        lp = uo.loss_and_grad(p + e, inp, tgt)[0]
+        lm = uo.loss_and_grad(p - e, inp, tgt)[0]
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
