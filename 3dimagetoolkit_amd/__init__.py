@@ -44,7 +44,8 @@ class RsfeParams(C.Structure):
         "tol", "ls_tol", "radius")] + [(n, C.c_int) for n in (
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "gs_order",
             "max_sweeps", "warm_start")] + [("ls_res_tol", C.c_double),
-                                            ("res_check_every", C.c_int)]
+                                            ("res_check_every", C.c_int),
+                                            ("ws_presweeps", C.c_int)]
 
 
 class IterStats(C.Structure):
@@ -93,6 +94,7 @@ def _load():
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
         "tk_poisson_rb_autotune": (i, [vp, vp, ip, dp]),
         "tk_poisson_solve_residual": (i, [vp, vp, vp, d, d, i, d, i, i, ip, dp]),
+        "tk_poisson_residual_field": (i, [vp, vp, vp, vp]),
         "tk_rsfe_rhs": (vp, [vp]),
         "tk_rsfe_set_warm_start": (i, [vp, vp]),
         "tk_poisson_rb_set_tile": (i, [vp, vp, i]),

@@ -825,6 +825,30 @@ __global__ __launch_bounds__(256) void k_residual(const T* __restrict__ phi, con
     }
 }
 
+// r = rhs + lap_h phi on the interior, 0 on the faces (the Dirichlet problem's residual)
+template <class T>
+__global__ void k_residual_field(const T* __restrict__ phi, const T* __restrict__ rhs, int X,
+                                 int Y, int Z, double ihx2, double ihy2, double ihz2,
+                                 T* __restrict__ out)
+{
+    const long n = (long)X * Y * Z, sY = Z, sX = (long)Y * Z;
+    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < n;
+         v += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(v % Z);
+        const long rr = v / Z;
+        const int j = (int)(rr % Y), i = (int)(rr / Y);
+        T r = 0;
+        if (i > 0 && i < X - 1 && j > 0 && j < Y - 1 && k > 0 && k < Z - 1) {
+            const double c = phi[v];
+            const double lap = ((double)phi[v + sX] + (double)phi[v - sX] - 2 * c) * ihx2 +
+                               ((double)phi[v + sY] + (double)phi[v - sY] - 2 * c) * ihy2 +
+                               ((double)phi[v + 1] + (double)phi[v - 1] - 2 * c) * ihz2;
+            r = (T)((double)rhs[v] + lap);
+        }
+        out[v] = r;
+    }
+}
+
 __global__ void k_sum_pairs_gs(const double* __restrict__ part, int nb, double* __restrict__ out)
 {
     double s1 = 0, s2 = 0;
@@ -934,6 +958,27 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
                              err_out);
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
                         err_out);
+}
+
+int tk_poisson_residual_field(tk_ctx ctx, tk_vol phi, tk_vol rhs, tk_vol out)
+{
+    TK_CHECK(ctx && out && out->d, TK_ERR_ARG, "tk_poisson_residual_field: argument");
+    TK_TRY(check_gs(phi, rhs));
+    TK_CHECK(same_dims(out, phi) && out->dtype == phi->dtype, TK_ERR_SHAPE,
+             "tk_poisson_residual_field: out must match phi");
+    const unsigned nb = blocks_for(phi->n(), 256, 8192);
+    const double ihx2 = 1. / (phi->hx * phi->hx), ihy2 = 1. / (phi->hy * phi->hy),
+                 ihz2 = 1. / (phi->hz * phi->hz);
+    if (phi->dtype == TK_F64)
+        hipLaunchKernelGGL(k_residual_field<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)phi->d, (const double*)rhs->d, phi->nx, phi->ny, phi->nz,
+                           ihx2, ihy2, ihz2, (double*)out->d);
+    else
+        hipLaunchKernelGGL(k_residual_field<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)phi->d, (const float*)rhs->d, phi->nx, phi->ny, phi->nz,
+                           ihx2, ihy2, ihz2, (float*)out->d);
+    TK_LAUNCHED();
+    return TK_OK;
 }
 
 int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,

@@ -39,7 +39,9 @@ struct Tap {
 __device__ __forceinline__ Tap tap_of(int t) { return {t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1}; }
 
 // forward: y[v][yo + o] = act(b[o] + sum_k w[o][k] x[v + tap(k)][xo + c(k)])
-template <bool RELU, bool OUT_F32>
+// CI16: ci % 16 == 0 and xs, xo % 4 == 0, so a 16-wide k chunk lies inside one tap and a
+// lane's four k values are four consecutive channels: one 8-byte load per operand.
+template <bool RELU, bool OUT_F32, bool CI16>
 __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ x, int xs, int xo,
                                                    int ci, const uint16_t* __restrict__ w,
                                                    int kpad, const float* __restrict__ bias,
@@ -62,23 +64,40 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
     const int ob = ct * 16 + m;      // B column: this lane's output channel
     const int K = 27 * ci;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < kpad; kc += 16) {
-        v4s a, b;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int k = kc + kq + q;
-            uint16_t av = 0;
-            if (k < K && za < Z) {
-                const int tp = k / ci, c = k - tp * ci;
-                const Tap d = tap_of(tp);
-                const int ii = i + d.dx, jj = j + d.dy, zz = za + d.dz;
-                if (ii >= 0 && ii < X && jj >= 0 && jj < Y && zz >= 0 && zz < Z)
-                    av = x[(((long)ii * Y + jj) * Z + zz) * xs + xo + c];
+    if constexpr (CI16) {
+        // K = 27 ci is a multiple of 16 here, so kpad == K
+        const uint16_t* wrow = w + (long)(ob < co ? ob : 0) * kpad + kq;
+        for (int tp = 0; tp < 27; ++tp) {
+            const Tap d = tap_of(tp);
+            const int ii = i + d.dx, jj = j + d.dy, zz = za + d.dz;
+            const bool in = za < Z && ii >= 0 && ii < X && jj >= 0 && jj < Y && zz >= 0 && zz < Z;
+            const uint16_t* xrow = in ? x + (((long)ii * Y + jj) * Z + zz) * xs + xo + kq : x;
+            for (int c0 = 0; c0 < ci; c0 += 16) {
+                v4s a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+                if (in) a = *(const v4s*)(xrow + c0);
+                if (ob < co) b = *(const v4s*)(wrow + tp * ci + c0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
             }
-            a[q] = (short)av;
-            b[q] = (short)(ob < co ? w[(long)ob * kpad + k] : (uint16_t)0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    } else {
+        for (int kc = 0; kc < kpad; kc += 16) {
+            v4s a, b;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = kc + kq + q;
+                uint16_t av = 0;
+                if (k < K && za < Z) {
+                    const int tp = k / ci, c = k - tp * ci;
+                    const Tap d = tap_of(tp);
+                    const int ii = i + d.dx, jj = j + d.dy, zz = za + d.dz;
+                    if (ii >= 0 && ii < X && jj >= 0 && jj < Y && zz >= 0 && zz < Z)
+                        av = x[(((long)ii * Y + jj) * Z + zz) * xs + xo + c];
+                }
+                a[q] = (short)av;
+                b[q] = (short)(ob < co ? w[(long)ob * kpad + k] : (uint16_t)0);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
     }
     const int od = ct * 16 + (lane & 15);
     if (od >= co) return;
@@ -332,13 +351,22 @@ int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const vo
     const unsigned nb = (unsigned)((tiles + 3) / 4);
     const uint16_t* xp = (const uint16_t*)x;
     const uint16_t* wp = (const uint16_t*)w;
-#define TK_FWD(R, F)                                                                           \
-    hipLaunchKernelGGL((k_conv3_fwd<R, F>), dim3(nb), dim3(256), 0, ctx->stream, xp, xs, xo, ci, \
-                       wp, kpad, bias, y, ys, yo, co, nx, ny, nz)
-    if (relu && out_f32) TK_FWD(true, true);
-    else if (relu) TK_FWD(true, false);
-    else if (out_f32) TK_FWD(false, true);
-    else TK_FWD(false, false);
+    const bool ci16 = ci % 16 == 0 && xs % 4 == 0 && xo % 4 == 0 && kpad == 27 * ci &&
+                      (uintptr_t)x % 8 == 0 && (uintptr_t)w % 8 == 0;
+#define TK_FWD(R, F, V)                                                                        \
+    hipLaunchKernelGGL((k_conv3_fwd<R, F, V>), dim3(nb), dim3(256), 0, ctx->stream, xp, xs, xo,  \
+                       ci, wp, kpad, bias, y, ys, yo, co, nx, ny, nz)
+    if (ci16) {
+        if (relu && out_f32) TK_FWD(true, true, true);
+        else if (relu) TK_FWD(true, false, true);
+        else if (out_f32) TK_FWD(false, true, true);
+        else TK_FWD(false, false, true);
+    } else {
+        if (relu && out_f32) TK_FWD(true, true, false);
+        else if (relu) TK_FWD(true, false, false);
+        else if (out_f32) TK_FWD(false, true, false);
+        else TK_FWD(false, false, false);
+    }
 #undef TK_FWD
     TK_LAUNCHED();
     return TK_OK;

@@ -167,6 +167,12 @@ __global__ void k_init_cube(T* __restrict__ phi, int X, int Y, int Z, int ix, in
     }
 }
 
+template <class T>
+__global__ void k_add_inplace(T* __restrict__ a, const T* __restrict__ b, long n)
+{
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) a[v] += b[v];
+}
+
 __global__ void k_sum_pairs_rsfe(const double* __restrict__ part, int nb, double* __restrict__ out)
 {
     double s1 = 0, s2 = 0;
@@ -199,12 +205,24 @@ struct tk_rsfe_s {
     void* alt = nullptr;       // red-black ping-pong partner of phi (this run's own)
     tk_unet net = nullptr;     // learned warm start (same context), or none
     tk_vol guess = nullptr;    // its guess
+    tk_vol resid = nullptr;    // the residual field it reads
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
     int iter = 0;
     double tollerance = 1000;
 };
 
 namespace {
+
+// a += b (same dims and type)
+template <class T>
+int launch_add(tk_ctx ctx, tk_vol a, tk_vol b)
+{
+    const long n = a->n();
+    hipLaunchKernelGGL(k_add_inplace<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream,
+                       (T*)a->d, (const T*)b->d, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
 
 // One outer iteration.  sync: read the tollerance back and wait (the reference's loop
 // condition needs it); without sync nothing waits on the host and nothing but kernels is
@@ -253,18 +271,36 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         // sweeps remove, while the sweep count is set by the smooth error: both candidates
         // get kJudge sweeps before their residuals are compared; all of them are counted.
         constexpr int kJudge = 16;
-        TK_TRY(tk_unet_predict(s->net, s->rhs, s->guess));
-        TK_TRY(poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge, 0,
-                                     0, s->alt ? &s->alt : nullptr, nullptr, nullptr));
-        TK_TRY(poisson_dir_solve_alt(ctx, s->guess, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge,
-                                     0, 0, nullptr, nullptr, nullptr));
-        TK_TRY(tk_poisson_residual(ctx, s->phi, s->rhs, &res_plain));
-        TK_TRY(tk_poisson_residual(ctx, s->guess, s->rhs, &res_guess));
-        if (res_guess < res_plain) {
+        const int pre = p.ws_presweeps > 0 ? p.ws_presweeps : 0;
+        for (int round = 0; round < p.warm_start; ++round) {
+            if (pre > 0) {
+                TK_TRY(poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order,
+                                             pre, 0, 0, s->alt ? &s->alt : nullptr, nullptr,
+                                             nullptr));
+                judge += pre;
+            }
+            TK_TRY(poisson_set_boundary(ctx, s->phi, chosen_bc));
+            TK_TRY(tk_poisson_residual_field(ctx, s->phi, s->rhs, s->resid));
+            TK_TRY(tk_unet_predict(s->net, s->resid, s->guess));
+            TK_TRY(launch_add<T>(ctx, s->guess, s->phi));
+            TK_TRY(poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge,
+                                         0, 0, s->alt ? &s->alt : nullptr, nullptr, nullptr));
+            TK_TRY(poisson_dir_solve_alt(ctx, s->guess, s->rhs, p.dt, chosen_bc, p.gs_order,
+                                         kJudge, 0, 0, nullptr, nullptr, nullptr));
+            double rp = 0, rg = 0;
+            TK_TRY(tk_poisson_residual(ctx, s->phi, s->rhs, &rp));
+            TK_TRY(tk_poisson_residual(ctx, s->guess, s->rhs, &rg));
+            if (round == 0) {
+                res_plain = rp;
+                res_guess = rg;
+            }
+            judge += 2 * kJudge;
+            if (!(rg < rp)) break;
             TK_TRY(tk_vol_copy(s->phi, s->guess));
-            used = 1;
+            ++used;
+            // the ls_res_tol gate may already hold
+            if (p.ls_res_tol > 0 && rg <= p.ls_res_tol) break;
         }
-        judge = 2 * kJudge;
     }
     int r;
     if (p.ls_res_tol > 0)
@@ -335,6 +371,7 @@ void free_state(tk_rsfe s)
     if (s->C) (void)hipFree(s->C);
     if (s->alt) (void)hipFree(s->alt);
     if (s->guess) tk_vol_free(s->guess);
+    if (s->resid) tk_vol_free(s->resid);
     if (s->tol_dev) (void)hipFree(s->tol_dev);
     if (s->flt) tk_filter_free(s->flt);
     delete s;
@@ -424,6 +461,9 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     p->gs_order = TK_GS_LEX;
     p->max_sweeps = 0;
     p->warm_start = 0;
+    p->ls_res_tol = 0;
+    p->res_check_every = 64;
+    p->ws_presweeps = 512;
     return TK_OK;
 }
 
@@ -435,7 +475,7 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
              "tk_rsfe_create: init dtype");
     TK_CHECK(u0->nx >= 3 && u0->ny >= 3 && u0->nz >= 3, TK_ERR_UNSUPPORTED,
              "tk_rsfe_create: 3-D volumes with every edge >= 3 only");
-    TK_CHECK(p->warm_start == 0 || p->warm_start == 1, TK_ERR_ARG, "tk_rsfe_create: warm_start");
+    TK_CHECK(p->warm_start >= 0 && p->warm_start <= 64, TK_ERR_ARG, "tk_rsfe_create: warm_start");
     TK_CHECK(p->gs_order == TK_GS_LEX || p->gs_order == TK_GS_REDBLACK, TK_ERR_ARG,
              "tk_rsfe_create: gs_order");
     tk_rsfe s = new tk_rsfe_s();
@@ -562,8 +602,11 @@ int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
     if (!s->guess)
         TK_TRY(tk_vol_alloc(s->ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], s->dtype,
                             &s->guess));
+    if (!s->resid)
+        TK_TRY(tk_vol_alloc(s->ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], s->dtype,
+                            &s->resid));
     s->net = net;
-    s->p.warm_start = 1;
+    if (s->p.warm_start < 1) s->p.warm_start = 1;
     return TK_OK;
 }
 tk_filter tk_rsfe_filter(tk_rsfe s) { return s ? s->flt : nullptr; }

@@ -145,6 +145,8 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
    ||rhs - (phi/dt - lap_h phi) + phi/dt||_2 over the interior / ||rhs||_2, i.e.
    ||rhs + lap_h phi|| / ||rhs|| with FP64 accumulation */
 int tk_poisson_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double* rel_res);
+/* out = rhs + lap_h phi on the interior, 0 on the faces (no reference counterpart) */
+int tk_poisson_residual_field(tk_ctx ctx, tk_vol phi, tk_vol rhs, tk_vol out);
 /* sweeps (in batches of check_every) until tk_poisson_residual <= res_tol, after setting
    phi's faces to bc (no reference counterpart: the reference gates on the relative update) */
 int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc,
@@ -207,21 +209,23 @@ typedef struct tk_rsfe_params {
     int gaussian_pixel_approach;
     int gs_order;             /* TK_GS_LEX / TK_GS_REDBLACK */
     int max_sweeps;           /* cap per Poisson solve when ls_tol > 0 */
-    int warm_start;           /* 0: previous iterate (reference); 1: learned guess (R15),
-                                 network attached with tk_rsfe_set_warm_start */
+    int warm_start;           /* 0: previous iterate (reference); R >= 1: up to R learned
+                                 correction rounds (R15), network attached with
+                                 tk_rsfe_set_warm_start */
     double ls_res_tol;        /* > 0: the Poisson step instead sweeps until the relative
                                  residual ||rhs + lap phi|| / ||rhs|| <= ls_res_tol (no
                                  reference counterpart; the learned-start evaluation) */
     int res_check_every;      /* sweeps between residual checks (default 64) */
+    int ws_presweeps;         /* R15: sweeps before each learned round (default 512) */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {
     int sweeps;               /* S_t: Gauss-Seidel sweeps in this outer iteration */
     double ls_err;            /* last relative update of the Poisson loop (-1 if not computed) */
     double tollerance;        /* ||phi - phiold||_L2 / ||phiold||_L2 (imp:1832) */
-    int guess_used;           /* R15: 1 if the learned guess was accepted */
-    double res_plain;         /* R15: relative residual of the previous iterate after 16 sweeps */
-    double res_guess;         /* R15: relative residual of the guess after 16 sweeps (-1: none) */
+    int guess_used;           /* R15: learned rounds accepted */
+    double res_plain;         /* R15: first round: residual of the previous iterate after 16 sweeps */
+    double res_guess;         /* R15: first round: residual of the candidate after 16 sweeps */
 } tk_rsfe_iter_stats;
 
 int tk_rsfe_default_params(tk_rsfe_params* p);
@@ -237,10 +241,14 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
 tk_vol tk_rsfe_phi(tk_rsfe s);
 /* the right-hand side of the last outer iteration's Poisson step (valid until the next) */
 tk_vol tk_rsfe_rhs(tk_rsfe s);
-/* R15: attach the learned warm start (created on the run's context for its dims).  Before
-   each Poisson step the network predicts a guess from rhs; the guess and the previous
-   iterate (the reference's start) each get 16 sweeps, and the one with the lower relative
-   residual continues (all 32 sweeps are counted in tk_rsfe_iter_stats.sweeps) */
+/* R15: attach the learned warm start (created on the run's context for its dims).  The
+   Poisson problem is linear, so the network that maps a right-hand side to its solution
+   also maps the current residual r = rhs + lap phi to the correction.  The network is
+   accurate on smooth residuals (those the sweeps leave) and poor on rough ones, so each
+   round first runs ws_presweeps sweeps, then candidate = phi + network(r); candidate and phi
+   each get 16 sweeps and the one with the lower relative residual continues; a round whose
+   candidate loses ends the rounds, then the Poisson step continues as configured.  All
+   sweeps are counted in tk_rsfe_iter_stats.sweeps. */
 int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net);
 /* the run's filter (FP32 runs autotune its per-axis methods at creation) */
 tk_filter tk_rsfe_filter(tk_rsfe s);
