@@ -361,9 +361,9 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     constexpr int OJ = TJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
     constexpr int RJ = TJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
     constexpr int NO = (OJ * WV + 255) / 256, NB = (RJ * WV + 255) / 256;
-    __shared__ T O[4][OJ * W];
-    __shared__ T Rp[3][RJ * W];
-    __shared__ T Bp[3][RJ * W];
+    __shared__ __align__(16) T O[4][OJ * W];
+    __shared__ __align__(16) T Rp[3][RJ * W];
+    __shared__ __align__(16) T Bp[3][RJ * W];
     const uint3 bl = xcd_block3();  // tiles sharing halo rows run on one XCD
     const int kb = bl.x * kRbTK, kbase = kb - 4;
     const int j0 = 1 + bl.y * TJ;
@@ -389,12 +389,13 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
 #pragma unroll
         for (int q = 0; q < n; ++q) {
             const int e = tid + q * 256;
-            if (e < rows * WV) {
-#pragma unroll
-                for (int u = 0; u < V; ++u) lds[e * V + u] = in[q].v[u];
-            }
+            if (e < rows * WV) strow(lds + e * V, in[q]);
         }
     };
+    // Each lane works on one 16-byte row of V voxels: the centre row, its four x/y
+    // neighbour rows and the rhs row are whole-row LDS reads (conflict-free), the two z
+    // neighbours outside the row are scalar reads; every voxel's update is computed and
+    // the colour selects it.
     auto red = [&](int p) {
         T* r = Rp[p % 3];
         const T* o = O[p & 3];
@@ -406,17 +407,27 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
             const int rr = e / WV, c4 = e - rr * WV;
             const int j = j0 - 1 + rr;
             const bool jin = inner_p && j >= 1 && j <= Y - 2;
+            const int oi = (rr + 1) * W + c4 * V;
+            const Row<T> ctr = ldrow(o + oi);
+            Row<T> out = ctr;
+            if (jin) {
+                const Row<T> xp = ldrow(op + oi), xm = ldrow(om + oi);
+                const Row<T> yp = ldrow(o + oi + W), ym = ldrow(o + oi - W);
+                const Row<T> bq = ldrow(b + rr * W + c4 * V);
+                const T zl = o[oi - 1], zr = o[oi + V];
 #pragma unroll
-            for (int u = 0; u < V; ++u) {
-                const int col = c4 * V + u, k = kbase + col;
-                const int oi = (rr + 1) * W + col;
-                T v = o[oi];
-                if (jin && col >= 3 && col <= kRbTK + 4 && k >= 1 && k <= Z - 2 &&
-                    ((p + j + k) & 1) == 0)
-                    v = gs_update<T, UNIT>(op[oi], om[oi], o[oi + W], o[oi - W], o[oi + 1], o[oi - 1],
-                                           b[rr * W + col], v, c);
-                r[rr * W + col] = v;
+                for (int u = 0; u < V; ++u) {
+                    const int col = c4 * V + u, k = kbase + col;
+                    const T zm = u > 0 ? ctr.v[u - 1] : zl;
+                    const T zp = u < V - 1 ? ctr.v[u + 1] : zr;
+                    const T v = gs_update<T, UNIT>(xp.v[u], xm.v[u], yp.v[u], ym.v[u], zp, zm,
+                                                   bq.v[u], ctr.v[u], c);
+                    if (col >= 3 && col <= kRbTK + 4 && k >= 1 && k <= Z - 2 &&
+                        ((p + j + k) & 1) == 0)
+                        out.v[u] = v;
+                }
             }
+            strow(r + rr * W + c4 * V, out);
         }
     };
     double d2 = 0, o2 = 0;
@@ -431,22 +442,32 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
             const int jj = e / TV, c4 = e - jj * TV;
             const int j = j0 + jj, k0 = kb + c4 * V;
             if (j > Y - 2 || k0 >= Z) continue;
-            Row<T> out;
+            const int ri = (jj + 1) * W + 4 + c4 * V;
+            const Row<T> ctr = ldrow(r + ri);
+            const Row<T> xp = ldrow(rp + ri), xm = ldrow(rm + ri);
+            const Row<T> yp = ldrow(r + ri + W), ym = ldrow(r + ri - W);
+            const Row<T> bq = ldrow(b + ri);
+            const T zl = r[ri - 1], zr = r[ri + V];
+            Row<T> out = ctr;
 #pragma unroll
             for (int u = 0; u < V; ++u) {
-                const int col = 4 + c4 * V + u, k = k0 + u;
-                const int ri = (jj + 1) * W + col;
-                T v = r[ri];
-                const bool kin = k >= 1 && k <= Z - 2;
-                if (kin && ((p + j + k) & 1) == 1)
-                    v = gs_update<T, UNIT>(rp[ri], rm[ri], r[ri + W], r[ri - W], r[ri + 1], r[ri - 1],
-                                           b[ri], v, c);
-                out.v[u] = v;
-                if (gated && kin) {
-                    const double old = (double)o[(jj + 2) * W + col];
-                    const double d = (double)v - old;
-                    d2 += d * d;
-                    o2 += old * old;
+                const int k = k0 + u;
+                const T zm = u > 0 ? ctr.v[u - 1] : zl;
+                const T zp = u < V - 1 ? ctr.v[u + 1] : zr;
+                const T v = gs_update<T, UNIT>(xp.v[u], xm.v[u], yp.v[u], ym.v[u], zp, zm,
+                                               bq.v[u], ctr.v[u], c);
+                if (k >= 1 && k <= Z - 2 && ((p + j + k) & 1) == 1) out.v[u] = v;
+            }
+            if (gated) {
+                const Row<T> old = ldrow(o + (jj + 2) * W + 4 + c4 * V);
+#pragma unroll
+                for (int u = 0; u < V; ++u) {
+                    const int k = k0 + u;
+                    if (k >= 1 && k <= Z - 2) {
+                        const double d = (double)out.v[u] - (double)old.v[u];
+                        d2 += d * d;
+                        o2 += (double)old.v[u] * (double)old.v[u];
+                    }
                 }
             }
             strow(dst + (long)p * sX + (long)j * Z + k0, out);
