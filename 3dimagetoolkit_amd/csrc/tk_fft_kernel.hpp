@@ -282,6 +282,11 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
 #pragma unroll
     for (int n = 0; n <= S; ++n) w[n] = wh[n];
     const int items = y.P * (L / B);
+    // z pass (pencils SQ apart, positions adjacent): a wave's 16- or 8-pencil lane groups
+    // take blocks 4 or 2 apart inside each run of 8 blocks, so their reads fall 32 or 16
+    // dwords apart in the LDS banks (a bijection on the blocks when L/B is a multiple of 8)
+    const int G = 32 >> y.lp;
+    const bool swz = y.SP == 1 && (G == 2 || G == 4) && (L / B) % 8 == 0;
     cplx<T> out[MAXI][B];
     int dst[MAXI];
 #pragma unroll
@@ -289,7 +294,8 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
         const int it = threadIdx.x + q * NT;
         dst[q] = -1;
         if (it < items) {
-            const int p = it & (y.P - 1), b = it >> y.lp;
+            const int p = it & (y.P - 1), br = it >> y.lp;
+            const int b = swz ? (br % G) * (8 / G) + (br / G) % (8 / G) + (br & ~7) : br;
             const int base = y.off + (b * B - S) * y.SP + p * y.SQ;
             cplx<T> x[W];
 #pragma unroll
@@ -511,10 +517,22 @@ __device__ __forceinline__ void for_quads(const AxisGeom& g, const Lay& y, long 
     }
 }
 
+// LDS stride between positions of the x/y passes (pencils adjacent).  The direct
+// window's reads are conflict-free in the 64 LDS banks when consecutive 4-position blocks
+// of a 16-pencil group start 32 dwords apart (P + 4) and of an 8-pencil group 16 dwords
+// apart (P + 2); the FFT path keeps P + 1.
+inline __host__ __device__ int conv_sp(int axis, int P, int S)
+{
+    if (axis == 2) return 1;
+    if (S > 0 && P == 16) return P + 4;
+    if (S > 0 && P == 8) return P + 2;
+    return P + 1;
+}
+
 // LDS elements of a pencil group; the direct path (S > 0) adds S halo positions per side
 inline __host__ __device__ int conv_nbuf(int axis, int L, int P, int S)
 {
-    return axis == 2 ? P * (L + 2 * S + 1) : (L + 2 * S) * (P + 1);
+    return axis == 2 ? P * (L + 2 * S + 1) : (L + 2 * S) * conv_sp(axis, P, S);
 }
 
 // per-axis convolution pass: S == 0 FFT (or the generic direct fallback), S > 0 the
@@ -528,7 +546,7 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     y.L = ax.L;
     y.P = g.P;
     y.lp = g.logP;
-    y.SP = g.axis == 2 ? 1 : g.P + 1;
+    y.SP = conv_sp(g.axis, g.P, S);
     y.SQ = g.axis == 2 ? ax.L + 2 * S + 1 : 1;
     y.off = S * y.SP;
     const int nbuf = conv_nbuf(g.axis, ax.L, g.P, S);
