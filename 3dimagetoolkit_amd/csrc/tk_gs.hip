@@ -665,9 +665,11 @@ static int set_boundary(tk_ctx ctx, tk_vol phi, double bc, double* acc)
 
 // sweep loop of rsfe_splitbregman_imp.hxx:1762-1797
 template <class T>
+// faces_set: phi's (and *alt's) six faces already hold bc and no sweep, cut or copy since
+// has written them, so the fixed-sweep paths skip setting them again
 static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order, int ls_steps,
                  double ls_tol, int max_sweeps, int* sweeps_out, double* err_out,
-                 void** alt = nullptr)
+                 void** alt = nullptr, bool faces_set = false)
 {
     const GsCoef c = gs_coef(phi, dt);
     const int X = phi->nx, Y = phi->ny, Z = phi->nz;
@@ -683,13 +685,13 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
         tk_vol_s tmp = *phi;
         tmp.d = other;
         tmp.owned = false;
-        TK_TRY(set_boundary<T>(ctx, &tmp, bc, nullptr));
+        if (!(faces_set && ls_tol == 0)) TK_TRY(set_boundary<T>(ctx, &tmp, bc, nullptr));
         void* buf[2] = {phi->d, other};
         int sweeps = 0;
         double err = -1;
         int r = TK_OK;
         if (ls_tol == 0) {
-            TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+            if (!faces_set) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
             RbGate g{};
             for (int q = 0; q < ls_steps; ++q)
                 TK_TRY(rb_sweep<T>(ctx, buf[q & 1], buf[(q + 1) & 1], rhs, c, q, 0, 0, g));
@@ -758,7 +760,7 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
     }
     if (ls_tol == 0) {
         // exactly ls_steps sweeps, no norms (the reference computes none either)
-        TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+        if (!faces_set) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
         TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr));
         if (sweeps_out) *sweeps_out = ls_steps;
         if (err_out) *err_out = -1;
@@ -940,14 +942,14 @@ int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc)
 
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, void** alt,
-                          int* sweeps_out, double* err_out)
+                          int* sweeps_out, double* err_out, bool faces_set)
 {
     TK_TRY(check_gs(phi, rhs));
     if (phi->dtype == TK_F64)
         return solve<double>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps,
-                             sweeps_out, err_out, alt);
+                             sweeps_out, err_out, alt, faces_set);
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
-                        err_out, alt);
+                        err_out, alt, faces_set);
 }
 }  // namespace tk
 

@@ -120,9 +120,10 @@ int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc);
 
 // tk_poisson_dir_solve with a caller-owned red-black ping-pong buffer *alt of phi's size
 // (swapped with phi->d when the sweep count is odd); alt == nullptr uses the context's slot
+// faces_set: the faces of phi and *alt already hold bc (the fixed-sweep paths skip them)
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, void** alt,
-                          int* sweeps_out, double* err_out);
+                          int* sweeps_out, double* err_out, bool faces_set = false);
 
 inline bool same_dims(const tk_vol a, const tk_vol b)
 {

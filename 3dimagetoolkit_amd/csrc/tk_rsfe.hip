@@ -209,6 +209,10 @@ struct tk_rsfe_s {
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
     int iter = 0;
     double tollerance = 1000;
+    // phi's and alt's faces hold the Poisson boundary value: set by the first solve; the
+    // sweeps never write the i and j faces, copy the k faces, and cut_phi keeps a value in
+    // [a0, b0] (bc is 0 or a0), so they stay set for the run
+    bool faces_set = false;
 };
 
 namespace {
@@ -310,8 +314,9 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
     else
         r = poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
                                   p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps,
-                                  &ls_err);
+                                  &ls_err, s->faces_set);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
+    if (chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
     // cut + norms + grad/shrink/Bregman, then div
     int nb = 0;
     double* part = ctx->acc + 64;
