@@ -46,13 +46,13 @@ BYTES_PER_VOXEL_W = {
 # WRITE x1.0); keyed by (size, dtype, sweep order, kernel tag).
 TRAFFIC_SOURCE = "profiles/rsfe256_f32_traffic.md"
 _TRAFFIC_256_F32_RB = {
-    "conv_x1": 335834595,
-    "conv_z2": 402997128,
-    "conv_x2": 404011419,
-    "conv_y": 268616707,
-    "gs": 225215412,
-    "sb": 942136823,
-    "div": 322915812,
+    "conv_x1": 335793751,
+    "conv_z2": 402997362,
+    "conv_x2": 403975106,
+    "conv_y": 268611545,
+    "gs": 225204009,
+    "sb": 942585162,
+    "div": 323881195,
 }
 MEASURED_TRAFFIC = {(256, "f32", "rb", k): v for k, v in _TRAFFIC_256_F32_RB.items()}
 
@@ -124,6 +124,39 @@ def bench_batch(tk, a):
     print(json.dumps(out), flush=True)
 
 
+def bench_warmstart(tk, a):
+    """64^3 ellipsoid phantoms (seeds 50200 + i, unseen in training), FP64 Split-Bregman with
+    red-black sweeps and every Poisson step gated on a relative residual of 1e-6, 15 outer
+    iterations, once from the previous iterate (the reference's start) and once with the
+    shipped network's correction rounds; a step is one seed.  value = plain time / learned
+    time over all steps (higher is better); the sweep counts are in config."""
+    ws = importlib.import_module("3dimagetoolkit_amd.warmstart")
+    ctx = tk.Context(0)
+    net = ws.load(ctx)
+    if a.warmup:
+        ws.evaluate(ctx, net, 1, seed0=50190)
+    rows = ws.evaluate(ctx, net, a.steps, seed0=50200)
+    net.close()
+    sp = sum(r["sweeps_plain"] for r in rows)
+    sl = sum(r["sweeps_learned"] for r in rows)
+    tp = sum(r["s_plain"] for r in rows)
+    tl = sum(r["s_learned"] for r in rows)
+    dices = [r["dice"] for r in rows]
+    out = {"metric": "time to converged Poisson solves, plain start / learned start",
+           "value": tp / tl, "unit": "x", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * tl / a.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 solver, bf16 network with fp32 accumulation",
+           "data": "synthetic (seeded ellipsoid phantoms, rescaled to [0,255])",
+           "config": {"workload": "rsfe_splitbregman 64^3 f64, sigma 3, red-black sweeps to "
+                                  "relative residual 1e-6, 15 outer iterations per seed, "
+                                  "learned rounds after 512 sweeps each (at most 8)",
+                      "sweeps_plain": sp, "sweeps_learned": sl, "sweep_ratio": sl / sp,
+                      "seconds_plain": tp, "seconds_learned": tl,
+                      "dice_median": float(np.median(dices)), "dice_min": float(min(dices))},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,8 +171,9 @@ def main():
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
-    ap.add_argument("--workload", choices=["c1", "batch"], default="c1",
-                    help="c1: one 256^3 volume (default); batch: 64 uneven volumes")
+    ap.add_argument("--workload", choices=["c1", "batch", "warmstart"], default="c1",
+                    help="c1: one 256^3 volume (default); batch: 64 uneven volumes; "
+                         "warmstart: learned vs plain Poisson start (64^3, residual 1e-6)")
     ap.add_argument("--no-graph", action="store_true", help="batch: eager launches")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 512 MiB volume copy (k_copy16): the known-size kernel "
@@ -156,6 +190,9 @@ def main():
         return
     if a.workload == "batch":
         bench_batch(tk, a)
+        return
+    if a.workload == "warmstart":
+        bench_warmstart(tk, a)
         return
     n = a.size
     shape = (n, n, n)
