@@ -326,7 +326,8 @@ class Filter:
         return out
 
     def methods(self):
-        """per-axis method: 0 FFT pass, 1 direct circular convolution"""
+        """per-axis method: 0 FFT pass, 1 direct circular convolution (vector ALUs),
+        2 direct circular convolution on the matrix cores"""
         out = []
         for a in range(3):
             m = C.c_int()
@@ -338,9 +339,9 @@ class Filter:
         _check(lib.tk_filter_set_method(self.h, axis, method))
 
     def autotune(self):
-        ms = (C.c_double * 6)()
+        ms = (C.c_double * 9)()
         _check(lib.tk_filter_autotune(self.ctx.h, self.h, ms))
-        return [(ms[2 * a], ms[2 * a + 1]) for a in range(3)]
+        return [(ms[3 * a], ms[3 * a + 1], ms[3 * a + 2]) for a in range(3)]
 
     def conv(self, out, inp):
         _check(lib.tk_conv_cyclic(self.ctx.h, self.h, out.h, inp.h))

@@ -83,6 +83,21 @@ static int upload_axis(tk_filter_s* f, int a)
         TK_HIP(hipMalloc(&f->half_dev[a], sizeof(T) * (s + 1)));
         TK_HIP(hipMemcpy(f->half_dev[a], half.data(), sizeof(T) * (s + 1), hipMemcpyHostToDevice));
         ax.half = f->half_dev[a];
+        if (sizeof(T) == 4 && direct_width_supported(nh)) {
+            // A operand of the matrix-core window, lane l = n + 16 g of K-step ks:
+            // w(|4 ks + g - s - n|) inside the band, 0 outside
+            const int KS = mfma_ksteps(s);
+            std::vector<float> band((size_t)KS * 64);
+            for (int ks = 0; ks < KS; ++ks)
+                for (int l = 0; l < 64; ++l) {
+                    const int d = 4 * ks + (l >> 4) - s - (l & 15);
+                    band[(size_t)ks * 64 + l] = (d >= -s && d <= s) ? (float)half[d < 0 ? -d : d] : 0.f;
+                }
+            TK_HIP(hipMalloc(&f->band_dev[a], sizeof(float) * band.size()));
+            TK_HIP(hipMemcpy(f->band_dev[a], band.data(), sizeof(float) * band.size(),
+                             hipMemcpyHostToDevice));
+            ax.band = f->band_dev[a];
+        }
         // FP32 computes every convolution directly.  An FFT pass has rounding errors
         // relative to the largest value along the line, while the region fit divides
         // small local quantities (f2 = (num - kmyim) / (den - kones) where H is near 0 or
@@ -91,6 +106,10 @@ static int upload_axis(tk_filter_s* f, int a)
         // one (~2(2s+1) fused operations per complex element against ~140).  The register
         // window needs a compiled half-width and 16-byte quads of pencils (dims % 4 == 0,
         // at least four pencils per group); otherwise the generic direct taps pass runs.
+        // The window runs on the vector ALUs (method 1) by default.  The matrix-core form
+        // (method 2, banded products) measured the same time at 256^3 (67-68 us per y or
+        // z pass, 89-93 us per x pass): both passes are bound by the global and LDS
+        // traffic around the arithmetic, not by the arithmetic itself.
         if (sizeof(T) == 4) {
             const AxisGeom g = make_geom(a, f->dims[0], f->dims[1], f->dims[2], L,
                                          conv_ept<T>() * 256);
@@ -221,6 +240,7 @@ int tk_filter_free(tk_filter f)
         if (f->tw_dev[a]) (void)hipFree(f->tw_dev[a]);
         if (f->spec_dev[a]) (void)hipFree(f->spec_dev[a]);
         if (f->half_dev[a]) (void)hipFree(f->half_dev[a]);
+        if (f->band_dev[a]) (void)hipFree(f->band_dev[a]);
     }
     delete f;
     return TK_OK;
@@ -249,13 +269,18 @@ int tk_filter_get_method(tk_filter f, int axis, int* method)
 
 int tk_filter_set_method(tk_filter f, int axis, int method)
 {
-    TK_CHECK(f && axis >= 0 && axis < 3 && (method == 0 || method == 1), TK_ERR_ARG,
+    TK_CHECK(f && axis >= 0 && axis < 3 && method >= 0 && method <= 2, TK_ERR_ARG,
              "tk_filter_set_method: argument");
     const AxisDev& ax = f->ax[axis];
-    if (method == 1)
+    if (method >= 1)
         TK_CHECK(direct_width_supported(ax.nh) && f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 &&
                      f->dims[2] % 4 == 0,
                  TK_ERR_UNSUPPORTED, "tk_filter_set_method: no direct path for %d taps", ax.nh);
+    if (method == 2)
+        TK_CHECK(f->dtype == TK_F32 && mfma_window_fits(axis, f->dims), TK_ERR_UNSUPPORTED,
+                 "tk_filter_set_method: the matrix-core window needs FP32, a length that is a "
+                 "multiple of 16 and groups of a multiple of 16 pencils (axis %d, length %d)",
+                 axis, ax.L);
     if (method == 0)
         TK_CHECK(ax.nstage > 0 || ax.L == 1, TK_ERR_UNSUPPORTED,
                  "tk_filter_set_method: length %d has no FFT plan", ax.L);
@@ -277,9 +302,9 @@ static int autotune(tk_ctx ctx, tk_filter_s* f, double* ms)
     TK_HIP(hipEventCreate(&e1));
     int r = TK_OK;
     for (int a = 0; a < 3 && r == TK_OK; ++a) {
-        double t[2] = {-1, -1};
+        double t[3] = {-1, -1, -1};
         const int keep = f->ax[a].direct;
-        for (int m = 0; m < 2 && r == TK_OK; ++m) {
+        for (int m = 0; m < 3 && r == TK_OK; ++m) {
             if (f->ax[a].L == 1 || tk_filter_set_method(f, a, m) != TK_OK) continue;
             // one warm-up pass, then three timed passes, complex -> complex in place
             for (int rep = 0; rep < 4 && r == TK_OK; ++rep) {
@@ -293,13 +318,12 @@ static int autotune(tk_ctx ctx, tk_filter_s* f, double* ms)
             t[m] = el / 3.0;
         }
         f->ax[a].direct = keep;
-        if (t[0] >= 0 && t[1] >= 0) f->ax[a].direct = t[1] < t[0] ? 1 : 0;
-        else if (t[1] >= 0) f->ax[a].direct = 1;
-        else if (t[0] >= 0) f->ax[a].direct = 0;
-        if (ms) {
-            ms[2 * a] = t[0];
-            ms[2 * a + 1] = t[1];
-        }
+        int best = -1;
+        for (int m = 0; m < 3; ++m)
+            if (t[m] >= 0 && (best < 0 || t[m] < t[best])) best = m;
+        if (best >= 0) f->ax[a].direct = best;
+        if (ms)
+            for (int m = 0; m < 3; ++m) ms[3 * a + m] = t[m];
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

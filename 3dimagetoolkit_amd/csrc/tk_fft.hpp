@@ -54,10 +54,12 @@ struct AxisDev {
     int nstage = 0;              // 0 with L > 1: length not 2/3/5-smooth, direct convolution
     int radix[kMaxStages] = {0};
     int nh = 1;                  // taps of the direct path (odd)
-    int direct = 0;              // 1: symmetric direct circular convolution (halo-padded LDS)
+    int direct = 0;              // symmetric direct circular convolution in halo-padded LDS:
+                                 // 1 register window on the vector ALUs, 2 on the matrix cores
     const void* tw = nullptr;    // cplx<T>[L], exp(-2 pi i t / L)
     const void* spec = nullptr;  // T[L], real spectrum / L (FFT path) or T[nh] taps (direct)
     const void* half = nullptr;  // T[nh/2 + 1]: w(0), w(1), .. w(s) of the direct path
+    const void* band = nullptr;  // FP32 [KS][64]: the matrix-core window's band operand
 };
 
 // how the pencils of one axis sit in memory
@@ -94,5 +96,6 @@ struct tk_filter_s {
     void* spec_dev[3] = {nullptr, nullptr, nullptr};
     void* tw_dev[3] = {nullptr, nullptr, nullptr};
     void* half_dev[3] = {nullptr, nullptr, nullptr};
+    void* band_dev[3] = {nullptr, nullptr, nullptr};
     tk::AxisDev ax[3];
 };

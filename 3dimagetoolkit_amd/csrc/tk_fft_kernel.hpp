@@ -323,6 +323,117 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
     __syncthreads();
 }
 
+// The same circular convolution on the matrix cores (FP32, v_mfma_f32_16x16x4_f32), for
+// pencil groups of a multiple of 8 and lengths L % 16 == 0.  A 16 x 16 output tile is 16
+// consecutive positions of 8 pencils x 2 fields (the real and imaginary parts of the
+// complex pencils), the product of the banded Toeplitz matrix
+// Wb[m][k] = w(k - S - m) (16 x 4*KS, zero outside |k - S - m| <= S) with the 4*KS input
+// positions from 16b - S of the same pencils and fields:
+//   out(16b + m, c) = sum_k Wb[m][k] x(16b - S + k, c),   c = 2 * pencil + field
+// Operand layouts of the instruction (lane l, n = l % 16, g = l / 16): A[m = n][k = g],
+// B[k = g][c = n], D[m = 4g + q][c = n] in register q.  K-step ks covers the input
+// positions 4ks + g - S, so the band Wb is the same for every tile (KS registers, read
+// from the filter's table) and consecutive tiles of a wave share KS - 4 K-steps of inputs
+// (a sliding register window).  Where the last K-step reaches past the S halo positions
+// (S = 9: two positions), its reads are clamped to the last halo position, so that no
+// unset LDS word enters a product (their weights are zero).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int mfma_ksteps(int S) { return (16 + 2 * S + 3) / 4; }
+
+template <int EPT, int NT, int S>
+__device__ __forceinline__ void mfma_win(cplx<float>* buf, const float* band, const Lay& y)
+{
+    constexpr int KS = mfma_ksteps(S);
+    constexpr bool CLAMP = 4 * KS - 16 - S > S;
+    constexpr int NW = NT / 64;
+    constexpr int MAXT = (EPT * NT / 128 + NW - 1) / NW;  // tiles per wave
+    const int L = y.L;
+    for (int e = threadIdx.x; e < 2 * S * y.P; e += NT) {
+        const int p = e & (y.P - 1), t = e >> y.lp;
+        const int pos = t < S ? t - S : L + (t - S);
+        int src = pos;
+        while (src < 0) src += L;
+        while (src >= L) src -= L;
+        buf[y.off + pos * y.SP + p * y.SQ] = buf[y.off + src * y.SP + p * y.SQ];
+    }
+    const int lane = threadIdx.x & 63;
+    float A[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) A[ks] = band[ks * 64 + lane];
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int n = lane & 15, g = lane >> 4;
+    const int NB = L >> 4, T = NB * (y.P >> 3);
+    const int t0 = wave * T / NW, t1 = (wave + 1) * T / NW;
+    const int last = L + S - 1;
+    const int sp2 = 2 * y.SP, st4 = 4 * sp2;
+    float* fb = (float*)buf;
+    // lane n reads field n & 1 of pencil n >> 1 of the tile's 8-pencil group
+    const int cofs = 2 * (y.off + (n >> 1) * y.SQ) + (n & 1);
+    f32x4 o[MAXT];
+    float R[KS];
+    int prev = -2;
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        const int t = t0 + j;
+        if (t < t1) {
+            const int pg = t / NB, b = t - pg * NB;
+            const float* col = fb + cofs + 16 * pg * y.SQ;
+            const int p0 = 16 * b + g - S;
+            if (t == prev + 1 && b > 0) {
+#pragma unroll
+                for (int i = 0; i < KS - 4; ++i) R[i] = R[i + 4];
+                int a = (p0 + 4 * (KS - 4)) * sp2;
+#pragma unroll
+                for (int i = KS - 4; i < KS; ++i, a += st4) {
+                    if constexpr (CLAMP) R[i] = col[min(p0 + 4 * i, last) * sp2];
+                    else R[i] = col[a];
+                }
+            } else {
+                int a = p0 * sp2;
+#pragma unroll
+                for (int i = 0; i < KS; ++i, a += st4) {
+                    if constexpr (CLAMP) R[i] = col[min(p0 + 4 * i, last) * sp2];
+                    else R[i] = col[a];
+                }
+            }
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[ks], R[ks], acc, 0, 0, 0);
+            o[j] = acc;
+            prev = t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MAXT; ++j) {
+        const int t = t0 + j;
+        if (t < t1) {
+            const int pg = t / NB, b = t - pg * NB;
+            float* col = fb + cofs + 16 * pg * y.SQ + (16 * b + 4 * g) * sp2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) col[q * sp2] = o[j][q];
+        }
+    }
+    __syncthreads();
+}
+
+// MF: the matrix-core window (FP32; the launcher checks the group shape), else the
+// vector-ALU window.  Separate kernels, so each gets the register budget it needs.
+template <class T, int EPT, int NT, int S, bool MF>
+__device__ __forceinline__ void direct_conv(cplx<T>* buf, const T* wh, const AxisDev& ax,
+                                            const Lay& y)
+{
+    if constexpr (MF) {
+        static_assert(sizeof(T) == 4, "matrix-core window: FP32 only");
+        mfma_win<EPT, NT, S>((cplx<float>*)buf, (const float*)ax.band, y);
+    } else {
+        direct_win<T, EPT, NT, S>(buf, wh, y);
+    }
+}
+
 // forward FFT, multiply by the (real, 1/L-scaled) spectrum, inverse FFT
 template <class T, int EPT, int NT>
 __device__ __forceinline__ void conv_lds(cplx<T>* buf, const cplx<T>* tw, const T* spec,
@@ -536,8 +647,8 @@ inline __host__ __device__ int conv_nbuf(int axis, int L, int P, int S)
 }
 
 // per-axis convolution pass: S == 0 FFT (or the generic direct fallback), S > 0 the
-// register-window direct path with half-width S
-template <class T, int EPT, int NT, bool QUAD, int S, class Ld, class Md, class St>
+// register-window direct path with half-width S, on the matrix cores when MF
+template <class T, int EPT, int NT, bool QUAD, int S, bool MF, class Ld, class Md, class St>
 __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int rounds, Ld ld,
                                                   Md md, St st)
 {
@@ -581,7 +692,7 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
         for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = ld(v); });
     }
     __syncthreads();
-    if constexpr (S > 0) direct_win<T, EPT, NT, S>(buf, spec, y);
+    if constexpr (S > 0) direct_conv<T, EPT, NT, S, MF>(buf, spec, ax, y);
     else conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
     if constexpr (!std::is_same<Md, MdNone>::value) {
         if (rounds > 1) {
@@ -598,7 +709,7 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
                 for_group<T>(g, y, c0, NT, [&](long v, int l) { buf[l] = md(v, buf[l]); });
             }
             __syncthreads();
-            if constexpr (S > 0) direct_win<T, EPT, NT, S>(buf, spec, y);
+            if constexpr (S > 0) direct_conv<T, EPT, NT, S, MF>(buf, spec, ax, y);
             else conv_lds<T, EPT, NT>(buf, tw, spec, ax, y);
         }
     }
@@ -638,38 +749,69 @@ inline AxisGeom make_geom(int axis, int nx, int ny, int nz, int L, int elems)
     return g;
 }
 
+// half-widths with a compiled register-window direct path
+inline bool direct_width_supported(int nh) { return nh == 19 || nh == 25 || nh == 37; }
+
+// The matrix-core window runs groups of 2048 complex elements (half the vector-ALU
+// window's), which halves the LDS of a workgroup and doubles the workgroups resident per
+// CU.
+constexpr int kMfmaEpt = 8;
+
+// the matrix-core direct window fits this axis: FP32, length a multiple of 16, groups of
+// a multiple of 8 pencils
+inline bool mfma_window_fits(int axis, const int dims[3])
+{
+    const int L = dims[axis];
+    const AxisGeom g = make_geom(axis, dims[0], dims[1], dims[2], L, kMfmaEpt * 256);
+    return L % 16 == 0 && g.P % 8 == 0 && L * g.P <= kMfmaEpt * 256;
+}
+
 template <class T, class Ld, class Md, class St>
 int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld ld, Md md, St st)
 {
     constexpr int NT = 256;
     constexpr int EPT = conv_ept<T>();
     const AxisDev& ax = f->ax[axis];
-    AxisGeom g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, EPT * NT);
+    const bool dims4 = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0;
+    // the matrix-core window (checked when the method was chosen; checked again here since
+    // the group shape decides the tiling)
+    const bool mf = std::is_same<T, float>::value && ax.direct == 2 && dims4 &&
+                    direct_width_supported(ax.nh) && mfma_window_fits(axis, f->dims);
+    const int elems = (mf ? kMfmaEpt : EPT) * NT;
+    AxisGeom g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, elems);
     TK_CHECK(ax.L * g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
              ax.L, EPT * NT);
     unsigned nb = (unsigned)((g.nlines + g.P - 1) / g.P);
-    const bool quad = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0 &&
-                      g.P >= 4;
+    const bool quad = dims4 && g.P >= 4;
     // register-window direct path for the instantiated half-widths (sigma 3, 4, 6 at h=1)
     const int S = ax.direct && quad ? ax.nh / 2 : 0;
     auto lds_for = [&](int s) {
         return (size_t)conv_nbuf(axis, ax.L, g.P, s) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
                (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
     };
-#define TK_CONV_LAUNCH(Q, SS)                                                                  \
-    hipLaunchKernelGGL((k_conv_axis<T, EPT, NT, Q, SS, Ld, Md, St>), dim3(nb), dim3(NT),        \
-                       lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
-    if (S == 9) TK_CONV_LAUNCH(true, 9);
-    else if (S == 12) TK_CONV_LAUNCH(true, 12);
-    else if (S == 18) TK_CONV_LAUNCH(true, 18);
-    else if (quad) TK_CONV_LAUNCH(true, 0);
-    else TK_CONV_LAUNCH(false, 0);
+#define TK_CONV_LAUNCH(Q, SS, MF)                                                              \
+    hipLaunchKernelGGL((k_conv_axis<T, MF ? kMfmaEpt : EPT, NT, Q, SS, MF, Ld, Md, St>),        \
+                       dim3(nb), dim3(NT), lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
+    if constexpr (std::is_same<T, float>::value) {
+        if (mf && S == 9) TK_CONV_LAUNCH(true, 9, true);
+        else if (mf && S == 12) TK_CONV_LAUNCH(true, 12, true);
+        else if (mf && S == 18) TK_CONV_LAUNCH(true, 18, true);
+        else if (S == 9) TK_CONV_LAUNCH(true, 9, false);
+        else if (S == 12) TK_CONV_LAUNCH(true, 12, false);
+        else if (S == 18) TK_CONV_LAUNCH(true, 18, false);
+        else if (quad) TK_CONV_LAUNCH(true, 0, false);
+        else TK_CONV_LAUNCH(false, 0, false);
+    } else {
+        if (S == 9) TK_CONV_LAUNCH(true, 9, false);
+        else if (S == 12) TK_CONV_LAUNCH(true, 12, false);
+        else if (S == 18) TK_CONV_LAUNCH(true, 18, false);
+        else if (quad) TK_CONV_LAUNCH(true, 0, false);
+        else TK_CONV_LAUNCH(false, 0, false);
+    }
 #undef TK_CONV_LAUNCH
     TK_LAUNCHED();
     return TK_OK;
 }
 
-// half-widths with a compiled register-window direct path
-inline bool direct_width_supported(int nh) { return nh == 19 || nh == 25 || nh == 37; }
 
 }  // namespace tk

@@ -108,14 +108,16 @@ int tk_filter_taps(tk_filter flt, int nh[3]);
 /* real spectrum along one axis (length dims[axis]), including nothing else */
 int tk_filter_axis_spectrum(tk_filter flt, int axis, double* out);
 /* per-axis method: 0 = FFT pass (forward, spectrum multiply, inverse in LDS),
-   1 = direct circular convolution with the symmetric taps (same result up to rounding).
-   No reference counterpart: the reference always uses FFTW. */
+   1 = direct circular convolution with the symmetric taps on the vector ALUs,
+   2 = the same direct convolution as banded matrix products on the matrix cores (FP32,
+   axis length a multiple of 16, pencil groups of a multiple of 16); all three give the
+   same result up to rounding.  No reference counterpart: the reference always uses FFTW. */
 int tk_filter_get_method(tk_filter flt, int axis, int* method);
 int tk_filter_set_method(tk_filter flt, int axis, int method);
-/* autotuner: time both methods of every axis on the device (on the filter's own
-   sizes) and keep the faster; ms (may be NULL) receives {fft, direct} per axis, -1
-   where a method is unavailable */
-int tk_filter_autotune(tk_ctx ctx, tk_filter flt, double ms[6]);
+/* autotuner: time every method of every axis on the device (on the filter's own
+   sizes) and keep the fastest; ms (may be NULL) receives {fft, direct, direct on the
+   matrix cores} per axis, -1 where a method is unavailable */
+int tk_filter_autotune(tk_ctx ctx, tk_filter flt, double ms[9]);
 
 /* ---- R5: cyclic (periodic) convolution ------------------------------------------
  * conv::cyclic_fftw_convolution (convolution_imp.hxx:657-757), the function the

@@ -21,8 +21,10 @@ tests/test_oracle_pins.py.  Inputs come from the project's seeded generator.
                  and 1-4 ellipsoids, rescaled to [0,255], sigma 3, 30 outer iterations,
                  ls_steps 5, ls_tol 0: per volume the bit-packed mask phi >= 0.5 and the
                  FP64 sum of phi.
+  c3_batch_it10.npz -- the same 64 runs stopped after 10 outer iterations (the state the
+                 FP32 loop is compared with before rounding-level differences can grow).
 
-Usage: python tests/golden/make_golden.py [c0|s5|c1|c3|all]
+Usage: python tests/golden/make_golden.py [c0|s5|c1|c3|c3_10|all]
 """
 import importlib
 import json
@@ -94,7 +96,7 @@ def c3_volume(v, dims, nell):
     return tk.gen_standard(tuple(int(x) for x in dims), C3_SEED + v, 1, int(nell))
 
 
-def run_c3(maxiter=30):
+def run_c3(maxiter=30, name="c3_batch.npz"):
     dims, nell = tk.gen_batch_dims(C3_SEED, 64)
     p = orc.default_params(sigma=3.0, maxiter=maxiter, tol=0.0, ls_steps=5, ls_tol=0.0)
     out = {}
@@ -106,7 +108,7 @@ def run_c3(maxiter=30):
         out[f"sum_{v}"] = np.float64(phi.sum())
     meta = dict(seed=C3_SEED, n_vol=64, sigma=3.0, maxiter=maxiter, ls_steps=5, ls_tol=0.0,
                 seconds=time.time() - t0, threads=orc.lib().or_num_threads())
-    np.savez_compressed(os.path.join(HERE, "c3_batch.npz"), dims=dims, nell=nell,
+    np.savez_compressed(os.path.join(HERE, name), dims=dims, nell=nell,
                         meta=json.dumps(meta), **out)
     print("c3", meta)
 
@@ -121,3 +123,5 @@ if __name__ == "__main__":
         run_c1()
     if which in ("c3", "all"):
         run_c3()
+    if which in ("c3_10", "all"):
+        run_c3(10, "c3_batch_it10.npz")

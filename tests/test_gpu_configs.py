@@ -256,6 +256,27 @@ def test_c3_graph_replays_match_eager(tk, ctx):
     ea.close()
 
 
+def test_c3_batch_f32_lex_10_iterations_vs_golden(tk, ctx):
+    """every one of the 64 volumes after 10 outer iterations, before rounding-level
+    differences can grow along an unstable trajectory: the FP32 batch with the reference's
+    (lexicographic) sweeps against the FP64 oracle stopped at the same iteration"""
+    d = np.load(os.path.join(GOLDEN, "c3_batch_it10.npz"))
+    dims, nell = d["dims"], d["nell"]
+    vols = list(range(64))
+    b = c3_batch(tk, ctx, dims, nell, vols, gs_order=tk.TK_GS_LEX)
+    b.iterate(10, use_graph=True)
+    dices = []
+    for v in vols:
+        n = int(np.prod(dims[v]))
+        ref = np.unpackbits(d[f"mask_{v}"])[:n].astype(bool)
+        dices.append(dice((b.phi(v) >= 0.5).reshape(-1), ref))
+    b.close()
+    dices = np.array(dices)
+    say(f"C3 64 volumes FP32 lex, 10 iterations, vs FP64 oracle: Dice min {dices.min():.6f} "
+        f"mean {dices.mean():.6f}")
+    assert dices.min() >= 0.999
+
+
 @pytest.mark.parametrize("order", ["lex", "redblack"])
 def test_c3_batch_f32_vs_golden(tk, ctx, order):
     """per-volume Dice of the FP32 batch against the FP64 oracle.  Lexicographic sweeps

@@ -66,7 +66,8 @@ def test_conv_cyclic_f64(tk, orc, ctx, shape, sigma, h, rng):
     assert np.max(np.abs(vb.numpy() - ref2)) < 4e-15 * np.log2(x.size) * np.max(np.abs(y2))
 
 
-@pytest.mark.parametrize("method", [0, 1])  # FFT pass, direct circular convolution
+# FFT pass, direct circular convolution on the vector ALUs, the same on the matrix cores
+@pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("shape,sigma", [((64, 64, 64), 3.0), ((48, 80, 96), 3.0),
                                          ((128, 64, 160), 4.0), ((32, 48, 64), 6.0)])
 def test_conv_cyclic_f32(tk, orc, ctx, shape, sigma, method, rng):
@@ -85,12 +86,37 @@ def test_conv_cyclic_f32(tk, orc, ctx, shape, sigma, method, rng):
     assert np.max(np.abs(vy.numpy().astype(np.float64) - ref)) < 1e-6
 
 
+def test_conv_f32_matrix_cores_accuracy(tk, orc, ctx, rng):
+    """the banded matrix products on the matrix cores against the vector-ALU window: the
+    same FP32 sums of 2s+1 products in another order, so the error against the FP64
+    oracle has the same size and no bias (a truncating accumulation would show as a mean
+    error of the order of the RMS error)"""
+    shape = (256, 48, 64)
+    x = rng.random(shape)
+    tf, nh, spec, nc = orc.gaussian(shape, 6.0)
+    ref = orc.cyclic_conv(spec, nc, x)
+    stats = {}
+    for method in (1, 2):
+        f = tk.Filter(ctx, shape, 6.0, dtype=np.float32)
+        for a in range(3):
+            f.set_method(a, method)
+        vx = ctx.from_numpy(x.astype(np.float32))
+        vy = ctx.volume(shape, np.float32)
+        f.conv(vy, vx)
+        e = vy.numpy().astype(np.float64) - ref
+        stats[method] = (np.abs(e).max(), np.sqrt(np.mean(e ** 2)), e.mean())
+        say(f"method {method}: max {stats[method][0]:.3e} rms {stats[method][1]:.3e} "
+            f"mean {stats[method][2]:.3e}")
+    (m1, r1, b1), (m2, r2, b2) = stats[1], stats[2]
+    assert m2 < 1e-6 and r2 < 1.5 * r1 and abs(b2) < 0.2 * r2
+
+
 def test_filter_autotune_keeps_results(tk, orc, ctx, rng):
     shape = (64, 48, 80)
     f = tk.Filter(ctx, shape, 3.0, dtype=np.float32)
     ms = f.autotune()
-    say("autotune ms (fft, direct) per axis:", ms, "->", f.methods())
-    assert all(t[0] > 0 and t[1] > 0 for t in ms)
+    say("autotune ms (fft, direct, direct on the matrix cores) per axis:", ms, "->", f.methods())
+    assert all(t[0] > 0 and t[1] > 0 and t[2] > 0 for t in ms)
     x = rng.random(shape)
     tf, nh, spec, nc = orc.gaussian(shape, 3.0)
     vx = ctx.from_numpy(x.astype(np.float32))
