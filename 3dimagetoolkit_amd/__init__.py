@@ -98,6 +98,7 @@ def _load():
         "tk_rsfe_rhs": (vp, [vp]),
         "tk_rsfe_set_warm_start": (i, [vp, vp]),
         "tk_poisson_rb_set_tile": (i, [vp, vp, i]),
+        "tk_sb_update": (i, [vp, vp, vp, vp, vp, vp, vp, d, d, dp]),
         "tk_heaviside": (i, [vp, vp, d, d, d, vp]),
         "tk_grad": (i, [vp, vp, vp, vp, vp]),
         "tk_div": (i, [vp, vp, vp, vp, vp]),
@@ -403,6 +404,16 @@ def poisson_rb_autotune(ctx, rhs):
 
 def poisson_rb_set_tile(ctx, rhs, rows):
     _check(lib.tk_poisson_rb_set_tile(ctx.h, rhs.h, int(rows)))
+
+
+def sb_update(ctx, phi, phiold, g, b, q, divdb, a0=0.0, b0=1.0):
+    """The Split-Bregman update and div (tk_sb_update): phi clamped, b updated in place,
+    q = d - b, divdb = div(q); returns (sum (phi-phiold)^2, sum phiold^2)."""
+    arr = lambda vs: (C.c_void_p * 3)(*[v.h for v in vs])
+    norms = (C.c_double * 2)()
+    _check(lib.tk_sb_update(ctx.h, phi.h, phiold.h, g.h, arr(b), arr(q), divdb.h, a0, b0,
+                            norms))
+    return norms[0], norms[1]
 
 
 class Rsfe:

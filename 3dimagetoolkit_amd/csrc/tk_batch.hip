@@ -13,7 +13,8 @@
 //  - graph mode captures two outer iterations of all volumes (forked over the lanes and
 //    joined back) into one hipGraph and replays it.  Two, because an odd sweep count swaps
 //    phi with its ping-pong partner every iteration: after two the buffers are back where
-//    the capture found them.  The graph holds kernel nodes only (no memset nodes).
+//    the capture found them.  After an odd number of eager iterations the graph is captured
+//    again.  The graph holds kernel nodes only (no memset nodes).
 // Otherwise (ls_tol > 0 gates the Poisson loop on the host, tol > 0 stops a run early) each
 // volume runs the reference's loop with its host checks, lane after lane.
 #include <hip/hip_runtime.h>
@@ -38,6 +39,7 @@ struct tk_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_nodes = 0;
+    int graph_parity = 0;  // iteration parity at capture (phi and its partner swap roles)
 };
 
 namespace {
@@ -80,6 +82,7 @@ int capture(tk_batch b, const std::vector<int>& order)
     hipStream_t s0 = b->lanes[0]->stream;
     std::vector<int> it0(b->runs.size());
     for (size_t v = 0; v < b->runs.size(); ++v) it0[v] = rsfe_iterations(b->runs[v]);
+    b->graph_parity = it0[0] & 1;
     TK_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
     int r = TK_OK;
     if (hipEventRecord(b->fork, s0) != hipSuccess) r = TK_ERR_HIP;
@@ -208,7 +211,9 @@ int tk_batch_iterate(tk_batch b, int n_iters, int use_graph)
     int n = std::min(n_iters, std::max(0, b->p.maxiter - it));
     const std::vector<int> order = lane_order(b);
     if (use_graph && n >= 2) {
-        if (!b->exec) {
+        // the graph holds the buffers as the capture found them: with an odd sweep count an
+        // odd number of iterations since then has swapped them, so capture again
+        if (!b->exec || (it & 1) != b->graph_parity) {
             TK_TRY(sync_lanes(b));
             TK_TRY(capture(b, order));
         }

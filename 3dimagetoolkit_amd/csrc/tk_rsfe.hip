@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <utility>
 
 #include "tk_device.hpp"
 #include "tk_fft_kernel.hpp"
@@ -317,7 +318,9 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
                                   &ls_err, s->faces_set);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
     if (chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
-    // cut + norms + grad/shrink/Bregman, then div
+    // cut + norms + grad/shrink/Bregman, then div.  (A single pass recomputing q of the
+    // six neighbours for div measured 281 us at 256^3 FP32 against 155 + 70 us for the two
+    // passes: 140 registers per lane, three waves per SIMD.)
     int nb = 0;
     double* part = ctx->acc + 64;
     {
@@ -615,6 +618,39 @@ int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
     return TK_OK;
 }
 tk_filter tk_rsfe_filter(tk_rsfe s) { return s ? s->flt : nullptr; }
+
+int tk_sb_update(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol g, const tk_vol* b,
+                 const tk_vol* q, tk_vol divdb, double a0, double b0, double* norms)
+{
+    TK_CHECK(ctx && phi && phiold && g && b && q && divdb, TK_ERR_ARG,
+             "tk_sb_update: null argument");
+    tk_vol all[] = {phiold, g, divdb, b[0], b[1], b[2], q[0], q[1], q[2]};
+    for (tk_vol v : all) {
+        TK_CHECK(v && v->d, TK_ERR_ARG, "tk_sb_update: null volume");
+        TK_CHECK(same_dims(v, phi) && v->dtype == phi->dtype, TK_ERR_SHAPE,
+                 "tk_sb_update: volumes differ in dims or type");
+    }
+    TK_CHECK(phi->dtype != TK_U8, TK_ERR_DTYPE, "tk_sb_update: U8 volume");
+    TK_CHECK(phi->nx >= 3 && phi->ny >= 3 && phi->nz >= 3, TK_ERR_UNSUPPORTED,
+             "tk_sb_update: 3-D volumes with every edge >= 3 only");
+    const bool dbl = phi->dtype == TK_F64;
+    int nb = 0;
+    double* part = ctx->acc + 64;
+    tk_vol bb[3] = {b[0], b[1], b[2]}, qq[3] = {q[0], q[1], q[2]};
+    TK_TRY(dbl ? launch_sb_update<double>(ctx, phi, phiold, bb, g, qq, a0, b0, &nb, part)
+               : launch_sb_update<float>(ctx, phi, phiold, bb, g, qq, a0, b0, &nb, part));
+    TK_TRY(dbl ? launch_div<double>(ctx, qq[0], qq[1], qq[2], divdb)
+               : launch_div<float>(ctx, qq[0], qq[1], qq[2], divdb));
+    hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb,
+                       ctx->acc + 8);
+    TK_LAUNCHED();
+    if (norms) {
+        TK_HIP(hipMemcpyAsync(norms, ctx->acc + 8, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return TK_OK;
+}
 
 int tk_rsfe_destroy(tk_rsfe s)
 {

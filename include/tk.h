@@ -174,6 +174,14 @@ int tk_div(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol out);
 int tk_edge_detect(tk_ctx ctx, tk_vol u0, double beta, double lamda, double nu, tk_vol g);
 /* rsfe_splitbregman::shrink (imp:343-365): d = sign(b) max(|b| - g, 0) */
 int tk_shrink(tk_ctx ctx, tk_vol b, tk_vol g, tk_vol d);
+/* The Split-Bregman update after the Poisson step (rsfe_splitbregman_imp.hxx:1800-1932):
+   cut_phi (phi clamped to [a0, b0] in place, imp:320-339), d = grad phi, b += d,
+   d = shrink(b, g), b -= d, d -= b (b[3] updated in place, d - b stored in q[3]) and
+   divdb = div(q) (image3d_imp.hxx:947-1383, with its quirks); norms (may be NULL) receives
+   sum (phi - phiold)^2 and sum phiold^2 in FP64 over every voxel (imp:1832 without the
+   spacing factor).  The two passes of the run loop. */
+int tk_sb_update(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol g, const tk_vol* b,
+                 const tk_vol* q, tk_vol divdb, double a0, double b0, double* norms);
 /* image3d::normL2 (image3d_imp.hxx:1419-1423) with FP64 accumulation */
 int tk_norm_l2(tk_ctx ctx, tk_vol a, double* out);
 /* ||a - b||_L2 without a temporary (rsfe_splitbregman_imp.hxx:1783,1832) */

@@ -252,6 +252,12 @@ def test_c3_graph_replays_match_eager(tk, ctx):
     assert info["graph_nodes"] > 0 and info["iterations"] == 7
     for q in range(len(vols)):
         assert np.array_equal(ga.phi(q), ea.phi(q))
+    # after an odd count the buffers have swapped roles: the graph is captured again
+    ga.iterate(2, use_graph=True)
+    ea.iterate(2, use_graph=False)
+    assert ga.info()["iterations"] == 9
+    for q in range(len(vols)):
+        assert np.array_equal(ga.phi(q), ea.phi(q))
     ga.close()
     ea.close()
 

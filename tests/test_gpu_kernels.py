@@ -360,3 +360,40 @@ def test_cc_label_matches_scipy(tk, ctx, rng):
         sel = ref == c
         expect[sel] = flat[sel].min() + 1
     assert np.array_equal(raw, expect)
+
+
+@pytest.mark.parametrize("shape,h", [((3, 3, 8), (1.0, 1.0, 1.0)), ((5, 7, 8), (0.7, 1.3, 1.0)),
+                                     ((16, 18, 20), (1.0, 1.0, 1.0)),
+                                     ((33, 17, 12), (1.0, 2.0, 0.5)),
+                                     ((40, 37, 136), (1.0, 1.0, 1.0)), ((9, 10, 11), (1.0, 1.0, 1.0))])
+def test_sb_update_vs_oracle(tk, orc, ctx, shape, h, rng):
+    """the Split-Bregman update and div of the run loop (tk_sb_update) in FP64 against the
+    oracle's cut_phi, grad, shrink and div composed in the reference's order
+    (rsfe_splitbregman_imp.hxx:1868-1932), bit for bit; rows of 16 bytes and scalar voxels"""
+    a0, b0 = 0.0, 1.0
+    phi = rng.uniform(-0.3, 1.3, shape)
+    phiold = rng.random(shape)
+    g = rng.uniform(0.0, 0.5, shape)
+    b = [0.3 * rng.standard_normal(shape) for _ in range(3)]
+    vp = ctx.from_numpy(phi, h=h)
+    vo = ctx.from_numpy(phiold, h=h)
+    vg = ctx.from_numpy(g, h=h)
+    vb = [ctx.from_numpy(x, h=h) for x in b]
+    vq = [ctx.volume(shape, np.float64, h=h) for _ in range(3)]
+    vd = ctx.volume(shape, np.float64, h=h)
+    n1 = tk.sb_update(ctx, vp, vo, vg, vb, vq, vd, a0, b0)
+    pc = np.clip(phi, a0, b0)
+    dgr = orc.grad(pc, h)
+    qs, bs = [], []
+    for a in range(3):
+        bb = b[a] + dgr[a]
+        dn = orc.shrink(bb, g)
+        bb = bb - dn
+        qs.append(dn - bb)
+        bs.append(bb)
+    assert np.array_equal(vp.numpy(), pc)
+    assert all(np.array_equal(v.numpy(), y) for v, y in zip(vb, bs))
+    assert all(np.array_equal(v.numpy(), y) for v, y in zip(vq, qs))
+    assert np.array_equal(vd.numpy(), orc.div(*qs, h))
+    assert np.isclose(n1[0], np.sum((pc - phiold) ** 2), rtol=1e-12)
+    assert np.isclose(n1[1], np.sum(phiold ** 2), rtol=1e-12)
