@@ -346,11 +346,12 @@ __global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* 
 // aligned to multiples of 64, so every plane row is moved with 16-byte accesses; the rhs
 // rows are staged in LDS with the old planes, and the planes two steps ahead are fetched
 // into registers (two alternating sets) while the current one is computed.  TJ (rows per
-// tile) is 8 or 16, picked per volume shape by timing both.
+// tile) is 8 (256 threads) or 16 (512 threads, the same work per thread), picked per volume
+// shape and type by timing both.
 constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
 
-template <class T, bool UNIT, int TJ>
-__global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
+template <class T, bool UNIT, int TJ, int NT>
+__global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
                                                   const T* __restrict__ rhs, int X, int Y, int Z,
                                                   int CI, GsCoef c, int s, int slot, int gated,
                                                   RbGate g)
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     constexpr int W = kRbW, WV = W / V;
     constexpr int OJ = TJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
     constexpr int RJ = TJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
-    constexpr int NO = (OJ * WV + 255) / 256, NB = (RJ * WV + 255) / 256;
+    constexpr int NO = (OJ * WV + NT - 1) / NT, NB = (RJ * WV + NT - 1) / NT;
     __shared__ __align__(16) T O[4][OJ * W];
     __shared__ __align__(16) T Rp[3][RJ * W];
     __shared__ __align__(16) T Bp[3][RJ * W];
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     auto fetch = [&](const T* base, int rows, int row0, Row<T>* out, int n) {
 #pragma unroll
         for (int q = 0; q < n; ++q) {
-            const int e = tid + q * 256;
+            const int e = tid + q * NT;
             Row<T> v = {};
             if (e < rows * WV) {
                 const int r = e / WV, c4 = e - r * WV;
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
     auto deposit = [&](T* lds, int rows, const Row<T>* in, int n) {
 #pragma unroll
         for (int q = 0; q < n; ++q) {
-            const int e = tid + q * 256;
+            const int e = tid + q * NT;
             if (e < rows * WV) strow(lds + e * V, in[q]);
         }
     };
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
         const T* op = O[(p + 1) & 3];
         const T* b = Bp[p % 3];
         const bool inner_p = p >= 1 && p <= X - 2;
-        for (int e = tid; e < RJ * WV; e += 256) {
+        for (int e = tid; e < RJ * WV; e += NT) {
             const int rr = e / WV, c4 = e - rr * WV;
             const int j = j0 - 1 + rr;
             const bool jin = inner_p && j >= 1 && j <= Y - 2;
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(256) void k_gs_rb_pv(const T* __restrict__ src, T* 
         const T* o = O[p & 3];
         const T* b = Bp[p % 3];
         constexpr int TV = kRbTK / V;
-        for (int e = tid; e < TJ * TV; e += 256) {
+        for (int e = tid; e < TJ * TV; e += NT) {
             const int jj = e / TV, c4 = e - jj * TV;
             const int j = j0 + jj, k0 = kb + c4 * V;
             if (j > Y - 2 || k0 >= Z) continue;
@@ -548,6 +549,7 @@ template <class T, int TJ>
 static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
                         int s, int slot, int gated, const RbGate& g, bool unit)
 {
+    constexpr int NT = TJ == 16 ? 512 : 256;
     const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
     // chunk length in x chosen so the grid is one round of as many workgroups per CU as
     // the LDS holds (160 KB per CU)
@@ -560,11 +562,11 @@ static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, cons
     const int CI = (int)((X - 2 + chunks - 1) / chunks);
     const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
     if (unit)
-        hipLaunchKernelGGL((k_gs_rb_pv<T, true, TJ>), grid, dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_gs_rb_pv<T, true, TJ, NT>), grid, dim3(NT), 0, ctx->stream,
                            (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
                            gated, g);
     else
-        hipLaunchKernelGGL((k_gs_rb_pv<T, false, TJ>), grid, dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_gs_rb_pv<T, false, TJ, NT>), grid, dim3(NT), 0, ctx->stream,
                            (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
                            gated, g);
     TK_LAUNCHED();
