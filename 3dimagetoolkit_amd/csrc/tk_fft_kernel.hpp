@@ -792,13 +792,17 @@ int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld 
 #define TK_CONV_LAUNCH(Q, SS, MF)                                                              \
     hipLaunchKernelGGL((k_conv_axis<T, MF ? kMfmaEpt : EPT, NT, Q, SS, MF, Ld, Md, St>),        \
                        dim3(nb), dim3(NT), lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
+    // the FP32 vector-ALU window: the same 4096-element group over 512 threads
+#define TK_CONV_LAUNCH_W(SS)                                                                   \
+    hipLaunchKernelGGL((k_conv_axis<T, EPT / 2, 2 * NT, true, SS, false, Ld, Md, St>),          \
+                       dim3(nb), dim3(2 * NT), lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
     if constexpr (std::is_same<T, float>::value) {
         if (mf && S == 9) TK_CONV_LAUNCH(true, 9, true);
         else if (mf && S == 12) TK_CONV_LAUNCH(true, 12, true);
         else if (mf && S == 18) TK_CONV_LAUNCH(true, 18, true);
-        else if (S == 9) TK_CONV_LAUNCH(true, 9, false);
-        else if (S == 12) TK_CONV_LAUNCH(true, 12, false);
-        else if (S == 18) TK_CONV_LAUNCH(true, 18, false);
+        else if (S == 9) TK_CONV_LAUNCH_W(9);
+        else if (S == 12) TK_CONV_LAUNCH_W(12);
+        else if (S == 18) TK_CONV_LAUNCH_W(18);
         else if (quad) TK_CONV_LAUNCH(true, 0, false);
         else TK_CONV_LAUNCH(false, 0, false);
     } else {
@@ -809,6 +813,7 @@ int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld 
         else TK_CONV_LAUNCH(false, 0, false);
     }
 #undef TK_CONV_LAUNCH
+#undef TK_CONV_LAUNCH_W
     TK_LAUNCHED();
     return TK_OK;
 }
