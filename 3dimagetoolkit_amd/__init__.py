@@ -45,7 +45,8 @@ class RsfeParams(C.Structure):
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "gs_order",
             "max_sweeps", "warm_start")] + [("ls_res_tol", C.c_double),
                                             ("res_check_every", C.c_int),
-                                            ("ws_presweeps", C.c_int)]
+                                            ("ws_presweeps", C.c_int),
+                                            ("acyclic", C.c_int)]
 
 
 class IterStats(C.Structure):
@@ -89,6 +90,8 @@ def _load():
         "tk_rsfe_filter": (vp, [vp]),
         "tk_conv_cyclic": (i, [vp, vp, vp, vp]),
         "tk_conv_cyclic_pair": (i, [vp, vp, vp, vp, vp, vp]),
+        "tk_conv_acyclic": (i, [vp, vp, vp, vp]),
+        "tk_conv_acyclic_pair": (i, [vp, vp, vp, vp, vp, vp]),
         "tk_poisson_dir_sweeps": (i, [vp, vp, vp, d, d, i, i]),
         "tk_poisson_dir_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
@@ -344,12 +347,15 @@ class Filter:
         _check(lib.tk_filter_autotune(self.ctx.h, self.h, ms))
         return [(ms[3 * a], ms[3 * a + 1], ms[3 * a + 2]) for a in range(3)]
 
-    def conv(self, out, inp):
-        _check(lib.tk_conv_cyclic(self.ctx.h, self.h, out.h, inp.h))
+    def conv(self, out, inp, acyclic=False):
+        """cyclic_fftw_convolution, or acyclic_fftw_convolution (zero outside the volume)"""
+        fn = lib.tk_conv_acyclic if acyclic else lib.tk_conv_cyclic
+        _check(fn(self.ctx.h, self.h, out.h, inp.h))
         return out
 
-    def conv_pair(self, oa, ob, ia, ib):
-        _check(lib.tk_conv_cyclic_pair(self.ctx.h, self.h, oa.h, ob.h, ia.h, ib.h))
+    def conv_pair(self, oa, ob, ia, ib, acyclic=False):
+        fn = lib.tk_conv_acyclic_pair if acyclic else lib.tk_conv_cyclic_pair
+        _check(fn(self.ctx.h, self.h, oa.h, ob.h, ia.h, ib.h))
 
     def free(self):
         if self.h:

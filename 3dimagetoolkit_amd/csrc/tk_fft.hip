@@ -83,6 +83,13 @@ static int upload_axis(tk_filter_s* f, int a)
         TK_HIP(hipMalloc(&f->half_dev[a], sizeof(T) * (s + 1)));
         TK_HIP(hipMemcpy(f->half_dev[a], half.data(), sizeof(T) * (s + 1), hipMemcpyHostToDevice));
         ax.half = f->half_dev[a];
+        // all taps, for the generic direct pass (any method, and every acyclic pass that
+        // has no register window)
+        std::vector<T> taps(nh);
+        for (int n = -s; n <= s; ++n) taps[n + s] = half[n < 0 ? -n : n];
+        TK_HIP(hipMalloc(&f->taps_dev[a], sizeof(T) * nh));
+        TK_HIP(hipMemcpy(f->taps_dev[a], taps.data(), sizeof(T) * nh, hipMemcpyHostToDevice));
+        ax.taps = f->taps_dev[a];
         if (sizeof(T) == 4 && direct_width_supported(nh)) {
             // A operand of the matrix-core window, lane l = n + 16 g of K-step ks:
             // w(|4 ks + g - s - n|) inside the band, 0 outside
@@ -164,7 +171,7 @@ static int upload_axis(tk_filter_s* f, int a)
 // run the convolution passes over the axes of length > 1.  The first pass loads with
 // `ld`, the last stores with `st`; intermediate passes go through the complex buffer.
 template <class T, class Ld, class St>
-static int conv_passes(tk_ctx ctx, const tk_filter_s* f, Ld ld, St st)
+static int conv_passes(tk_ctx ctx, const tk_filter_s* f, Ld ld, St st, bool acyclic)
 {
     int axes[3], na = 0;
     for (int a = 0; a < 3; ++a)
@@ -176,24 +183,28 @@ static int conv_passes(tk_ctx ctx, const tk_filter_s* f, Ld ld, St st)
     cplx<T>* C = (cplx<T>*)wk;
     for (int q = 0; q < na; ++q) {
         const bool first = q == 0, last = q == na - 1;
-        if (first && last) TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, st)));
+        if (first && last)
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, st, acyclic)));
         else if (first)
-            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, StComplex<T>{C})));
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, ld, MdNone{}, StComplex<T>{C},
+                                        acyclic)));
         else if (last)
-            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, LdComplex<T>{C}, MdNone{}, st)));
+            TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, LdComplex<T>{C}, MdNone{}, st,
+                                        acyclic)));
         else
             TK_TRY((launch_conv_axis<T>(ctx, f, axes[q], 1, LdComplex<T>{C}, MdNone{},
-                                        StComplex<T>{C})));
+                                        StComplex<T>{C}, acyclic)));
     }
     return TK_OK;
 }
 
 template <class T>
-static int conv_pair(tk_ctx ctx, const tk_filter_s* f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib)
+static int conv_pair(tk_ctx ctx, const tk_filter_s* f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib,
+                     bool acyclic)
 {
     LdPair<T> ld{ia ? (const T*)ia->d : nullptr, ib ? (const T*)ib->d : nullptr, (T)0, (T)0};
     StPair<T> st{oa ? (T*)oa->d : nullptr, ob ? (T*)ob->d : nullptr};
-    return conv_passes<T>(ctx, f, ld, st);
+    return conv_passes<T>(ctx, f, ld, st, acyclic);
 }
 
 }  // namespace tk
@@ -241,6 +252,7 @@ int tk_filter_free(tk_filter f)
         if (f->spec_dev[a]) (void)hipFree(f->spec_dev[a]);
         if (f->half_dev[a]) (void)hipFree(f->half_dev[a]);
         if (f->band_dev[a]) (void)hipFree(f->band_dev[a]);
+        if (f->taps_dev[a]) (void)hipFree(f->taps_dev[a]);
     }
     delete f;
     return TK_OK;
@@ -353,8 +365,8 @@ int tk_conv_cyclic(tk_ctx ctx, tk_filter f, tk_vol out, tk_vol in)
     TK_CHECK(ctx && f, TK_ERR_ARG, "tk_conv_cyclic: null argument");
     TK_TRY(check_conv(f, out));
     TK_TRY(check_conv(f, in));
-    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, out, nullptr, in, nullptr);
-    return conv_pair<float>(ctx, f, out, nullptr, in, nullptr);
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, out, nullptr, in, nullptr, false);
+    return conv_pair<float>(ctx, f, out, nullptr, in, nullptr, false);
 }
 
 int tk_conv_cyclic_pair(tk_ctx ctx, tk_filter f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib)
@@ -364,8 +376,28 @@ int tk_conv_cyclic_pair(tk_ctx ctx, tk_filter f, tk_vol oa, tk_vol ob, tk_vol ia
     TK_TRY(check_conv(f, ob));
     TK_TRY(check_conv(f, ia));
     TK_TRY(check_conv(f, ib));
-    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, oa, ob, ia, ib);
-    return conv_pair<float>(ctx, f, oa, ob, ia, ib);
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, oa, ob, ia, ib, false);
+    return conv_pair<float>(ctx, f, oa, ob, ia, ib, false);
+}
+
+int tk_conv_acyclic(tk_ctx ctx, tk_filter f, tk_vol out, tk_vol in)
+{
+    TK_CHECK(ctx && f, TK_ERR_ARG, "tk_conv_acyclic: null argument");
+    TK_TRY(check_conv(f, out));
+    TK_TRY(check_conv(f, in));
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, out, nullptr, in, nullptr, true);
+    return conv_pair<float>(ctx, f, out, nullptr, in, nullptr, true);
+}
+
+int tk_conv_acyclic_pair(tk_ctx ctx, tk_filter f, tk_vol oa, tk_vol ob, tk_vol ia, tk_vol ib)
+{
+    TK_CHECK(ctx && f, TK_ERR_ARG, "tk_conv_acyclic_pair: null argument");
+    TK_TRY(check_conv(f, oa));
+    TK_TRY(check_conv(f, ob));
+    TK_TRY(check_conv(f, ia));
+    TK_TRY(check_conv(f, ib));
+    if (f->dtype == TK_F64) return conv_pair<double>(ctx, f, oa, ob, ia, ib, true);
+    return conv_pair<float>(ctx, f, oa, ob, ia, ib, true);
 }
 
 }  // extern "C"

@@ -60,6 +60,8 @@ struct AxisDev {
     const void* spec = nullptr;  // T[L], real spectrum / L (FFT path) or T[nh] taps (direct)
     const void* half = nullptr;  // T[nh/2 + 1]: w(0), w(1), .. w(s) of the direct path
     const void* band = nullptr;  // FP32 [KS][64]: the matrix-core window's band operand
+    const void* taps = nullptr;  // T[nh]: the normalised taps w(-s) .. w(s)
+    int zero_halo = 0;           // set per launch: 1 = acyclic (zero outside the volume)
 };
 
 // how the pencils of one axis sit in memory
@@ -97,5 +99,6 @@ struct tk_filter_s {
     void* tw_dev[3] = {nullptr, nullptr, nullptr};
     void* half_dev[3] = {nullptr, nullptr, nullptr};
     void* band_dev[3] = {nullptr, nullptr, nullptr};
+    void* taps_dev[3] = {nullptr, nullptr, nullptr};
     tk::AxisDev ax[3];
 };

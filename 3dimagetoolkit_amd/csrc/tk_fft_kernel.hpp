@@ -152,7 +152,8 @@ __device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n
 // SQ = L+1 (pencils as padded rows), so the contiguous global rows map to LDS rows.
 struct Lay {
     int L, P, lp, SP, SQ;
-    int off;  // LDS index of position 0 (the direct path keeps a wrapped halo before it)
+    int off;   // LDS index of position 0 (the direct path keeps a halo before it)
+    int zero;  // acyclic pass: the halo holds zeros instead of the wrapped positions
 };
 
 // one Stockham stage; butterfly b handles pencil p = b mod P at index j = b / P
@@ -234,6 +235,7 @@ __device__ __forceinline__ void direct_lds(cplx<T>* buf, const T* w, const AxisD
             const int p = e & (y.P - 1), pos = e >> y.lp;
             T ax_ = 0, ay = 0;
             for (int n = -s; n <= s; ++n) {
+                if (y.zero && (pos - n < 0 || pos - n >= L)) continue;
                 int m = (pos - n) % L;
                 if (m < 0) m += L;
                 const cplx<T> v = buf[m * y.SP + p * y.SQ];
@@ -269,13 +271,14 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
 {
     constexpr int B = 4, W = B + 2 * S, MAXI = (EPT + B - 1) / B;
     const int L = y.L;
-    // halo: positions -S..-1 and L..L+S-1 hold x((pos mod L))
+    // halo: positions -S..-1 and L..L+S-1 hold x((pos mod L)), or zeros (acyclic)
     for (int e = threadIdx.x; e < 2 * S * y.P; e += NT) {
         const int p = e & (y.P - 1), t = e >> y.lp;
         const int pos = t < S ? t - S : L + (t - S);
         int src = pos % L;
         if (src < 0) src += L;
-        buf[y.off + pos * y.SP + p * y.SQ] = buf[y.off + src * y.SP + p * y.SQ];
+        buf[y.off + pos * y.SP + p * y.SQ] =
+            y.zero ? cplx<T>{0, 0} : buf[y.off + src * y.SP + p * y.SQ];
     }
     __syncthreads();
     T w[S + 1];
@@ -355,7 +358,8 @@ __device__ __forceinline__ void mfma_win(cplx<float>* buf, const float* band, co
         int src = pos;
         while (src < 0) src += L;
         while (src >= L) src -= L;
-        buf[y.off + pos * y.SP + p * y.SQ] = buf[y.off + src * y.SP + p * y.SQ];
+        buf[y.off + pos * y.SP + p * y.SQ] =
+            y.zero ? cplx<float>{0.f, 0.f} : buf[y.off + src * y.SP + p * y.SQ];
     }
     const int lane = threadIdx.x & 63;
     float A[KS];
@@ -439,7 +443,7 @@ template <class T, int EPT, int NT>
 __device__ __forceinline__ void conv_lds(cplx<T>* buf, const cplx<T>* tw, const T* spec,
                                          const AxisDev& ax, const Lay& y)
 {
-    if (ax.nstage == 0) {
+    if (ax.nstage == 0 || y.zero) {
         direct_lds<T, EPT, NT>(buf, spec, ax, y);
         return;
     }
@@ -660,19 +664,20 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     y.SP = conv_sp(g.axis, g.P, S);
     y.SQ = g.axis == 2 ? ax.L + 2 * S + 1 : 1;
     y.off = S * y.SP;
+    y.zero = ax.zero_halo;
     const int nbuf = conv_nbuf(g.axis, ax.L, g.P, S);
     cplx<T>* buf = (cplx<T>*)smem;
     cplx<T>* tw = buf + nbuf;
     T* spec = (T*)(tw + ax.L);
     if constexpr (S > 0) {
         for (int t = threadIdx.x; t <= S; t += NT) spec[t] = ((const T*)ax.half)[t];
-    } else if (ax.nstage > 0) {
+    } else if (ax.nstage > 0 && !ax.zero_halo) {
         for (int t = threadIdx.x; t < ax.L; t += NT) {
             tw[t] = ((const cplx<T>*)ax.tw)[t];
             spec[t] = ((const T*)ax.spec)[t];
         }
     } else {
-        for (int t = threadIdx.x; t < ax.nh; t += NT) spec[t] = ((const T*)ax.spec)[t];
+        for (int t = threadIdx.x; t < ax.nh; t += NT) spec[t] = ((const T*)ax.taps)[t];
     }
     // consecutive groups share 128-byte lines along x and y: keep them on one XCD
     const long c0 = (long)xcd_remap(blockIdx.x, gridDim.x) * g.P;
@@ -767,11 +772,15 @@ inline bool mfma_window_fits(int axis, const int dims[3])
 }
 
 template <class T, class Ld, class Md, class St>
-int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld ld, Md md, St st)
+int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld ld, Md md, St st,
+                     bool acyclic = false)
 {
     constexpr int NT = 256;
     constexpr int EPT = conv_ept<T>();
-    const AxisDev& ax = f->ax[axis];
+    // acyclic (zero outside the volume): direct passes with zero halos; an axis on the FFT
+    // method runs the register window where one is compiled, else the generic direct pass
+    AxisDev ax = f->ax[axis];
+    ax.zero_halo = acyclic ? 1 : 0;
     const bool dims4 = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0;
     // the matrix-core window (checked when the method was chosen; checked again here since
     // the group shape decides the tiling)
@@ -784,7 +793,7 @@ int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld 
     unsigned nb = (unsigned)((g.nlines + g.P - 1) / g.P);
     const bool quad = dims4 && g.P >= 4;
     // register-window direct path for the instantiated half-widths (sigma 3, 4, 6 at h=1)
-    const int S = ax.direct && quad ? ax.nh / 2 : 0;
+    const int S = (ax.direct || (acyclic && direct_width_supported(ax.nh))) && quad ? ax.nh / 2 : 0;
     auto lds_for = [&](int s) {
         return (size_t)conv_nbuf(axis, ax.L, g.P, s) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
                (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);

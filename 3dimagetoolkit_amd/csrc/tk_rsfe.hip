@@ -239,31 +239,34 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
     const tk_rsfe_params& p = s->p;
     cplx<T>* C = (cplx<T>*)s->C;
     const tk_filter_s* f = s->flt;
+    const bool acy = p.acyclic != 0;
     // conv pair 1: x, y, then z with the region fit in the middle
     LdHeaviside<T> ld{(const T*)s->phi->d, (const T*)s->u0->d, (T*)s->phiold->d,
                       (T)((p.a0 + p.b0) / 2.), (T)p.epsilon};
     {
         TimedLaunch tl(ctx, TK_K_CONV_X1);
-        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C})));
+        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C}, acy)));
     }
     {
         TimedLaunch tl(ctx, TK_K_CONV_Y);
-        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C},
+                                    acy)));
     }
     MdRegionFit<T> md{(const T*)s->kmyim->d, (const T*)s->kones->d, (T)p.lamda1, (T)p.lamda2};
     {
         TimedLaunch tl(ctx, TK_K_CONV_Z2);
-        TK_TRY((launch_conv_axis<T>(ctx, f, 2, 2, LdComplex<T>{C}, md, StComplex<T>{C})));
+        TK_TRY((launch_conv_axis<T>(ctx, f, 2, 2, LdComplex<T>{C}, md, StComplex<T>{C}, acy)));
     }
     {
         TimedLaunch tl(ctx, TK_K_CONV_Y);
-        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+        TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C},
+                                    acy)));
     }
     StRhs<T> sr{(const T*)s->u0->d, (const T*)s->kones->d, (const T*)s->divdb->d, (T*)s->rhs->d,
                 (T)(p.lamda1 - p.lamda2), (T)p.lamda};
     {
         TimedLaunch tl(ctx, TK_K_CONV_X2);
-        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, LdComplex<T>{C}, MdNone{}, sr)));
+        TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, LdComplex<T>{C}, MdNone{}, sr, acy)));
     }
     // Poisson loop (rsfe_splitbregman_imp.hxx:1735-1797): bc value, then the sweeps from
     // the previous iterate (or from the learned guess when its residual is lower)
@@ -356,14 +359,14 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
 // kones = K*1 and kmyim = K*u0 as one complex transform
 template <class T>
 int conv_passes_pair_ones(tk_ctx ctx, const tk_filter_s* f, tk_vol kones, tk_vol kmyim, tk_vol u0,
-                          void* work)
+                          void* work, bool acy)
 {
     cplx<T>* C = (cplx<T>*)work;
     LdPair<T> ld{nullptr, (const T*)u0->d, (T)1, (T)0};
-    TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C})));
-    TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C})));
+    TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, ld, MdNone{}, StComplex<T>{C}, acy)));
+    TK_TRY((launch_conv_axis<T>(ctx, f, 1, 1, LdComplex<T>{C}, MdNone{}, StComplex<T>{C}, acy)));
     StPair<T> st{(T*)kones->d, (T*)kmyim->d};
-    TK_TRY((launch_conv_axis<T>(ctx, f, 2, 1, LdComplex<T>{C}, MdNone{}, st)));
+    TK_TRY((launch_conv_axis<T>(ctx, f, 2, 1, LdComplex<T>{C}, MdNone{}, st, acy)));
     return TK_OK;
 }
 
@@ -472,6 +475,7 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     p->ls_res_tol = 0;
     p->res_check_every = 64;
     p->ws_presweeps = 512;
+    p->acyclic = 0;
     return TK_OK;
 }
 
@@ -528,9 +532,11 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     // kones = cv(1), kmyim = cv(myim) in one complex transform (imp:1648-1651)
     if (r == TK_OK) {
         if (dt == TK_F64)
-            r = conv_passes_pair_ones<double>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C);
+            r = conv_passes_pair_ones<double>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
+                                              p->acyclic != 0);
         else
-            r = conv_passes_pair_ones<float>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C);
+            r = conv_passes_pair_ones<float>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
+                                             p->acyclic != 0);
     }
     // gforshrink = edge_detect(myim) (imp:1655)
     if (r == TK_OK)

@@ -128,6 +128,17 @@ int tk_conv_cyclic(tk_ctx ctx, tk_filter flt, tk_vol out, tk_vol in);
 int tk_conv_cyclic_pair(tk_ctx ctx, tk_filter flt, tk_vol out_a, tk_vol out_b, tk_vol in_a,
                         tk_vol in_b);
 
+/* ---- R5, acyclic: zero outside the volume -------------------------------------------
+ * conv::acyclic_fftw_convolution (convolution_imp.hxx:535-653), the reference's default cv
+ * (rsfe_splitbregman_imp.hxx:25-41): the image zero-padded to dims + nh - 1, the padded
+ * spectrum applied, the first dims voxels kept.  That equals the separable convolution with
+ * the normalised taps and zero values outside the volume (conv::direct_convolution,
+ * convolution_imp.hxx:761-812), which is what runs here, for every method of the filter
+ * (direct passes whose halos hold zeros).  out may alias in. */
+int tk_conv_acyclic(tk_ctx ctx, tk_filter flt, tk_vol out, tk_vol in);
+int tk_conv_acyclic_pair(tk_ctx ctx, tk_filter flt, tk_vol out_a, tk_vol out_b, tk_vol in_a,
+                         tk_vol in_b);
+
 /* ---- R8 / R9: Dirichlet Gauss-Seidel Poisson sweeps -------------------------------
  * lapl::DirGaussSeidel (poisson_imp.hxx:340-429) behind
  * lapl::unsteady_poisson_functor<T>::operator() (poisson.hxx:90-96), and the
@@ -227,6 +238,8 @@ typedef struct tk_rsfe_params {
                                  reference counterpart; the learned-start evaluation) */
     int res_check_every;      /* sweeps between residual checks (default 64) */
     int ws_presweeps;         /* R15: sweeps before each learned round (default 512) */
+    int acyclic;              /* 1: set_cv(acyclic_fftw_convolution), the reference's default
+                                 cv (imp:25-41); 0 (default here): cyclic_fftw_convolution */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {

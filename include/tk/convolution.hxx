@@ -1,6 +1,6 @@
 /*
- * tk/convolution.hxx -- conv::filtering<T> and conv::cyclic_fftw_convolution over the
- * device filters of libtk.
+ * tk/convolution.hxx -- conv::filtering<T>, conv::cyclic_fftw_convolution and
+ * conv::acyclic_fftw_convolution over the device filters of libtk.
  *
  * Mirrors conv::filtering<T> (header/convolution.hxx:65-315): a functor holding a plain
  * function pointer void (*f)(image3d<T>& result, image3d<T> const& factor) (79), its
@@ -8,7 +8,9 @@
  * class-wide static state built by the static build_gaussian_filter (226-229) and
  * build_freq_filters (299).  cyclic_fftw_convolution (391; convolution_imp.hxx:657-757) is
  * the circular convolution of factor with that filter: it resizes result to the factor's
- * dims and copies its spacing (convolution_imp.hxx:678-686).
+ * dims and copies its spacing (convolution_imp.hxx:678-686).  acyclic_fftw_convolution
+ * (368; convolution_imp.hxx:535-653) is the convolution with zero values outside the
+ * volume (the reference pads to dims + Nh - 1 and keeps the first dims voxels).
  *
  * The static state is per thread here (the reference's is per process and not re-entrant):
  * the Gaussian parameters of the last build_gaussian_filter and a device filter built for
@@ -86,7 +88,7 @@ public:
             s.dims[1] != factor.getdimy() || s.dims[2] != factor.getdimz() ||
             s.h[0] != factor.gethx() || s.h[1] != factor.gethy() || s.h[2] != factor.gethz()) {
             if (s.flt)
-                tk::warning("cyclic_fftw_convolution: factor dims differ from the filter's; "
+                tk::warning("fftw_convolution: factor dims differ from the filter's; "
                             "rebuilding the frequency filter");
             build_freq_filters(factor);
         }
@@ -125,6 +127,17 @@ void cyclic_fftw_convolution(im3d::image3d<T>& result, im3d::image3d<T> const& f
     result.shape_like(factor);
     tk::check(tk_conv_cyclic(factor.ctx(), flt, result.handle(), factor.handle()),
               "cyclic_fftw_convolution");
+}
+
+// result = factor (*) filter, zero outside the volume (convolution_imp.hxx:535-653)
+template <typename T>
+void acyclic_fftw_convolution(im3d::image3d<T>& result, im3d::image3d<T> const& factor)
+{
+    tk_filter flt = filtering<T>::filter_for(factor);
+    if (!flt) return;
+    result.shape_like(factor);
+    tk::check(tk_conv_acyclic(factor.ctx(), flt, result.handle(), factor.handle()),
+              "acyclic_fftw_convolution");
 }
 
 }  // namespace conv

@@ -9,9 +9,9 @@
  * with the reference's validation and WARNING-and-keep behaviour (imp:990-1330), set_cv and
  * set_onestep_poisson (imp:836-852), set_bc (imp:1294-1322), and apply (imp:1547-1995).
  *
- * What this layer runs: cv = conv::cyclic_fftw_convolution (the reference's default cv is
- * the acyclic FFT convolution, not built here: set_cv with any other function keeps the
- * cyclic one and prints a WARNING), onestep_poisson = lapl::DirGaussSeidel (bit-exact
+ * What this layer runs: cv = conv::acyclic_fftw_convolution (the reference's default) or
+ * conv::cyclic_fftw_convolution (set_cv with any other function keeps the current one and
+ * prints a WARNING), onestep_poisson = lapl::DirGaussSeidel (bit-exact
  * lexicographic sweeps) or lapl::DirRedBlackGaussSeidel (red-black), edge detector GRAD
  * with edge_detector_sigma = 0.  set_bc(NEU) would switch the reference to NeuGaussSeidel,
  * which is not built: it prints a WARNING and keeps the Dirichlet sweep (the reference's
@@ -49,7 +49,7 @@ public:
     enum ed_type { GRAD = 1, GRAD_THRESHOLD, LBP };
 
     // constructor defaults (rsfe_splitbregman_imp.hxx:25-41)
-    rsfe_splitbregman() : cv_(conv::cyclic_fftw_convolution<T>), onestep_(lapl::DirGaussSeidel<T>)
+    rsfe_splitbregman() : cv_(conv::acyclic_fftw_convolution<T>), onestep_(lapl::DirGaussSeidel<T>)
     {
         tk_rsfe_default_params(&p_);
         this->alpha = (T)0.5;
@@ -134,8 +134,11 @@ public:
     }
     void set_cv(conv::filtering<T> const cv)
     {
-        if (cv.getf() == &conv::cyclic_fftw_convolution<T>) cv_ = cv;
-        else tk::warning("set_cv: only conv::cyclic_fftw_convolution is built; kept");
+        if (cv.getf() == &conv::cyclic_fftw_convolution<T> ||
+            cv.getf() == &conv::acyclic_fftw_convolution<T>)
+            cv_ = cv;
+        else tk::warning("set_cv: only conv::acyclic_fftw_convolution and "
+                         "conv::cyclic_fftw_convolution are built; kept");
     }
     void set_onestep_poisson(lapl::unsteady_poisson_functor<T> const osl)
     {
@@ -173,6 +176,7 @@ private:
         tk_rsfe_params p = p_;
         p.gs_order = onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ? TK_GS_REDBLACK
                                                                            : TK_GS_LEX;
+        p.acyclic = cv_.getf() == &conv::acyclic_fftw_convolution<T> ? 1 : 0;
         tk_rsfe s = nullptr;
         tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
         sweeps_.clear();

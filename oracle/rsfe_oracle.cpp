@@ -101,10 +101,12 @@ struct FftPlan {
         cd* a = x;
         cd* b = tmp;
         int Ns = 1;
-        cd v[64];
+        std::vector<cd> v, o;
         for (size_t st = 0; st < radix.size(); ++st) {
             int R = radix[st];
             int NR = n / R;
+            v.resize(R);
+            o.resize(R);
             for (int j = 0; j < NR; ++j) {
                 int k = j % Ns;
                 for (int r = 0; r < R; ++r) {
@@ -112,7 +114,6 @@ struct FftPlan {
                     v[r] = a[j + r * NR] * w[t % n];
                 }
                 /* size-R DFT */
-                cd o[64];
                 for (int q = 0; q < R; ++q) {
                     cd s = 0;
                     for (int r = 0; r < R; ++r) s += v[r] * w[((long)r * q * (n / R)) % n];
@@ -274,6 +275,68 @@ void or_cyclic_conv(const double* spec, const int nc[3], const double* in, int d
                 out[idx(i, j, k, dy, dz)] =
                     (buf[idx(i, j, k, Ny, Nz)].real()) * pixel_measure / (Nx * Ny * Nz);
     g_t_fft += now_s() - t0;
+}
+
+/* ---- R5, acyclic: conv::acyclic_fftw_convolution (convolution_imp.hxx:535-653), the
+   reference's default cv.  Zero-padded transform size dims + Nh - 1 per axis (244-246);
+   the spectrum is the same wrapped 8-block placement on that size (278-402), so
+   or_cyclic_spectrum(tf, nh, na, spec) builds it.  The input sits at [0, dims) in a zero
+   array, forward FFT, multiply, inverse FFT, [0, dims) times h^3 / (Nx Ny Nz). */
+void or_acyclic_dims(int nx, int ny, int nz, const int nh[3], int na_out[3])
+{
+    na_out[0] = nx + nh[0] - 1;
+    na_out[1] = ny + nh[1] - 1;
+    na_out[2] = nz + nh[2] - 1;
+}
+
+void or_acyclic_conv(const double* spec, const int na[3], const double* in, int dx, int dy,
+                     int dz, double hx, double hy, double hz, double* out)
+{
+    double t0 = now_s();
+    int Nx = na[0], Ny = na[1], Nz = na[2];
+    long n = (long)Nx * Ny * Nz;
+    std::vector<cd> buf(n, cd(0., 0.));
+#pragma omp parallel for
+    for (int i = 0; i < dx; ++i)
+        for (int j = 0; j < dy; ++j)
+            for (int k = 0; k < dz; ++k)
+                buf[idx(i, j, k, Ny, Nz)] = cd(in[idx(i, j, k, dy, dz)], 0.);
+    fft3d(buf.data(), Nx, Ny, Nz, -1);
+#pragma omp parallel for
+    for (long t = 0; t < n; ++t) buf[t] = cd(buf[t].real() * spec[t], buf[t].imag() * spec[t]);
+    fft3d(buf.data(), Nx, Ny, Nz, +1);
+    double const pixel_measure = hx * hy * hz;
+#pragma omp parallel for
+    for (int i = 0; i < dx; ++i)
+        for (int j = 0; j < dy; ++j)
+            for (int k = 0; k < dz; ++k)
+                out[idx(i, j, k, dy, dz)] =
+                    (buf[idx(i, j, k, Ny, Nz)].real()) * pixel_measure / (Nx * Ny * Nz);
+    g_t_fft += now_s() - t0;
+}
+
+/* closed form of the acyclic R5 (conv::direct_convolution, convolution_imp.hxx:761-812):
+   y(p) = sum_q tf(q) * x(p - q + s) with x = 0 outside the volume */
+void or_direct_acyclic_conv(const double* tf, const int nh[3], const double* in, int nx, int ny,
+                            int nz, double* out)
+{
+    int sx = nh[0] / 2, sy = nh[1] / 2, sz = nh[2] / 2;
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                long double s = 0;
+                for (int a = 0; a < nh[0]; ++a)
+                    for (int b = 0; b < nh[1]; ++b)
+                        for (int c = 0; c < nh[2]; ++c) {
+                            int ii = i - (a - sx), jj = j - (b - sy), kk = k - (c - sz);
+                            if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kk < 0 || kk >= nz)
+                                continue;
+                            s += (long double)tf[idx(a, b, c, nh[1], nh[2])] *
+                                 in[idx(ii, jj, kk, ny, nz)];
+                        }
+                out[idx(i, j, k, ny, nz)] = (double)s;
+            }
 }
 
 /* closed form of R5: y(p) = sum_q tf(q) * x(p - q + s) circularly, times h^3 --
@@ -666,7 +729,7 @@ void or_default_params(or_params* p)
     p->sigma = 3; p->lamda = 5.e-2; p->lamda1 = 1.e-5; p->lamda2 = 1.e-5;
     p->beta = 100.; p->dt = 1; p->ls_steps = 5; p->epsilon = 1.e-3; p->a0 = 0.; p->b0 = 1.;
     p->nu = 1.; p->tol = 1.e-5; p->ls_tol = 0; p->maxiter = 100; p->bc_dir = 0;
-    p->gaussian_pixel_approach = 0; p->radius = 3; p->verbosity = 0;
+    p->gaussian_pixel_approach = 0; p->radius = 3; p->verbosity = 0; p->acyclic = 0;
 }
 
 /* ---- R6 + R7: forcing term of one outer iteration (imp:1700-1759) ----------------
@@ -744,11 +807,15 @@ void or_rsfe_forcing(const double* myim, const double* phi, const double* divdb,
     std::vector<double> tf((long)nh[0] * nh[1] * nh[2]);
     or_gaussian_time_filter(Z, hx, hy, hz, p->sigma, p->radius, p->gaussian_pixel_approach,
                             tf.data());
-    or_cyclic_dims(X, Y, Z, nh, nc);
+    if (p->acyclic) or_acyclic_dims(X, Y, Z, nh, nc);
+    else or_cyclic_dims(X, Y, Z, nh, nc);
     std::vector<double> spec((long)nc[0] * nc[1] * nc[2]);
     or_cyclic_spectrum(tf.data(), nh, nc, spec.data());
     auto cv = [&](std::vector<double>& out, const std::vector<double>& in) {
-        or_cyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
+        if (p->acyclic)
+            or_acyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
+        else
+            or_cyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
     };
     std::vector<double> ones(n, 1.), u0(myim, myim + n), kones(n), kmyim(n);
     cv(kones, ones);
@@ -824,12 +891,17 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
     V tf((long)nh[0] * nh[1] * nh[2]);
     or_gaussian_time_filter(Z, hx, hy, hz, p->sigma, p->radius, p->gaussian_pixel_approach,
                             tf.data());
-    or_cyclic_dims(X, Y, Z, nh, nc);
+    /* cv: cyclic_fftw_convolution, or acyclic_fftw_convolution (the reference's default) */
+    if (p->acyclic) or_acyclic_dims(X, Y, Z, nh, nc);
+    else or_cyclic_dims(X, Y, Z, nh, nc);
     V spec((long)nc[0] * nc[1] * nc[2]);
     or_cyclic_spectrum(tf.data(), nh, nc, spec.data());
 
     auto cv = [&](V& out, const V& in) {
-        or_cyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
+        if (p->acyclic)
+            or_acyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
+        else
+            or_cyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
     };
     {
         V aux(n, 1.);

@@ -37,6 +37,14 @@ void or_direct_cyclic_conv(const double* tf, const int nh[3], const double* in,
 
 /* plain 1-D complex DFT helpers (forward sign -1), used by the pins */
 void or_fft_inplace(double* re, double* im, int n, int sign);
+/* R5 acyclic (convolution_imp.hxx:244-246, 278-402, 535-653): padded dims dims + Nh - 1;
+   the spectrum comes from or_cyclic_spectrum on those dims */
+void or_acyclic_dims(int nx, int ny, int nz, const int nh[3], int na_out[3]);
+void or_acyclic_conv(const double* spec, const int na[3], const double* in,
+                     int dx, int dy, int dz, double hx, double hy, double hz, double* out);
+/* closed form of the acyclic R5 (convolution_imp.hxx:761-812), zero outside the volume */
+void or_direct_acyclic_conv(const double* tf, const int nh[3], const double* in,
+                            int nx, int ny, int nz, double* out);
 
 /* ---- R9: one Dirichlet Gauss-Seidel sweep (poisson_imp.hxx:340-429) -------- */
 void or_dir_gauss_seidel(double* phi, const double* rhs, int nx, int ny, int nz,
@@ -86,6 +94,7 @@ typedef struct or_params {
     int bc_dir;             /* 0: flag NEU (boundary value 0), 1: DIR (value a0) */
     int gaussian_pixel_approach;
     int verbosity;          /* 1 adds the extra phi copy/norm of imp:1776-1793 (timing only) */
+    int acyclic;            /* cv: 0 cyclic_fftw_convolution, 1 acyclic_fftw_convolution */
 } or_params;
 
 typedef struct or_stats {

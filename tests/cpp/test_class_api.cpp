@@ -99,6 +99,17 @@ static void run_cases()
         report("conv::filtering cyclic_fftw_convolution", maxabs(out.download(), ref), 1e-13);
         report("convolution result dims", (out.getdimx() == X && out.getdimy() == Y &&
                                            out.getdimz() == Z) ? 0 : 1, 0);
+
+        // acyclic_fftw_convolution: the oracle's padded-FFT form
+        cv.changef(conv::acyclic_fftw_convolution<double>);
+        im3d::image3d<double> outa;
+        cv(outa, f);
+        int na[3];
+        or_acyclic_dims(X, Y, Z, nh, na);
+        std::vector<double> speca((long)na[0] * na[1] * na[2]), refa(n);
+        or_cyclic_spectrum(tf.data(), nh, na, speca.data());
+        or_acyclic_conv(speca.data(), na, raw.data(), X, Y, Z, 1., 1., 1., refa.data());
+        report("conv::filtering acyclic_fftw_convolution", maxabs(outa.download(), refa), 1e-13);
     }
 
     // ---- lapl::unsteady_poisson_functor(DirGaussSeidel): bitwise --------------------------
@@ -167,7 +178,7 @@ static void run_cases()
             sweeps_equal &= rsfe.get_sweeps()[t] == sw[t];
         report("rsfe_splitbregman sweep counts", sweeps_equal ? 0 : 1, 0);
 
-        // apply(myim): cube initial contour
+        // apply(myim): cube initial contour, the default cv (acyclic_fftw_convolution)
         segm::rsfe_splitbregman<double> cube;
         cube.set_verbosity(false);
         cube.set_sigma(2.0);
@@ -177,6 +188,7 @@ static void run_cases()
         std::vector<double> cinit(m), cref(m);
         or_init_phi(nullptr, 0, 0, 0, SX, SY, SZ, 0., 1., cinit.data());
         p.maxiter = 3;
+        p.acyclic = 1;
         or_rsfe_apply(u255.data(), cinit.data(), SX, SY, SZ, 1., 1., 1., &p, cref.data(),
                       nullptr, nullptr, nullptr);
         report("rsfe_splitbregman apply(myim) cube FP64", maxabs(cube.getoutput().download(), cref),

@@ -12,7 +12,8 @@ class OrParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "sigma", "lamda", "lamda1", "lamda2", "beta", "dt", "epsilon", "a0", "b0", "nu", "tol",
         "ls_tol", "radius")] + [(n, C.c_int) for n in (
-            "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "verbosity")]
+            "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "verbosity",
+            "acyclic")]
 
 
 class OrStats(C.Structure):
@@ -79,6 +80,32 @@ def direct_cyclic_conv(tf, nh, x):
     x = np.ascontiguousarray(x, np.float64)
     out = np.zeros_like(x)
     lib().or_direct_cyclic_conv(dp(tf), ip(nh), dp(x), *[C.c_int(s) for s in x.shape], dp(out))
+    return out
+
+
+def acyclic_spectrum(tf, nh, shape):
+    """spectrum of conv::acyclic_fftw_convolution on the padded dims shape + nh - 1"""
+    na = np.zeros(3, np.int32)
+    lib().or_acyclic_dims(C.c_int(shape[0]), C.c_int(shape[1]), C.c_int(shape[2]), ip(nh),
+                          ip(na))
+    spec = np.zeros(tuple(na), np.float64)
+    lib().or_cyclic_spectrum(dp(tf), ip(nh), ip(na), dp(spec))
+    return spec, na
+
+
+def acyclic_conv(spec, na, x, h=(1.0, 1.0, 1.0)):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().or_acyclic_conv(dp(spec), ip(na), dp(x), C.c_int(x.shape[0]), C.c_int(x.shape[1]),
+                          C.c_int(x.shape[2]), C.c_double(h[0]), C.c_double(h[1]),
+                          C.c_double(h[2]), dp(out))
+    return out
+
+
+def direct_acyclic_conv(tf, nh, x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    lib().or_direct_acyclic_conv(dp(tf), ip(nh), dp(x), *[C.c_int(s) for s in x.shape], dp(out))
     return out
 
 

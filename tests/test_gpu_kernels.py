@@ -111,6 +111,56 @@ def test_conv_f32_matrix_cores_accuracy(tk, orc, ctx, rng):
     assert m2 < 1e-6 and r2 < 1.5 * r1 and abs(b2) < 0.2 * r2
 
 
+# acyclic_fftw_convolution (the reference's default cv): zero outside the volume.  The
+# oracle pads to dims + nh - 1 and runs the FFT; the device runs direct passes with zero
+# halos (the generic taps pass in FP64 unless a register window is compiled for the width).
+@pytest.mark.parametrize("shape,sigma,h", CONV_CASES)
+def test_conv_acyclic_f64(tk, orc, ctx, shape, sigma, h, rng):
+    say("conv acyclic f64", shape, sigma, h)
+    x = rng.random(shape)
+    tf, nh, _, _ = orc.gaussian(shape, sigma, 3.0, False, h)
+    spec, na = orc.acyclic_spectrum(tf, nh, shape)
+    ref = orc.acyclic_conv(spec, na, x, h)
+    f = tk.Filter(ctx, shape, sigma, h=h)
+    vx = ctx.from_numpy(x, h=h)
+    vy = ctx.volume(shape, h=h)
+    f.conv(vy, vx, acyclic=True)
+    n_pad = int(np.prod(na))
+    assert np.max(np.abs(vy.numpy() - ref)) < 4e-15 * np.log2(n_pad) * np.max(np.abs(x))
+    # pair, outputs aliasing inputs
+    y2 = rng.standard_normal(shape)
+    va, vb = ctx.from_numpy(x, h=h), ctx.from_numpy(y2, h=h)
+    f.conv_pair(va, vb, va, vb, acyclic=True)
+    ref2 = orc.acyclic_conv(spec, na, y2, h)
+    assert np.max(np.abs(va.numpy() - ref)) < 4e-15 * np.log2(n_pad) * np.max(np.abs(y2))
+    assert np.max(np.abs(vb.numpy() - ref2)) < 4e-15 * np.log2(n_pad) * np.max(np.abs(y2))
+    # the boundary matters: the cyclic result differs by far more than the tolerance
+    vc = ctx.volume(shape, h=h)
+    f.conv(vc, vx)
+    assert np.max(np.abs(vc.numpy() - ref)) > 1e-6
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("shape,sigma", [((64, 64, 64), 3.0), ((48, 80, 96), 3.0),
+                                         ((32, 48, 64), 6.0), ((28, 21, 26), 1.5)])
+def test_conv_acyclic_f32(tk, orc, ctx, shape, sigma, method, rng):
+    say("conv acyclic f32", shape, sigma, method)
+    x = rng.random(shape)
+    tf, nh, _, _ = orc.gaussian(shape, sigma)
+    spec, na = orc.acyclic_spectrum(tf, nh, shape)
+    ref = orc.acyclic_conv(spec, na, x)
+    f = tk.Filter(ctx, shape, sigma, dtype=np.float32)
+    for a in range(3):
+        try:
+            f.set_method(a, method)
+        except tk.TkError:
+            pass  # no FFT plan (lengths with a factor 7) or no window for this width/shape
+    vx = ctx.from_numpy(x.astype(np.float32))
+    vy = ctx.volume(shape, np.float32)
+    f.conv(vy, vx, acyclic=True)
+    assert np.max(np.abs(vy.numpy().astype(np.float64) - ref)) < 1e-6
+
+
 def test_filter_autotune_keeps_results(tk, orc, ctx, rng):
     shape = (64, 48, 80)
     f = tk.Filter(ctx, shape, 3.0, dtype=np.float32)

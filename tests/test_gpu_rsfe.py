@@ -52,6 +52,31 @@ def test_loop_f64_matches_oracle_small(tk, orc, ctx, shape, ls_tol):
     assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
 
 
+@pytest.mark.parametrize("shape,sigma", [((24, 20, 28), 2.0), ((32, 32, 32), 3.0),
+                                         ((32, 28, 36), 3.0)])
+def test_loop_acyclic_f64_matches_oracle(tk, orc, ctx, shape, sigma):
+    """set_cv(acyclic_fftw_convolution), the reference's default cv: every convolution of
+    the loop (kones, kmyim, the region fit, the forcing) with zero outside the volume.
+    At sigma 3 (19 taps) on dims that are multiples of 4 the FP64 register window runs with
+    zero halos; 24x20x28 at sigma 2 runs the generic taps pass."""
+    say("loop acyclic f64", shape, sigma)
+    u0, phi0 = tk.gen_standard(shape, 7, 1, 3)
+    gs = tk.TK_GS_LEX
+    p = orc.default_params(sigma=sigma, maxiter=5, tol=0.0, ls_steps=5, ls_tol=0.0, acyclic=1)
+    ref, sw, tl, _ = orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0, p)
+    phi, st = run_gpu(tk, ctx, u0, phi0, np.float64, 5, sigma=sigma, maxiter=5, tol=0.0,
+                      ls_steps=5, ls_tol=0.0, gs_order=gs, acyclic=1)
+    err = np.max(np.abs(phi - ref))
+    say(f"loop acyclic {shape}: max|phi-ref| = {err:.3e}")
+    assert [s["sweeps"] for s in st] == list(sw)
+    assert err < 1e-9
+    assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
+    # and it is not the cyclic loop
+    phic, _ = run_gpu(tk, ctx, u0, phi0, np.float64, 5, sigma=sigma, maxiter=5, tol=0.0,
+                      ls_steps=5, ls_tol=0.0, gs_order=gs)
+    assert np.max(np.abs(phic - ref)) > 1e-6
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("init_shape", ["same", (15, 16, 11), (16, 15, 12), None, (10, 16, 12)])
 def test_init_phi_cases_vs_oracle(tk, orc, ctx, rng, init_shape, dtype):

@@ -204,6 +204,40 @@ def test_cyclic_conv_vs_direct_and_scipy(orc, shape, sigma, h, rng):
     assert np.max(np.abs(one - 2.5)) < 1e-14
 
 
+@pytest.mark.parametrize("shape,sigma,h", [
+    ((12, 10, 14), 1.0, (1.0, 1.0, 1.0)),
+    ((8, 9, 10), 1.2, (0.5, 1.0, 2.0)),
+    ((6, 7, 5), 2.0, (1.0, 1.0, 1.0)),
+    ((20, 9, 11), 3.0, (1.0, 1.0, 1.0)),
+])
+def test_acyclic_conv_vs_direct_and_scipy(orc, shape, sigma, h, rng):
+    """conv::acyclic_fftw_convolution (convolution_imp.hxx:535-653), the reference's default
+    cv: against the closed-form zero-padded convolution of direct_convolution (761-812) and
+    an independent separable route (scipy, zero boundary); K*1 is the product of the
+    per-axis partial weight sums"""
+    from scipy import ndimage
+    tf, nh, _, _ = orc.gaussian(shape, sigma, 3.0, False, h)
+    spec, na = orc.acyclic_spectrum(tf, nh, shape)
+    assert list(na) == [shape[a] + nh[a] - 1 for a in range(3)]
+    x = rng.random(shape)
+    y = orc.acyclic_conv(spec, na, x, h)
+    d = orc.direct_acyclic_conv(tf, nh, x) * (h[0] * h[1] * h[2])
+    assert np.max(np.abs(y - d)) < 1e-13 * np.max(np.abs(x))
+    z = x.copy()
+    for a in range(3):
+        z = ndimage.convolve1d(z, gauss1d(sigma, h[a], nh[a]), axis=a, mode="constant", cval=0.0)
+    assert np.max(np.abs(y - z)) < 1e-13
+    # K*1: 1 inside, the partial sums of the normalised taps near the faces
+    k1 = orc.acyclic_conv(spec, na, np.ones(shape), h)
+    ref = np.ones(shape)
+    for a in range(3):
+        w = gauss1d(sigma, h[a], nh[a])
+        s = nh[a] // 2
+        part = np.array([w[max(0, s - i): s + (shape[a] - i)].sum() for i in range(shape[a])])
+        ref = ref * part.reshape([-1 if b == a else 1 for b in range(3)])
+    assert np.max(np.abs(k1 - ref)) < 1e-13
+
+
 # ---- Gauss-Seidel ---------------------------------------------------------------------
 def gs_python(phi, rhs, dt, bc, h):
     """poisson_imp.hxx:370-425 evaluated with Python floats (IEEE double, no FMA)."""
@@ -442,6 +476,34 @@ def test_apply_matches_composition_of_pinned_pieces(orc, tk):
             d[a] = d[a] - b[a]
         divdb = orc.div(*d)
     assert np.array_equal(phi, phi_or)
+
+
+def test_apply_acyclic_matches_composition(orc, tk):
+    """the same composition with the reference's default cv, acyclic_fftw_convolution"""
+    shape = (14, 12, 10)
+    u0, phi0 = tk.gen_standard(shape, 11, 1, 2)
+    p = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0, acyclic=1)
+    phi_or, sweeps, tols, st = orc.rsfe_apply(u0, phi0, p)
+    phi = orc.change_range(phi0, 1.0, 0.0)
+    divdb = np.zeros(shape)
+    b = [np.zeros(shape) for _ in range(3)]
+    g = orc.edge_detect(u0, p.beta, p.lamda, p.nu)
+    for t in range(2):
+        rhs = orc.rsfe_forcing(u0, phi, divdb, p)
+        phi, s, _ = orc.poisson_loop(phi, rhs, 3, 0.0)
+        phi = np.clip(phi, 0.0, 1.0)
+        d = orc.grad(phi)
+        for a in range(3):
+            b[a] = b[a] + d[a]
+        d = [orc.shrink(b[a], g) for a in range(3)]
+        for a in range(3):
+            b[a] = b[a] - d[a]
+            d[a] = d[a] - b[a]
+        divdb = orc.div(*d)
+    assert np.array_equal(phi, phi_or)
+    # and it differs from the cyclic run (the boundary sees zeros instead of wrapped values)
+    pc = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0, acyclic=0)
+    assert not np.array_equal(orc.rsfe_apply(u0, phi0, pc)[0], phi_or)
 
 
 def test_init_phi_cases(orc):
