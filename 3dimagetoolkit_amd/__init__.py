@@ -46,7 +46,8 @@ class RsfeParams(C.Structure):
             "max_sweeps", "warm_start")] + [("ls_res_tol", C.c_double),
                                             ("res_check_every", C.c_int),
                                             ("ws_presweeps", C.c_int),
-                                            ("acyclic", C.c_int)]
+                                            ("acyclic", C.c_int),
+                                            ("neumann", C.c_int)]
 
 
 class IterStats(C.Structure):
@@ -94,6 +95,8 @@ def _load():
         "tk_conv_acyclic_pair": (i, [vp, vp, vp, vp, vp, vp]),
         "tk_poisson_dir_sweeps": (i, [vp, vp, vp, d, d, i, i]),
         "tk_poisson_dir_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
+        "tk_poisson_neu_sweeps": (i, [vp, vp, vp, d, d, i, i]),
+        "tk_poisson_neu_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
         "tk_poisson_rb_autotune": (i, [vp, vp, ip, dp]),
         "tk_poisson_solve_residual": (i, [vp, vp, vp, d, d, i, d, i, i, ip, dp]),
@@ -369,16 +372,19 @@ class Filter:
             pass
 
 
-def poisson_sweeps(ctx, phi, rhs, n, dt=1.0, bc=0.0, order=TK_GS_LEX):
-    _check(lib.tk_poisson_dir_sweeps(ctx.h, phi.h, rhs.h, dt, bc, order, n))
+def poisson_sweeps(ctx, phi, rhs, n, dt=1.0, bc=0.0, order=TK_GS_LEX, neumann=False):
+    """n DirGaussSeidel sweeps (bc: boundary value), or NeuGaussSeidel (bc: derivative)"""
+    fn = lib.tk_poisson_neu_sweeps if neumann else lib.tk_poisson_dir_sweeps
+    _check(fn(ctx.h, phi.h, rhs.h, dt, bc, order, n))
 
 
 def poisson_solve(ctx, phi, rhs, ls_steps, ls_tol, dt=1.0, bc=0.0, order=TK_GS_LEX,
-                  max_sweeps=0, allow_cap=False):
+                  max_sweeps=0, allow_cap=False, neumann=False):
     s = C.c_int()
     e = C.c_double()
-    rc = lib.tk_poisson_dir_solve(ctx.h, phi.h, rhs.h, dt, bc, order, ls_steps, ls_tol,
-                                  max_sweeps, C.byref(s), C.byref(e))
+    fn = lib.tk_poisson_neu_solve if neumann else lib.tk_poisson_dir_solve
+    rc = fn(ctx.h, phi.h, rhs.h, dt, bc, order, ls_steps, ls_tol, max_sweeps, C.byref(s),
+            C.byref(e))
     if not (allow_cap and rc == TK_ERR_SWEEPS):
         _check(rc)
     return s.value, e.value

@@ -20,6 +20,12 @@
 // inside a chunk the chunk is replayed from a saved copy up to the stopping sweep
 // (deterministic, hence identical).
 //
+// Neumann (lapl::NeuGaussSeidel, poisson_imp.hxx:115-334, 3-D branch): the same
+// pipelined wavefront over every voxel, the boundary ones included.  The reference's loop
+// runs k outer and i fastest, the Dirichlet one i outer and k fastest; for a 7-point stencil
+// every lexicographic order reads new values at all three minus neighbours and old values
+// at the plus neighbours, so both give the same result and share the schedule.
+//
 // TK_GS_REDBLACK -- the FP32 fast mode: red (i+j+k even) then black, each pass
 // in place; FP64 per-sweep update norms; the black pass's last workgroup evaluates
 // the stop rule on the device and raises a flag that makes later sweeps return at once.
@@ -48,6 +54,31 @@ inline GsCoef gs_coef(const tk_vol phi, double dt)
     c.htilde = 1. / (1. / dt + 2. / (phi->hx * phi->hx) + 2. / (phi->hy * phi->hy) +
                      2. / (phi->hz * phi->hz));
     return c;
+}
+
+// Neumann boundary coefficients: htilde per boundary mask (bit 0: i on a face, 1: j, 2: k;
+// 1/h^2 instead of 2/h^2 on a boundary axis, poisson_imp.hxx:175-197) and the derivative
+// term (n bc) h per number of boundary planes (n = 1, 1/sqrt 2, 1/sqrt 3, 201-205) and axis
+struct NeuCoef {
+    double ht[8];
+    double q[4][3];
+};
+
+inline NeuCoef neu_coef(const tk_vol phi, double dt, double bc)
+{
+    NeuCoef nc;
+    const double hx = phi->hx, hy = phi->hy, hz = phi->hz;
+    for (int m = 0; m < 8; ++m) {
+        const double cx = (m & 1) ? 1. : 2., cy = (m & 2) ? 1. : 2., cz = (m & 4) ? 1. : 2.;
+        nc.ht[m] = 1. / (1. / dt + cx / (hx * hx) + cy / (hy * hy) + cz / (hz * hz));
+    }
+    const double n_of[4] = {0., 1., 1. / std::sqrt(2.), 1. / std::sqrt(3.)};
+    for (int c = 0; c < 4; ++c) {
+        nc.q[c][0] = n_of[c] * bc * hx;
+        nc.q[c][1] = n_of[c] * bc * hy;
+        nc.q[c][2] = n_of[c] * bc * hz;
+    }
+    return nc;
 }
 
 // poisson_imp.hxx:420-424, same operand order; UNIT skips the exact divisions by 1
@@ -132,29 +163,58 @@ inline long boundary_count(int X, int Y, int Z)
 constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
 constexpr int kLexTZ = 32;  // tile depth in z
 
+// one Neumann update of a voxel on the boundary (mask m != 0): per axis (p+ - q) on the
+// low face, (p- + q) on the high face, (p+ + p-) inside; then the interior expression with
+// the boundary htilde (poisson_imp.hxx:200-331)
 template <class T, bool UNIT>
+__device__ __forceinline__ T neu_update(T xp, T xm, T yp, T ym, T zp, T zm, T b, T self,
+                                        const GsCoef& c, const double* ht, const double* q,
+                                        int m, bool lx, bool ly, bool lz)
+{
+    const T tx = (m & 1) ? (lx ? xp - (T)q[0] : xm + (T)q[0]) : xp + xm;
+    const T ty = (m & 2) ? (ly ? yp - (T)q[1] : ym + (T)q[1]) : yp + ym;
+    const T tz = (m & 4) ? (lz ? zp - (T)q[2] : zm + (T)q[2]) : zp + zm;
+    if constexpr (UNIT) return (tx + ty + tz + b + self) * (T)ht[m];
+    else
+        return (tx / (T)c.hx2 + ty / (T)c.hy2 + tz / (T)c.hz2 + b + self / (T)c.dt) * (T)ht[m];
+}
+
+// NEU: tiles cover every voxel (Neumann sweep), else the interior (Dirichlet sweep)
+template <class T, bool UNIT, bool NEU>
 __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __restrict__ rhs,
                                                int X, int Y, int Z, int nTx, int nTy, int nTz,
-                                               int tau, int C, GsCoef c, double* __restrict__ acc)
+                                               int tau, int C, GsCoef c, NeuCoef nc,
+                                               double* __restrict__ acc)
 {
     constexpr int TX = kLexT, TY = kLexT, TZ = kLexTZ;
     constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    constexpr int O = NEU ? 0 : 1;  // first updated index along each axis
     __shared__ T ph[HX * HY * HZ];
     __shared__ T rb[TX * TY * TZ];
+    __shared__ double nct[8 + 12];
     const int tile = blockIdx.x;
     const int K = tile % nTz, J = (tile / nTz) % nTy, I = tile / (nTz * nTy);
     const int s2 = tau - (I + J + K);
     if (s2 < 0 || (s2 & 1) || (s2 >> 1) >= C) return;
     const int s = s2 >> 1;
-    const int i0 = 1 + I * TX, j0 = 1 + J * TY, k0 = 1 + K * TZ;
-    const int tx = min(TX, X - 1 - i0), ty = min(TY, Y - 1 - j0), tz = min(TZ, Z - 1 - k0);
+    const int i0 = O + I * TX, j0 = O + J * TY, k0 = O + K * TZ;
+    const int tx = min(TX, X - O - i0), ty = min(TY, Y - O - j0), tz = min(TZ, Z - O - k0);
     const long sY = Z, sX = (long)Y * Z;
     const int lane = threadIdx.x;
-    // load phi tile + halo and the rhs tile (rows along z are contiguous)
+    if constexpr (NEU) {
+        if (lane < 8) nct[lane] = nc.ht[lane];
+        else if (lane < 20) nct[lane] = nc.q[(lane - 8) / 3][(lane - 8) % 3];
+    }
+    // load phi tile + halo and the rhs tile (rows along z are contiguous); the Neumann
+    // tiles' halo positions outside the volume are never read and hold 0
     const int hz = tz + 2;
     for (int e = lane; e < (tx + 2) * (ty + 2) * hz; e += 64) {
         const int kk = e % hz, r = e / hz, jj = r % (ty + 2), ii = r / (ty + 2);
-        ph[(ii * HY + jj) * HZ + kk] = phi[(i0 - 1 + ii) * sX + (j0 - 1 + jj) * sY + (k0 - 1 + kk)];
+        const int gi = i0 - 1 + ii, gj = j0 - 1 + jj, gk = k0 - 1 + kk;
+        if (NEU && (gi < 0 || gi >= X || gj < 0 || gj >= Y || gk < 0 || gk >= Z))
+            ph[(ii * HY + jj) * HZ + kk] = (T)0;
+        else
+            ph[(ii * HY + jj) * HZ + kk] = phi[gi * sX + gj * sY + gk];
     }
     for (int e = lane; e < tx * ty * tz; e += 64) {
         const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
@@ -169,8 +229,23 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
         if (a < tx && b < ty && k >= 0 && k < tz) {
             T* p = &ph[((a + 1) * HY + (b + 1)) * HZ + (k + 1)];
             const T old = p[0];
-            const T v = gs_update<T, UNIT>(p[HY * HZ], p[-HY * HZ], p[HZ], p[-HZ], p[1], p[-1],
+            T v;
+            if constexpr (NEU) {
+                const int gi = i0 + a, gj = j0 + b, gk = k0 + k;
+                const bool lx = gi == 0, ly = gj == 0, lz = gk == 0;
+                const int bx = lx || gi == X - 1, by = ly || gj == Y - 1, bz = lz || gk == Z - 1;
+                const int m = bx | (by << 1) | (bz << 2);
+                if (m == 0)
+                    v = gs_update<T, UNIT>(p[HY * HZ], p[-HY * HZ], p[HZ], p[-HZ], p[1], p[-1],
                                            rb[(a * TY + b) * TZ + k], old, c);
+                else
+                    v = neu_update<T, UNIT>(p[HY * HZ], p[-HY * HZ], p[HZ], p[-HZ], p[1], p[-1],
+                                            rb[(a * TY + b) * TZ + k], old, c, nct,
+                                            nct + 8 + 3 * (bx + by + bz), m, lx, ly, lz);
+            } else {
+                v = gs_update<T, UNIT>(p[HY * HZ], p[-HY * HZ], p[HZ], p[-HZ], p[1], p[-1],
+                                       rb[(a * TY + b) * TZ + k], old, c);
+            }
             p[0] = v;
             const double d = (double)v - (double)old;
             d2 += d * d;
@@ -193,24 +268,33 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
     }
 }
 
+// C lexicographic sweeps; neu: the Neumann sweep over every voxel with its coefficients
 template <class T>
-static int lex_chunk(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int C, double* acc)
+static int lex_chunk(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int C, double* acc,
+                     const NeuCoef* neu = nullptr)
 {
     const int X = phi->nx, Y = phi->ny, Z = phi->nz;
-    const int nTx = (X - 2 + kLexT - 1) / kLexT, nTy = (Y - 2 + kLexT - 1) / kLexT,
-              nTz = (Z - 2 + kLexTZ - 1) / kLexTZ;
+    const int e = neu ? 0 : 2;  // voxels left out along each axis
+    const int nTx = (X - e + kLexT - 1) / kLexT, nTy = (Y - e + kLexT - 1) / kLexT,
+              nTz = (Z - e + kLexTZ - 1) / kLexTZ;
     const int nT = nTx * nTy * nTz;
     const int last = (nTx - 1) + (nTy - 1) + (nTz - 1) + 2 * (C - 1);
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+    const NeuCoef nc = neu ? *neu : NeuCoef{};
+#define TK_LEX(U, N)                                                                           \
+    hipLaunchKernelGGL((k_gs_lex<T, U, N>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,     \
+                       (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, nc, acc)
     for (int tau = 0; tau <= last; ++tau) {
         TimedLaunch tl(ctx, TK_K_GS);
-        if (unit)
-            hipLaunchKernelGGL((k_gs_lex<T, true>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,
-                               (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, acc);
-        else
-            hipLaunchKernelGGL((k_gs_lex<T, false>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,
-                               (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, acc);
+        if (neu) {
+            if (unit) TK_LEX(true, true);
+            else TK_LEX(false, true);
+        } else {
+            if (unit) TK_LEX(true, false);
+            else TK_LEX(false, false);
+        }
     }
+#undef TK_LEX
     TK_LAUNCHED();
     return TK_OK;
 }
@@ -668,15 +752,26 @@ static int set_boundary(tk_ctx ctx, tk_vol phi, double bc, double* acc)
 // sweep loop of rsfe_splitbregman_imp.hxx:1762-1797
 template <class T>
 // faces_set: phi's (and *alt's) six faces already hold bc and no sweep, cut or copy since
-// has written them, so the fixed-sweep paths skip setting them again
+// has written them, so the fixed-sweep paths skip setting them again.  neumann: the sweep is
+// NeuGaussSeidel (lexicographic; every voxel updated, bc is the boundary derivative)
 static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order, int ls_steps,
                  double ls_tol, int max_sweeps, int* sweeps_out, double* err_out,
-                 void** alt = nullptr, bool faces_set = false)
+                 void** alt = nullptr, bool faces_set = false, bool neumann = false)
 {
     const GsCoef c = gs_coef(phi, dt);
     const int X = phi->nx, Y = phi->ny, Z = phi->nz;
     const double h3 = phi->hx * phi->hy * phi->hz;
-    const double bnd_o2 = (double)boundary_count(X, Y, Z) * (double)(T)bc * (double)(T)bc;
+    NeuCoef ncoef{};
+    const NeuCoef* neu = nullptr;
+    if (neumann) {
+        TK_CHECK(order == TK_GS_LEX, TK_ERR_UNSUPPORTED,
+                 "poisson: the Neumann sweep runs in the reference's lexicographic order only");
+        ncoef = neu_coef(phi, dt, bc);
+        neu = &ncoef;
+    }
+    // Dirichlet: the faces (bc) enter the update norms; Neumann: the kernel covers them
+    const double bnd_o2 =
+        neu ? 0. : (double)boundary_count(X, Y, Z) * (double)(T)bc * (double)(T)bc;
     if (ls_steps < 1) ls_steps = 1;
     if (order == TK_GS_REDBLACK) {
         // ping-pong between phi and a scratch volume (the caller's *alt, else the context's
@@ -762,8 +857,8 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
     }
     if (ls_tol == 0) {
         // exactly ls_steps sweeps, no norms (the reference computes none either)
-        if (!faces_set) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
-        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr));
+        if (!faces_set && !neu) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, ls_steps, nullptr, neu));
         if (sweeps_out) *sweeps_out = ls_steps;
         if (err_out) *err_out = -1;
         return TK_OK;
@@ -781,8 +876,8 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
         C = std::min(C, std::min(kMaxChunk, max_sweeps - done));
         TK_HIP(hipMemcpyAsync(save_p, phi->d, phi->bytes(), hipMemcpyDeviceToDevice, ctx->stream));
         hipLaunchKernelGGL(k_zero_d, dim3(16), dim3(256), 0, ctx->stream, acc, 2 * C);
-        if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, acc));
-        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, C, acc));
+        if (done == 0 && !neu) TK_TRY(set_boundary<T>(ctx, phi, bc, acc));
+        TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, C, acc, neu));
         TK_HIP(hipMemcpyAsync(ctx->h_acc, acc, sizeof(double) * 2 * C, hipMemcpyDeviceToHost,
                               ctx->stream));
         TK_HIP(hipStreamSynchronize(ctx->stream));
@@ -803,8 +898,8 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
                 // replay the chunk up to the stopping sweep from the saved state
                 TK_HIP(hipMemcpyAsync(phi->d, save_p, phi->bytes(), hipMemcpyDeviceToDevice,
                                       ctx->stream));
-                if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
-                TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, stop_at + 1, nullptr));
+                if (done == 0 && !neu) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
+                TK_TRY(lex_chunk<T>(ctx, phi, rhs, c, stop_at + 1, nullptr, neu));
             }
             if (sweeps_out) *sweeps_out = done + stop_at + 1;
             if (err_out) *err_out = err;
@@ -892,14 +987,16 @@ __global__ void k_sum_pairs_gs(const double* __restrict__ part, int nb, double* 
 
 using namespace tk;
 
-static int check_gs(tk_vol phi, tk_vol rhs)
+// min_edge: 3 for the Dirichlet sweep (an interior to update), 2 for the Neumann sweep
+// (its 2-D branch, Z == 1, is not built)
+static int check_gs(tk_vol phi, tk_vol rhs, int min_edge = 3)
 {
     TK_CHECK(phi && rhs && phi->d && rhs->d, TK_ERR_ARG, "poisson: null volume");
     TK_CHECK(same_dims(phi, rhs), TK_ERR_SHAPE, "poisson: phi and rhs dims differ");
     TK_CHECK(phi->dtype == rhs->dtype, TK_ERR_DTYPE, "poisson: dtypes differ");
     TK_CHECK(phi->dtype == TK_F32 || phi->dtype == TK_F64, TK_ERR_DTYPE, "poisson: not FP");
-    TK_CHECK(phi->nx >= 3 && phi->ny >= 3 && phi->nz >= 3, TK_ERR_UNSUPPORTED,
-             "poisson: 3-D volumes with every edge >= 3 only");
+    TK_CHECK(phi->nx >= min_edge && phi->ny >= min_edge && phi->nz >= min_edge,
+             TK_ERR_UNSUPPORTED, "poisson: 3-D volumes with every edge >= %d only", min_edge);
     return TK_OK;
 }
 
@@ -953,6 +1050,18 @@ int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double 
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
                         err_out, alt, faces_set);
 }
+
+int poisson_neu_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
+                          double* err_out)
+{
+    TK_TRY(check_gs(phi, rhs, 2));
+    if (phi->dtype == TK_F64)
+        return solve<double>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps,
+                             sweeps_out, err_out, nullptr, false, true);
+    return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
+                        err_out, nullptr, false, true);
+}
 }  // namespace tk
 
 extern "C" {
@@ -983,6 +1092,29 @@ int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double b
                              err_out);
     return solve<float>(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps, sweeps_out,
                         err_out);
+}
+
+int tk_poisson_neu_sweeps(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int n_sweeps)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_neu_sweeps: null ctx");
+    TK_TRY(check_gs(phi, rhs, 2));
+    TK_CHECK(order == TK_GS_LEX || order == TK_GS_REDBLACK, TK_ERR_ARG, "bad order");
+    TK_CHECK(dt > 0, TK_ERR_ARG, "tk_poisson_neu_sweeps: dt <= 0");
+    if (n_sweeps <= 0) return TK_OK;
+    return poisson_neu_solve_alt(ctx, phi, rhs, dt, bc, order, n_sweeps, 0, 0, nullptr, nullptr);
+}
+
+int tk_poisson_neu_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                         int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
+                         double* err_out)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_neu_solve: null ctx");
+    TK_TRY(check_gs(phi, rhs, 2));
+    TK_CHECK(order == TK_GS_LEX || order == TK_GS_REDBLACK, TK_ERR_ARG, "bad order");
+    TK_CHECK(dt > 0 && ls_tol >= 0, TK_ERR_ARG, "tk_poisson_neu_solve: dt <= 0 or ls_tol < 0");
+    return poisson_neu_solve_alt(ctx, phi, rhs, dt, bc, order, ls_steps, ls_tol, max_sweeps,
+                                 sweeps_out, err_out);
 }
 
 int tk_poisson_residual_field(tk_ctx ctx, tk_vol phi, tk_vol rhs, tk_vol out)

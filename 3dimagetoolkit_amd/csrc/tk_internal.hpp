@@ -124,6 +124,10 @@ int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc);
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, void** alt,
                           int* sweeps_out, double* err_out, bool faces_set = false);
+// NeuGaussSeidel sweeps (lexicographic only), the same loop rule
+int poisson_neu_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
+                          double* err_out);
 
 inline bool same_dims(const tk_vol a, const tk_vol b)
 {

@@ -311,7 +311,10 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         }
     }
     int r;
-    if (p.ls_res_tol > 0)
+    if (p.neumann)
+        r = poisson_neu_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
+                                  p.ls_tol, p.max_sweeps, &sweeps, &ls_err);
+    else if (p.ls_res_tol > 0)
         r = poisson_solve_residual_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order,
                                        p.ls_res_tol, p.res_check_every > 0 ? p.res_check_every : 64,
                                        p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps, &ls_err);
@@ -320,7 +323,7 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
                                   p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps,
                                   &ls_err, s->faces_set);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
-    if (chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
+    if (!p.neumann && chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
     // cut + norms + grad/shrink/Bregman, then div.  (A single pass recomputing q of the
     // six neighbours for div measured 281 us at 256^3 FP32 against 155 + 70 us for the two
     // passes: 140 registers per lane, three waves per SIMD.)
@@ -476,6 +479,7 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     p->res_check_every = 64;
     p->ws_presweeps = 512;
     p->acyclic = 0;
+    p->neumann = 0;
     return TK_OK;
 }
 
@@ -490,6 +494,14 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     TK_CHECK(p->warm_start >= 0 && p->warm_start <= 64, TK_ERR_ARG, "tk_rsfe_create: warm_start");
     TK_CHECK(p->gs_order == TK_GS_LEX || p->gs_order == TK_GS_REDBLACK, TK_ERR_ARG,
              "tk_rsfe_create: gs_order");
+    TK_CHECK(p->acyclic == 0 || p->acyclic == 1, TK_ERR_ARG, "tk_rsfe_create: acyclic");
+    TK_CHECK(p->neumann == 0 || p->neumann == 1, TK_ERR_ARG, "tk_rsfe_create: neumann");
+    // the Neumann sweep is the reference's lexicographic one; the learned start and the
+    // residual gate are built on the Dirichlet problem
+    TK_CHECK(!p->neumann || (p->gs_order == TK_GS_LEX && p->warm_start == 0 &&
+                             p->ls_res_tol <= 0),
+             TK_ERR_UNSUPPORTED,
+             "tk_rsfe_create: neumann needs gs_order TK_GS_LEX, no warm_start and no ls_res_tol");
     tk_rsfe s = new tk_rsfe_s();
     s->ctx = ctx;
     s->p = *p;
@@ -607,6 +619,8 @@ int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
         s->p.warm_start = 0;
         return TK_OK;
     }
+    TK_CHECK(!s->p.neumann, TK_ERR_UNSUPPORTED,
+             "tk_rsfe_set_warm_start: the learned start serves the Dirichlet sweep only");
     TK_CHECK(unet_ctx(net) == s->ctx, TK_ERR_ARG,
              "tk_rsfe_set_warm_start: the network must live on the run's context");
     int d[3];

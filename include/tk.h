@@ -154,6 +154,19 @@ int tk_poisson_dir_sweeps(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double 
 int tk_poisson_dir_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                          int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,
                          double* ls_err_out);
+
+/* ---- lapl::NeuGaussSeidel (poisson_imp.hxx:115-334, 3-D branch) --------------------
+ * One Gauss-Seidel sweep over every voxel with the derivative bc prescribed on the
+ * boundary: per axis (p(+1) - n bc h)/h^2 on a low face, (p(-1) + n bc h)/h^2 on a high face,
+ * n = 1, 1/sqrt 2, 1/sqrt 3 on faces, edges, corners, and htilde with 1/h^2 for each
+ * boundary axis; FP64 is bitwise the reference's sweep.  Installed by set_bc(NEU)
+ * (rsfe_splitbregman_imp.hxx:1294-1322).  order must be TK_GS_LEX (the reference's order;
+ * TK_GS_REDBLACK returns TK_ERR_UNSUPPORTED); every edge >= 2 (the 2-D branch is not
+ * built).  The solve variant runs the loop rule of tk_poisson_dir_solve. */
+int tk_poisson_neu_sweeps(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                          int n_sweeps);
+int tk_poisson_neu_solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
+                         int ls_steps, double ls_tol, int max_sweeps, int* sweeps, double* err);
 /* build-defined residual (no reference counterpart):
    ||rhs - (phi/dt - lap_h phi) + phi/dt||_2 over the interior / ||rhs||_2, i.e.
    ||rhs + lap_h phi|| / ||rhs|| with FP64 accumulation */
@@ -240,6 +253,9 @@ typedef struct tk_rsfe_params {
     int ws_presweeps;         /* R15: sweeps before each learned round (default 512) */
     int acyclic;              /* 1: set_cv(acyclic_fftw_convolution), the reference's default
                                  cv (imp:25-41); 0 (default here): cyclic_fftw_convolution */
+    int neumann;              /* 1: onestep_poisson = NeuGaussSeidel, as set_bc(NEU) installs
+                                 (imp:1294-1322; bc value 0 unless bc_dir); lexicographic
+                                 only, no warm_start / ls_res_tol.  0: DirGaussSeidel */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {

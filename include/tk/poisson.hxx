@@ -1,6 +1,6 @@
 /*
- * tk/poisson.hxx -- lapl::unsteady_poisson_functor<T> and lapl::DirGaussSeidel over the
- * device Gauss-Seidel sweeps of libtk.
+ * tk/poisson.hxx -- lapl::unsteady_poisson_functor<T>, lapl::DirGaussSeidel and
+ * lapl::NeuGaussSeidel over the device Gauss-Seidel sweeps of libtk.
  *
  * Mirrors lapl::unsteady_poisson_functor<T> (header/poisson.hxx:54-111): a functor holding
  * void (*f)(image3d<T>& res, image3d<T> const& b, T const& dt, T const& bc,
@@ -12,7 +12,9 @@
  * res to bc and runs one lexicographic sweep: on the device this is a pipelined wavefront
  * whose result is bitwise the reference's sequential order.  DirRedBlackGaussSeidel (no
  * reference counterpart) is the red-black ordering of the same update: the same fixed
- * point, other iterates.  Like the reference, res must already have b's dims; a mismatch
+ * point, other iterates.  NeuGaussSeidel (poisson.hxx:193; poisson_imp.hxx:115-334) updates
+ * every voxel with the derivative bc prescribed on the boundary (3-D branch; bitwise the
+ * reference in FP64).  Like the reference, res must already have b's dims; a mismatch
  * prints a WARNING and leaves res unchanged (the reference does not check at all).
  */
 #ifndef TK_POISSON_HXX
@@ -46,16 +48,21 @@ private:
 namespace detail {
 template <typename T>
 void dir_sweep(im3d::image3d<T>& res, im3d::image3d<T> const& b, T const& dt, T const& bc,
-               int order, const char* name)
+               int order, const char* name, bool neumann = false)
 {
     if (res.getdimx() != b.getdimx() || res.getdimy() != b.getdimy() ||
         res.getdimz() != b.getdimz()) {
         tk::warning(std::string(name) + ": res and b must have the same dimensions");
         return;
     }
-    tk::check(tk_poisson_dir_sweeps(res.ctx(), res.handle(), b.handle(), (double)dt, (double)bc,
-                                    order, 1),
-              name);
+    if (neumann)
+        tk::check(tk_poisson_neu_sweeps(res.ctx(), res.handle(), b.handle(), (double)dt,
+                                        (double)bc, order, 1),
+                  name);
+    else
+        tk::check(tk_poisson_dir_sweeps(res.ctx(), res.handle(), b.handle(), (double)dt,
+                                        (double)bc, order, 1),
+                  name);
 }
 }  // namespace detail
 
@@ -66,6 +73,15 @@ void DirGaussSeidel(im3d::image3d<T>& res, im3d::image3d<T> const& b, T const& d
 {
     (void)input;
     detail::dir_sweep(res, b, dt, bc, TK_GS_LEX, "DirGaussSeidel");
+}
+
+// one Neumann Gauss-Seidel sweep, lexicographic (poisson_imp.hxx:115-334)
+template <typename T>
+void NeuGaussSeidel(im3d::image3d<T>& res, im3d::image3d<T> const& b, T const& dt, T const& bc,
+                    im3d::image3d<T> const& input)
+{
+    (void)input;
+    detail::dir_sweep(res, b, dt, bc, TK_GS_LEX, "NeuGaussSeidel", true);
 }
 
 // one Dirichlet Gauss-Seidel sweep, red-black order (build addition)

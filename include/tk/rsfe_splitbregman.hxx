@@ -12,10 +12,10 @@
  * What this layer runs: cv = conv::acyclic_fftw_convolution (the reference's default) or
  * conv::cyclic_fftw_convolution (set_cv with any other function keeps the current one and
  * prints a WARNING), onestep_poisson = lapl::DirGaussSeidel (bit-exact
- * lexicographic sweeps) or lapl::DirRedBlackGaussSeidel (red-black), edge detector GRAD
- * with edge_detector_sigma = 0.  set_bc(NEU) would switch the reference to NeuGaussSeidel,
- * which is not built: it prints a WARNING and keeps the Dirichlet sweep (the reference's
- * default state is the NEU flag, i.e. boundary value 0, with DirGaussSeidel).
+ * lexicographic sweeps), lapl::DirRedBlackGaussSeidel (red-black) or lapl::NeuGaussSeidel
+ * (installed by set_bc(NEU), as in the reference), edge detector GRAD with
+ * edge_detector_sigma = 0.  The default state is the reference's: the NEU flag (boundary
+ * value 0) with DirGaussSeidel.
  */
 #ifndef TK_RSFE_SPLITBREGMAN_HXX
 #define TK_RSFE_SPLITBREGMAN_HXX
@@ -119,15 +119,16 @@ public:
         if (eds == 0) return;
         tk::warning("set_edge_detector_sigma: only edge_detector_sigma = 0 is built");
     }
-    // bc flag: NEU -> boundary value 0 with the Dirichlet sweep kept; DIR -> value a0
-    // (imp:1294-1322, 1735-1743)
+    // bc flag and sweep (imp:1294-1322): NEU -> NeuGaussSeidel with derivative 0, DIR ->
+    // DirGaussSeidel with boundary value a0 (imp:1735-1743)
     void set_bc(bc_type const bc)
     {
         if (bc == DIR) {
             p_.bc_dir = 1;
             onestep_.changef(lapl::DirGaussSeidel<T>);
         } else if (bc == NEU) {
-            tk::warning("set_bc: NeuGaussSeidel is not built; the Dirichlet sweep is kept");
+            p_.bc_dir = 0;
+            onestep_.changef(lapl::NeuGaussSeidel<T>);
         } else {
             tk::warning("set_bc: specify a proper bc_type condition");
         }
@@ -142,10 +143,11 @@ public:
     }
     void set_onestep_poisson(lapl::unsteady_poisson_functor<T> const osl)
     {
-        if (osl.getf() == &lapl::DirGaussSeidel<T> || osl.getf() == &lapl::DirRedBlackGaussSeidel<T>)
+        if (osl.getf() == &lapl::DirGaussSeidel<T> || osl.getf() == &lapl::DirRedBlackGaussSeidel<T> ||
+            osl.getf() == &lapl::NeuGaussSeidel<T>)
             onestep_ = osl;
-        else tk::warning("set_onestep_poisson: only DirGaussSeidel and DirRedBlackGaussSeidel "
-                         "are built; kept");
+        else tk::warning("set_onestep_poisson: only DirGaussSeidel, DirRedBlackGaussSeidel and "
+                         "NeuGaussSeidel are built; kept");
     }
     // cap on the sweeps of one Poisson solve when ls_tol > 0 (0: none)
     void set_max_sweeps(int const m) { p_.max_sweeps = m; }
@@ -177,6 +179,7 @@ private:
         p.gs_order = onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ? TK_GS_REDBLACK
                                                                            : TK_GS_LEX;
         p.acyclic = cv_.getf() == &conv::acyclic_fftw_convolution<T> ? 1 : 0;
+        p.neumann = onestep_.getf() == &lapl::NeuGaussSeidel<T> ? 1 : 0;
         tk_rsfe s = nullptr;
         tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
         sweeps_.clear();

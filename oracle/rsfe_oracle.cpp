@@ -401,6 +401,50 @@ void or_dir_gauss_seidel(double* res, const double* b, int X, int Y, int Z, doub
         }
 }
 
+/* lapl::NeuGaussSeidel, 3-D branch (poisson_imp.hxx:115-334).  The reference writes 27
+ * cases (interior, 6 faces, 12 edges, 8 corners) out one by one; they all follow one rule,
+ * restated here per voxel:
+ *  - along an axis where the voxel is interior the term is (p(+1) + p(-1)) / h^2; on the
+ *    low face it is (p(+1) - n bc h) / h^2, on the high face (p(-1) + n bc h) / h^2, with
+ *    n = 1, 1/sqrt(2), 1/sqrt(3) when the voxel lies on 1, 2, 3 boundary planes (face,
+ *    edge, corner: the oblique normal, 201-205), the product evaluated as (n bc) h;
+ *  - htilde = 1 / (1/dt + c_x/hx^2 + c_y/hy^2 + c_z/hz^2), c = 1 on a boundary axis and 2
+ *    elsewhere (175-197);
+ *  - p = (term_x + term_y + term_z + b + p/dt) htilde, summed left to right.
+ * The loops run k outer, j, i inner (the reference's order: face k = 0, then k = 1..Z-2,
+ * then k = Z-1, each row by row). */
+int or_neu_gauss_seidel(double* res, const double* b, int X, int Y, int Z, double hx,
+                        double hy, double hz, double dt, double bc)
+{
+    if (Z == 1 || X < 2 || Y < 2 || Z < 2) return -1;
+    const double n_of[4] = {1., 1., 1. / std::sqrt(2.), 1. / std::sqrt(3.)};
+    double ht[8];
+    for (int m = 0; m < 8; ++m) {
+        const double cx = (m & 1) ? 1. : 2., cy = (m & 2) ? 1. : 2., cz = (m & 4) ? 1. : 2.;
+        ht[m] = 1. / (1. / dt + cx / (hx * hx) + cy / (hy * hy) + cz / (hz * hz));
+    }
+    const long sx = (long)Y * Z, sy = Z;
+    for (int k = 0; k < Z; ++k)
+        for (int j = 0; j < Y; ++j)
+            for (int i = 0; i < X; ++i) {
+                const int lx = i == 0, ux = i == X - 1, ly = j == 0, uy = j == Y - 1,
+                          lz = k == 0, uz = k == Z - 1;
+                const int m = (lx | ux) | ((ly | uy) << 1) | ((lz | uz) << 2);
+                const double n = n_of[(lx | ux) + (ly | uy) + (lz | uz)];
+                const long v = idx(i, j, k, Y, Z);
+                double* r = res + v;
+                const double tx = lx ? r[sx] - n * bc * hx : ux ? r[-sx] + n * bc * hx
+                                                                 : r[sx] + r[-sx];
+                const double ty = ly ? r[sy] - n * bc * hy : uy ? r[-sy] + n * bc * hy
+                                                                 : r[sy] + r[-sy];
+                const double tz = lz ? r[1] - n * bc * hz : uz ? r[-1] + n * bc * hz
+                                                                : r[1] + r[-1];
+                r[0] = (tx / (hx * hx) + ty / (hy * hy) + tz / (hz * hz) + b[v] + r[0] / dt) *
+                       ht[m];
+            }
+    return 0;
+}
+
 /* ---- R10 ------------------------------------------------------------------- */
 double or_scalarprod(const double* a, const double* b, long n)
 {
@@ -443,13 +487,22 @@ int or_poisson_loop(double* phi, const double* rhs, int nx, int ny, int nz, doub
                     double hy, double hz, double dt, double bc, int ls_steps, double ls_tol,
                     double* ls_err_out)
 {
+    return or_poisson_loop_bc(phi, rhs, nx, ny, nz, hx, hy, hz, dt, bc, ls_steps, ls_tol,
+                              ls_err_out, 0);
+}
+
+int or_poisson_loop_bc(double* phi, const double* rhs, int nx, int ny, int nz, double hx,
+                       double hy, double hz, double dt, double bc, int ls_steps, double ls_tol,
+                       double* ls_err_out, int neumann)
+{
     long n = (long)nx * ny * nz;
     std::vector<double> aux;
     double ls_err = ls_tol ? 1000. : -1.;
     unsigned s;
     for (s = 0; s < (unsigned)ls_steps || ls_err > ls_tol; ++s) {
         if (ls_tol != 0) aux.assign(phi, phi + n);
-        or_dir_gauss_seidel(phi, rhs, nx, ny, nz, hx, hy, hz, dt, bc);
+        if (neumann) or_neu_gauss_seidel(phi, rhs, nx, ny, nz, hx, hy, hz, dt, bc);
+        else or_dir_gauss_seidel(phi, rhs, nx, ny, nz, hx, hy, hz, dt, bc);
         if (ls_tol != 0 && s >= (unsigned)ls_steps - 1) {
             std::vector<double> d(n);
 #pragma omp parallel for
@@ -730,6 +783,7 @@ void or_default_params(or_params* p)
     p->beta = 100.; p->dt = 1; p->ls_steps = 5; p->epsilon = 1.e-3; p->a0 = 0.; p->b0 = 1.;
     p->nu = 1.; p->tol = 1.e-5; p->ls_tol = 0; p->maxiter = 100; p->bc_dir = 0;
     p->gaussian_pixel_approach = 0; p->radius = 3; p->verbosity = 0; p->acyclic = 0;
+    p->neumann = 0;
 }
 
 /* ---- R6 + R7: forcing term of one outer iteration (imp:1700-1759) ----------------
@@ -866,8 +920,9 @@ int or_init_phi(const double* init, int ix, int iy, int iz, int X, int Y, int Z,
 }
 
 /* ---- R2: apply(myim, init), rsfe_splitbregman_imp.hxx:1547-1995 ------------------
-   cv = cyclic_fftw_convolution, onestep_poisson = DirGaussSeidel, edge detector GRAD
-   with edge_detector_sigma == 0, no display / dump / on-the-go / auto CC. */
+   cv = cyclic_fftw_convolution or acyclic_fftw_convolution (p->acyclic), onestep_poisson =
+   DirGaussSeidel or NeuGaussSeidel (p->neumann), edge detector GRAD with
+   edge_detector_sigma == 0, no display / dump / on-the-go / auto CC. */
 int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, double hx,
                   double hy, double hz, const or_params* p, double* phi_out,
                   int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
@@ -922,8 +977,8 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
         forcing(myim, phi.data(), kones.data(), kmyim.data(), divdb.data(), n, p, cv, kr2.data());
         /* Poisson loop, imp:1762-1797 */
         double ls_err;
-        int s = or_poisson_loop(phi.data(), kr2.data(), X, Y, Z, hx, hy, hz, p->dt, chosen_bc,
-                                p->ls_steps, p->ls_tol, &ls_err);
+        int s = or_poisson_loop_bc(phi.data(), kr2.data(), X, Y, Z, hx, hy, hz, p->dt, chosen_bc,
+                                   p->ls_steps, p->ls_tol, &ls_err, p->neumann);
         if (sweeps_per_iter) sweeps_per_iter[t - 1] = s;
         /* cut_phi, imp:320-339 (serial) */
         for (long q = 0; q < n; ++q) {

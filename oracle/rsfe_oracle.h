@@ -50,11 +50,22 @@ void or_direct_acyclic_conv(const double* tf, const int nh[3], const double* in,
 void or_dir_gauss_seidel(double* phi, const double* rhs, int nx, int ny, int nz,
                          double hx, double hy, double hz, double dt, double bc);
 
+/* ---- lapl::NeuGaussSeidel, 3-D branch (poisson_imp.hxx:115-334): one lexicographic
+   sweep over every voxel (k outer, i fastest, the reference's loop order) with the
+   outward normal derivative bc prescribed on the boundary; every edge must be >= 2.  The
+   2-D branch (Z == 1, poisson_imp.hxx:32-113) is not restated; the call returns -1. */
+int or_neu_gauss_seidel(double* phi, const double* rhs, int nx, int ny, int nz,
+                        double hx, double hy, double hz, double dt, double bc);
+
 /* ---- R8: Poisson driver loop (rsfe_splitbregman_imp.hxx:1762-1797) ---------- */
 /* returns the number of sweeps run; *ls_err_out = last relative-update error */
 int or_poisson_loop(double* phi, const double* rhs, int nx, int ny, int nz,
                     double hx, double hy, double hz, double dt, double bc,
                     int ls_steps, double ls_tol, double* ls_err_out);
+/* the same loop around onestep_poisson = NeuGaussSeidel (neumann != 0) or DirGaussSeidel */
+int or_poisson_loop_bc(double* phi, const double* rhs, int nx, int ny, int nz,
+                       double hx, double hy, double hz, double dt, double bc,
+                       int ls_steps, double ls_tol, double* ls_err_out, int neumann);
 
 /* ---- R6: Heaviside (rsfe_splitbregman_imp.hxx:174-192) ---------------------- */
 void or_heaviside(const double* phi, long n, double a0, double b0, double eps, double* H);
@@ -95,6 +106,8 @@ typedef struct or_params {
     int gaussian_pixel_approach;
     int verbosity;          /* 1 adds the extra phi copy/norm of imp:1776-1793 (timing only) */
     int acyclic;            /* cv: 0 cyclic_fftw_convolution, 1 acyclic_fftw_convolution */
+    int neumann;            /* onestep_poisson: 0 DirGaussSeidel (the ctor's), 1 NeuGaussSeidel
+                               (set_bc(NEU), rsfe_splitbregman_imp.hxx:1294-1322) */
 } or_params;
 
 typedef struct or_stats {

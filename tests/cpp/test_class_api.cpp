@@ -127,6 +127,19 @@ static void run_cases()
         }
         report("unsteady_poisson_functor DirGaussSeidel x3 (bitwise)", maxabs(phi.download(), p0),
                0);
+
+        // NeuGaussSeidel with a non-zero derivative
+        std::vector<double> q0(n);
+        for (auto& x : q0) x = U(gen);
+        im3d::image3d<double> psi(X, Y, Z, 1., 2., .5);
+        psi.upload(q0);
+        osl.changef(lapl::NeuGaussSeidel<double>);
+        for (int s = 0; s < 3; ++s) {
+            osl(psi, r, 0.8, 0.3);
+            or_neu_gauss_seidel(q0.data(), rhs.data(), X, Y, Z, 1., 2., .5, 0.8, 0.3);
+        }
+        report("unsteady_poisson_functor NeuGaussSeidel x3 (bitwise)", maxabs(psi.download(), q0),
+               0);
     }
 
     // ---- connected component ----------------------------------------------------------
@@ -193,6 +206,25 @@ static void run_cases()
                       nullptr, nullptr, nullptr);
         report("rsfe_splitbregman apply(myim) cube FP64", maxabs(cube.getoutput().download(), cref),
                1e-9);
+
+        // set_bc(NEU): NeuGaussSeidel with derivative 0 (imp:1294-1322, 1736-1739)
+        segm::rsfe_splitbregman<double> neu;
+        neu.set_verbosity(false);
+        neu.set_cv(conv::filtering<double>(conv::cyclic_fftw_convolution<double>));
+        neu.set_bc(segm::rsfe_splitbregman<double>::NEU);
+        neu.set_sigma(2.0);
+        neu.set_maxiter(4);
+        neu.set_tol(0.);
+        neu.apply(myim, phi0);
+        std::vector<double> nref(m);
+        p.maxiter = 4;
+        p.acyclic = 0;
+        p.neumann = 1;
+        or_rsfe_apply(u255.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p, nref.data(), nullptr,
+                      nullptr, nullptr);
+        report("rsfe_splitbregman set_bc(NEU) FP64", maxabs(neu.getoutput().download(), nref),
+               1e-9);
+        p.neumann = 0;
 
         // setter validation keeps the old value, as the reference
         segm::rsfe_splitbregman<double> v;

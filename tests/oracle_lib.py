@@ -13,7 +13,7 @@ class OrParams(C.Structure):
         "sigma", "lamda", "lamda1", "lamda2", "beta", "dt", "epsilon", "a0", "b0", "nu", "tol",
         "ls_tol", "radius")] + [(n, C.c_int) for n in (
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "verbosity",
-            "acyclic")]
+            "acyclic", "neumann")]
 
 
 class OrStats(C.Structure):
@@ -37,6 +37,8 @@ def lib():
         L.or_rsfe_apply.restype = i
         L.or_connected_component.restype = i
         L.or_poisson_loop.restype = i
+        L.or_poisson_loop_bc.restype = i
+        L.or_neu_gauss_seidel.restype = i
         _lib = L
     return _lib
 
@@ -124,13 +126,24 @@ def gs_sweep(phi, rhs, dt=1.0, bc=0.0, h=(1.0, 1.0, 1.0)):
     return phi
 
 
-def poisson_loop(phi, rhs, ls_steps, ls_tol, dt=1.0, bc=0.0, h=(1.0, 1.0, 1.0)):
+def neu_sweep(phi, rhs, dt=1.0, bc=0.0, h=(1.0, 1.0, 1.0)):
+    phi = np.ascontiguousarray(phi, np.float64).copy()
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    rc = lib().or_neu_gauss_seidel(dp(phi), dp(rhs), *[C.c_int(s) for s in phi.shape],
+                                   *[C.c_double(x) for x in h], C.c_double(dt), C.c_double(bc))
+    if rc != 0:
+        raise ValueError("or_neu_gauss_seidel: 3-D volumes with every edge >= 2 only")
+    return phi
+
+
+def poisson_loop(phi, rhs, ls_steps, ls_tol, dt=1.0, bc=0.0, h=(1.0, 1.0, 1.0), neumann=False):
     phi = np.ascontiguousarray(phi, np.float64).copy()
     rhs = np.ascontiguousarray(rhs, np.float64)
     err = C.c_double()
-    s = lib().or_poisson_loop(dp(phi), dp(rhs), *[C.c_int(q) for q in phi.shape],
-                              *[C.c_double(x) for x in h], C.c_double(dt), C.c_double(bc),
-                              C.c_int(ls_steps), C.c_double(ls_tol), C.byref(err))
+    s = lib().or_poisson_loop_bc(dp(phi), dp(rhs), *[C.c_int(q) for q in phi.shape],
+                                 *[C.c_double(x) for x in h], C.c_double(dt), C.c_double(bc),
+                                 C.c_int(ls_steps), C.c_double(ls_tol), C.byref(err),
+                                 C.c_int(int(bool(neumann))))
     return phi, s, err.value
 
 

@@ -218,6 +218,65 @@ def test_gs_solve_lex_matches_oracle_loop(tk, orc, ctx, ls_steps, ls_tol, rng):
     assert abs(e - e_ref) <= 1e-10 * e_ref
 
 
+# NeuGaussSeidel (poisson_imp.hxx:115-334): every voxel updated, the derivative bc on the
+# boundary; FP64 bit for bit against the oracle's sweep in the reference's loop order.
+NEU_SHAPES = [(2, 2, 2), (3, 4, 2), (5, 6, 7), (9, 17, 33), (16, 8, 64), (20, 30, 70)]
+
+
+@pytest.mark.parametrize("shape", NEU_SHAPES)
+@pytest.mark.parametrize("dt,bc,h", [(1.0, 0.0, (1.0, 1.0, 1.0)), (0.7, 0.3, (0.5, 1.0, 2.0)),
+                                     (1.0, -1.2, (1.0, 1.0, 1.0))])
+def test_gs_neumann_bitwise(tk, orc, ctx, shape, dt, bc, h, rng):
+    say("gs neumann", shape, dt, bc, h)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    ref = phi
+    for _ in range(3):
+        ref = orc.neu_sweep(ref, rhs, dt, bc, h)
+    vp, vr = ctx.from_numpy(phi, h=h), ctx.from_numpy(rhs, h=h)
+    tk.poisson_sweeps(ctx, vp, vr, 3, dt, bc, tk.TK_GS_LEX, neumann=True)
+    assert np.array_equal(vp.numpy(), ref)
+
+
+def test_gs_neumann_f32_and_refusals(tk, orc, ctx, rng):
+    """FP32 storage against the FP64 oracle (error stated against the field's scale), and
+    the orders and shapes the Neumann sweep refuses"""
+    shape = (33, 20, 40)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    ref = phi
+    for _ in range(4):
+        ref = orc.neu_sweep(ref, rhs, 1.0, 0.25)
+    vp, vr = ctx.from_numpy(phi.astype(np.float32)), ctx.from_numpy(rhs.astype(np.float32))
+    tk.poisson_sweeps(ctx, vp, vr, 4, 1.0, 0.25, tk.TK_GS_LEX, neumann=True)
+    err = np.max(np.abs(vp.numpy().astype(np.float64) - ref)) / np.max(np.abs(ref))
+    say(f"neumann FP32 4 sweeps: max rel err {err:.2e}")
+    assert err < 2e-6
+    with pytest.raises(tk.TkError):
+        tk.poisson_sweeps(ctx, vp, vr, 1, 1.0, 0.0, tk.TK_GS_REDBLACK, neumann=True)
+    flat = ctx.from_numpy(rng.random((8, 8, 1)))
+    with pytest.raises(tk.TkError):
+        tk.poisson_sweeps(ctx, flat, flat, 1, 1.0, 0.0, tk.TK_GS_LEX, neumann=True)
+
+
+@pytest.mark.parametrize("ls_steps,ls_tol", [(5, 1e-6), (40, 1e-3)])
+def test_gs_neumann_solve_matches_oracle_loop(tk, orc, ctx, ls_steps, ls_tol, rng):
+    """the pure Neumann problem is singular: a right-hand side with a non-zero mean has no
+    steady state (phi drifts and the relative update falls like 1/sweeps, 1e6 sweeps to
+    reach 1e-6), so the gated cases use a zero-mean one, which converges (~2000 sweeps)"""
+    say("gs neumann solve", ls_steps, ls_tol)
+    shape = (24, 22, 26)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    rhs -= rhs.mean()
+    ref, s_ref, e_ref = orc.poisson_loop(phi, rhs, ls_steps, ls_tol, neumann=True)
+    vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+    s, e = tk.poisson_solve(ctx, vp, vr, ls_steps, ls_tol, order=tk.TK_GS_LEX, neumann=True)
+    assert s == s_ref
+    assert np.array_equal(vp.numpy(), ref)
+    assert abs(e - e_ref) <= 1e-10 * e_ref
+
+
 def rb_sweep_f32(phi, rhs):
     """red (i+j+k even) then black, float32 throughout, operand order of the kernel."""
     X, Y, Z = phi.shape

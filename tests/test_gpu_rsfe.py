@@ -77,6 +77,38 @@ def test_loop_acyclic_f64_matches_oracle(tk, orc, ctx, shape, sigma):
     assert np.max(np.abs(phic - ref)) > 1e-6
 
 
+@pytest.mark.parametrize("shape,ls_tol", [((24, 20, 28), 0.0), ((32, 30, 20), 0.02)])
+def test_loop_neumann_f64_matches_oracle(tk, orc, ctx, shape, ls_tol):
+    """set_bc(NEU): onestep_poisson = NeuGaussSeidel with derivative 0 (imp:1294-1322).
+    The pure Neumann problem is singular and the loop's right-hand side has a non-zero mean,
+    so phi drifts while the relative update falls like 1/sweeps.  A tight gate lets it drift
+    out of [a0, b0] for good (ls_tol 1e-3: ~1000 sweeps, then the cut leaves phi = 0 and the
+    next right-hand side is rounding noise, whose sweep count no two implementations share);
+    the gate here, 0.02, keeps a mask (44-45 sweeps per iteration)."""
+    say("loop neumann f64", shape, ls_tol)
+    u0, phi0 = tk.gen_standard(shape, 5, 1, 2)
+    p = orc.default_params(sigma=2.0, maxiter=6, tol=0.0, ls_steps=5, ls_tol=ls_tol, neumann=1)
+    ref, sw, tl, _ = orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0, p)
+    phi, st = run_gpu(tk, ctx, u0, phi0, np.float64, 6, sigma=2.0, maxiter=6, tol=0.0,
+                      ls_steps=5, ls_tol=ls_tol, gs_order=tk.TK_GS_LEX, neumann=1)
+    assert [s["sweeps"] for s in st] == list(sw)
+    assert (ref >= 0.5).sum() > 500
+    err = np.max(np.abs(phi - ref))
+    say(f"loop neumann {shape} ls_tol {ls_tol}: max|phi-ref| = {err:.3e}, sweeps {list(sw)}")
+    assert err < 1e-9
+    assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
+    pd = orc.default_params(sigma=2.0, maxiter=6, tol=0.0, ls_steps=5, ls_tol=ls_tol)
+    assert np.max(np.abs(orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0, pd)[0] - ref)) > 1e-6
+
+
+def test_loop_neumann_refusals(tk, ctx):
+    u0, phi0 = tk.gen_standard((16, 16, 16), 5, 1, 2)
+    vu, vi = ctx.from_numpy(u0), ctx.from_numpy(phi0)
+    for kw in (dict(gs_order=tk.TK_GS_REDBLACK), dict(ls_res_tol=1e-6)):
+        with pytest.raises(tk.TkError):
+            tk.Rsfe(ctx, vu, vi, tk.default_params(neumann=1, **kw))
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("init_shape", ["same", (15, 16, 11), (16, 15, 12), None, (10, 16, 12)])
 def test_init_phi_cases_vs_oracle(tk, orc, ctx, rng, init_shape, dtype):

@@ -279,6 +279,148 @@ def test_gauss_seidel_bitwise_vs_sequential(orc, dt, bc, h, rng):
     assert np.array_equal(a2, gs_python(b, rhs, dt, bc, h))
 
 
+# ---- NeuGaussSeidel (poisson_imp.hxx:115-334, 3-D branch) ----------------------------
+def neu_flux_form(phi, b, dt, bc, h):
+    """One lexicographic sweep written as a cell balance rather than the reference's case
+    tables: each voxel solves a p = sum(in-box neighbours)/h^2 + flux + b + p_old/dt, where
+    a = 1/dt + (in-box neighbour count per axis)/h^2 and every side of the voxel on the
+    volume's boundary adds the prescribed derivative as a flux: -n bc / h on a low side,
+    +n bc / h on a high side, with n = 1/sqrt(number of boundary planes through the voxel)
+    (the oblique normal of edges and corners)."""
+    r = phi.copy()
+    X, Y, Z = r.shape
+    dims = (X, Y, Z)
+    for k in range(Z):
+        for j in range(Y):
+            for i in range(X):
+                v = (i, j, k)
+                nplanes = sum(1 for a in range(3) if v[a] in (0, dims[a] - 1))
+                n = 1.0 / np.sqrt(nplanes) if nplanes else 1.0
+                a_coef = 1.0 / dt
+                rhs = b[v] + r[v] / dt
+                for ax in range(3):
+                    hh = h[ax]
+                    for step in (-1, 1):
+                        w = list(v)
+                        w[ax] += step
+                        if 0 <= w[ax] < dims[ax]:
+                            rhs += r[tuple(w)] / (hh * hh)
+                            a_coef += 1.0 / (hh * hh)
+                        else:
+                            rhs += (-n * bc / hh) if step < 0 else (n * bc / hh)
+                r[v] = rhs / a_coef
+    return r
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 4, 2), (5, 3, 4), (6, 7, 5)])
+@pytest.mark.parametrize("dt,bc,h", [(1.0, 0.0, (1.0, 1.0, 1.0)), (0.7, 0.3, (0.5, 1.0, 2.0)),
+                                     (1.0, -1.2, (1.0, 1.0, 1.0))])
+def test_neumann_sweep_vs_flux_form(orc, shape, dt, bc, h, rng):
+    """the oracle's sweep against the cell-balance form above (two sweeps; the forms round
+    differently, so 1e-13 relative), and the 2-D branch is refused"""
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    a = orc.neu_sweep(phi, rhs, dt, bc, h)
+    b = neu_flux_form(phi, rhs, dt, bc, h)
+    scale = np.max(np.abs(b))
+    assert np.max(np.abs(a - b)) < 1e-13 * scale
+    a2 = orc.neu_sweep(a, rhs, dt, bc, h)
+    assert np.max(np.abs(a2 - neu_flux_form(b, rhs, dt, bc, h))) < 1e-13 * scale
+    with pytest.raises(ValueError):
+        orc.neu_sweep(rng.random((4, 4, 1)), rng.random((4, 4, 1)))
+
+
+def test_neumann_fixed_point_is_graph_laplacian_solution(orc, rng):
+    """bc = 0: the fixed point of the sweep solves -L phi = rhs with L the Laplacian of the
+    grid graph (edge weights 1/h^2, no flux through the boundary: the Neumann matrix),
+    assembled here from the grid's neighbour pairs; rhs has zero sum (the compatibility
+    condition of the singular system) and phi is compared up to its constant."""
+    X, Y, Z = 5, 4, 6
+    h = (1.0, 0.8, 1.3)
+    rhs = rng.standard_normal((X, Y, Z))
+    rhs -= rhs.mean()
+    phi = np.zeros((X, Y, Z))
+    for _ in range(3000):
+        phi = orc.neu_sweep(phi, rhs, 1.0, 0.0, h)
+    n = X * Y * Z
+    L = np.zeros((n, n))
+    ids = np.arange(n).reshape(X, Y, Z)
+    for ax, hh in enumerate(h):
+        a = np.moveaxis(ids, ax, 0)
+        for p, q in zip(a[:-1].ravel(), a[1:].ravel()):
+            w = 1.0 / (hh * hh)
+            L[p, p] -= w
+            L[q, q] -= w
+            L[p, q] += w
+            L[q, p] += w
+    sol = np.linalg.lstsq(-L, rhs.ravel(), rcond=None)[0].reshape(X, Y, Z)
+    d = (phi - phi.mean()) - (sol - sol.mean())
+    assert np.max(np.abs(d)) < 1e-9 * np.max(np.abs(sol))
+    assert np.max(np.abs(L @ phi.ravel() + rhs.ravel())) < 1e-9
+
+
+def test_neumann_fixed_point_mirror_symmetry(orc, rng):
+    """bc != 0: mirroring the volume along all three axes swaps low and high faces, which
+    flips the sign of the prescribed derivative: fixed_point(mirror(rhs), -bc) =
+    mirror(fixed_point(rhs, bc)) up to the constant.  A sign or face swap in one of the
+    boundary cases breaks it."""
+    X, Y, Z = 5, 6, 4
+    h = (0.9, 1.1, 1.0)
+    rhs = rng.standard_normal((X, Y, Z))
+    rhs -= rhs.mean()
+    bc = 0.4
+    p1 = np.zeros((X, Y, Z))
+    p2 = np.zeros((X, Y, Z))
+    m = rhs[::-1, ::-1, ::-1].copy()
+    for _ in range(3000):
+        p1 = orc.neu_sweep(p1, rhs, 1.0, bc, h)
+        p2 = orc.neu_sweep(p2, m, 1.0, -bc, h)
+    back = p2[::-1, ::-1, ::-1]
+    d = (p1 - p1.mean()) - (back - back.mean())
+    assert np.max(np.abs(d)) < 1e-9 * np.max(np.abs(p1 - p1.mean()))
+    # and the boundary term is live: bc = 0 gives another field
+    p0 = np.zeros((X, Y, Z))
+    for _ in range(3000):
+        p0 = orc.neu_sweep(p0, rhs, 1.0, 0.0, h)
+    assert np.max(np.abs((p0 - p0.mean()) - (p1 - p1.mean()))) > 1e-3
+
+
+def test_neumann_poisson_loop_and_apply(orc, tk):
+    """the ls_steps/ls_tol loop around NeuGaussSeidel, and apply() with set_bc(NEU) (bc
+    value 0, imp:1736-1739) equal to the composition with the Neumann loop"""
+    rng = np.random.default_rng(3)
+    phi0 = rng.random((6, 5, 7))
+    rhs = rng.standard_normal((6, 5, 7))
+    ref = phi0
+    for _ in range(4):
+        ref = orc.neu_sweep(ref, rhs)
+    got, s, _ = orc.poisson_loop(phi0, rhs, 4, 0.0, neumann=True)
+    assert s == 4 and np.array_equal(got, ref)
+    shape = (14, 12, 10)
+    u0, phi0 = tk.gen_standard(shape, 13, 1, 2)
+    p = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0, neumann=1)
+    phi_or, sweeps, tols, st = orc.rsfe_apply(u0, phi0, p)
+    phi = orc.change_range(phi0, 1.0, 0.0)
+    divdb = np.zeros(shape)
+    b = [np.zeros(shape) for _ in range(3)]
+    g = orc.edge_detect(u0, p.beta, p.lamda, p.nu)
+    for t in range(2):
+        rhs = orc.rsfe_forcing(u0, phi, divdb, p)
+        phi, s, _ = orc.poisson_loop(phi, rhs, 3, 0.0, neumann=True)
+        phi = np.clip(phi, 0.0, 1.0)
+        d = orc.grad(phi)
+        for a in range(3):
+            b[a] = b[a] + d[a]
+        d = [orc.shrink(b[a], g) for a in range(3)]
+        for a in range(3):
+            b[a] = b[a] - d[a]
+            d[a] = d[a] - b[a]
+        divdb = orc.div(*d)
+    assert np.array_equal(phi, phi_or)
+    pd = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0)
+    assert not np.array_equal(orc.rsfe_apply(u0, phi0, pd)[0], phi_or)
+
+
 def test_gauss_seidel_fixed_point_is_dirichlet_poisson(orc, rng):
     """The fixed point of the sweep solves -lap_h phi = rhs with phi = bc on the faces
     (the 1/dt terms cancel); checked against a dense solve."""
