@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libtk.so")
 
 TK_OK = 0
 TK_ERR_ARG, TK_ERR_HIP, TK_ERR_NOMEM, TK_ERR_SHAPE = -1, -2, -3, -4
-TK_ERR_DTYPE, TK_ERR_UNSUPPORTED, TK_ERR_SWEEPS = -5, -6, -7
+TK_ERR_DTYPE, TK_ERR_UNSUPPORTED, TK_ERR_SWEEPS, TK_ERR_IO = -5, -6, -7, -8
 TK_F32, TK_F64, TK_U8 = 1, 2, 3
 TK_GS_LEX, TK_GS_REDBLACK = 0, 1
 # tk_kernel_tag order
@@ -47,12 +47,21 @@ class RsfeParams(C.Structure):
                                             ("res_check_every", C.c_int),
                                             ("ws_presweeps", C.c_int),
                                             ("acyclic", C.c_int),
-                                            ("neumann", C.c_int)]
+                                            ("neumann", C.c_int),
+                                            ("auto_extract_conn_comp", C.c_int),
+                                            ("auto_extract_conn_comp_freq", C.c_int),
+                                            ("cc_init_pixel", C.c_double * 3),
+                                            ("cc_threshold", C.c_double),
+                                            ("cc_binary_output", C.c_int),
+                                            ("cc_init_variables", C.c_int),
+                                            ("dumpfrequency", C.c_int),
+                                            ("outputname", C.c_char * 256)]
 
 
 class IterStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("ls_err", C.c_double), ("tollerance", C.c_double),
-                ("guess_used", C.c_int), ("res_plain", C.c_double), ("res_guess", C.c_double)]
+                ("guess_used", C.c_int), ("res_plain", C.c_double), ("res_guess", C.c_double),
+                ("cc_extracted", C.c_int)]
 
 
 def _load():
@@ -81,6 +90,8 @@ def _load():
         "tk_vol_download": (i, [vp, vp, i]),
         "tk_vol_fill": (i, [vp, d]),
         "tk_vol_copy": (i, [vp, vp]),
+        "tk_vol_write_mhd": (i, [vp, C.c_char_p]),
+        "tk_vol_read_mhd": (i, [vp, C.c_char_p, i, C.POINTER(vp)]),
         "tk_filter_gaussian": (i, [vp, i, i, i, d, d, d, d, d, i, i, C.POINTER(vp)]),
         "tk_filter_free": (i, [vp]),
         "tk_filter_taps": (i, [vp, ip]),
@@ -212,6 +223,12 @@ def default_params(**kw):
     for k, v in kw.items():
         if not hasattr(p, k):
             raise AttributeError(k)
+        if k == "cc_init_pixel":
+            v = (C.c_double * 3)(*[float(x) for x in v])
+        elif k == "outputname":
+            v = v.encode() if isinstance(v, str) else bytes(v)
+            if len(v) >= 256:
+                raise ValueError("outputname: at most 255 bytes")
         setattr(p, k, v)
     return p
 
@@ -266,15 +283,30 @@ class Context:
         v.upload(a)
         return v
 
+    def read_mhd(self, path, dtype=np.float64):
+        """a MetaImage file (the reference's dump format) as a new volume"""
+        hv = C.c_void_p()
+        _check(lib.tk_vol_read_mhd(self.h, os.fsencode(path), _NP2TK[np.dtype(dtype)],
+                                   C.byref(hv)))
+        dims = (C.c_int * 3)()
+        h = (C.c_double * 3)()
+        dt = C.c_int()
+        _check(lib.tk_vol_info(hv, dims, h, C.byref(dt)))
+        return Volume(self, tuple(dims), _TK2NP[dt.value], tuple(h), handle=hv)
+
 
 class Volume:
     """tk_vol: a device-resident volume (im3d::image3d storage)."""
 
-    def __init__(self, ctx, shape, dtype=np.float64, h=(1.0, 1.0, 1.0)):
+    def __init__(self, ctx, shape, dtype=np.float64, h=(1.0, 1.0, 1.0), handle=None):
         self.ctx = ctx
         ctx._adopt(self)
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
+        self.h_spacing = tuple(float(x) for x in h)
+        if handle is not None:
+            self.h = handle
+            return
         hv = C.c_void_p()
         _check(lib.tk_vol_alloc(ctx.h, *self.shape, *[float(x) for x in h],
                                 _NP2TK[self.dtype], C.byref(hv)))
@@ -295,6 +327,10 @@ class Volume:
     def fill(self, value):
         _check(lib.tk_vol_fill(self.h, float(value)))
         return self
+
+    def write_mhd(self, basename):
+        """<basename>.mhd + <basename>.raw (MetaImage, double, x fastest)"""
+        _check(lib.tk_vol_write_mhd(self.h, os.fsencode(basename)))
 
     def free(self):
         if self.h:
@@ -448,7 +484,8 @@ class Rsfe:
         _check(lib.tk_rsfe_iterate(self.h, n, st, C.byref(done)))
         return [dict(sweeps=st[q].sweeps, ls_err=st[q].ls_err, tollerance=st[q].tollerance,
                      guess_used=st[q].guess_used, res_plain=st[q].res_plain,
-                     res_guess=st[q].res_guess) for q in range(done.value)]
+                     res_guess=st[q].res_guess, cc_extracted=st[q].cc_extracted)
+                for q in range(done.value)]
 
     def phi(self, dtype=np.float64):
         out = np.empty(self.shape, dtype)

@@ -17,6 +17,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
 #include <utility>
 
 #include "tk_device.hpp"
@@ -174,6 +177,41 @@ __global__ void k_add_inplace(T* __restrict__ a, const T* __restrict__ b, long n
     for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) a[v] += b[v];
 }
 
+// sum (phi - phiold)^2 and sum phiold^2 per block (the tollerance of imp:1832 when the
+// Split-Bregman pass, which forms them otherwise, does not run)
+template <class T>
+__global__ __launch_bounds__(256) void k_tol_partial(const T* __restrict__ phi,
+                                                     const T* __restrict__ old, long n,
+                                                     double* __restrict__ part)
+{
+    double s1 = 0, s2 = 0;
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        const double o = (double)old[v];
+        const double d = (double)phi[v] - o;
+        s1 += d * d;
+        s2 += o * o;
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+// the non-binary output of connected_component (image3d_imp.hxx:1806-1816):
+// phi := ((cc + negative bw) * (phi - min)) + min, in T
+template <class T>
+__global__ void k_cc_values(T* __restrict__ phi, const uint8_t* __restrict__ cc,
+                            const uint8_t* __restrict__ neg, T mn, long n)
+{
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        T r = (T)cc[v];
+        r += (T)neg[v];
+        r *= phi[v] - mn;
+        phi[v] = r + mn;
+    }
+}
+
 __global__ void k_sum_pairs_rsfe(const double* __restrict__ part, int nb, double* __restrict__ out)
 {
     double s1 = 0, s2 = 0;
@@ -207,6 +245,8 @@ struct tk_rsfe_s {
     tk_unet net = nullptr;     // learned warm start (same context), or none
     tk_vol guess = nullptr;    // its guess
     tk_vol resid = nullptr;    // the residual field it reads
+    tk_vol ccmask = nullptr;   // U8: the automatic extraction's black-and-white / component
+    tk_vol ccneg = nullptr;    // U8: its negative black-and-white image (non-binary output)
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
     int iter = 0;
     double tollerance = 1000;
@@ -214,6 +254,7 @@ struct tk_rsfe_s {
     // sweeps never write the i and j faces, copy the k faces, and cut_phi keeps a value in
     // [a0, b0] (bc is 0 or a0), so they stay set for the run
     bool faces_set = false;
+    bool final_dumped = false;  // the end-of-run dump has been written
 };
 
 namespace {
@@ -226,6 +267,57 @@ int launch_add(tk_ctx ctx, tk_vol a, tk_vol b)
     hipLaunchKernelGGL(k_add_inplace<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream,
                        (T*)a->d, (const T*)b->d, n);
     TK_LAUNCHED();
+    return TK_OK;
+}
+
+// extract_connected_component, automatic branch (rsfe_splitbregman_imp.hxx:447-491), on the
+// cut phi: seed floor(cc_pixel / h); outside the volume -> the reference's WARNING, no
+// change; a seed that is black at threshold (alpha - a0)/(b0 - a0) -> no change; else
+// phi := connected_component(seed, 0.5, 6-connected, cc_binary_output) rescaled to
+// [a0, b0].  Reads one voxel of the seed test back to the host (waits on the stream).
+template <class T>
+int auto_extract_cc(tk_rsfe s, int* extracted)
+{
+    *extracted = 0;
+    tk_ctx ctx = s->ctx;
+    const tk_rsfe_params& p = s->p;
+    const int i = (int)std::floor(p.cc_init_pixel[0] / s->h[0]),
+              j = (int)std::floor(p.cc_init_pixel[1] / s->h[1]),
+              k = (int)std::floor(p.cc_init_pixel[2] / s->h[2]);
+    if (i < 0 || j < 0 || k < 0 || i > s->X - 1 || j > s->Y - 1 || k > s->Z - 1) {
+        std::fprintf(stderr, "WARNING::apply::extract_connected_component: coordinates outside "
+                             "allowed range\n");
+        return TK_OK;
+    }
+    if (!s->ccmask)
+        TK_TRY(tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], TK_U8, &s->ccmask));
+    TK_TRY(tk_im_to_black_and_white(ctx, s->phi, p.cc_threshold, 0, s->ccmask));
+    uint8_t seed_bw = 0;
+    const long v = ((long)i * s->Y + j) * s->Z + k;
+    TK_HIP(hipMemcpyAsync(&seed_bw, (const uint8_t*)s->ccmask->d + v, 1, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    if (seed_bw == 0) return TK_OK;
+    double mn = 0, mx = 0;
+    if (!p.cc_binary_output) {
+        if (!s->ccneg)
+            TK_TRY(tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], TK_U8,
+                                &s->ccneg));
+        TK_TRY(tk_im_to_black_and_white(ctx, s->phi, 0.5, 1, s->ccneg));
+        TK_TRY(tk_minmax(ctx, s->phi, &mn, &mx));
+    }
+    TK_TRY(tk_cc_seeded(ctx, s->phi, i, j, k, 0.5, 0, s->ccmask, nullptr));
+    if (p.cc_binary_output) {
+        TK_TRY(tk_vol_copy(s->phi, s->ccmask));
+    } else {
+        const long n = s->phi->n();
+        hipLaunchKernelGGL(k_cc_values<T>, dim3(blocks_for(n, 256, 8192)), dim3(256), 0,
+                           ctx->stream, (T*)s->phi->d, (const uint8_t*)s->ccmask->d,
+                           (const uint8_t*)s->ccneg->d, (T)mn, n);
+        TK_LAUNCHED();
+    }
+    TK_TRY(tk_change_range_of_intensity(ctx, s->phi, p.b0, p.a0));
+    *extracted = 1;
     return TK_OK;
 }
 
@@ -324,21 +416,50 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
                                   &ls_err, s->faces_set);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
     if (!p.neumann && chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
-    // cut + norms + grad/shrink/Bregman, then div.  (A single pass recomputing q of the
-    // six neighbours for div measured 281 us at 256^3 FP32 against 155 + 70 us for the two
-    // passes: 140 registers per lane, three waves per SIMD.)
+    // automatic connected component (imp:1808-1812) on the cut phi; the extracted phi is
+    // not cut again, and with cc_init_variables b and divdb restart from zero in place of
+    // this iteration's Split-Bregman update (imp:1849-1859)
+    int extracted = 0;
+    double lo = p.a0, hi = p.b0;
+    if (p.auto_extract_conn_comp && p.auto_extract_conn_comp_freq > 0 &&
+        (s->iter + 1) % p.auto_extract_conn_comp_freq == 0) {
+        TK_TRY(tk_cut(ctx, s->phi, p.a0, p.b0));
+        TK_TRY(auto_extract_cc<T>(s, &extracted));
+        if (extracted) {
+            s->faces_set = false;  // the component may reach the faces
+            lo = -std::numeric_limits<double>::infinity();
+            hi = std::numeric_limits<double>::infinity();
+        }
+    }
     int nb = 0;
     double* part = ctx->acc + 64;
-    {
-        TimedLaunch tl(ctx, TK_K_SB);
-        TK_TRY(launch_sb_update<T>(ctx, s->phi, s->phiold, s->b, s->g, s->q, p.a0, p.b0, &nb,
-                                   part));
-    }
-    hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb, s->tol_dev);
-    TK_LAUNCHED();
-    {
-        TimedLaunch tl(ctx, TK_K_DIV);
-        TK_TRY(launch_div<T>(ctx, s->q[0], s->q[1], s->q[2], s->divdb));
+    if (extracted && p.cc_init_variables) {
+        const long n = s->phi->n();
+        nb = (int)blocks_for(n, 256, 1024);
+        hipLaunchKernelGGL(k_tol_partial<T>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const T*)s->phi->d, (const T*)s->phiold->d, n, part);
+        TK_LAUNCHED();
+        hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb,
+                           s->tol_dev);
+        TK_LAUNCHED();
+        for (int a = 0; a < 3; ++a) TK_TRY(tk_vol_fill(s->b[a], 0.));
+        TK_TRY(tk_vol_fill(s->divdb, 0.));
+    } else {
+        // cut + norms + grad/shrink/Bregman, then div.  (A single pass recomputing q of the
+        // six neighbours for div measured 281 us at 256^3 FP32 against 155 + 70 us for the
+        // two passes: 140 registers per lane, three waves per SIMD.)
+        {
+            TimedLaunch tl(ctx, TK_K_SB);
+            TK_TRY(launch_sb_update<T>(ctx, s->phi, s->phiold, s->b, s->g, s->q, lo, hi, &nb,
+                                       part));
+        }
+        hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb,
+                           s->tol_dev);
+        TK_LAUNCHED();
+        {
+            TimedLaunch tl(ctx, TK_K_DIV);
+            TK_TRY(launch_div<T>(ctx, s->q[0], s->q[1], s->q[2], s->divdb));
+        }
     }
     s->iter++;
     if (!sync) return TK_OK;
@@ -355,6 +476,7 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         st->guess_used = used;
         st->res_plain = res_plain;
         st->res_guess = res_guess;
+        st->cc_extracted = extracted;
     }
     return TK_OK;
 }
@@ -386,6 +508,8 @@ void free_state(tk_rsfe s)
     if (s->alt) (void)hipFree(s->alt);
     if (s->guess) tk_vol_free(s->guess);
     if (s->resid) tk_vol_free(s->resid);
+    if (s->ccmask) tk_vol_free(s->ccmask);
+    if (s->ccneg) tk_vol_free(s->ccneg);
     if (s->tol_dev) (void)hipFree(s->tol_dev);
     if (s->flt) tk_filter_free(s->flt);
     delete s;
@@ -480,6 +604,17 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     p->ws_presweeps = 512;
     p->acyclic = 0;
     p->neumann = 0;
+    // rsfe_splitbregman_imp.hxx:25-41: auto_extract_conn_comp false, cc_init_pixel -1,
+    // cc_binary_output and cc_init_variables true, freq 0, dumpfrequency 0, "result"
+    p->auto_extract_conn_comp = 0;
+    p->auto_extract_conn_comp_freq = 0;
+    p->cc_init_pixel[0] = p->cc_init_pixel[1] = p->cc_init_pixel[2] = -1;
+    p->cc_threshold = 0.5;
+    p->cc_binary_output = 1;
+    p->cc_init_variables = 1;
+    p->dumpfrequency = 0;
+    std::memset(p->outputname, 0, sizeof p->outputname);
+    std::strcpy(p->outputname, "result");
     return TK_OK;
 }
 
@@ -496,6 +631,12 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
              "tk_rsfe_create: gs_order");
     TK_CHECK(p->acyclic == 0 || p->acyclic == 1, TK_ERR_ARG, "tk_rsfe_create: acyclic");
     TK_CHECK(p->neumann == 0 || p->neumann == 1, TK_ERR_ARG, "tk_rsfe_create: neumann");
+    TK_CHECK(p->auto_extract_conn_comp_freq >= 0 && p->dumpfrequency >= 0, TK_ERR_ARG,
+             "tk_rsfe_create: auto_extract_conn_comp_freq and dumpfrequency must be >= 0");
+    TK_CHECK(p->cc_threshold >= 0 && p->cc_threshold <= 1, TK_ERR_ARG,
+             "tk_rsfe_create: cc_threshold must be in [0, 1]");
+    TK_CHECK(std::memchr(p->outputname, 0, sizeof p->outputname) != nullptr, TK_ERR_ARG,
+             "tk_rsfe_create: outputname is not terminated");
     // the Neumann sweep is the reference's lexicographic one; the learned start and the
     // residual gate are built on the Dirichlet problem
     TK_CHECK(!p->neumann || (p->gs_order == TK_GS_LEX && p->warm_start == 0 &&
@@ -569,15 +710,30 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
     TK_CHECK(s->p.warm_start == 0 || s->net, TK_ERR_ARG,
              "tk_rsfe_iterate: warm_start set but no network attached (tk_rsfe_set_warm_start)");
     int done = 0;
+    const tk_rsfe_params& p = s->p;
     // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
-    while (done < n_iters && s->tollerance > s->p.tol && s->iter < s->p.maxiter) {
+    while (done < n_iters && s->tollerance > p.tol && s->iter < p.maxiter) {
         int r = s->dtype == TK_F64 ? rsfe_iteration<double>(s, stats ? &stats[done] : nullptr)
                                    : rsfe_iteration<float>(s, stats ? &stats[done] : nullptr);
+        // dump every dumpfrequency iterations, not at maxiter (imp:1939-1951)
+        const int t = s->iter;
+        if (r == TK_OK && p.dumpfrequency != 0 && t % p.dumpfrequency == 0 && t != p.maxiter)
+            r = tk_vol_write_mhd(s->phi, (std::string(p.outputname) + "_" + std::to_string(t)).c_str());
         if (r != TK_OK) {
             if (n_done) *n_done = done;
             return r;
         }
         ++done;
+    }
+    // the loop has ended: the final dump (imp:1968-1978), once
+    if (p.dumpfrequency > 0 && !s->final_dumped && (s->tollerance <= p.tol || s->iter >= p.maxiter)) {
+        s->final_dumped = true;
+        const int r = tk_vol_write_mhd(
+            s->phi, (std::string(p.outputname) + "_" + std::to_string(s->iter) + "_final").c_str());
+        if (r != TK_OK) {
+            if (n_done) *n_done = done;
+            return r;
+        }
     }
     if (n_done) *n_done = done;
     return TK_OK;

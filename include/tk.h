@@ -32,7 +32,8 @@ enum tk_status {
     TK_ERR_SHAPE = -4,       /* dimensions of the operands do not agree */
     TK_ERR_DTYPE = -5,       /* element types of the operands do not agree */
     TK_ERR_UNSUPPORTED = -6, /* valid request outside what this build implements */
-    TK_ERR_SWEEPS = -7       /* Poisson solve hit max_sweeps before ls_tol */
+    TK_ERR_SWEEPS = -7,      /* Poisson solve hit max_sweeps before ls_tol */
+    TK_ERR_IO = -8           /* a file could not be opened, read, parsed or written */
 };
 
 typedef enum tk_dtype { TK_F32 = 1, TK_F64 = 2, TK_U8 = 3 } tk_dtype;
@@ -94,6 +95,18 @@ int tk_vol_upload(tk_vol vol, const void* host, tk_dtype host_dtype);
 int tk_vol_download(tk_vol vol, void* host, tk_dtype host_dtype);
 int tk_vol_fill(tk_vol vol, double value);
 int tk_vol_copy(tk_vol dst, tk_vol src); /* converts F32 <-> F64 */
+
+/* ---- MetaImage files: the reference's dump format -------------------------------------
+ * im3d::interface<S>::write(name, ".mhd") (interface_imp.hxx:672-690) of the image
+ * convertfromimage3d made (104-122): <basename>.mhd (text header: DimSize nx ny nz,
+ * ElementSpacing hx hy hz, ElementType MET_DOUBLE, little-endian, uncompressed) and
+ * <basename>.raw (the voxels as double, x fastest: i + nx (j + ny k)).  The reader takes
+ * any uncompressed MetaImage (NDims 2 or 3, MET_CHAR .. MET_DOUBLE, either byte order,
+ * ElementDataFile LOCAL or a file, HeaderSize, channels: component 0), as
+ * interface<S>(name.mhd) + convert2image3d (interface_imp.hxx:33-47, 125-150) do, into a
+ * new volume of dtype. */
+int tk_vol_write_mhd(tk_vol vol, const char* basename);
+int tk_vol_read_mhd(tk_ctx ctx, const char* path, tk_dtype dtype, tk_vol* out);
 
 /* ---- R4: Gaussian filter and its cyclic spectrum ---------------------------------
  * conv::filtering<T>::build_gaussian_filter (convolution_imp.hxx:22-91) and the
@@ -256,6 +269,22 @@ typedef struct tk_rsfe_params {
     int neumann;              /* 1: onestep_poisson = NeuGaussSeidel, as set_bc(NEU) installs
                                  (imp:1294-1322; bc value 0 unless bc_dir); lexicographic
                                  only, no warm_start / ls_res_tol.  0: DirGaussSeidel */
+    /* automatic connected component (extract_connected_component, imp:447-491, called at
+       1808-1812): when auto_extract_conn_comp and iteration t % auto_extract_conn_comp_freq
+       == 0, the cut phi is replaced by its component through the voxel
+       floor(cc_init_pixel / h) at relative threshold cc_threshold ((alpha - a0)/(b0 - a0),
+       0.5 for the default alpha), binary or not (cc_binary_output), rescaled to [a0, b0];
+       cc_init_variables then zeroes b and divdb in place of that iteration's Split-Bregman
+       update (1849-1859).  A seed outside the volume prints the reference's WARNING. */
+    int auto_extract_conn_comp, auto_extract_conn_comp_freq;
+    double cc_init_pixel[3];
+    double cc_threshold;
+    int cc_binary_output, cc_init_variables;
+    /* dumps (imp:1939-1951, 1968-1978): every dumpfrequency iterations (not at maxiter)
+       phi is written to <outputname>_<t>.mhd/.raw, and when the run ends to
+       <outputname>_<t>_final.mhd/.raw (tk_vol_write_mhd).  0: no dumps */
+    int dumpfrequency;
+    char outputname[256];
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {
@@ -265,6 +294,7 @@ typedef struct tk_rsfe_iter_stats {
     int guess_used;           /* R15: learned rounds accepted */
     double res_plain;         /* R15: first round: residual of the previous iterate after 16 sweeps */
     double res_guess;         /* R15: first round: residual of the candidate after 16 sweeps */
+    int cc_extracted;         /* 1: the automatic extraction replaced phi in this iteration */
 } tk_rsfe_iter_stats;
 
 int tk_rsfe_default_params(tk_rsfe_params* p);

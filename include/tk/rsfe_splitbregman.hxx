@@ -20,6 +20,8 @@
 #ifndef TK_RSFE_SPLITBREGMAN_HXX
 #define TK_RSFE_SPLITBREGMAN_HXX
 
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "convolution.hxx"
@@ -108,6 +110,33 @@ public:
         else tk::warning("set_ls_tol: ls_tol must be non negative");
     }
     void set_ls_steps(unsigned const steps) { p_.ls_steps = (int)steps; }
+    // automatic connected component (imp:903-911, 968-1030, 1380-1417) and alpha (1422-1440)
+    void set_auto_extract_conn_comp(bool const auto_cc) { p_.auto_extract_conn_comp = auto_cc ? 1 : 0; }
+    void set_auto_extract_conn_comp_freq(unsigned const& cc_freq)
+    {
+        p_.auto_extract_conn_comp_freq = (int)cc_freq;
+    }
+    void set_cc_init_pixel_x(double const& v) { pixel(0, v, "set_cc_init_pixel_x"); }
+    void set_cc_init_pixel_y(double const& v) { pixel(1, v, "set_cc_init_pixel_y"); }
+    void set_cc_init_pixel_z(double const& v) { pixel(2, v, "set_cc_init_pixel_z"); }
+    void set_cc_binary_output(bool const cc_bo) { p_.cc_binary_output = cc_bo ? 1 : 0; }
+    void set_cc_init_variables(bool const cc_iv) { p_.cc_init_variables = cc_iv ? 1 : 0; }
+    void set_alpha(T const& alpha) override
+    {
+        if (alpha > (T)p_.a0 && alpha < (T)p_.b0) this->alpha = alpha;
+        else tk::warning("set_alpha: alpha must be greater than a0 and less than b0");
+    }
+    // dumps of phi as MetaImage files (imp:916-941, 1939-1978)
+    void set_dumpfrequency(unsigned const& dump) { p_.dumpfrequency = (int)dump; }
+    void set_outputname(std::string const& name)
+    {
+        if (name.size() < sizeof p_.outputname) {
+            std::memset(p_.outputname, 0, sizeof p_.outputname);
+            std::memcpy(p_.outputname, name.data(), name.size());
+        } else {
+            tk::warning("set_outputname: at most 255 characters; kept");
+        }
+    }
     void set_gaussian_pixel_approach(bool const gpa) { p_.gaussian_pixel_approach = gpa ? 1 : 0; }
     void set_edge_detector(ed_type const ed)
     {
@@ -167,6 +196,11 @@ public:
     tk_rsfe_params const& params() const { return p_; }
 
 private:
+    void pixel(int a, double const& v, const char* name)
+    {
+        if (v >= 0) p_.cc_init_pixel[a] = v;
+        else tk::warning(std::string(name) + ": coordinates must be positive");
+    }
     void positive(T const& v, double& slot, const char* msg)
     {
         if (v > 0) slot = (double)v;
@@ -180,6 +214,8 @@ private:
                                                                            : TK_GS_LEX;
         p.acyclic = cv_.getf() == &conv::acyclic_fftw_convolution<T> ? 1 : 0;
         p.neumann = onestep_.getf() == &lapl::NeuGaussSeidel<T> ? 1 : 0;
+        // extract_connected_component's threshold (imp:481), in T as the reference
+        p.cc_threshold = (double)((this->alpha - (T)p_.a0) / ((T)p_.b0 - (T)p_.a0));
         tk_rsfe s = nullptr;
         tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
         sweeps_.clear();

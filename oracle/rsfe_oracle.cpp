@@ -784,6 +784,9 @@ void or_default_params(or_params* p)
     p->nu = 1.; p->tol = 1.e-5; p->ls_tol = 0; p->maxiter = 100; p->bc_dir = 0;
     p->gaussian_pixel_approach = 0; p->radius = 3; p->verbosity = 0; p->acyclic = 0;
     p->neumann = 0;
+    p->auto_cc = 0; p->auto_cc_freq = 0;
+    p->cc_pixel[0] = p->cc_pixel[1] = p->cc_pixel[2] = -1;
+    p->cc_binary_output = 1; p->cc_init_variables = 1;
 }
 
 /* ---- R6 + R7: forcing term of one outer iteration (imp:1700-1759) ----------------
@@ -919,6 +922,43 @@ int or_init_phi(const double* init, int ix, int iy, int iz, int X, int Y, int Z,
     return 0;
 }
 
+/* extract_connected_component, automatic branch (rsfe_splitbregman_imp.hxx:447-491):
+ * the seed is the voxel floor(cc_pixel / h); outside the volume (or negative) the call
+ * warns and changes nothing (462-479); so does a seed that is black in phi's
+ * black-and-white image at threshold (alpha - a0)/(b0 - a0), alpha = (a0+b0)/2 (481-487).
+ * Otherwise phi := connected_component(seed, 0.5, 6-connected, cc_binary_output)
+ * (image3d_imp.hxx:1694-1757; the non-binary output is (cc + negative bw) (phi - min) + min,
+ * 1806-1816), then change_range_of_intensity(b0, a0).  Returns 1 when phi was replaced. */
+static int auto_extract_cc(std::vector<double>& phi, int X, int Y, int Z, double hx, double hy,
+                           double hz, const or_params* p)
+{
+    const long n = (long)X * Y * Z;
+    const int i = (int)std::floor(p->cc_pixel[0] / hx), j = (int)std::floor(p->cc_pixel[1] / hy),
+              k = (int)std::floor(p->cc_pixel[2] / hz);
+    if (i < 0 || j < 0 || k < 0 || i > X - 1 || j > Y - 1 || k > Z - 1) {
+        std::fprintf(stderr, "WARNING::apply::extract_connected_component: coordinates outside "
+                             "allowed range\n");
+        return 0;
+    }
+    const double alpha = (p->a0 + p->b0) / 2.;
+    std::vector<double> bw(n), cc(n);
+    or_im_to_black_and_white(phi.data(), n, (alpha - p->a0) / (p->b0 - p->a0), 0, bw.data());
+    if (bw[idx(i, j, k, Y, Z)] == 0) return 0;
+    or_connected_component(phi.data(), X, Y, Z, i, j, k, 0.5, 0, cc.data());
+    if (!p->cc_binary_output) {
+        or_im_to_black_and_white(phi.data(), n, 0.5, 1, bw.data());
+        const double mn = or_min(phi.data(), n);
+        for (long q = 0; q < n; ++q) {
+            cc[q] += bw[q];
+            cc[q] *= phi[q] - mn;
+            cc[q] += mn;
+        }
+    }
+    phi.swap(cc);
+    or_change_range_of_intensity(phi.data(), n, p->b0, p->a0);
+    return 1;
+}
+
 /* ---- R2: apply(myim, init), rsfe_splitbregman_imp.hxx:1547-1995 ------------------
    cv = cyclic_fftw_convolution or acyclic_fftw_convolution (p->acyclic), onestep_poisson =
    DirGaussSeidel or NeuGaussSeidel (p->neumann), edge detector GRAD with
@@ -970,7 +1010,7 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
     double t_loop0 = now_s();
     double tollerance = 1000;
     int t;
-    int iters = 0;
+    int iters = 0, cc_extractions = 0;
     for (t = 1; tollerance > p->tol && t <= p->maxiter; ++t) {
         V phiold(phi);
         V kr2(n);
@@ -985,6 +1025,13 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
             if (phi[q] < p->a0) phi[q] = p->a0;
             else if (phi[q] > p->b0) phi[q] = p->b0;
         }
+        /* automatic connected component, imp:1808-1812 */
+        bool init_variables = false;
+        if (p->auto_cc && p->auto_cc_freq != 0 && t % p->auto_cc_freq == 0 &&
+            auto_extract_cc(phi, X, Y, Z, hx, hy, hz, p)) {
+            ++cc_extractions;
+            if (p->cc_init_variables) init_variables = true;
+        }
         /* imp:1832 */
         {
             V dd(n);
@@ -994,6 +1041,13 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
                          or_norm_l2(phiold.data(), n, hx, hy, hz);
         }
         if (tol_per_iter) tol_per_iter[t - 1] = tollerance;
+        if (init_variables) {
+            /* imp:1849-1859: divdb = 0, b = 0, no update this iteration */
+            std::fill(divdb.begin(), divdb.end(), 0.);
+            for (int a = 0; a < space_dim; ++a) std::fill(b[a].begin(), b[a].end(), 0.);
+            ++iters;
+            continue;
+        }
         /* imp:1868-1936: d = grad(phi); b += d; d = shrink(b, g); b -= d; d -= b; div */
         or_grad(phi.data(), X, Y, Z, hx, hy, hz, d[0].data(), d[1].data(), d[2].data());
         for (int a = 0; a < space_dim; ++a) {
@@ -1015,6 +1069,7 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
     std::memcpy(phi_out, phi.data(), sizeof(double) * n);
     if (st) {
         st->iters = iters;
+        st->cc_extractions = cc_extractions;
         st->final_tollerance = tollerance;
         st->t_setup = t_loop0 - t_start;
         st->t_loop = t_end - t_loop0;

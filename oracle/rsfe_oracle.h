@@ -108,10 +108,18 @@ typedef struct or_params {
     int acyclic;            /* cv: 0 cyclic_fftw_convolution, 1 acyclic_fftw_convolution */
     int neumann;            /* onestep_poisson: 0 DirGaussSeidel (the ctor's), 1 NeuGaussSeidel
                                (set_bc(NEU), rsfe_splitbregman_imp.hxx:1294-1322) */
+    /* automatic connected-component extraction (imp:447-491, called at 1808-1812):
+       every auto_cc_freq iterations when auto_cc, phi := its component through the voxel at
+       the physical point cc_pixel, rescaled to [a0,b0]; cc_init_variables then zeroes b
+       and divdb in place of that iteration's Split-Bregman update (1849-1859) */
+    int auto_cc, auto_cc_freq;
+    double cc_pixel[3];
+    int cc_binary_output, cc_init_variables;
 } or_params;
 
 typedef struct or_stats {
     int iters;              /* outer iterations run */
+    int cc_extractions;     /* iterations whose phi the automatic extraction replaced */
     double final_tollerance;
     double t_setup, t_loop, t_fft;   /* seconds */
 } or_stats;

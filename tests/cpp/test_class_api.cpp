@@ -12,6 +12,9 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
+#include "../../include/tk/interface.hxx"
 #include "../../include/tk/rsfe_splitbregman.hxx"
 #include "../../oracle/rsfe_oracle.h"
 
@@ -233,6 +236,106 @@ static void run_cases()
         report("setter validation", std::fabs(v.params().sigma - 3.) +
                                         std::fabs(v.params().a0 + 3.), 0);
     }
+
+    // ---- im3d::interface: MetaImage write / read ---------------------------------------
+    char dir_t[] = "/tmp/tk_cpp_io_XXXXXX";
+    const char* dir = mkdtemp(dir_t);
+    if (!dir) {
+        report("mkdtemp", 1, 0);
+        return;
+    }
+    const std::string base = std::string(dir) + "/img";
+    {
+        im3d::image3d<double> img(X, Y, Z, 0.5, 1.25, 2.);
+        img.upload(raw);
+        im3d::interface<double> out(img);
+        out.write(base);
+        im3d::interface<double> in(base + ".mhd");
+        im3d::image3d<double> back;
+        in.convert2image3d(back);
+        report("interface write/read (.mhd) round trip", maxabs(back.download(), raw), 0);
+        report("interface spacing and dims", (back.gethx() == 0.5 && back.gethy() == 1.25 &&
+                                              back.gethz() == 2. && back.getdimx() == X &&
+                                              back.getdimy() == Y && back.getdimz() == Z) ? 0 : 1,
+               0);
+        // the .raw holds doubles with i fastest (VTK's order)
+        std::vector<double> file(n);
+        FILE* f = std::fopen((base + ".raw").c_str(), "rb");
+        size_t got = f ? std::fread(file.data(), sizeof(double), n, f) : 0;
+        if (f) std::fclose(f);
+        double bad = got == (size_t)n ? 0 : 1;
+        for (unsigned k = 0; k < Z && got == (size_t)n; ++k)
+            for (unsigned j = 0; j < Y; ++j)
+                for (unsigned i = 0; i < X; ++i)
+                    bad += file[i + (long)X * (j + (long)Y * k)] != raw[((long)i * Y + j) * Z + k];
+        report("interface .raw voxel order (x fastest)", bad, 0);
+    }
+
+    // ---- rsfe_splitbregman: automatic connected component and dumps ---------------------
+    {
+        // two bright balls; the initial contour covers both, the seed lies in the larger
+        const int SX = 20, SY = 18, SZ = 16;
+        const long m = (long)SX * SY * SZ;
+        std::vector<double> u0(m), init(m);
+        for (int i = 0; i < SX; ++i)
+            for (int j = 0; j < SY; ++j)
+                for (int k = 0; k < SZ; ++k) {
+                    const double r1 = (i - 6) * (i - 6) + (j - 6) * (j - 6) + (k - 8) * (k - 8);
+                    const double r2 = (i - 14) * (i - 14) + (j - 12) * (j - 12) + (k - 8) * (k - 8);
+                    const long v = ((long)i * SY + j) * SZ + k;
+                    u0[v] = 255. * (0.2 + 0.6 * (r1 <= 16) + 0.5 * (r2 <= 9)) + 10. * (U(gen) - 0.5);
+                    init[v] = (r1 <= 25 || r2 <= 16) ? 1. : 0.;
+                }
+        im3d::image3d<double> myim(SX, SY, SZ), phi0(SX, SY, SZ);
+        myim.upload(u0);
+        phi0.upload(init);
+        segm::rsfe_splitbregman<double> cc;
+        cc.set_verbosity(false);
+        cc.set_cv(conv::filtering<double>(conv::cyclic_fftw_convolution<double>));
+        cc.set_sigma(1.5);
+        cc.set_ls_steps(3);
+        cc.set_maxiter(4);
+        cc.set_tol(0.);
+        cc.set_auto_extract_conn_comp(true);
+        cc.set_auto_extract_conn_comp_freq(2);
+        cc.set_cc_init_pixel_x(6.5);
+        cc.set_cc_init_pixel_y(6.5);
+        cc.set_cc_init_pixel_z(8.5);
+        cc.set_dumpfrequency(3);
+        cc.set_outputname(std::string(dir) + "/res");
+        cc.apply(myim, phi0);
+        or_params p;
+        or_default_params(&p);
+        p.sigma = 1.5;
+        p.ls_steps = 3;
+        p.maxiter = 4;
+        p.tol = 0.;
+        p.auto_cc = 1;
+        p.auto_cc_freq = 2;
+        p.cc_pixel[0] = 6.5;
+        p.cc_pixel[1] = 6.5;
+        p.cc_pixel[2] = 8.5;
+        std::vector<double> ref(m);
+        or_stats st;
+        or_rsfe_apply(u0.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p, ref.data(), nullptr,
+                      nullptr, &st);
+        std::vector<double> got = cc.getoutput().download();
+        report("rsfe_splitbregman auto connected component FP64", maxabs(got, ref), 1e-9);
+        report("auto connected component: extractions", st.cc_extractions == 2 ? 0 : 1, 0);
+        // dumps: res_3 (3 % 3 == 0, not maxiter) and res_4_final, no res_4
+        im3d::interface<double> fin(std::string(dir) + "/res_4_final.mhd");
+        im3d::image3d<double> fin_im;
+        fin.convert2image3d(fin_im);
+        report("dump <outputname>_4_final.mhd equals the result", maxabs(fin_im.download(), got), 0);
+        const bool has3 = access((std::string(dir) + "/res_3.mhd").c_str(), F_OK) == 0 &&
+                          access((std::string(dir) + "/res_3.raw").c_str(), F_OK) == 0;
+        const bool no4 = access((std::string(dir) + "/res_4.mhd").c_str(), F_OK) != 0;
+        report("dumps every dumpfrequency iterations, not at maxiter", (has3 && no4) ? 0 : 1, 0);
+    }
+    for (const char* f : {"img.mhd", "img.raw", "res_3.mhd", "res_3.raw", "res_4_final.mhd",
+                          "res_4_final.raw"})
+        std::remove((std::string(dir) + "/" + f).c_str());
+    rmdir(dir);
 }
 
 extern "C" int tk_cpp_class_api_test(char* log, int cap)

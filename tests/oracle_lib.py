@@ -13,11 +13,14 @@ class OrParams(C.Structure):
         "sigma", "lamda", "lamda1", "lamda2", "beta", "dt", "epsilon", "a0", "b0", "nu", "tol",
         "ls_tol", "radius")] + [(n, C.c_int) for n in (
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "verbosity",
-            "acyclic", "neumann")]
+            "acyclic", "neumann", "auto_cc", "auto_cc_freq")] + [
+                ("cc_pixel", C.c_double * 3), ("cc_binary_output", C.c_int),
+                ("cc_init_variables", C.c_int)]
 
 
 class OrStats(C.Structure):
-    _fields_ = [("iters", C.c_int), ("final_tollerance", C.c_double), ("t_setup", C.c_double),
+    _fields_ = [("iters", C.c_int), ("cc_extractions", C.c_int), ("final_tollerance", C.c_double),
+                ("t_setup", C.c_double),
                 ("t_loop", C.c_double), ("t_fft", C.c_double)]
 
 
@@ -259,6 +262,8 @@ def default_params(**kw):
     for k, v in kw.items():
         if not hasattr(p, k):
             raise AttributeError(k)
+        if k == "cc_pixel":
+            v = (C.c_double * 3)(*v)
         setattr(p, k, v)
     return p
 

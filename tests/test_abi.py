@@ -36,9 +36,13 @@ def test_params_struct_matches_header(tk):
     src = open(os.path.join(ROOT, "include", "tk.h")).read()
     body = re.search(r"typedef struct tk_rsfe_params \{(.*?)\} tk_rsfe_params;", src, re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    fields = re.findall(r"\b([a-z_0-9]+)\s*[,;]", body)
+    fields = re.findall(r"\b([a-z_][a-z_0-9]*)\s*(?:\[\d+\])?\s*[,;]", body)
     assert [f for f, _ in tk.RsfeParams._fields_] == fields
     p = tk.default_params()
+    # the connected-component and dump defaults of the same constructor
+    assert (p.auto_extract_conn_comp, p.auto_extract_conn_comp_freq, list(p.cc_init_pixel),
+            p.cc_binary_output, p.cc_init_variables) == (0, 0, [-1, -1, -1], 1, 1)
+    assert (p.dumpfrequency, p.outputname, p.cc_threshold) == (0, b"result", 0.5)
     # rsfe_splitbregman_imp.hxx:25-41
     assert (p.sigma, p.lamda, p.lamda1, p.lamda2, p.beta, p.dt) == (3, 5e-2, 1e-5, 1e-5, 100, 1)
     assert (p.ls_steps, p.epsilon, p.a0, p.b0, p.nu, p.tol, p.ls_tol) == (5, 1e-3, 0, 1, 1, 1e-5, 0)

@@ -648,6 +648,97 @@ def test_apply_acyclic_matches_composition(orc, tk):
     assert not np.array_equal(orc.rsfe_apply(u0, phi0, pc)[0], phi_or)
 
 
+def _compose_auto_cc(orc, u0, phi0, p, maxiter, freq, pixel, binary, init_vars):
+    """apply() with the automatic connected-component extraction, composed from the pinned
+    pieces in the order of rsfe_splitbregman_imp.hxx:1798-1936 and 447-491"""
+    shape = u0.shape
+    phi = orc.change_range(phi0, 1.0, 0.0)
+    divdb = np.zeros(shape)
+    b = [np.zeros(shape) for _ in range(3)]
+    g = orc.edge_detect(u0, p.beta, p.lamda, p.nu)
+    n_cc = 0
+    for t in range(1, maxiter + 1):
+        rhs = orc.rsfe_forcing(u0, phi, divdb, p)
+        phi, _, _ = orc.poisson_loop(phi, rhs, p.ls_steps, 0.0)
+        phi = np.clip(phi, 0.0, 1.0)
+        reset = False
+        if t % freq == 0:
+            seed = tuple(int(np.floor(x)) for x in pixel)
+            if all(0 <= seed[a] < shape[a] for a in range(3)):
+                if orc.black_and_white(phi, 0.5)[seed] != 0:
+                    cc = orc.connected_component(phi, seed, 0.5, False)[0]
+                    if not binary:
+                        cc = (cc + orc.black_and_white(phi, 0.5, True)) * (phi - phi.min()) + phi.min()
+                    phi = orc.change_range(cc, 1.0, 0.0)
+                    n_cc += 1
+                    reset = init_vars
+        if reset:
+            divdb = np.zeros(shape)
+            b = [np.zeros(shape) for _ in range(3)]
+            continue
+        d = orc.grad(phi)
+        for a in range(3):
+            b[a] = b[a] + d[a]
+        d = [orc.shrink(b[a], g) for a in range(3)]
+        for a in range(3):
+            b[a] = b[a] - d[a]
+            d[a] = d[a] - b[a]
+        divdb = orc.div(*d)
+    return phi, n_cc
+
+
+@pytest.mark.parametrize("binary,init_vars", [(1, 1), (0, 1), (1, 0)])
+def test_apply_auto_cc_matches_composition(orc, tk, binary, init_vars):
+    """the in-loop automatic extraction (auto_extract_conn_comp, every 2 iterations) against
+    the composition; the seed lies in the largest of several 6-connected components of the
+    mask (scipy.ndimage.label), so the extraction removes the others: after it (binary
+    output) phi takes the two values a0, b0 and its mask is that one component"""
+    from scipy import ndimage
+    from cases import two_balls
+    shape = (20, 18, 16)
+    u0, phi0, seed = two_balls(shape)
+    base = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0)
+    phi2 = orc.rsfe_apply(u0, phi0, base)[0]
+    lab, nlab = ndimage.label(phi2 >= 0.5)
+    assert nlab >= 2
+    big = lab[seed]
+    assert big > 0
+    pixel = tuple(x + 0.5 for x in seed)
+    p = orc.default_params(sigma=1.5, maxiter=4, tol=0.0, ls_steps=3, ls_tol=0.0, auto_cc=1,
+                           auto_cc_freq=2, cc_pixel=pixel, cc_binary_output=binary,
+                           cc_init_variables=init_vars)
+    phi_or, _, _, st = orc.rsfe_apply(u0, phi0, p)
+    ref, n_cc = _compose_auto_cc(orc, u0, phi0, p, 4, 2, pixel, binary, init_vars)
+    assert st.cc_extractions == n_cc == 2
+    assert np.array_equal(phi_or, ref)
+    if binary:
+        p2 = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0, auto_cc=1,
+                                auto_cc_freq=2, cc_pixel=pixel)
+        phi_cc = orc.rsfe_apply(u0, phi0, p2)[0]
+        assert set(np.unique(phi_cc)) <= {0.0, 1.0}
+        lab_cc, n_cc_lab = ndimage.label(phi_cc >= 0.5)
+        assert n_cc_lab == 1 and phi_cc[seed] == 1.0
+        # the component kept is the labelled one (up to the reference's boundary rule:
+        # propagation only from interior voxels)
+        inner = (slice(1, -1),) * 3
+        assert np.array_equal((phi_cc >= 0.5)[inner], (lab == big)[inner])
+
+
+def test_apply_auto_cc_seed_outside_or_black_changes_nothing(orc, tk):
+    """a seed outside the volume (the reference's WARNING) or on a black voxel leaves the
+    run as it is without the extraction"""
+    shape = (16, 14, 12)
+    u0, phi0 = tk.gen_standard(shape, 7, 1, 2)
+    base = orc.default_params(sigma=1.5, maxiter=3, tol=0.0, ls_steps=3, ls_tol=0.0)
+    ref = orc.rsfe_apply(u0, phi0, base)[0]
+    for pixel in ((-1.0, 3.0, 3.0), (3.0, 99.0, 3.0), (0.2, 0.2, 0.2)):
+        p = orc.default_params(sigma=1.5, maxiter=3, tol=0.0, ls_steps=3, ls_tol=0.0, auto_cc=1,
+                               auto_cc_freq=1, cc_pixel=pixel)
+        phi, _, _, st = orc.rsfe_apply(u0, phi0, p)
+        assert st.cc_extractions == 0
+        assert np.array_equal(phi, ref)
+
+
 def test_init_phi_cases(orc):
     """initialize_phi_with_init / _with_cube (rsfe_splitbregman_imp.hxx:48-170), pinned by
     closed forms: same dims = min-max rescale; one voxel short = edge padding (numpy) then
