@@ -55,7 +55,8 @@ class RsfeParams(C.Structure):
                                             ("cc_binary_output", C.c_int),
                                             ("cc_init_variables", C.c_int),
                                             ("dumpfrequency", C.c_int),
-                                            ("outputname", C.c_char * 256)]
+                                            ("outputname", C.c_char * 256),
+                                            ("edge_detector_sigma", C.c_double)]
 
 
 class IterStats(C.Structure):

@@ -615,6 +615,7 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     p->dumpfrequency = 0;
     std::memset(p->outputname, 0, sizeof p->outputname);
     std::strcpy(p->outputname, "result");
+    p->edge_detector_sigma = 0;
     return TK_OK;
 }
 
@@ -635,6 +636,8 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
              "tk_rsfe_create: auto_extract_conn_comp_freq and dumpfrequency must be >= 0");
     TK_CHECK(p->cc_threshold >= 0 && p->cc_threshold <= 1, TK_ERR_ARG,
              "tk_rsfe_create: cc_threshold must be in [0, 1]");
+    TK_CHECK(p->edge_detector_sigma >= 0, TK_ERR_ARG,
+             "tk_rsfe_create: edge_detector_sigma must be >= 0");
     TK_CHECK(std::memchr(p->outputname, 0, sizeof p->outputname) != nullptr, TK_ERR_ARG,
              "tk_rsfe_create: outputname is not terminated");
     // the Neumann sweep is the reference's lexicographic one; the learned start and the
@@ -691,10 +694,28 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
             r = conv_passes_pair_ones<float>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
                                              p->acyclic != 0);
     }
-    // gforshrink = edge_detect(myim) (imp:1655)
+    // gforshrink = edge_detect(myim) (imp:1655); with edge_detector_sigma != 0 the edge
+    // detector reads cv(myim) with a Gaussian of that sigma and radius 3 (imp:214-227)
+    tk_vol edge_src = s->u0, blurred = nullptr;
+    tk_filter fe = nullptr;
+    if (r == TK_OK && p->edge_detector_sigma != 0) {
+        r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2],
+                               p->edge_detector_sigma, 3., p->gaussian_pixel_approach, dt, &fe);
+        if (r == TK_OK)
+            r = tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], dt, &blurred);
+        if (r == TK_OK)
+            r = p->acyclic ? tk_conv_acyclic(ctx, fe, blurred, s->u0)
+                           : tk_conv_cyclic(ctx, fe, blurred, s->u0);
+        edge_src = blurred;
+    }
     if (r == TK_OK)
-        r = dt == TK_F64 ? launch_edge<double>(ctx, s->u0, p->beta, p->lamda, p->nu, s->g)
-                         : launch_edge<float>(ctx, s->u0, p->beta, p->lamda, p->nu, s->g);
+        r = dt == TK_F64 ? launch_edge<double>(ctx, edge_src, p->beta, p->lamda, p->nu, s->g)
+                         : launch_edge<float>(ctx, edge_src, p->beta, p->lamda, p->nu, s->g);
+    if (fe || blurred) {
+        if (r == TK_OK) r = tk_ctx_sync(ctx);
+        if (blurred) tk_vol_free(blurred);
+        if (fe) tk_filter_free(fe);
+    }
     if (r == TK_OK) r = tk_ctx_sync(ctx);
     if (r != TK_OK) {
         free_state(s);

@@ -285,6 +285,9 @@ typedef struct tk_rsfe_params {
        <outputname>_<t>_final.mhd/.raw (tk_vol_write_mhd).  0: no dumps */
     int dumpfrequency;
     char outputname[256];
+    double edge_detector_sigma; /* > 0: the edge detector (GRAD) reads the image convolved by
+                                   the run's cv with a Gaussian of this sigma, radius 3
+                                   (imp:214-227); 0: the image itself (default) */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "convolution.hxx"
+#include "getpot.hxx"
 #include "poisson.hxx"
 
 namespace segm {
@@ -141,12 +142,67 @@ public:
     void set_edge_detector(ed_type const ed)
     {
         if (ed == GRAD) return;
-        tk::warning("set_edge_detector: only the GRAD edge detector is built");
+        if (ed == GRAD_THRESHOLD || ed == LBP)
+            tk::warning("set_edge_detector: the interactive GRAD_THRESHOLD and LBP detectors are "
+                        "not built; GRAD is kept");
+        else tk::warning("set_edge_detector: specify a proper ed_type condition");
     }
+    // edge detector on the image blurred with this sigma (imp:1135-1153, 214-227)
     void set_edge_detector_sigma(T const& eds)
     {
-        if (eds == 0) return;
-        tk::warning("set_edge_detector_sigma: only edge_detector_sigma = 0 is built");
+        if (eds >= 0) p_.edge_detector_sigma = (double)eds;
+        else tk::warning("set_edge_detector_sigma: edge_detector_sigma must be greater than zero");
+    }
+    // parameter file (imp:807-813) read by apply (imp:1559-1571), GetPot section
+    // splitbregman/init/; showfrequency and onthego are kept but nothing is displayed
+    void set_getpotfile(std::string const& name) { getpotfile_ = name; }
+    void set_onthego(bool const onthego) { onthego_ = onthego; }
+    void set_showfrequency(unsigned const& sf) { showfrequency_ = sf; }
+
+    // set_param_from_getpot (imp:582-796): every name present in section
+    // splitbregman/<section> of the parameter file goes through its setter, with the
+    // reference's "absent" sentinels
+    void set_param_from_getpot(std::string const& section)
+    {
+        tk::getpot f(getpotfile_.c_str(), "#", "\n");
+        f.set_prefix(("splitbregman/" + section).c_str());
+        int it;
+        double t;
+        std::string st;
+        if ((it = f("verbosity", -1)) != -1) set_verbosity((bool)it);
+        if ((it = f("maxiter", -1)) > 0) set_maxiter((unsigned)it);
+        if ((it = f("showfrequency", -1)) >= 0) set_showfrequency((unsigned)it);
+        if ((it = f("dumpfrequency", -1)) >= 0) set_dumpfrequency((unsigned)it);
+        if ((st = f("outputname", "")).size() != 0) set_outputname(st);
+        if (f("save_current", -1) != -1 || f("end_now", -1) != -1)
+            tk::warning("set_param_from_getpot: save_current and end_now are on-the-go "
+                        "controls; not built");
+        if ((t = f("sigma", -1.)) != -1.) set_sigma((T)t);
+        if ((t = f("lamda", -1.)) != -1.) set_lamda((T)t);
+        if ((t = f("lamda1", -1.)) != -1.) set_lamda1((T)t);
+        if ((t = f("lamda2", -1.)) != -1.) set_lamda2((T)t);
+        if ((t = f("nu", -1.)) != -1.) set_nu((T)t);
+        if ((t = f("beta", -1.)) != -1.) set_beta((T)t);
+        if ((t = f("dt", -1.)) != -1.) set_dt((T)t);
+        if ((it = f("ls_steps", -1)) != -1) set_ls_steps((unsigned)it);
+        if ((t = f("epsilon", -1.)) != -1.) set_epsilon((T)t);
+        if ((t = f("a0", 2.1e-6)) != 2.1e-6) set_a0((T)t);
+        if ((t = f("b0", -2.1e-6)) != -2.1e-6) set_b0((T)t);
+        if ((it = f("auto_extract_conn_comp", -1)) != -1) set_auto_extract_conn_comp(it != 0);
+        if ((it = f("cc_binary_output", -1)) != -1) set_cc_binary_output(it != 0);
+        if ((it = f("cc_init_variables", -1)) != -1) set_cc_init_variables(it != 0);
+        if ((t = f("cc_init_pixel_x", -1.)) != -1.) set_cc_init_pixel_x(t);
+        if ((t = f("cc_init_pixel_y", -1.)) != -1.) set_cc_init_pixel_y(t);
+        if ((t = f("cc_init_pixel_z", -1.)) != -1.) set_cc_init_pixel_z(t);
+        if ((it = f("auto_extract_conn_comp_freq", -1)) >= 0)
+            set_auto_extract_conn_comp_freq((unsigned)it);
+        if ((it = f("bc", -1)) != -1) set_bc(static_cast<bc_type>(it));
+        if ((t = f("ls_tol", -1.)) != -1.) set_ls_tol((T)t);
+        if ((it = f("edge_detector", -1)) != -1) set_edge_detector(static_cast<ed_type>(it));
+        if ((t = f("edge_detector_sigma", -1.)) != -1.) set_edge_detector_sigma((T)t);
+        if ((it = f("gaussian_pixel_approach", -1)) != -1) set_gaussian_pixel_approach(it != 0);
+        if ((t = f("tol", -1.)) != -1.) set_tol((T)t);
+        if ((t = f("alpha", 1.e-9)) != 1.e-9) set_alpha((T)t);
     }
     // bc flag and sweep (imp:1294-1322): NEU -> NeuGaussSeidel with derivative 0, DIR ->
     // DirGaussSeidel with boundary value a0 (imp:1735-1743)
@@ -209,6 +265,9 @@ private:
 
     void run(im3d::image3d<T> const& myim, tk_vol init)
     {
+        // the parameter file's init section first (imp:1559-1571)
+        if (getpotfile_ != "") set_param_from_getpot("init/");
+        else onthego_ = false;
         tk_rsfe_params p = p_;
         p.gs_order = onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ? TK_GS_REDBLACK
                                                                            : TK_GS_LEX;
@@ -242,6 +301,9 @@ private:
 
     tk_rsfe_params p_;
     bool verbosity_ = true;
+    std::string getpotfile_ = "data";  // the constructor's default (imp:27)
+    bool onthego_ = true;
+    unsigned showfrequency_ = 0;
     conv::filtering<T> cv_;
     lapl::unsteady_poisson_functor<T> onestep_;
     std::vector<int> sweeps_;

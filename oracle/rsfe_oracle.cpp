@@ -784,7 +784,7 @@ void or_default_params(or_params* p)
     p->nu = 1.; p->tol = 1.e-5; p->ls_tol = 0; p->maxiter = 100; p->bc_dir = 0;
     p->gaussian_pixel_approach = 0; p->radius = 3; p->verbosity = 0; p->acyclic = 0;
     p->neumann = 0;
-    p->auto_cc = 0; p->auto_cc_freq = 0;
+    p->auto_cc = 0; p->auto_cc_freq = 0; p->edge_detector_sigma = 0;
     p->cc_pixel[0] = p->cc_pixel[1] = p->cc_pixel[2] = -1;
     p->cc_binary_output = 1; p->cc_init_variables = 1;
 }
@@ -961,8 +961,9 @@ static int auto_extract_cc(std::vector<double>& phi, int X, int Y, int Z, double
 
 /* ---- R2: apply(myim, init), rsfe_splitbregman_imp.hxx:1547-1995 ------------------
    cv = cyclic_fftw_convolution or acyclic_fftw_convolution (p->acyclic), onestep_poisson =
-   DirGaussSeidel or NeuGaussSeidel (p->neumann), edge detector GRAD with
-   edge_detector_sigma == 0, no display / dump / on-the-go / auto CC. */
+   DirGaussSeidel or NeuGaussSeidel (p->neumann), edge detector GRAD (on the image blurred
+   with edge_detector_sigma when that is not 0), the automatic connected component
+   (p->auto_cc); no display / dump / on-the-go. */
 int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, double hx,
                   double hy, double hz, const or_params* p, double* phi_out,
                   int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
@@ -1004,7 +1005,30 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
         V u0(myim, myim + n);
         cv(kmyim, u0);
     }
-    or_edge_detect(myim, X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu, gforshrink.data());
+    /* edge_detect (imp:196-229, called at 1655): with edge_detector_sigma != 0 the image is
+       first convolved by cv with a Gaussian of that sigma (radius 3); the main filter is
+       rebuilt afterwards, so kones / kmyim above are unaffected.  (imp:1877-1882 recomputes g
+       from the same inputs when sigma_ed is not 0 or 1; the result is the same g.) */
+    if (p->edge_detector_sigma != 0) {
+        int nhe[3], nce[3];
+        or_gaussian_dims(Z, hx, hy, hz, p->edge_detector_sigma, 3., p->gaussian_pixel_approach,
+                         nhe);
+        V tfe((long)nhe[0] * nhe[1] * nhe[2]);
+        or_gaussian_time_filter(Z, hx, hy, hz, p->edge_detector_sigma, 3.,
+                                p->gaussian_pixel_approach, tfe.data());
+        if (p->acyclic) or_acyclic_dims(X, Y, Z, nhe, nce);
+        else or_cyclic_dims(X, Y, Z, nhe, nce);
+        V spece((long)nce[0] * nce[1] * nce[2]), blurred(n);
+        or_cyclic_spectrum(tfe.data(), nhe, nce, spece.data());
+        if (p->acyclic)
+            or_acyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
+        else
+            or_cyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
+        or_edge_detect(blurred.data(), X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu,
+                       gforshrink.data());
+    } else {
+        or_edge_detect(myim, X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu, gforshrink.data());
+    }
 
     double chosen_bc = (p->bc_dir == 0) ? 0 : p->a0;
     double t_loop0 = now_s();

@@ -115,6 +115,8 @@ typedef struct or_params {
     int auto_cc, auto_cc_freq;
     double cc_pixel[3];
     int cc_binary_output, cc_init_variables;
+    double edge_detector_sigma; /* != 0: g from the image blurred by cv with this sigma
+                                   (rsfe_splitbregman_imp.hxx:214-227) */
 } or_params;
 
 typedef struct or_stats {

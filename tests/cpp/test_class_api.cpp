@@ -332,6 +332,48 @@ static void run_cases()
         const bool no4 = access((std::string(dir) + "/res_4.mhd").c_str(), F_OK) != 0;
         report("dumps every dumpfrequency iterations, not at maxiter", (has3 && no4) ? 0 : 1, 0);
     }
+    // ---- rsfe_splitbregman: parameters from a GetPot file, blurred edge detector ---------
+    {
+        const std::string pot = std::string(dir) + "/params.pot";
+        FILE* f = std::fopen(pot.c_str(), "w");
+        if (f) {
+            std::fputs("# parameters of the run\n[splitbregman]\n  [./init]\n"
+                       "    verbosity = 0\n    maxiter = 3   # three outer iterations\n"
+                       "    sigma = 1.5\n    ls_steps=3\n    tol = 0\n"
+                       "    edge_detector_sigma = 1.2\n    beta = 50\n  [../]\n"
+                       "  [./onthego]\n    maxiter = 99\n  [../]\n[../]\n", f);
+            std::fclose(f);
+        }
+        const int SX = 18, SY = 16, SZ = 20;
+        const long m = (long)SX * SY * SZ;
+        std::vector<double> u0(m), init(m);
+        tk_gen_standard(SX, SY, SZ, 9, 1, 2, u0.data(), init.data());
+        im3d::image3d<double> myim(SX, SY, SZ), phi0(SX, SY, SZ);
+        myim.upload(u0);
+        phi0.upload(init);
+        myim.change_range_of_intensity(255.);
+        segm::rsfe_splitbregman<double> gp;
+        gp.set_cv(conv::filtering<double>(conv::cyclic_fftw_convolution<double>));
+        gp.set_getpotfile(pot);
+        gp.apply(myim, phi0);
+        or_params p;
+        or_default_params(&p);
+        p.maxiter = 3;
+        p.sigma = 1.5;
+        p.ls_steps = 3;
+        p.tol = 0.;
+        p.edge_detector_sigma = 1.2;
+        p.beta = 50;
+        std::vector<double> u255(u0), ref(m);
+        or_change_range_of_intensity(u255.data(), m, 255., 0.);
+        or_rsfe_apply(u255.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p, ref.data(), nullptr,
+                      nullptr, nullptr);
+        report("rsfe_splitbregman from a GetPot file, blurred edge detector FP64",
+               maxabs(gp.getoutput().download(), ref), 1e-9);
+        report("GetPot: the file's maxiter (init section, not onthego)",
+               gp.get_iterations() == 3 ? 0 : 1, 0);
+        std::remove(pot.c_str());
+    }
     for (const char* f : {"img.mhd", "img.raw", "res_3.mhd", "res_3.raw", "res_4_final.mhd",
                           "res_4_final.raw"})
         std::remove((std::string(dir) + "/" + f).c_str());

@@ -15,7 +15,7 @@ class OrParams(C.Structure):
             "maxiter", "ls_steps", "bc_dir", "gaussian_pixel_approach", "verbosity",
             "acyclic", "neumann", "auto_cc", "auto_cc_freq")] + [
                 ("cc_pixel", C.c_double * 3), ("cc_binary_output", C.c_int),
-                ("cc_init_variables", C.c_int)]
+                ("cc_init_variables", C.c_int), ("edge_detector_sigma", C.c_double)]
 
 
 class OrStats(C.Structure):

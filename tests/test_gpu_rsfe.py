@@ -101,6 +101,30 @@ def test_loop_neumann_f64_matches_oracle(tk, orc, ctx, shape, ls_tol):
     assert np.max(np.abs(orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0, pd)[0] - ref)) > 1e-6
 
 
+@pytest.mark.parametrize("acyclic", [0, 1])
+def test_loop_blurred_edge_detector_f64_vs_oracle(tk, orc, ctx, acyclic):
+    """edge_detector_sigma != 0 (imp:214-227): g from the image convolved by cv with a
+    Gaussian of that sigma; FP64 against the oracle, FP32 mask Dice"""
+    shape = (24, 20, 28)
+    u0, phi0 = tk.gen_standard(shape, 9, 1, 2)
+    kw = dict(sigma=2.0, maxiter=5, tol=0.0, ls_steps=5, ls_tol=0.0, acyclic=acyclic,
+              edge_detector_sigma=1.5)
+    ref, sw, tl, _ = orc.rsfe_apply(orc.change_range(u0, 255.0, 0.0), phi0,
+                                    orc.default_params(**kw))
+    phi, st = run_gpu(tk, ctx, u0, phi0, np.float64, 5, gs_order=tk.TK_GS_LEX, **kw)
+    err = np.max(np.abs(phi - ref))
+    say(f"blurred edge detector acyclic {acyclic}: max|phi-ref| = {err:.3e}")
+    assert err < 1e-9
+    assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
+    plain, _ = run_gpu(tk, ctx, u0, phi0, np.float64, 5, gs_order=tk.TK_GS_LEX,
+                       **dict(kw, edge_detector_sigma=0.0))
+    assert np.max(np.abs(plain - ref)) > 1e-6
+    p32, _ = run_gpu(tk, ctx, u0, phi0, np.float32, 5, gs_order=tk.TK_GS_LEX, **kw)
+    d = dice(p32 >= 0.5, ref >= 0.5)
+    say(f"  FP32 Dice {d:.6f}")
+    assert d > 0.999
+
+
 def test_loop_neumann_refusals(tk, ctx):
     u0, phi0 = tk.gen_standard((16, 16, 16), 5, 1, 2)
     vu, vi = ctx.from_numpy(u0), ctx.from_numpy(phi0)
