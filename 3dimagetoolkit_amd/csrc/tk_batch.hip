@@ -17,6 +17,14 @@
 //    again.  The graph holds kernel nodes only (no memset nodes).
 // Otherwise (ls_tol > 0 gates the Poisson loop on the host, tol > 0 stops a run early) each
 // volume runs the reference's loop with its host checks, lane after lane.
+//
+// Red-black sweeps with a fixed count run staged, one launch per stage over all volumes,
+// on lane 0: each convolution pass of every volume (k_conv_axis_batch, one launch per pass
+// and filter variant), each sweep (k_gs_rb_pv_batch), the Split-Bregman update with the
+// tolerance sums (k_sb_update_batch, k_sum_pairs_batch) and div (k_div_batch): about 13
+// launches per outer iteration whatever the number of volumes.  The job tables are built
+// once, for both roles of phi and its red-black partner (an odd sweep count swaps the roles
+// each iteration).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +48,14 @@ struct tk_batch_s {
     hipGraphExec_t exec = nullptr;
     int graph_nodes = 0;
     int graph_parity = 0;  // iteration parity at capture (phi and its partner swap roles)
+    // staged path: job tables phi -> partner (fwd) and partner -> phi (bwd) as built, and
+    // whether phi and its partner have swapped roles since
+    void* rb_fwd = nullptr;
+    void* rb_bwd = nullptr;
+    int rb_blocks = 0;
+    bool rb_swapped = false;
+    RsfeBatchConv* conv = nullptr;
+    RsfeBatchUpdate* upd = nullptr;
 };
 
 namespace {
@@ -50,6 +66,35 @@ void drop_graph(tk_batch b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+void drop_tables(tk_batch b)
+{
+    if (b->rb_fwd) (void)hipFree(b->rb_fwd);
+    if (b->rb_bwd) (void)hipFree(b->rb_bwd);
+    b->rb_fwd = b->rb_bwd = nullptr;
+    b->rb_blocks = 0;
+    b->rb_swapped = false;
+    if (b->conv) rsfe_batch_conv_free(b->conv);
+    if (b->upd) rsfe_batch_update_free(b->upd);
+    b->conv = nullptr;
+    b->upd = nullptr;
+}
+
+// the staged path applies: red-black, a fixed sweep count, no host gate, 16-byte rows
+bool staged(tk_batch b)
+{
+    const tk_rsfe_params& p = b->p;
+    if (p.gs_order != TK_GS_REDBLACK || p.ls_tol > 0 || p.tol > 0 || p.neumann ||
+        p.warm_start > 0 || p.ls_res_tol > 0)
+        return false;
+    // the faces keep the boundary value from the first sweep on (cut_phi leaves it)
+    const double bc = p.bc_dir ? p.a0 : 0.;
+    if (!(bc >= p.a0 && bc <= p.b0)) return false;
+    const int V = b->dtype == TK_F64 ? 2 : 4;
+    for (size_t v = 0; v < b->runs.size(); ++v)
+        if (b->dims[3 * v + 2] % V != 0) return false;
+    return true;
 }
 
 // volumes in an order that alternates lanes, so consecutive launches go to different streams
@@ -70,8 +115,79 @@ std::vector<int> lane_order(tk_batch b)
     return order;
 }
 
+int join_into_lane0(tk_batch b)
+{
+    for (size_t k = 1; k < b->lanes.size(); ++k) {
+        TK_HIP(hipEventRecord(b->join[k], b->lanes[k]->stream));
+        TK_HIP(hipStreamWaitEvent(b->lanes[0]->stream, b->join[k], 0));
+    }
+    return TK_OK;
+}
+
+// the staged path's job tables, built for the buffers' present roles: allocation and a
+// host copy, so never inside a capture (tk_batch_iterate builds them first).  Each run's
+// faces get the boundary value here if they have not yet.
+int ensure_tables(tk_batch b)
+{
+    if (b->rb_fwd) return TK_OK;
+    std::vector<RbBatchVol> vols(b->runs.size());
+    for (size_t v = 0; v < b->runs.size(); ++v) {
+        RbBatchVol& q = vols[v];
+        void *phi, *alt;
+        const void* rhs;
+        TK_TRY(rsfe_rb_prepare(b->runs[v], &phi, &alt, &rhs));
+        q.src = phi;
+        q.dst = alt;
+        q.rhs = rhs;
+        q.X = b->dims[3 * v];
+        q.Y = b->dims[3 * v + 1];
+        q.Z = b->dims[3 * v + 2];
+    }
+    std::vector<RbBatchVol> back(vols);
+    for (RbBatchVol& q : back) {
+        void* from = const_cast<void*>(q.src);
+        q.src = q.dst;
+        q.dst = from;
+    }
+    for (tk_ctx c : b->lanes) TK_HIP(hipStreamSynchronize(c->stream));
+    int n1 = 0, n2 = 0;
+    TK_TRY(rb_batch_table(b->lanes[0], b->dtype, vols.data(), (int)vols.size(), &b->rb_fwd, &n1));
+    TK_TRY(rb_batch_table(b->lanes[0], b->dtype, back.data(), (int)back.size(), &b->rb_bwd, &n2));
+    b->rb_blocks = n1;
+    b->rb_swapped = false;
+    TK_TRY(rsfe_batch_conv_build(b->runs.data(), (int)b->runs.size(), &b->conv));
+    TK_TRY(rsfe_batch_update_build(b->runs.data(), (int)b->runs.size(), &b->upd));
+    return TK_OK;
+}
+
+// one outer iteration of every volume, staged (see the file comment); the tables exist
+int staged_step(tk_batch b)
+{
+    const tk_rsfe_params& p = b->p;
+    const int S = p.ls_steps < 1 ? 1 : p.ls_steps;
+    // the previous iteration's updates, on every lane, come first
+    TK_TRY(join_into_lane0(b));
+    TK_TRY(rsfe_batch_conv_run(b->lanes[0], b->conv, b->rb_swapped ? 1 : 0));
+    for (int q = 0; q < S; ++q) {
+        const bool fwd = ((q & 1) == 0) != b->rb_swapped;
+        TK_TRY(rb_batch_sweep(b->lanes[0], b->dtype, fwd ? b->rb_fwd : b->rb_bwd,
+                              (int)b->runs.size(), b->rb_blocks, p.dt, 1., 1., 1.));
+    }
+    if (S & 1) {
+        for (tk_rsfe run : b->runs) rsfe_rb_exchange(run);
+        b->rb_swapped = !b->rb_swapped;
+    }
+    return rsfe_batch_update_run(b->lanes[0], b->upd, b->runs.data(), (int)b->runs.size(),
+                                 b->rb_swapped ? 1 : 0);
+}
+
 int step_all(tk_batch b, const std::vector<int>& order)
 {
+    if (staged(b)) {
+        for (tk_rsfe run : b->runs)
+            if (rsfe_iterations(run) >= rsfe_params(run).maxiter) return TK_OK;
+        return staged_step(b);
+    }
     for (int v : order) TK_TRY(rsfe_step_async(b->runs[v]));
     return TK_OK;
 }
@@ -186,6 +302,7 @@ int tk_batch_set_volume(tk_batch b, int v, const void* u0, const void* init, tk_
     if (r == TK_OK && init) r = tk_vol_upload(vi, init, hd);
     if (r == TK_OK) {
         drop_graph(b);
+        drop_tables(b);
         if (b->runs[v]) tk_rsfe_destroy(b->runs[v]);
         b->runs[v] = nullptr;
         r = tk_rsfe_create(c, vu, vi, &b->p, &b->runs[v]);
@@ -215,6 +332,7 @@ int tk_batch_iterate(tk_batch b, int n_iters, int use_graph)
                  "tk_batch_iterate: volumes are at different iterations");
     int n = std::min(n_iters, std::max(0, b->p.maxiter - it));
     const std::vector<int> order = lane_order(b);
+    if (staged(b) && n > 0) TK_TRY(ensure_tables(b));
     if (use_graph && n >= 2) {
         // the graph holds the buffers as the capture found them: with an odd sweep count an
         // odd number of iterations since then has swapped them, so capture again
@@ -260,6 +378,7 @@ int tk_batch_destroy(tk_batch b)
     if (!b) return TK_OK;
     for (tk_ctx c : b->lanes) (void)hipStreamSynchronize(c->stream);
     drop_graph(b);
+    drop_tables(b);
     for (tk_rsfe r : b->runs)
         if (r) tk_rsfe_destroy(r);
     if (b->fork) (void)hipEventDestroy(b->fork);

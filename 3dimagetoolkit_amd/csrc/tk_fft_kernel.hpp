@@ -650,13 +650,14 @@ inline __host__ __device__ int conv_nbuf(int axis, int L, int P, int S)
     return axis == 2 ? P * (L + 2 * S + 1) : (L + 2 * S) * conv_sp(axis, P, S);
 }
 
-// per-axis convolution pass: S == 0 FFT (or the generic direct fallback), S > 0 the
-// register-window direct path with half-width S, on the matrix cores when MF
+// per-axis convolution of the pencil group starting at line c0: S == 0 FFT (or the generic
+// direct fallback), S > 0 the register-window direct path with half-width S, on the matrix
+// cores when MF
 template <class T, int EPT, int NT, bool QUAD, int S, bool MF, class Ld, class Md, class St>
-__global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int rounds, Ld ld,
-                                                  Md md, St st)
+__device__ __forceinline__ void conv_group(const AxisDev& ax, const AxisGeom& g, int rounds,
+                                           const Ld& ld, const Md& md, const St& st, long c0,
+                                           unsigned char* smem)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     Lay y;
     y.L = ax.L;
     y.P = g.P;
@@ -679,8 +680,6 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     } else {
         for (int t = threadIdx.x; t < ax.nh; t += NT) spec[t] = ((const T*)ax.taps)[t];
     }
-    // consecutive groups share 128-byte lines along x and y: keep them on one XCD
-    const long c0 = (long)xcd_remap(blockIdx.x, gridDim.x) * g.P;
     // pencils past the end of the volume stay zero
     if ((c0 + g.P) > g.nlines) {
         for (int e = threadIdx.x; e < nbuf; e += NT) buf[e] = {0, 0};
@@ -730,6 +729,51 @@ __global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int ro
     }
 }
 
+template <class T, int EPT, int NT, bool QUAD, int S, bool MF, class Ld, class Md, class St>
+__global__ __launch_bounds__(NT) void k_conv_axis(AxisDev ax, AxisGeom g, int rounds, Ld ld,
+                                                  Md md, St st)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    // consecutive groups share 128-byte lines along x and y: keep them on one XCD
+    conv_group<T, EPT, NT, QUAD, S, MF>(ax, g, rounds, ld, md, st,
+                                        (long)xcd_remap(blockIdx.x, gridDim.x) * g.P, smem);
+}
+
+// one volume of a batched convolution pass (k_conv_axis_batch): its axis, geometry, the
+// stage's load / middle / store operators on its fields, and its block range (first a
+// multiple of kNumXcd, nblk padded to one; groups: the real pencil groups)
+template <class Ld, class Md, class St>
+struct ConvJob {
+    AxisDev ax;
+    AxisGeom g;
+    int rounds;
+    Ld ld;
+    Md md;
+    St st;
+    int first, nblk, groups;
+};
+
+// one convolution pass of many volumes in one launch; each block finds its volume by a
+// binary search of the first blocks and runs that volume's group in the XCD-aware order
+// of the volume's own launch (padding blocks exit at once)
+template <class T, int EPT, int NT, bool QUAD, int S, bool MF, class Ld, class Md, class St>
+__global__ __launch_bounds__(NT) void k_conv_axis_batch(const ConvJob<Ld, Md, St>* __restrict__ jobs,
+                                                        int njobs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const ConvJob<Ld, Md, St> jb = jobs[lo];
+    const unsigned w = xcd_remap(blockIdx.x - (unsigned)jb.first, (unsigned)jb.nblk);
+    if (w >= (unsigned)jb.groups) return;
+    conv_group<T, EPT, NT, QUAD, S, MF>(jb.ax, jb.g, jb.rounds, jb.ld, jb.md, jb.st,
+                                        (long)w * jb.g.P, smem);
+}
+
 // elements per workgroup: 4096 complex FP32 / 2048 complex FP64 (<= ~60 KB of LDS)
 template <class T>
 constexpr int conv_ept()
@@ -771,61 +815,101 @@ inline bool mfma_window_fits(int axis, const int dims[3])
     return L % 16 == 0 && g.P % 8 == 0 && L * g.P <= kMfmaEpt * 256;
 }
 
+// what a convolution pass of one filter axis runs: the axis with its boundary mode, the
+// pencil-group geometry, the variant (register window of half-width S on the vector ALUs
+// or the matrix cores, or the FFT / generic direct pass), the grid and the LDS bytes
+struct ConvPlan {
+    AxisDev ax;
+    AxisGeom g;
+    bool mf = false, quad = false;
+    int S = 0;
+    unsigned nb = 0;
+    size_t lds = 0;
+};
+
+template <class T>
+int conv_plan(const tk_filter_s* f, int axis, bool acyclic, ConvPlan* out)
+{
+    constexpr int NT = 256;
+    constexpr int EPT = conv_ept<T>();
+    ConvPlan p;
+    // acyclic (zero outside the volume): direct passes with zero halos; an axis on the FFT
+    // method runs the register window where one is compiled, else the generic direct pass
+    p.ax = f->ax[axis];
+    p.ax.zero_halo = acyclic ? 1 : 0;
+    const AxisDev& ax = p.ax;
+    const bool dims4 = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0;
+    // the matrix-core window (checked when the method was chosen; checked again here since
+    // the group shape decides the tiling)
+    p.mf = std::is_same<T, float>::value && ax.direct == 2 && dims4 &&
+           direct_width_supported(ax.nh) && mfma_window_fits(axis, f->dims);
+    const int elems = (p.mf ? kMfmaEpt : EPT) * NT;
+    p.g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, elems);
+    TK_CHECK(ax.L * p.g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
+             ax.L, EPT * NT);
+    p.nb = (unsigned)((p.g.nlines + p.g.P - 1) / p.g.P);
+    p.quad = dims4 && p.g.P >= 4;
+    // register-window direct path for the instantiated half-widths (sigma 3, 4, 6 at h=1)
+    p.S = (ax.direct || (acyclic && direct_width_supported(ax.nh))) && p.quad ? ax.nh / 2 : 0;
+    p.lds = (size_t)conv_nbuf(axis, ax.L, p.g.P, p.S) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
+            (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
+    *out = p;
+    return TK_OK;
+}
+
+// the kernel instance of a plan's variant, launched as KERNEL_LAUNCH(kernel<...>, threads)
+#define TK_CONV_DISPATCH(plan, KERNEL_LAUNCH)                                                  \
+    do {                                                                                       \
+        constexpr int NT_ = 256, EPT_ = conv_ept<T>();                                         \
+        const int S_ = (plan).S;                                                               \
+        if constexpr (std::is_same<T, float>::value) {                                         \
+            if ((plan).mf && S_ == 9) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 9, true), NT_);    \
+            else if ((plan).mf && S_ == 12) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 12, true), NT_); \
+            else if ((plan).mf && S_ == 18) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 18, true), NT_); \
+            else if (S_ == 9) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 9, false), 2 * NT_);   \
+            else if (S_ == 12) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 12, false), 2 * NT_); \
+            else if (S_ == 18) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 18, false), 2 * NT_); \
+            else if ((plan).quad) KERNEL_LAUNCH((T, EPT_, NT_, true, 0, false), NT_);          \
+            else KERNEL_LAUNCH((T, EPT_, NT_, false, 0, false), NT_);                          \
+        } else {                                                                               \
+            if (S_ == 9) KERNEL_LAUNCH((T, EPT_, NT_, true, 9, false), NT_);                    \
+            else if (S_ == 12) KERNEL_LAUNCH((T, EPT_, NT_, true, 12, false), NT_);             \
+            else if (S_ == 18) KERNEL_LAUNCH((T, EPT_, NT_, true, 18, false), NT_);             \
+            else if ((plan).quad) KERNEL_LAUNCH((T, EPT_, NT_, true, 0, false), NT_);          \
+            else KERNEL_LAUNCH((T, EPT_, NT_, false, 0, false), NT_);                          \
+        }                                                                                      \
+    } while (0)
+
+#define TK_UNPACK_(...) __VA_ARGS__
+
 template <class T, class Ld, class Md, class St>
 int launch_conv_axis(tk_ctx ctx, const tk_filter_s* f, int axis, int rounds, Ld ld, Md md, St st,
                      bool acyclic = false)
 {
-    constexpr int NT = 256;
-    constexpr int EPT = conv_ept<T>();
-    // acyclic (zero outside the volume): direct passes with zero halos; an axis on the FFT
-    // method runs the register window where one is compiled, else the generic direct pass
-    AxisDev ax = f->ax[axis];
-    ax.zero_halo = acyclic ? 1 : 0;
-    const bool dims4 = f->dims[0] % 4 == 0 && f->dims[1] % 4 == 0 && f->dims[2] % 4 == 0;
-    // the matrix-core window (checked when the method was chosen; checked again here since
-    // the group shape decides the tiling)
-    const bool mf = std::is_same<T, float>::value && ax.direct == 2 && dims4 &&
-                    direct_width_supported(ax.nh) && mfma_window_fits(axis, f->dims);
-    const int elems = (mf ? kMfmaEpt : EPT) * NT;
-    AxisGeom g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, elems);
-    TK_CHECK(ax.L * g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
-             ax.L, EPT * NT);
-    unsigned nb = (unsigned)((g.nlines + g.P - 1) / g.P);
-    const bool quad = dims4 && g.P >= 4;
-    // register-window direct path for the instantiated half-widths (sigma 3, 4, 6 at h=1)
-    const int S = (ax.direct || (acyclic && direct_width_supported(ax.nh))) && quad ? ax.nh / 2 : 0;
-    auto lds_for = [&](int s) {
-        return (size_t)conv_nbuf(axis, ax.L, g.P, s) * sizeof(cplx<T>) + ax.L * sizeof(cplx<T>) +
-               (size_t)(ax.L > ax.nh ? ax.L : ax.nh) * sizeof(T);
-    };
-#define TK_CONV_LAUNCH(Q, SS, MF)                                                              \
-    hipLaunchKernelGGL((k_conv_axis<T, MF ? kMfmaEpt : EPT, NT, Q, SS, MF, Ld, Md, St>),        \
-                       dim3(nb), dim3(NT), lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
-    // the FP32 vector-ALU window: the same 4096-element group over 512 threads
-#define TK_CONV_LAUNCH_W(SS)                                                                   \
-    hipLaunchKernelGGL((k_conv_axis<T, EPT / 2, 2 * NT, true, SS, false, Ld, Md, St>),          \
-                       dim3(nb), dim3(2 * NT), lds_for(SS), ctx->stream, ax, g, rounds, ld, md, st)
-    if constexpr (std::is_same<T, float>::value) {
-        if (mf && S == 9) TK_CONV_LAUNCH(true, 9, true);
-        else if (mf && S == 12) TK_CONV_LAUNCH(true, 12, true);
-        else if (mf && S == 18) TK_CONV_LAUNCH(true, 18, true);
-        else if (S == 9) TK_CONV_LAUNCH_W(9);
-        else if (S == 12) TK_CONV_LAUNCH_W(12);
-        else if (S == 18) TK_CONV_LAUNCH_W(18);
-        else if (quad) TK_CONV_LAUNCH(true, 0, false);
-        else TK_CONV_LAUNCH(false, 0, false);
-    } else {
-        if (S == 9) TK_CONV_LAUNCH(true, 9, false);
-        else if (S == 12) TK_CONV_LAUNCH(true, 12, false);
-        else if (S == 18) TK_CONV_LAUNCH(true, 18, false);
-        else if (quad) TK_CONV_LAUNCH(true, 0, false);
-        else TK_CONV_LAUNCH(false, 0, false);
-    }
-#undef TK_CONV_LAUNCH
-#undef TK_CONV_LAUNCH_W
+    ConvPlan p;
+    TK_TRY(conv_plan<T>(f, axis, acyclic, &p));
+#define TK_CONV_ONE(ARGS, THREADS)                                                             \
+    hipLaunchKernelGGL((k_conv_axis<TK_UNPACK_ ARGS, Ld, Md, St>), dim3(p.nb), dim3(THREADS), \
+                       p.lds, ctx->stream, p.ax, p.g, rounds, ld, md, st)
+    TK_CONV_DISPATCH(p, TK_CONV_ONE);
+#undef TK_CONV_ONE
     TK_LAUNCHED();
     return TK_OK;
 }
 
+// one pass of many volumes whose plans share the variant of `like` (jobs in device memory,
+// nblocks blocks in all, lds the largest of their LDS sizes)
+template <class T, class Ld, class Md, class St>
+int launch_conv_batch(tk_ctx ctx, const ConvPlan& like, const ConvJob<Ld, Md, St>* jobs, int njobs,
+                      unsigned nblocks, size_t lds)
+{
+#define TK_CONV_MANY(ARGS, THREADS)                                                            \
+    hipLaunchKernelGGL((k_conv_axis_batch<TK_UNPACK_ ARGS, Ld, Md, St>), dim3(nblocks),       \
+                       dim3(THREADS), lds, ctx->stream, jobs, njobs)
+    TK_CONV_DISPATCH(like, TK_CONV_MANY);
+#undef TK_CONV_MANY
+    TK_LAUNCHED();
+    return TK_OK;
+}
 
 }  // namespace tk

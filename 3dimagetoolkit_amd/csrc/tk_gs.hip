@@ -302,6 +302,16 @@ static int lex_chunk(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int C,
 // ---- red-black -----------------------------------------------------------------------
 // Device stop protocol (slots in ctx->cnt): cnt[0] = 1 + stopping sweep (0: running);
 // cnt[1 + slot] = arrivals of the black pass of the sweep using acc slot `slot`.
+// one volume of a batched sweep launch (k_gs_rb_pv_batch)
+struct RbJob {
+    const void* src;
+    void* dst;
+    const void* rhs;
+    int X, Y, Z, CI;
+    int gx, gy, gz;  // tiles in k, j, i
+    int first, nblk; // first block of the volume (a multiple of kNumXcd) and its block count
+};
+
 struct RbGate {
     double* acc;        // pair per sweep slot: sum (new-old)^2, sum old^2 (interior)
     double* err;        // per slot: relative update error
@@ -434,11 +444,13 @@ __global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* 
 // shape and type by timing both.
 constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
 
+// one workgroup's tile (bl: tile column in k, row block in j, chunk in i; nblk: tiles of
+// the launch, for the gate's arrival count)
 template <class T, bool UNIT, int TJ, int NT>
-__global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
-                                                  const T* __restrict__ rhs, int X, int Y, int Z,
-                                                  int CI, GsCoef c, int s, int slot, int gated,
-                                                  RbGate g)
+__device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restrict__ dst,
+                                           const T* __restrict__ rhs, int X, int Y, int Z, int CI,
+                                           const GsCoef& c, int s, int slot, int gated,
+                                           const RbGate& g, uint3 bl, unsigned nblk)
 {
     if (gated && *(volatile unsigned*)g.stop) return;
     constexpr int V = Vec16<T>::n;
@@ -449,7 +461,6 @@ __global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* _
     __shared__ __align__(16) T O[4][OJ * W];
     __shared__ __align__(16) T Rp[3][RJ * W];
     __shared__ __align__(16) T Bp[3][RJ * W];
-    const uint3 bl = xcd_block3();  // tiles sharing halo rows run on one XCD
     const int kb = bl.x * kRbTK, kbase = kb - 4;
     const int j0 = 1 + bl.y * TJ;
     const int ia = 1 + bl.z * CI;
@@ -600,7 +611,6 @@ __global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* _
         atomicAdd(&g.acc[2 * slot], d2);
         atomicAdd(&g.acc[2 * slot + 1], o2);
         __threadfence();
-        const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
         const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
         if (ticket == nblk - 1) {
             const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
@@ -612,6 +622,45 @@ __global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* _
             if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
         }
     }
+}
+
+template <class T, bool UNIT, int TJ, int NT>
+__global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
+                                                  const T* __restrict__ rhs, int X, int Y, int Z,
+                                                  int CI, GsCoef c, int s, int slot, int gated,
+                                                  RbGate g)
+{
+    // tiles sharing halo rows run on one XCD
+    rb_pv_tile<T, UNIT, TJ, NT>(src, dst, rhs, X, Y, Z, CI, c, s, slot, gated, g, xcd_block3(),
+                                gridDim.x * gridDim.y * gridDim.z);
+}
+
+// Many volumes, one sweep each, in one launch (the batch's fixed-sweep path): the job table
+// gives each volume a block range starting at a multiple of the XCD count and padded to
+// one, so the XCD-aware order inside a volume is that of its own launch; padding blocks
+// exit at once.  The sweep of each voxel does not depend on the tiling, so every volume
+// gets the values of its own one-volume sweep.
+template <class T, bool UNIT, int TJ, int NT>
+__global__ __launch_bounds__(NT) void k_gs_rb_pv_batch(const RbJob* __restrict__ jobs, int njobs,
+                                                        GsCoef c)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const RbJob jb = jobs[lo];
+    const unsigned n = (unsigned)jb.gx * jb.gy * jb.gz;
+    unsigned w = xcd_remap(blockIdx.x - (unsigned)jb.first, (unsigned)jb.nblk);
+    if (w >= n) return;
+    uint3 bl;
+    bl.x = w % jb.gx;
+    w /= jb.gx;
+    bl.y = w % jb.gy;
+    bl.z = w / jb.gy;
+    rb_pv_tile<T, UNIT, TJ, NT>((const T*)jb.src, (T*)jb.dst, (const T*)jb.rhs, jb.X, jb.Y, jb.Z,
+                                jb.CI, c, 0, 0, 0, RbGate{}, bl, n);
 }
 
 __global__ void k_zero_d(double* p, int n)
@@ -654,6 +703,60 @@ static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, cons
                            (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
                            gated, g);
     TK_LAUNCHED();
+    return TK_OK;
+}
+
+// the batched sweep: 8-row tiles of 256 threads (the per-volume autotuner's choice at
+// every size measured), the chunk length in x common to all volumes so that the whole
+// launch is about one round of resident workgroups
+constexpr int kRbBatchTJ = 8;
+constexpr int kRbBatchRounds = 6;
+
+template <class T>
+static int rb_batch_table_t(tk_ctx ctx, const RbBatchVol* vols, int n, void** table, int* nblocks)
+{
+    constexpr int TJ = kRbBatchTJ;
+    const long lds = (long)sizeof(T) * (4L * (TJ + 4) + 6L * (TJ + 2)) * kRbW + 256;
+    const long per_cu = std::max(1L, 163840L / lds);
+    long planes = 0;  // tile-planes of the whole batch
+    for (int v = 0; v < n; ++v) {
+        const int kt = (vols[v].Z + kRbTK - 1) / kRbTK, jt = (vols[v].Y - 2 + TJ - 1) / TJ;
+        planes += (long)kt * jt * (vols[v].X - 2);
+    }
+    // several rounds of resident workgroups: chunks of different volumes differ in length,
+    // and one round of whole-length chunks left a long tail (382 us per sweep of config 3
+    // against 180 us at the 256^3 rate)
+    const long target = std::max(1L, (long)ctx->num_cu * per_cu * kRbBatchRounds);
+    const int CI0 = (int)std::max(4L, (planes + target - 1) / target);
+    std::vector<RbJob> jobs((size_t)n);
+    int first = 0;
+    for (int v = 0; v < n; ++v) {
+        RbJob& j = jobs[(size_t)v];
+        j.src = vols[v].src;
+        j.dst = vols[v].dst;
+        j.rhs = vols[v].rhs;
+        j.X = vols[v].X;
+        j.Y = vols[v].Y;
+        j.Z = vols[v].Z;
+        j.CI = std::min(CI0, j.X - 2);
+        j.gx = (j.Z + kRbTK - 1) / kRbTK;
+        j.gy = (j.Y - 2 + TJ - 1) / TJ;
+        j.gz = (j.X - 2 + j.CI - 1) / j.CI;
+        const int nb = j.gx * j.gy * j.gz;
+        j.nblk = (nb + kNumXcd - 1) / kNumXcd * kNumXcd;
+        j.first = first;
+        first += j.nblk;
+    }
+    void* d = nullptr;
+    TK_CHECK(hipMalloc(&d, sizeof(RbJob) * (size_t)n) == hipSuccess, TK_ERR_NOMEM,
+             "batched sweep: job table allocation failed");
+    if (hipMemcpy(d, jobs.data(), sizeof(RbJob) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        set_error("batched sweep: job table upload failed");
+        return TK_ERR_HIP;
+    }
+    *table = d;
+    *nblocks = first;
     return TK_OK;
 }
 
@@ -986,6 +1089,51 @@ __global__ void k_sum_pairs_gs(const double* __restrict__ part, int nb, double* 
 }  // namespace tk
 
 using namespace tk;
+
+namespace tk {
+int rb_batch_table(tk_ctx ctx, tk_dtype dt, const RbBatchVol* vols, int n, void** table,
+                   int* nblocks)
+{
+    TK_CHECK(ctx && vols && n > 0 && table && nblocks, TK_ERR_ARG, "batched sweep: argument");
+    for (int v = 0; v < n; ++v)
+        TK_CHECK(vols[v].X >= 3 && vols[v].Y >= 3 && vols[v].Z >= 3 &&
+                     vols[v].Z % (dt == TK_F64 ? 2 : 4) == 0,
+                 TK_ERR_UNSUPPORTED, "batched sweep: volume %d: 16-byte rows needed", v);
+    return dt == TK_F64 ? rb_batch_table_t<double>(ctx, vols, n, table, nblocks)
+                        : rb_batch_table_t<float>(ctx, vols, n, table, nblocks);
+}
+
+int rb_batch_sweep(tk_ctx ctx, tk_dtype dt, const void* table, int n, int nblocks, double dt_step,
+                   double hx, double hy, double hz)
+{
+    tk_vol_s like{};
+    like.hx = hx;
+    like.hy = hy;
+    like.hz = hz;
+    const GsCoef c = gs_coef(&like, dt_step);
+    const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+    constexpr int TJ = kRbBatchTJ, NT = 256;
+    TimedLaunch tl(ctx, TK_K_GS);
+    const RbJob* jobs = (const RbJob*)table;
+    if (dt == TK_F64) {
+        if (unit)
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, true, TJ, NT>), dim3(nblocks), dim3(NT), 0,
+                               ctx->stream, jobs, n, c);
+        else
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, false, TJ, NT>), dim3(nblocks), dim3(NT),
+                               0, ctx->stream, jobs, n, c);
+    } else {
+        if (unit)
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, true, TJ, NT>), dim3(nblocks), dim3(NT), 0,
+                               ctx->stream, jobs, n, c);
+        else
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, false, TJ, NT>), dim3(nblocks), dim3(NT),
+                               0, ctx->stream, jobs, n, c);
+    }
+    TK_LAUNCHED();
+    return TK_OK;
+}
+}  // namespace tk
 
 // min_edge: 3 for the Dirichlet sweep (an interior to update), 2 for the Neumann sweep
 // (its 2-D branch, Z == 1, is not built)

@@ -103,6 +103,24 @@ int ctx_work(tk_ctx ctx, int slot, size_t bytes, void** out);
 // the tollerance of the last iteration read back (waits on the run's stream), the
 // iteration counter
 int rsfe_step_async(tk_rsfe s);
+// the run's red-black buffers (faces set to the boundary value if not yet) and rhs; the
+// exchange of phi and its partner after an odd number of sweeps
+int rsfe_rb_prepare(tk_rsfe s, void** phi, void** alt, const void** rhs);
+void rsfe_rb_exchange(tk_rsfe s);
+// the five convolution passes of many runs, one launch per pass and variant; tables for
+// both roles of phi and its red-black partner (role 0: as when built), built outside any
+// capture
+struct RsfeBatchConv;
+struct RsfeBatchUpdate;
+// the Split-Bregman update, tolerance sums and div of many runs, one launch each (tables for
+// both roles of phi, as the convolution tables); then each run's iteration count
+int rsfe_batch_update_build(tk_rsfe* runs, int n, RsfeBatchUpdate** out);
+int rsfe_batch_update_run(tk_ctx ctx, const RsfeBatchUpdate* t, tk_rsfe* runs, int n, int role);
+void rsfe_batch_update_free(RsfeBatchUpdate* t);
+int rsfe_batch_conv_build(tk_rsfe* runs, int n, RsfeBatchConv** out);
+int rsfe_batch_conv_run(tk_ctx ctx, const RsfeBatchConv* t, int role);
+void rsfe_batch_conv_free(RsfeBatchConv* t);
+const tk_rsfe_params& rsfe_params(tk_rsfe s);
 int rsfe_read_tollerance(tk_rsfe s, double* tol);
 int rsfe_iterations(tk_rsfe s);
 void rsfe_set_iterations(tk_rsfe s, int iter);
@@ -124,6 +142,18 @@ int poisson_set_boundary(tk_ctx ctx, tk_vol phi, double bc);
 int poisson_dir_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, void** alt,
                           int* sweeps_out, double* err_out, bool faces_set = false);
+// red-black sweeps of many volumes in one launch (fixed sweep count): the job table of one
+// sweep direction (src -> dst per volume; device memory, free with hipFree) and the launch
+struct RbBatchVol {
+    const void* src;
+    void* dst;
+    const void* rhs;
+    int X, Y, Z;
+};
+int rb_batch_table(tk_ctx ctx, tk_dtype dt, const RbBatchVol* vols, int n, void** table,
+                   int* nblocks);
+int rb_batch_sweep(tk_ctx ctx, tk_dtype dt, const void* table, int n, int nblocks, double dt_step,
+                   double hx, double hy, double hz);
 // NeuGaussSeidel sweeps (lexicographic only), the same loop rule
 int poisson_neu_solve_alt(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int order,
                           int ls_steps, double ls_tol, int max_sweeps, int* sweeps_out,

@@ -15,20 +15,26 @@
 // start on the axis where pair 1 ends.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <limits>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "tk_device.hpp"
 #include "tk_fft_kernel.hpp"
+#include "tk_stencil.hpp"
 
 namespace tk {
 
 template <class T>
 int launch_div(tk_ctx, tk_vol, tk_vol, tk_vol, tk_vol);
+int stencil_batch_grid(tk_dtype dt, int X, int Y, int Z, int* gy, int* nparts);
+int launch_sb_batch(tk_ctx ctx, tk_dtype dt, const SbJob* jobs, int n, unsigned nblocks);
+int launch_div_batch(tk_ctx ctx, tk_dtype dt, const DivJob* jobs, int n, unsigned nblocks);
 template <class T>
 int launch_edge(tk_ctx, tk_vol, double, double, double, tk_vol);
 template <class T>
@@ -325,7 +331,7 @@ int auto_extract_cc(tk_rsfe s, int* extracted)
 // condition needs it); without sync nothing waits on the host and nothing but kernels is
 // enqueued, so the iteration can be captured into a graph (the caller reads tol_dev later).
 template <class T>
-int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
+int rsfe_forcing(tk_rsfe s)
 {
     tk_ctx ctx = s->ctx;
     const tk_rsfe_params& p = s->p;
@@ -360,6 +366,22 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         TimedLaunch tl(ctx, TK_K_CONV_X2);
         TK_TRY((launch_conv_axis<T>(ctx, f, 0, 1, LdComplex<T>{C}, MdNone{}, sr, acy)));
     }
+    return TK_OK;
+}
+
+// after the Poisson step: automatic connected component, cut + tolerance sums + the
+// Split-Bregman update and div (or the reset of b and divdb), iteration count
+template <class T>
+int rsfe_update(tk_rsfe s, int* extracted_out);
+int rsfe_finish(tk_rsfe s, tk_rsfe_iter_stats* st, int sweeps, double ls_err, int used,
+                double res_plain, double res_guess, int extracted);
+
+template <class T>
+int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
+{
+    tk_ctx ctx = s->ctx;
+    const tk_rsfe_params& p = s->p;
+    TK_TRY(rsfe_forcing<T>(s));
     // Poisson loop (rsfe_splitbregman_imp.hxx:1735-1797): bc value, then the sweeps from
     // the previous iterate (or from the learned guess when its residual is lower)
     const double chosen_bc = p.bc_dir ? p.a0 : 0.;
@@ -416,6 +438,17 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
                                   &ls_err, s->faces_set);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
     if (!p.neumann && chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
+    int extracted = 0;
+    TK_TRY(rsfe_update<T>(s, &extracted));
+    if (!sync) return TK_OK;
+    return rsfe_finish(s, st, sweeps + judge, ls_err, used, res_plain, res_guess, extracted);
+}
+
+template <class T>
+int rsfe_update(tk_rsfe s, int* extracted_out)
+{
+    tk_ctx ctx = s->ctx;
+    const tk_rsfe_params& p = s->p;
     // automatic connected component (imp:1808-1812) on the cut phi; the extracted phi is
     // not cut again, and with cc_init_variables b and divdb restart from zero in place of
     // this iteration's Split-Bregman update (imp:1849-1859)
@@ -462,7 +495,15 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
         }
     }
     s->iter++;
-    if (!sync) return TK_OK;
+    *extracted_out = extracted;
+    return TK_OK;
+}
+
+// the reference's loop condition needs the tollerance: read it back and wait; the stats
+int rsfe_finish(tk_rsfe s, tk_rsfe_iter_stats* st, int sweeps, double ls_err, int used,
+                double res_plain, double res_guess, int extracted)
+{
+    tk_ctx ctx = s->ctx;
     TK_HIP(hipMemcpyAsync(ctx->h_acc, s->tol_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
     TK_HIP(hipStreamSynchronize(ctx->stream));
@@ -470,7 +511,7 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
     const double h3 = s->h[0] * s->h[1] * s->h[2];
     s->tollerance = std::sqrt(h3 * ctx->h_acc[0]) / std::sqrt(h3 * ctx->h_acc[1]);
     if (st) {
-        st->sweeps = sweeps + judge;
+        st->sweeps = sweeps;
         st->ls_err = ls_err;
         st->tollerance = s->tollerance;
         st->guess_used = used;
@@ -780,6 +821,281 @@ int rsfe_read_tollerance(tk_rsfe s, double* tol)
     return TK_OK;
 }
 int rsfe_iterations(tk_rsfe s) { return s->iter; }
+
+// ---- the convolution passes of many runs, one launch per pass and variant -----------------
+// Runs are grouped by the variant their filter axis plans (half-width, quads, matrix cores),
+// so each run computes exactly what its own launch would; a pass then costs one launch per
+// group.  The first x pass reads phi, whose buffer alternates with the red-black partner:
+// its tables exist for both roles.
+struct BatchGroup {
+    ConvPlan like;
+    void* jobs = nullptr;
+    int njobs = 0;
+    unsigned nblocks = 0;
+    size_t lds = 0;
+};
+
+struct RsfeBatchConv {
+    tk_dtype dt = TK_F32;
+    std::vector<BatchGroup> x1[2], y, z2, x2;
+    ~RsfeBatchConv()
+    {
+        for (auto* v : {&x1[0], &x1[1], &y, &z2, &x2})
+            for (BatchGroup& g : *v)
+                if (g.jobs) (void)hipFree(g.jobs);
+    }
+};
+
+namespace {
+
+template <class T, class Ld, class Md, class St, class Ops>
+int build_stage(std::vector<BatchGroup>& out, tk_rsfe* runs, int n, int axis, int rounds, Ops ops)
+{
+    typedef ConvJob<Ld, Md, St> Job;
+    std::vector<std::vector<Job>> jobs;
+    for (int v = 0; v < n; ++v) {
+        tk_rsfe s = runs[v];
+        ConvPlan pl;
+        TK_TRY(conv_plan<T>(s->flt, axis, s->p.acyclic != 0, &pl));
+        size_t q = 0;
+        while (q < out.size() && !(out[q].like.S == pl.S && out[q].like.quad == pl.quad &&
+                                   out[q].like.mf == pl.mf))
+            ++q;
+        if (q == out.size()) {
+            out.emplace_back();
+            out[q].like = pl;
+            jobs.emplace_back();
+        }
+        Job j;
+        j.ax = pl.ax;
+        j.g = pl.g;
+        j.rounds = rounds;
+        ops(s, &j.ld, &j.md, &j.st);
+        j.groups = (int)pl.nb;
+        j.nblk = (int)((pl.nb + kNumXcd - 1) / kNumXcd * kNumXcd);
+        j.first = (int)out[q].nblocks;
+        out[q].nblocks += (unsigned)j.nblk;
+        out[q].lds = std::max(out[q].lds, pl.lds);
+        jobs[q].push_back(j);
+    }
+    for (size_t q = 0; q < out.size(); ++q) {
+        out[q].njobs = (int)jobs[q].size();
+        TK_CHECK(hipMalloc(&out[q].jobs, sizeof(Job) * jobs[q].size()) == hipSuccess,
+                 TK_ERR_NOMEM, "batch: convolution job table allocation failed");
+        TK_HIP(hipMemcpy(out[q].jobs, jobs[q].data(), sizeof(Job) * jobs[q].size(),
+                         hipMemcpyHostToDevice));
+    }
+    return TK_OK;
+}
+
+template <class T, class Ld, class Md, class St>
+int run_stage(tk_ctx ctx, const std::vector<BatchGroup>& groups, int tag)
+{
+    TimedLaunch tl(ctx, tag);
+    for (const BatchGroup& g : groups)
+        TK_TRY((launch_conv_batch<T, Ld, Md, St>(ctx, g.like,
+                                                 (const ConvJob<Ld, Md, St>*)g.jobs, g.njobs,
+                                                 g.nblocks, g.lds)));
+    return TK_OK;
+}
+
+template <class T>
+int batch_conv_build(tk_rsfe* runs, int n, RsfeBatchConv* t)
+{
+    for (int role = 0; role < 2; ++role)
+        TK_TRY((build_stage<T, LdHeaviside<T>, MdNone, StComplex<T>>(
+            t->x1[role], runs, n, 0, 1,
+            [role](tk_rsfe s, LdHeaviside<T>* ld, MdNone*, StComplex<T>* st) {
+                const tk_rsfe_params& p = s->p;
+                const T* phi = (const T*)(role == 0 ? s->phi->d : s->alt);
+                *ld = LdHeaviside<T>{phi, (const T*)s->u0->d, (T*)s->phiold->d,
+                                     (T)((p.a0 + p.b0) / 2.), (T)p.epsilon};
+                *st = StComplex<T>{(cplx<T>*)s->C};
+            })));
+    TK_TRY((build_stage<T, LdComplex<T>, MdNone, StComplex<T>>(
+        t->y, runs, n, 1, 1, [](tk_rsfe s, LdComplex<T>* ld, MdNone*, StComplex<T>* st) {
+            *ld = LdComplex<T>{(cplx<T>*)s->C};
+            *st = StComplex<T>{(cplx<T>*)s->C};
+        })));
+    TK_TRY((build_stage<T, LdComplex<T>, MdRegionFit<T>, StComplex<T>>(
+        t->z2, runs, n, 2, 2,
+        [](tk_rsfe s, LdComplex<T>* ld, MdRegionFit<T>* md, StComplex<T>* st) {
+            const tk_rsfe_params& p = s->p;
+            *ld = LdComplex<T>{(cplx<T>*)s->C};
+            *md = MdRegionFit<T>{(const T*)s->kmyim->d, (const T*)s->kones->d, (T)p.lamda1,
+                                 (T)p.lamda2};
+            *st = StComplex<T>{(cplx<T>*)s->C};
+        })));
+    TK_TRY((build_stage<T, LdComplex<T>, MdNone, StRhs<T>>(
+        t->x2, runs, n, 0, 1, [](tk_rsfe s, LdComplex<T>* ld, MdNone*, StRhs<T>* st) {
+            const tk_rsfe_params& p = s->p;
+            *ld = LdComplex<T>{(cplx<T>*)s->C};
+            *st = StRhs<T>{(const T*)s->u0->d, (const T*)s->kones->d, (const T*)s->divdb->d,
+                           (T*)s->rhs->d, (T)(p.lamda1 - p.lamda2), (T)p.lamda};
+        })));
+    return TK_OK;
+}
+
+template <class T>
+int batch_conv_run(tk_ctx ctx, const RsfeBatchConv* t, int role)
+{
+    TK_TRY((run_stage<T, LdHeaviside<T>, MdNone, StComplex<T>>(ctx, t->x1[role], TK_K_CONV_X1)));
+    TK_TRY((run_stage<T, LdComplex<T>, MdNone, StComplex<T>>(ctx, t->y, TK_K_CONV_Y)));
+    TK_TRY((run_stage<T, LdComplex<T>, MdRegionFit<T>, StComplex<T>>(ctx, t->z2, TK_K_CONV_Z2)));
+    TK_TRY((run_stage<T, LdComplex<T>, MdNone, StComplex<T>>(ctx, t->y, TK_K_CONV_Y)));
+    TK_TRY((run_stage<T, LdComplex<T>, MdNone, StRhs<T>>(ctx, t->x2, TK_K_CONV_X2)));
+    return TK_OK;
+}
+
+}  // namespace
+
+int rsfe_batch_conv_build(tk_rsfe* runs, int n, RsfeBatchConv** out)
+{
+    TK_CHECK(runs && n > 0 && out, TK_ERR_ARG, "batch: convolution tables: argument");
+    for (int v = 0; v < n; ++v)
+        TK_CHECK(runs[v] && runs[v]->alt && runs[v]->dtype == runs[0]->dtype, TK_ERR_ARG,
+                 "batch: convolution tables: run %d", v);
+    RsfeBatchConv* t = new RsfeBatchConv();
+    t->dt = runs[0]->dtype;
+    const int r = t->dt == TK_F64 ? batch_conv_build<double>(runs, n, t)
+                                  : batch_conv_build<float>(runs, n, t);
+    if (r != TK_OK) {
+        delete t;
+        return r;
+    }
+    *out = t;
+    return TK_OK;
+}
+int rsfe_batch_conv_run(tk_ctx ctx, const RsfeBatchConv* t, int role)
+{
+    return t->dt == TK_F64 ? batch_conv_run<double>(ctx, t, role)
+                           : batch_conv_run<float>(ctx, t, role);
+}
+void rsfe_batch_conv_free(RsfeBatchConv* t) { delete t; }
+
+struct RsfeBatchUpdate {
+    tk_dtype dt = TK_F32;
+    void* sb[2] = {nullptr, nullptr};  // SbJob tables, phi in each role
+    void* div = nullptr;               // DivJob table
+    double* parts = nullptr;           // every run's block partials
+    unsigned sb_blocks = 0, div_blocks = 0;
+    int n = 0;
+    ~RsfeBatchUpdate()
+    {
+        for (void* p : {sb[0], sb[1], div, (void*)parts})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int rsfe_batch_update_build(tk_rsfe* runs, int n, RsfeBatchUpdate** out)
+{
+    TK_CHECK(runs && n > 0 && out, TK_ERR_ARG, "batch: update tables: argument");
+    RsfeBatchUpdate* t = new RsfeBatchUpdate();
+    t->dt = runs[0]->dtype;
+    t->n = n;
+    std::vector<SbJob> sb((size_t)n);
+    std::vector<DivJob> dv((size_t)n);
+    long parts = 0;
+    int first = 0;
+    for (int v = 0; v < n; ++v) {
+        tk_rsfe s = runs[v];
+        SbJob& j = sb[(size_t)v];
+        j.phiold = s->phiold->d;
+        j.g = s->g->d;
+        for (int a = 0; a < 3; ++a) {
+            j.b[a] = s->b[a]->d;
+            j.q[a] = s->q[a]->d;
+        }
+        j.X = s->X;
+        j.Y = s->Y;
+        j.Z = s->Z;
+        j.hx = s->h[0];
+        j.hy = s->h[1];
+        j.hz = s->h[2];
+        j.lo = s->p.a0;
+        j.hi = s->p.b0;
+        j.tol = s->tol_dev;
+        stencil_batch_grid(t->dt, s->X, s->Y, s->Z, &j.gy, &j.nparts);
+        j.nblk = (j.nparts + (int)kNumXcd - 1) / (int)kNumXcd * (int)kNumXcd;
+        j.first = first;
+        first += j.nblk;
+        j.part = nullptr;
+        j.phi = nullptr;
+        parts += 2L * j.nparts;
+        DivJob& d = dv[(size_t)v];
+        for (int a = 0; a < 3; ++a) d.q[a] = s->q[a]->d;
+        d.out = s->divdb->d;
+        d.X = s->X;
+        d.Y = s->Y;
+        d.Z = s->Z;
+        d.hx = s->h[0];
+        d.hy = s->h[1];
+        d.hz = s->h[2];
+        d.gy = j.gy;
+        d.nblk = j.nblk;
+        d.first = j.first;
+    }
+    t->sb_blocks = t->div_blocks = (unsigned)first;
+    int r = hipMalloc(&t->parts, sizeof(double) * (size_t)parts) == hipSuccess ? TK_OK : TK_ERR_NOMEM;
+    long off = 0;
+    for (int v = 0; v < n; ++v) {
+        sb[(size_t)v].part = t->parts + off;
+        off += 2L * sb[(size_t)v].nparts;
+    }
+    for (int role = 0; r == TK_OK && role < 2; ++role) {
+        for (int v = 0; v < n; ++v) sb[(size_t)v].phi = role == 0 ? runs[v]->phi->d : runs[v]->alt;
+        if (hipMalloc(&t->sb[role], sizeof(SbJob) * (size_t)n) != hipSuccess) r = TK_ERR_NOMEM;
+        else if (hipMemcpy(t->sb[role], sb.data(), sizeof(SbJob) * (size_t)n,
+                           hipMemcpyHostToDevice) != hipSuccess)
+            r = TK_ERR_HIP;
+    }
+    if (r == TK_OK && hipMalloc(&t->div, sizeof(DivJob) * (size_t)n) != hipSuccess) r = TK_ERR_NOMEM;
+    if (r == TK_OK &&
+        hipMemcpy(t->div, dv.data(), sizeof(DivJob) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        r = TK_ERR_HIP;
+    if (r != TK_OK) {
+        delete t;
+        set_error("batch: update tables: allocation or upload failed");
+        return r;
+    }
+    *out = t;
+    return TK_OK;
+}
+
+// cut + tolerance sums + grad/shrink/Bregman, the sums per run, div; the iteration counts.
+// (The staged path has no automatic extraction and no dumps: the batch refuses them.)
+int rsfe_batch_update_run(tk_ctx ctx, const RsfeBatchUpdate* t, tk_rsfe* runs, int n, int role)
+{
+    TK_TRY(launch_sb_batch(ctx, t->dt, (const SbJob*)t->sb[role], t->n, t->sb_blocks));
+    TK_TRY(launch_div_batch(ctx, t->dt, (const DivJob*)t->div, t->n, t->div_blocks));
+    for (int v = 0; v < n; ++v) runs[v]->iter++;
+    return TK_OK;
+}
+void rsfe_batch_update_free(RsfeBatchUpdate* t) { delete t; }
+
+// the staged batch path: the red-black buffers (faces set once), the pointer exchange
+// after an odd sweep count
+int rsfe_rb_prepare(tk_rsfe s, void** phi, void** alt, const void** rhs)
+{
+    TK_CHECK(s->alt, TK_ERR_ARG, "batch: the run has no red-black partner buffer");
+    const tk_rsfe_params& p = s->p;
+    const double bc = p.bc_dir ? p.a0 : 0.;
+    if (!s->faces_set) {
+        // as the first fixed-sweep solve does: both buffers carry the boundary value
+        tk_vol_s tmp = *s->phi;
+        tmp.d = s->alt;
+        tmp.owned = false;
+        TK_TRY(poisson_set_boundary(s->ctx, s->phi, bc));
+        TK_TRY(poisson_set_boundary(s->ctx, &tmp, bc));
+        if (bc >= p.a0 && bc <= p.b0) s->faces_set = true;
+    }
+    *phi = s->phi->d;
+    *alt = s->alt;
+    *rhs = s->rhs->d;
+    return TK_OK;
+}
+void rsfe_rb_exchange(tk_rsfe s) { std::swap(s->phi->d, s->alt); }
+const tk_rsfe_params& rsfe_params(tk_rsfe s) { return s->p; }
 void rsfe_set_iterations(tk_rsfe s, int iter) { s->iter = iter; }
 }  // namespace tk
 

@@ -2,6 +2,8 @@
 // Split-Bregman update (rsfe_splitbregman_imp.hxx:1868-1936).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tk_device.hpp"
 #include "tk_internal.hpp"
 #include "tk_stencil.hpp"
@@ -188,20 +190,28 @@ __device__ __forceinline__ void drow(const T* __restrict__ f, long v0, long st, 
     }
 }
 
+// The rows a thread visits in a gx x gy grid of bdx x bdy blocks, as thread (tx, ty) of
+// block (bx, by): the grid of one launch, or one volume's share of a batched launch.
+#define TK_FOR_ROWS_IN(X, Y, Z, V, i, j, k0, v0, TX, TY, BDX, BDY, BX, BY, GX, GY)            \
+    for (int r_ = (int)((BY) * (BDY) + (TY)); r_ < (X) * (Y); r_ += (int)((GY) * (BDY)))      \
+        for (int k0 = (int)((BX) * (BDX) + (TX)) * (V), i = r_ / (Y), j = r_ - i * (Y);      \
+             k0 < (Z); k0 += (int)((GX) * (BDX)) * (V))                                       \
+            for (long v0 = (long)r_ * (Z) + k0, once_ = 1; once_; once_ = 0)
+
+// the Split-Bregman update on the rows of one thread (sums of the tolerance into s1, s2)
 template <class T>
-__global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const T* __restrict__ phiold,
-                                                     T* __restrict__ b0, T* __restrict__ b1,
-                                                     T* __restrict__ b2, const T* __restrict__ g,
-                                                     T* __restrict__ q0, T* __restrict__ q1,
-                                                     T* __restrict__ q2, int X, int Y, int Z,
-                                                     double hx, double hy, double hz, T lo, T hi,
-                                                     double* __restrict__ part)
+__device__ __forceinline__ void sb_rows(T* __restrict__ phi, const T* __restrict__ phiold,
+                                        T* __restrict__ b0, T* __restrict__ b1, T* __restrict__ b2,
+                                        const T* __restrict__ g, T* __restrict__ q0,
+                                        T* __restrict__ q1, T* __restrict__ q2, int X, int Y, int Z,
+                                        double hx, double hy, double hz, T lo, T hi, double& s1,
+                                        double& s2, int tx, int ty, int bdx, int bdy, unsigned bx,
+                                        unsigned by, unsigned gx, unsigned gy)
 {
     constexpr int V = Vec16<T>::n;
     const long sX = (long)Y * Z, sY = Z;
     auto cl = [&](T x) { return clampv(x, lo, hi); };
-    double s1 = 0, s2 = 0;
-    TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)
+    TK_FOR_ROWS_IN(X, Y, Z, V, i, j, k0, v0, tx, ty, bdx, bdy, bx, by, gx, gy)
     {
         T d[3][V];
         drow(phi, v0, sX, i, X, hx, cl, d[0]);
@@ -246,6 +256,23 @@ __global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const 
         }
         strow(phi + v0, pc);
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const T* __restrict__ phiold,
+                                                     T* __restrict__ b0, T* __restrict__ b1,
+                                                     T* __restrict__ b2, const T* __restrict__ g,
+                                                     T* __restrict__ q0, T* __restrict__ q1,
+                                                     T* __restrict__ q2, int X, int Y, int Z,
+                                                     double hx, double hy, double hz, T lo, T hi,
+                                                     double* __restrict__ part)
+{
+    // blocks renumbered so that rows j-1, j+1 of a block's edge rows are read on one XCD
+    const unsigned w = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    double s1 = 0, s2 = 0;
+    sb_rows<T>(phi, phiold, b0, b1, b2, g, q0, q1, q2, X, Y, Z, hx, hy, hz, lo, hi, s1, s2,
+               threadIdx.x, threadIdx.y, blockDim.x, blockDim.y, w % gridDim.x, w / gridDim.x,
+               gridDim.x, gridDim.y);
     block_sum2(s1, s2);
     if (flat_tid() == 0) {
         const int b = blockIdx.x + gridDim.x * blockIdx.y;
@@ -255,14 +282,16 @@ __global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const 
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_div_v(const T* __restrict__ f0, const T* __restrict__ f1,
-                                               const T* __restrict__ f2, T* __restrict__ out, int X,
-                                               int Y, int Z, double hx, double hy, double hz)
+__device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __restrict__ f1,
+                                         const T* __restrict__ f2, T* __restrict__ out, int X,
+                                         int Y, int Z, double hx, double hy, double hz, int tx,
+                                         int ty, int bdx, int bdy, unsigned bx, unsigned by,
+                                         unsigned gx, unsigned gy)
 {
     constexpr int V = Vec16<T>::n;
     const long sX = (long)Y * Z, sY = Z;
     auto id = [](T x) { return x; };
-    TK_FOR_ROWS(X, Y, Z, V, i, j, k0, v0)
+    TK_FOR_ROWS_IN(X, Y, Z, V, i, j, k0, v0, tx, ty, bdx, bdy, bx, by, gx, gy)
     {
         const int ci = plane_class(i, X), cj = plane_class(j, Y);
         T tx[V], ty[V];
@@ -299,6 +328,97 @@ __global__ __launch_bounds__(256) void k_div_v(const T* __restrict__ f0, const T
             o.v[e] = tx[e] + ty[e] + tz;
         }
         strow(out + v0, o);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_div_v(const T* __restrict__ f0, const T* __restrict__ f1,
+                                               const T* __restrict__ f2, T* __restrict__ out, int X,
+                                               int Y, int Z, double hx, double hy, double hz)
+{
+    const unsigned w = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    div_rows<T>(f0, f1, f2, out, X, Y, Z, hx, hy, hz, threadIdx.x, threadIdx.y, blockDim.x,
+                blockDim.y, w % gridDim.x, w / gridDim.x, gridDim.x, gridDim.y);
+}
+
+// ---- many volumes per launch (the batch's staged path) --------------------------------
+// Blocks of 256 threads laid out as bdx = Z/V quads (one row of 16-byte words) by
+// bdy = 256 / bdx rows, so short rows leave few lanes idle (a 16 x 16 shape left 25-37 %
+// idle for z = 48 or 80); each volume's block range starts at a multiple of the XCD count
+// and is padded to one.  A block finds its volume by a binary search of the first blocks.
+template <class J>
+__device__ __forceinline__ const J& batch_job(const J* jobs, int njobs)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+__host__ __device__ __forceinline__ void batch_block_shape(int Zv, int* bdx, int* bdy)
+{
+    *bdx = Zv >= 64 ? 64 : Zv;
+    *bdy = 256 / *bdx;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sb_update_batch(const SbJob* __restrict__ jobs, int njobs)
+{
+    const SbJob& jb = batch_job(jobs, njobs);
+    constexpr int V = Vec16<T>::n;
+    int bdx, bdy;
+    batch_block_shape(jb.Z / V, &bdx, &bdy);
+    const unsigned gx = (unsigned)((jb.Z / V + bdx - 1) / bdx), gy = (unsigned)jb.gy;
+    const unsigned w = xcd_remap(blockIdx.x - (unsigned)jb.first, (unsigned)jb.nblk);
+    if (w >= gx * gy) return;
+    double s1 = 0, s2 = 0;
+    const int ty = threadIdx.x / bdx;
+    if (ty < bdy)
+        sb_rows<T>((T*)jb.phi, (const T*)jb.phiold, (T*)jb.b[0], (T*)jb.b[1], (T*)jb.b[2],
+                   (const T*)jb.g, (T*)jb.q[0], (T*)jb.q[1], (T*)jb.q[2], jb.X, jb.Y, jb.Z,
+                   jb.hx, jb.hy, jb.hz, (T)jb.lo, (T)jb.hi, s1, s2, threadIdx.x % bdx, ty, bdx,
+                   bdy, w % gx, w / gx, gx, gy);
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        jb.part[2 * w] = s1;
+        jb.part[2 * w + 1] = s2;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_div_batch(const DivJob* __restrict__ jobs, int njobs)
+{
+    const DivJob& jb = batch_job(jobs, njobs);
+    constexpr int V = Vec16<T>::n;
+    int bdx, bdy;
+    batch_block_shape(jb.Z / V, &bdx, &bdy);
+    const unsigned gx = (unsigned)((jb.Z / V + bdx - 1) / bdx), gy = (unsigned)jb.gy;
+    const unsigned w = xcd_remap(blockIdx.x - (unsigned)jb.first, (unsigned)jb.nblk);
+    if (w >= gx * gy) return;
+    const int ty = threadIdx.x / bdx;
+    if (ty < bdy)
+        div_rows<T>((const T*)jb.q[0], (const T*)jb.q[1], (const T*)jb.q[2], (T*)jb.out, jb.X,
+                    jb.Y, jb.Z, jb.hx, jb.hy, jb.hz, threadIdx.x % bdx, ty, bdx, bdy, w % gx,
+                    w / gx, gx, gy);
+}
+
+// each volume's tolerance sums from its block partials, one block per volume, in the order
+// of k_sum_pairs_rsfe
+__global__ __launch_bounds__(1024) void k_sum_pairs_batch(const SbJob* __restrict__ jobs)
+{
+    const SbJob& jb = jobs[blockIdx.x];
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < jb.nparts; i += blockDim.x) {
+        s1 += jb.part[2 * i];
+        s2 += jb.part[2 * i + 1];
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        jb.tol[0] = s1;
+        jb.tol[1] = s2;
     }
 }
 
@@ -368,6 +488,46 @@ int launch_sb_update(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol* b, tk_vol g,
                        phi->hy, phi->hz, (T)a0, (T)b0, part);
     TK_LAUNCHED();
     *nblocks = (int)nb;
+    return TK_OK;
+}
+
+int stencil_batch_grid(tk_dtype dt, int X, int Y, int Z, int* gy, int* nparts)
+{
+    const int V = dt == TK_F64 ? 2 : 4;
+    const int zv = Z / V;
+    int bdx, bdy;
+    batch_block_shape(zv, &bdx, &bdy);
+    const long gx = (zv + bdx - 1) / bdx;
+    long rows = ((long)X * Y + bdy - 1) / bdy;
+    rows = std::min(rows, std::max(1L, (long)kMaxPart / gx));
+    *gy = (int)rows;
+    *nparts = (int)(gx * rows);
+    return TK_OK;
+}
+
+int launch_sb_batch(tk_ctx ctx, tk_dtype dt, const SbJob* jobs, int n, unsigned nblocks)
+{
+    TimedLaunch tl(ctx, TK_K_SB);
+    if (dt == TK_F64)
+        hipLaunchKernelGGL(k_sb_update_batch<double>, dim3(nblocks), dim3(256), 0, ctx->stream,
+                           jobs, n);
+    else
+        hipLaunchKernelGGL(k_sb_update_batch<float>, dim3(nblocks), dim3(256), 0, ctx->stream,
+                           jobs, n);
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_pairs_batch, dim3((unsigned)n), dim3(1024), 0, ctx->stream, jobs);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int launch_div_batch(tk_ctx ctx, tk_dtype dt, const DivJob* jobs, int n, unsigned nblocks)
+{
+    TimedLaunch tl(ctx, TK_K_DIV);
+    if (dt == TK_F64)
+        hipLaunchKernelGGL(k_div_batch<double>, dim3(nblocks), dim3(256), 0, ctx->stream, jobs, n);
+    else
+        hipLaunchKernelGGL(k_div_batch<float>, dim3(nblocks), dim3(256), 0, ctx->stream, jobs, n);
+    TK_LAUNCHED();
     return TK_OK;
 }
 

@@ -5,6 +5,34 @@
 
 namespace tk {
 
+// one volume of a batched Split-Bregman update launch (k_sb_update_batch): its fields, the
+// cut bounds, its block partials and tolerance sums, its row grid (gy block rows, nparts
+// blocks) and block range (first a multiple of the XCD count, nblk padded to one)
+struct SbJob {
+    void* phi;
+    const void* phiold;
+    void* b[3];
+    const void* g;
+    void* q[3];
+    int X, Y, Z;
+    double hx, hy, hz;
+    double lo, hi;
+    double* part;
+    double* tol;
+    int gy, nparts;
+    int first, nblk;
+};
+
+// one volume of a batched div launch (k_div_batch)
+struct DivJob {
+    const void* q[3];
+    void* out;
+    int X, Y, Z;
+    double hx, hy, hz;
+    int gy;
+    int first, nblk;
+};
+
 // central inside, one-sided second order on the first / last plane
 // (image3d_imp.hxx:767-825); c = 0 first plane, 1 inside, 2 last plane
 template <class T>

@@ -262,6 +262,33 @@ def test_c3_graph_replays_match_eager(tk, ctx):
     ea.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_c3_batch_one_launch_sweeps_equal_single_runs(tk, ctx, dtype):
+    """the staged batch (each red-black sweep of every volume in one launch) gives bitwise
+    the values of each volume run on its own (its own sweep launches and tile choice), in
+    eager mode and through the captured graph"""
+    dims, nell = tk.gen_batch_dims(C3_SEED, 64)
+    vols = list(range(12))
+    b = c3_batch(tk, ctx, dims, nell, vols, dtype=dtype, gs_order=tk.TK_GS_REDBLACK)
+    g = c3_batch(tk, ctx, dims, nell, vols, dtype=dtype, gs_order=tk.TK_GS_REDBLACK)
+    b.iterate(3, use_graph=False)
+    g.iterate(3, use_graph=True)
+    for q, v in enumerate(vols):
+        shape = tuple(int(x) for x in dims[v])
+        u0, phi0 = tk.gen_standard(shape, C3_SEED + v, 1, int(nell[v]))
+        vu, vi = ctx.from_numpy(rescale255(u0).astype(dtype)), ctx.from_numpy(phi0.astype(dtype))
+        r = tk.Rsfe(ctx, vu, vi, tk.default_params(sigma=3.0, maxiter=30, tol=0.0, ls_steps=5,
+                                                   ls_tol=0.0, gs_order=tk.TK_GS_REDBLACK))
+        r.iterate(3)
+        single = r.phi(dtype)
+        r.close()
+        assert np.array_equal(b.phi(q).astype(dtype), single), v
+        assert np.array_equal(g.phi(q).astype(dtype), single), v
+    say("C3 staged batch:", g.info())
+    b.close()
+    g.close()
+
+
 def test_c3_batch_f32_lex_10_iterations_vs_golden(tk, ctx):
     """every one of the 64 volumes after 10 outer iterations, before rounding-level
     differences can grow along an unstable trajectory: the FP32 batch with the reference's
