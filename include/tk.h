@@ -383,11 +383,14 @@ int tk_unet_get_output(tk_unet net, float* host);
 /* ---- batched uneven volumes (config: many independent runs of uneven sizes) --------
  * No reference counterpart (the reference's static filter state serves one size at a time,
  * convolution.hxx:307-314).  Each volume is one rsfe_splitbregman::apply run with its own
- * state; volumes are spread over up to four lanes (a context and HIP stream each) by
- * longest-processing-time.  With ls_tol == 0 and tol == 0, tk_batch_iterate enqueues the
- * iterations of all volumes without host waits; use_graph != 0 captures two outer
- * iterations of every volume into one hipGraph (forked over the lanes) and replays it.
- * Otherwise each volume runs the reference's host-checked loop in turn. */
+ * state.  With red-black sweeps, ls_tol == 0 and tol == 0 (and 16-byte rows), every stage
+ * of an outer iteration is ONE launch over all volumes (each convolution pass per filter
+ * variant, each sweep, the update with its tolerance sums, div), with results bitwise those
+ * of each volume run alone; use_graph != 0 captures two outer iterations into one hipGraph
+ * and replays it.  Lexicographic sweeps with ls_tol == 0 and tol == 0 enqueue each
+ * volume's own launches over up to four lanes (a context and HIP stream each, volumes
+ * assigned by longest-processing-time), captured the same way.  Otherwise each volume runs
+ * the reference's host-checked loop in turn. */
 typedef struct tk_batch_s* tk_batch;
 int tk_batch_create(tk_ctx ctx, int n_vol, const int* dims3, tk_dtype dtype,
                     const tk_rsfe_params* p, tk_batch* out);
