@@ -468,24 +468,35 @@ __device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restr
     const long sX = (long)Y * Z;
     const int tid = threadIdx.x;
     Row<T> poA[NO], pbA[NB], poB[NO], pbB[NB];
-    auto fetch = [&](const T* base, int rows, int row0, Row<T>* out, int n) {
+    // Every lane issues its n loads on every path (a position outside the volume reads a
+    // clamped row, and the deposit writes zeros for it instead): with a fixed number of
+    // loads per step, and nothing touching their registers before the deposit, the
+    // compiler waits only for the set two steps old, not for every load in flight.
+    auto inside = [&](int rows, int row0, int n) {
+        unsigned m = 0;
+        for (int q = 0; q < n; ++q) {
+            const int e = tid + q * NT;
+            const int r = e / WV, c4 = e - r * WV;
+            const int j = row0 + r, k = kbase + c4 * V;
+            if (e < rows * WV && j >= 0 && j < Y && k >= 0 && k < Z) m |= 1u << q;
+        }
+        return m;
+    };
+    const unsigned in_o = inside(OJ, j0 - 2, NO), in_b = inside(RJ, j0 - 1, NB);
+    auto fetch = [&](const T* base, int row0, Row<T>* out, int n) {
 #pragma unroll
         for (int q = 0; q < n; ++q) {
             const int e = tid + q * NT;
-            Row<T> v = {};
-            if (e < rows * WV) {
-                const int r = e / WV, c4 = e - r * WV;
-                const int j = row0 + r, k = kbase + c4 * V;
-                if (j >= 0 && j < Y && k >= 0 && k < Z) v = ldrow(base + (long)j * Z + k);
-            }
-            out[q] = v;
+            const int r = e / WV, c4 = e - r * WV;
+            const int jc = min(max(row0 + r, 0), Y - 1), kc = min(max(kbase + c4 * V, 0), Z - V);
+            out[q] = ldrow(base + (long)jc * Z + kc);
         }
     };
-    auto deposit = [&](T* lds, int rows, const Row<T>* in, int n) {
+    auto deposit = [&](T* lds, int rows, const Row<T>* in, int n, unsigned m) {
 #pragma unroll
         for (int q = 0; q < n; ++q) {
             const int e = tid + q * NT;
-            if (e < rows * WV) strow(lds + e * V, in[q]);
+            if (e < rows * WV) strow(lds + e * V, (m >> q) & 1u ? in[q] : Row<T>{});
         }
     };
     // Each lane works on one 16-byte row of V voxels: the centre row, its four x/y
@@ -572,18 +583,21 @@ __device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restr
     // prologue: old planes ia-2 .. ia+1 and rhs planes ia-1, ia into LDS
     for (int p = ia - 2; p <= ia + 1; ++p)
         if (p >= 0 && p <= X - 1) {
-            fetch(src + (long)p * sX, OJ, j0 - 2, poA, NO);
-            deposit(O[p & 3], OJ, poA, NO);
+            fetch(src + (long)p * sX, j0 - 2, poA, NO);
+            deposit(O[p & 3], OJ, poA, NO, in_o);
         }
     for (int p = ia - 1; p <= ia; ++p) {
-        fetch(rhs + (long)p * sX, RJ, j0 - 1, pbA, NB);
-        deposit(Bp[p % 3], RJ, pbA, NB);
+        fetch(rhs + (long)p * sX, j0 - 1, pbA, NB);
+        deposit(Bp[p % 3], RJ, pbA, NB, in_b);
     }
-    // set A holds old plane ia+2 and rhs plane ia+1, set B old ia+3 and rhs ia+2
-    if (ia + 2 <= X - 1) fetch(src + (long)(ia + 2) * sX, OJ, j0 - 2, poA, NO);
-    fetch(rhs + (long)(ia + 1) * sX, RJ, j0 - 1, pbA, NB);
-    if (ia + 3 <= X - 1) fetch(src + (long)(ia + 3) * sX, OJ, j0 - 2, poB, NO);
-    if (ia + 2 <= X - 1) fetch(rhs + (long)(ia + 2) * sX, RJ, j0 - 1, pbB, NB);
+    // set A holds old plane ia+2 and rhs plane ia+1, set B old ia+3 and rhs ia+2 (plane
+    // indices past X-1 are clamped: those sets are fetched but never deposited)
+    const auto old_plane = [&](int p) { return src + (long)min(p, X - 1) * sX; };
+    const auto rhs_plane = [&](int p) { return rhs + (long)min(p, X - 1) * sX; };
+    fetch(old_plane(ia + 2), j0 - 2, poA, NO);
+    fetch(rhs_plane(ia + 1), j0 - 1, pbA, NB);
+    fetch(old_plane(ia + 3), j0 - 2, poB, NO);
+    fetch(rhs_plane(ia + 2), j0 - 1, pbB, NB);
     __syncthreads();
     red(ia - 1);
     red(ia);
@@ -592,19 +606,23 @@ __device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restr
     // refills the same register set with old plane p+4 and rhs plane p+3
     auto step = [&](int p, Row<T>* po, Row<T>* pb) {
         // slots refilled here were last read two planes ago (separated by barriers)
-        if (p + 2 <= X - 1) deposit(O[(p + 2) & 3], OJ, po, NO);
-        deposit(Bp[(p + 1) % 3], RJ, pb, NB);
-        if (p + 4 <= X - 1) fetch(src + (long)(p + 4) * sX, OJ, j0 - 2, po, NO);
-        if (p + 3 <= X - 1) fetch(rhs + (long)(p + 3) * sX, RJ, j0 - 1, pb, NB);
+        if (p + 2 <= X - 1) deposit(O[(p + 2) & 3], OJ, po, NO, in_o);
+        deposit(Bp[(p + 1) % 3], RJ, pb, NB, in_b);
+        fetch(old_plane(p + 4), j0 - 2, po, NO);
+        fetch(rhs_plane(p + 3), j0 - 1, pb, NB);
         __syncthreads();
         red(p + 1);
         __syncthreads();
         black(p);
     };
-    for (int p = ia; p < ib; p += 2) {
+    // both steps of the pair run on every trip, so each deposit has the other set's
+    // loads behind it on every path into it
+    int p = ia;
+    for (; p + 1 < ib; p += 2) {
         step(p, poA, pbA);
-        if (p + 1 < ib) step(p + 1, poB, pbB);
+        step(p + 1, poB, pbB);
     }
+    if (p < ib) step(p, poA, pbA);
     if (!gated) return;
     block_sum2(d2, o2);
     if (tid == 0) {
