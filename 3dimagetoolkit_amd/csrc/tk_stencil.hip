@@ -162,31 +162,50 @@ __device__ __forceinline__ T dz_win(const T* c, int e, int k, int Z, double h)
     return (c[e + 2] - c[e]) / (T)(2. * h);
 }
 
-// rows of f at the three lines needed for d/dx (or d/dy) of the row at index `idx`
-// along an axis of length n with line stride st: central -> (+1, -1); first -> (1, 0, 2);
-// last -> (n-1, n-2, n-3)
+// The rows of f that d/dx (or d/dy) of the row at index `idx` reads, along an axis of
+// length n with line stride st: central -> (idx+1, idx-1); first -> (0, 1, 2); last ->
+// (n-1, n-2, n-3).  Loading is split from the arithmetic so that a thread issues every
+// load of its row (all axes, all fields) before it uses any of them: a load followed by
+// its use inside a branch makes the thread wait for it before issuing the next axis.
+template <class T>
+struct DRows {
+    Row<T> r[3];
+    int c;  // plane_class(idx, n)
+};
+
+template <class T>
+__device__ __forceinline__ DRows<T> drow_load(const T* __restrict__ f, long v0, long st, int idx,
+                                              int n)
+{
+    DRows<T> d;
+    d.c = plane_class(idx, n);
+    const long base = v0 - (long)idx * st;
+    const long o0 = d.c == 1 ? v0 + st : (d.c == 0 ? base : base + (long)(n - 1) * st);
+    const long o1 = d.c == 1 ? v0 - st : (d.c == 0 ? base + st : base + (long)(n - 2) * st);
+    d.r[0] = ldrow(f + o0);
+    d.r[1] = ldrow(f + o1);
+    if (d.c != 1) d.r[2] = ldrow(f + (d.c == 0 ? base + 2 * st : base + (long)(n - 3) * st));
+    return d;
+}
+
+// central -> (f(+1) - f(-1)) / 2h; first -> (4 f(1) - 3 f(0) - f(2)) / 2h; last ->
+// (3 f(n-1) - 4 f(n-2) + f(n-3)) / 2h, each value through clampf
 template <class T, class Fn>
-__device__ __forceinline__ void drow(const T* __restrict__ f, long v0, long st, int idx, int n,
-                                     double h, Fn clampf, T* out)
+__device__ __forceinline__ void drow_eval(const DRows<T>& d, double h, Fn clampf, T* out)
 {
     constexpr int V = Vec16<T>::n;
-    const int c = plane_class(idx, n);
-    const long base = v0 - (long)idx * st;
-    if (c == 1) {
-        Row<T> p = ldrow(f + v0 + st), m = ldrow(f + v0 - st);
+    const Row<T>&a = d.r[0], &b = d.r[1], &c = d.r[2];
+    if (d.c == 1) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) out[e] = (clampf(p.v[e]) - clampf(m.v[e])) / (T)(2. * h);
-    } else if (c == 0) {
-        Row<T> a0 = ldrow(f + base), a1 = ldrow(f + base + st), a2 = ldrow(f + base + 2 * st);
+        for (int e = 0; e < V; ++e) out[e] = (clampf(a.v[e]) - clampf(b.v[e])) / (T)(2. * h);
+    } else if (d.c == 0) {
 #pragma unroll
         for (int e = 0; e < V; ++e)
-            out[e] = (4 * clampf(a1.v[e]) - 3 * clampf(a0.v[e]) - clampf(a2.v[e])) / (T)(2. * h);
+            out[e] = (4 * clampf(b.v[e]) - 3 * clampf(a.v[e]) - clampf(c.v[e])) / (T)(2. * h);
     } else {
-        Row<T> a1 = ldrow(f + base + (long)(n - 1) * st), a2 = ldrow(f + base + (long)(n - 2) * st),
-               a3 = ldrow(f + base + (long)(n - 3) * st);
 #pragma unroll
         for (int e = 0; e < V; ++e)
-            out[e] = (3 * clampf(a1.v[e]) - 4 * clampf(a2.v[e]) + clampf(a3.v[e])) / (T)(2. * h);
+            out[e] = (3 * clampf(a.v[e]) - 4 * clampf(b.v[e]) + clampf(c.v[e])) / (T)(2. * h);
     }
 }
 
@@ -213,19 +232,26 @@ __device__ __forceinline__ void sb_rows(T* __restrict__ phi, const T* __restrict
     auto cl = [&](T x) { return clampv(x, lo, hi); };
     TK_FOR_ROWS_IN(X, Y, Z, V, i, j, k0, v0, tx, ty, bdx, bdy, bx, by, gx, gy)
     {
-        T d[3][V];
-        drow(phi, v0, sX, i, X, hx, cl, d[0]);
-        drow(phi, v0, sY, j, Y, hy, cl, d[1]);
-        T c[V + 2];
+        T* bp[3] = {b0, b1, b2};
+        T* qp[3] = {q0, q1, q2};
+        const DRows<T> rx = drow_load(phi, v0, sX, i, X), ry = drow_load(phi, v0, sY, j, Y);
         const Row<T> cc = ldrow(phi + v0);
-        c[0] = k0 > 0 ? cl(phi[v0 - 1]) : (T)0;
-        c[V + 1] = k0 + V < Z ? cl(phi[v0 + V]) : (T)0;
+        const T zl = k0 > 0 ? phi[v0 - 1] : (T)0, zr = k0 + V < Z ? phi[v0 + V] : (T)0;
+        const Row<T> po = ldrow(phiold + v0);
+        const Row<T> gv = ldrow(g + v0);
+        Row<T> bb[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) bb[a] = ldrow(bp[a] + v0);
+        T d[3][V];
+        drow_eval(rx, hx, cl, d[0]);
+        drow_eval(ry, hy, cl, d[1]);
+        T c[V + 2];
+        c[0] = k0 > 0 ? cl(zl) : (T)0;
+        c[V + 1] = k0 + V < Z ? cl(zr) : (T)0;
 #pragma unroll
         for (int e = 0; e < V; ++e) c[e + 1] = cl(cc.v[e]);
 #pragma unroll
         for (int e = 0; e < V; ++e) d[2][e] = dz_win<T, V>(c, e, k0 + e, Z, hz);
-        const Row<T> po = ldrow(phiold + v0);
-        const Row<T> gv = ldrow(g + v0);
         Row<T> pc;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -234,24 +260,22 @@ __device__ __forceinline__ void sb_rows(T* __restrict__ phi, const T* __restrict
             s1 += dd * dd;
             s2 += (double)po.v[e] * (double)po.v[e];
         }
-        T* bp[3] = {b0, b1, b2};
-        T* qp[3] = {q0, q1, q2};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            Row<T> bb = ldrow(bp[a] + v0), qq;
+            Row<T> qq;
 #pragma unroll
             for (int e = 0; e < V; ++e) {
-                T b = bb.v[e];
+                T b = bb[a].v[e];
                 b += d[a][e];
                 const T m = fabs(b) - gv.v[e];
                 const T z = 0;
                 T dn = (T)((int)(b > 0) - (int)(b < 0)) * ((m < z) ? z : m);
                 b -= dn;
                 dn -= b;
-                bb.v[e] = b;
+                bb[a].v[e] = b;
                 qq.v[e] = dn;
             }
-            strow(bp[a] + v0, bb);
+            strow(bp[a] + v0, bb[a]);
             strow(qp[a] + v0, qq);
         }
         strow(phi + v0, pc);
@@ -293,26 +317,26 @@ __device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __re
     auto id = [](T x) { return x; };
     TK_FOR_ROWS_IN(X, Y, Z, V, i, j, k0, v0, tx, ty, bdx, bdy, bx, by, gx, gy)
     {
-        const int ci = plane_class(i, X), cj = plane_class(j, Y);
+        const DRows<T> rx = drow_load(f0, v0, sX, i, X), ry = drow_load(f1, v0, sY, j, Y);
+        const Row<T> cc = ldrow(f2 + v0);
+        const T zl = k0 > 0 ? f2[v0 - 1] : (T)0, zr = k0 + V < Z ? f2[v0 + V] : (T)0;
+        const int ci = rx.c, cj = ry.c;
         T tx[V], ty[V];
-        drow(f0, v0, sX, i, X, hx, id, tx);
+        drow_eval(rx, hx, id, tx);
         if (ci == 2 && cj == 1) {
-            // face i = X-1 (j, k inside): "/ 2. * (hx)" (image3d_imp.hxx:1092-1095)
-            const long base = v0 - (long)i * sX;
-            const Row<T> a1 = ldrow(f0 + base + (long)(X - 1) * sX),
-                         a2 = ldrow(f0 + base + (long)(X - 2) * sX),
-                         a3 = ldrow(f0 + base + (long)(X - 3) * sX);
+            // face i = X-1 (j, k inside): "/ 2. * (hx)" (image3d_imp.hxx:1092-1095); the
+            // rows X-1, X-2, X-3 are those of the last-plane derivative
 #pragma unroll
             for (int e = 0; e < V; ++e) {
                 const int k = k0 + e;
-                if (k >= 1 && k <= Z - 2) tx[e] = (3 * a1.v[e] - 4 * a2.v[e] + a3.v[e]) / (T)2. * (T)hx;
+                if (k >= 1 && k <= Z - 2)
+                    tx[e] = (3 * rx.r[0].v[e] - 4 * rx.r[1].v[e] + rx.r[2].v[e]) / (T)2. * (T)hx;
             }
         }
-        drow(f1, v0, sY, j, Y, hy, id, ty);
+        drow_eval(ry, hy, id, ty);
         T c[V + 2];
-        const Row<T> cc = ldrow(f2 + v0);
-        c[0] = k0 > 0 ? f2[v0 - 1] : (T)0;
-        c[V + 1] = k0 + V < Z ? f2[v0 + V] : (T)0;
+        c[0] = zl;
+        c[V + 1] = zr;
 #pragma unroll
         for (int e = 0; e < V; ++e) c[e + 1] = cc.v[e];
         Row<T> o;
