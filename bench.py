@@ -46,13 +46,13 @@ BYTES_PER_VOXEL_W = {
 # WRITE x1.0); keyed by (size, dtype, sweep order, kernel tag).
 TRAFFIC_SOURCE = "profiles/rsfe256_f32_traffic.md"
 _TRAFFIC_256_F32_RB = {
-    "conv_x1": 335705892,
-    "conv_z2": 402867767,
-    "conv_x2": 404442805,
-    "conv_y": 268556381,
-    "gs": 225188769,
-    "sb": 942213193,
-    "div": 322449399,
+    "conv_x1": 335710073,
+    "conv_z2": 402854007,
+    "conv_x2": 404448658,
+    "conv_y": 268556898,
+    "gs": 231320792,
+    "sb": 962540570,
+    "div": 326225325,
 }
 MEASURED_TRAFFIC = {(256, "f32", "rb", k): v for k, v in _TRAFFIC_256_F32_RB.items()}
 
