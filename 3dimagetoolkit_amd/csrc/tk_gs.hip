@@ -162,6 +162,7 @@ inline long boundary_count(int X, int Y, int Z)
 // ---- lexicographic pipelined tile wavefront --------------------------------------
 constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
 constexpr int kLexTZ = 32;  // tile depth in z
+constexpr int kLexBatch = 8;  // tile loads per lane in flight at once
 
 // one Neumann update of a voxel on the boundary (mask m != 0): per axis (p+ - q) on the
 // low face, (p- + q) on the high face, (p+ + p-) inside; then the interior expression with
@@ -207,18 +208,44 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
     }
     // load phi tile + halo and the rhs tile (rows along z are contiguous); the Neumann
     // tiles' halo positions outside the volume are never read and hold 0
+    // kLexBatch loads per lane are issued before any is written to LDS (a load written in
+    // the iteration that issues it is waited for before the next one goes out), and every
+    // lane issues all of them: an index past the end is clamped to the last element and its
+    // write skipped, so the compiler need not wait between conditional loads
     const int hz = tz + 2;
-    for (int e = lane; e < (tx + 2) * (ty + 2) * hz; e += 64) {
-        const int kk = e % hz, r = e / hz, jj = r % (ty + 2), ii = r / (ty + 2);
-        const int gi = i0 - 1 + ii, gj = j0 - 1 + jj, gk = k0 - 1 + kk;
-        if (NEU && (gi < 0 || gi >= X || gj < 0 || gj >= Y || gk < 0 || gk >= Z))
-            ph[(ii * HY + jj) * HZ + kk] = (T)0;
-        else
-            ph[(ii * HY + jj) * HZ + kk] = phi[gi * sX + gj * sY + gk];
+    const int nh = (tx + 2) * (ty + 2) * hz, nr = tx * ty * tz;
+    for (int e0 = lane; e0 < nh; e0 += 64 * kLexBatch) {
+        T v[kLexBatch];
+        int at[kLexBatch];
+        unsigned zero = 0;  // Neumann halo positions outside the volume hold 0
+#pragma unroll
+        for (int u = 0; u < kLexBatch; ++u) {
+            const int e = min(e0 + 64 * u, nh - 1);
+            const int kk = e % hz, r = e / hz, jj = r % (ty + 2), ii = r / (ty + 2);
+            const int gi = i0 - 1 + ii, gj = j0 - 1 + jj, gk = k0 - 1 + kk;
+            const bool out =
+                NEU && (gi < 0 || gi >= X || gj < 0 || gj >= Y || gk < 0 || gk >= Z);
+            v[u] = phi[out ? 0 : gi * sX + gj * sY + gk];
+            zero |= (unsigned)out << u;
+            at[u] = e0 + 64 * u < nh ? (ii * HY + jj) * HZ + kk : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kLexBatch; ++u)
+            if (at[u] >= 0) ph[at[u]] = (zero >> u) & 1u ? (T)0 : v[u];
     }
-    for (int e = lane; e < tx * ty * tz; e += 64) {
-        const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
-        rb[(ii * TY + jj) * TZ + kk] = rhs[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)];
+    for (int e0 = lane; e0 < nr; e0 += 64 * kLexBatch) {
+        T v[kLexBatch];
+        int at[kLexBatch];
+#pragma unroll
+        for (int u = 0; u < kLexBatch; ++u) {
+            const int e = min(e0 + 64 * u, nr - 1);
+            const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
+            v[u] = rhs[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)];
+            at[u] = e0 + 64 * u < nr ? (ii * TY + jj) * TZ + kk : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kLexBatch; ++u)
+            if (at[u] >= 0) rb[at[u]] = v[u];
     }
     __syncthreads();
     const int a = lane / TY, b = lane % TY;
