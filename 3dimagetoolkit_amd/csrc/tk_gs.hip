@@ -162,7 +162,7 @@ inline long boundary_count(int X, int Y, int Z)
 // ---- lexicographic pipelined tile wavefront --------------------------------------
 constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
 constexpr int kLexTZ = 32;  // tile depth in z
-constexpr int kLexBatch = 8;  // tile loads per lane in flight at once
+constexpr int kLexBatch = 32;  // tile loads per lane in flight at once
 
 // one Neumann update of a voxel on the boundary (mask m != 0): per axis (p+ - q) on the
 // low face, (p- + q) on the high face, (p+ + p-) inside; then the interior expression with
