@@ -254,6 +254,8 @@ struct tk_rsfe_s {
     tk_vol ccmask = nullptr;   // U8: the automatic extraction's black-and-white / component
     tk_vol ccneg = nullptr;    // U8: its negative black-and-white image (non-binary output)
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
+    hipEvent_t tol_ev = nullptr; // after the read-back of tol_dev, when work follows it
+    double* h_tol = nullptr;     // the device's address of the context's pinned h_acc
     int iter = 0;
     double tollerance = 1000;
     // phi's and alt's faces hold the Poisson boundary value: set by the first solve; the
@@ -374,14 +376,21 @@ int rsfe_forcing(tk_rsfe s)
 template <class T>
 int rsfe_update(tk_rsfe s, int* extracted_out);
 int rsfe_finish(tk_rsfe s, tk_rsfe_iter_stats* st, int sweeps, double ls_err, int used,
-                double res_plain, double res_guess, int extracted);
+                double res_plain, double res_guess, int extracted, int (*next)(tk_rsfe) = nullptr);
 
+// ahead (may be null): in, the forcing passes of this iteration are already enqueued; out,
+// those of the next one are (enqueued when `speculate`, between the read-back of this
+// iteration's tollerance and the wait for it: they write only phiold, the convolution
+// buffer and rhs, so a loop that then stops has lost nothing but their time)
 template <class T>
-int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
+int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true, bool* ahead = nullptr,
+                   bool speculate = false)
 {
     tk_ctx ctx = s->ctx;
     const tk_rsfe_params& p = s->p;
-    TK_TRY(rsfe_forcing<T>(s));
+    const bool ready = ahead && *ahead;
+    if (ahead) *ahead = false;
+    if (!ready) TK_TRY(rsfe_forcing<T>(s));
     // Poisson loop (rsfe_splitbregman_imp.hxx:1735-1797): bc value, then the sweeps from
     // the previous iterate (or from the learned guess when its residual is lower)
     const double chosen_bc = p.bc_dir ? p.a0 : 0.;
@@ -441,7 +450,11 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true)
     int extracted = 0;
     TK_TRY(rsfe_update<T>(s, &extracted));
     if (!sync) return TK_OK;
-    return rsfe_finish(s, st, sweeps + judge, ls_err, used, res_plain, res_guess, extracted);
+    const bool next = ahead && speculate;
+    TK_TRY(rsfe_finish(s, st, sweeps + judge, ls_err, used, res_plain, res_guess, extracted,
+                       next ? &rsfe_forcing<T> : nullptr));
+    if (next) *ahead = true;
+    return TK_OK;
 }
 
 template <class T>
@@ -499,14 +512,36 @@ int rsfe_update(tk_rsfe s, int* extracted_out)
     return TK_OK;
 }
 
-// the reference's loop condition needs the tollerance: read it back and wait; the stats
+// the tollerance sums into pinned host memory (a copy into it returns only once done, so
+// work enqueued after it would reach an idle device); the fence makes them visible to the
+// host by the time the kernel has finished
+__global__ void k_tol_to_host(const double* __restrict__ src, double* dst)
+{
+    if (threadIdx.x < 2) dst[threadIdx.x] = src[threadIdx.x];
+    __threadfence_system();
+}
+
+// the reference's loop condition needs the tollerance: read it back and wait (enqueueing
+// next(s) after the read-back, when given, so the device has work while the host waits);
+// the stats
 int rsfe_finish(tk_rsfe s, tk_rsfe_iter_stats* st, int sweeps, double ls_err, int used,
-                double res_plain, double res_guess, int extracted)
+                double res_plain, double res_guess, int extracted, int (*next)(tk_rsfe))
 {
     tk_ctx ctx = s->ctx;
-    TK_HIP(hipMemcpyAsync(ctx->h_acc, s->tol_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    TK_HIP(hipStreamSynchronize(ctx->stream));
+    if (next) {
+        if (!s->tol_ev) TK_HIP(hipEventCreateWithFlags(&s->tol_ev, hipEventDisableTiming));
+        if (!s->h_tol) TK_HIP(hipHostGetDevicePointer((void**)&s->h_tol, ctx->h_acc, 0));
+        hipLaunchKernelGGL(k_tol_to_host, dim3(1), dim3(64), 0, ctx->stream,
+                           (const double*)s->tol_dev, s->h_tol);
+        TK_LAUNCHED();
+        TK_HIP(hipEventRecord(s->tol_ev, ctx->stream));
+        TK_TRY(next(s));
+        TK_HIP(hipEventSynchronize(s->tol_ev));
+    } else {
+        TK_HIP(hipMemcpyAsync(ctx->h_acc, s->tol_dev, 2 * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+    }
     TK_TRY(timing_flush(ctx));
     const double h3 = s->h[0] * s->h[1] * s->h[2];
     s->tollerance = std::sqrt(h3 * ctx->h_acc[0]) / std::sqrt(h3 * ctx->h_acc[1]);
@@ -552,6 +587,7 @@ void free_state(tk_rsfe s)
     if (s->ccmask) tk_vol_free(s->ccmask);
     if (s->ccneg) tk_vol_free(s->ccneg);
     if (s->tol_dev) (void)hipFree(s->tol_dev);
+    if (s->tol_ev) (void)hipEventDestroy(s->tol_ev);
     if (s->flt) tk_filter_free(s->flt);
     delete s;
 }
@@ -773,10 +809,19 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
              "tk_rsfe_iterate: warm_start set but no network attached (tk_rsfe_set_warm_start)");
     int done = 0;
     const tk_rsfe_params& p = s->p;
+    // The next iteration's forcing passes go out before the wait for this one's tollerance
+    // when this call runs another iteration unless the tollerance stops it, and tol <= 0
+    // (then only an exactly stationary phi stops the loop, the one case in which rhs is left
+    // holding the forcing of an iteration that did not run); not with per-kernel timing,
+    // whose read-out would wait for them.  Nothing is carried over to the next call.
+    bool ahead = false;
     // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
     while (done < n_iters && s->tollerance > p.tol && s->iter < p.maxiter) {
-        int r = s->dtype == TK_F64 ? rsfe_iteration<double>(s, stats ? &stats[done] : nullptr)
-                                   : rsfe_iteration<float>(s, stats ? &stats[done] : nullptr);
+        const bool spec = done + 1 < n_iters && s->iter + 1 < p.maxiter && p.tol <= 0 &&
+                          !s->ctx->timing;
+        tk_rsfe_iter_stats* st = stats ? &stats[done] : nullptr;
+        int r = s->dtype == TK_F64 ? rsfe_iteration<double>(s, st, true, &ahead, spec)
+                                   : rsfe_iteration<float>(s, st, true, &ahead, spec);
         // dump every dumpfrequency iterations, not at maxiter (imp:1939-1951)
         const int t = s->iter;
         if (r == TK_OK && p.dumpfrequency != 0 && t % p.dumpfrequency == 0 && t != p.maxiter)

@@ -311,7 +311,9 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
    stats may be NULL, else has room for n_iters entries */
 int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done);
 tk_vol tk_rsfe_phi(tk_rsfe s);
-/* the right-hand side of the last outer iteration's Poisson step (valid until the next) */
+/* the right-hand side of the last outer iteration's Poisson step (valid until the next;
+ * with tol <= 0, a tk_rsfe_iterate call that stops on an exactly stationary phi leaves in it
+ * the forcing of the iteration it did not run, enqueued while the tollerance was read back) */
 tk_vol tk_rsfe_rhs(tk_rsfe s);
 /* R15: attach the learned warm start (created on the run's context for its dims).  The
    Poisson problem is linear, so the network that maps a right-hand side to its solution
