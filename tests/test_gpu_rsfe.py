@@ -223,3 +223,30 @@ def test_c0_s5_f32_redblack_vs_golden(tk, ctx):
     d = dice(phi >= 0.5, ref >= 0.5)
     say(f"s5 64^3 FP32 red-black: max|phi-ref| = {err:.3e}, Dice = {d:.5f}")
     assert d >= 0.98
+
+
+@pytest.mark.parametrize("dtype,order", [(np.float32, "redblack"), (np.float64, "lex")])
+def test_iterate_ahead_equals_one_call_per_iteration(tk, ctx, dtype, order):
+    """tk_rsfe_iterate(n) enqueues each next iteration's convolution passes before it waits
+    for the tollerance (tol <= 0, another iteration due in the same call); n calls of one
+    iteration each never do.  Both give the same phi, bit for bit, and the same stats."""
+    shape = (36, 28, 32)
+    u0, phi0 = tk.gen_standard(shape, 5, 1, 2)
+    gs = tk.TK_GS_REDBLACK if order == "redblack" else tk.TK_GS_LEX
+    p = tk.default_params(sigma=2.0, maxiter=50, tol=0.0, ls_steps=5, ls_tol=0.0, gs_order=gs)
+    runs = []
+    for _ in range(2):
+        vu = ctx.from_numpy(u0.astype(dtype))
+        tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+        vi = ctx.from_numpy(phi0.astype(dtype))
+        runs.append((tk.Rsfe(ctx, vu, vi, p), vu, vi))
+    a, b = runs[0][0], runs[1][0]
+    st_a = a.iterate(6)
+    st_b = [b.iterate(1)[0] for _ in range(6)]
+    phi_a, phi_b = a.phi(), b.phi()
+    a.close()
+    b.close()
+    say(f"ahead {order}: tollerances {[s['tollerance'] for s in st_a]}")
+    assert np.array_equal(phi_a, phi_b)
+    assert [s["sweeps"] for s in st_a] == [s["sweeps"] for s in st_b]
+    assert [s["tollerance"] for s in st_a] == [s["tollerance"] for s in st_b]
