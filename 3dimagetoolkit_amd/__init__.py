@@ -56,13 +56,17 @@ class RsfeParams(C.Structure):
                                             ("cc_init_variables", C.c_int),
                                             ("dumpfrequency", C.c_int),
                                             ("outputname", C.c_char * 256),
-                                            ("edge_detector_sigma", C.c_double)]
+                                            ("edge_detector_sigma", C.c_double),
+                                            ("ws_judge_sweeps", C.c_int),
+                                            ("ws_continue", C.c_int),
+                                            ("ws_coarse_sweeps", C.c_int),
+                                            ("verbosity", C.c_int)]
 
 
 class IterStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("ls_err", C.c_double), ("tollerance", C.c_double),
                 ("guess_used", C.c_int), ("res_plain", C.c_double), ("res_guess", C.c_double),
-                ("cc_extracted", C.c_int)]
+                ("cc_extracted", C.c_int), ("coarse_sweeps", C.c_int), ("tol_inf", C.c_double)]
 
 
 def _load():
@@ -144,6 +148,17 @@ def _load():
         "tk_nn_upsample2": (i, [vp, vp, i, i, i, i, i, i, vp, i, i]),
         "tk_nn_upsample2_bwd": (i, [vp, vp, i, i, i, i, i, i, vp, i, i]),
         "tk_nn_add_slice": (i, [vp, vp, i, i, vp, i, i, i, C.c_long]),
+        "tk_vol_op_scalar": (i, [vp, vp, i, d]),
+        "tk_vol_op_vol": (i, [vp, vp, i, vp]),
+        "tk_vol_get": (i, [vp, i, i, i, dp]),
+        "tk_vol_set": (i, [vp, i, i, i, d]),
+        "tk_scalarprod": (i, [vp, vp, vp, dp]),
+        "tk_norm1": (i, [vp, vp, dp]),
+        "tk_norminf": (i, [vp, vp, dp]),
+        "tk_norminf_diff": (i, [vp, vp, vp, dp, dp]),
+        "tk_vector_abs": (i, [vp, vp, vp, vp, vp]),
+        "tk_vol_restrict": (i, [vp, vp, vp]),
+        "tk_vol_prolong_add": (i, [vp, vp, vp, d]),
         "tk_nn_f32_to_bf16": (i, [vp, vp, C.c_long, vp]),
         "tk_nn_bf16_to_f32": (i, [vp, vp, C.c_long, vp]),
         "tk_unet_create": (i, [vp, i, i, i, u64, C.POINTER(vp)]),
@@ -485,7 +500,8 @@ class Rsfe:
         _check(lib.tk_rsfe_iterate(self.h, n, st, C.byref(done)))
         return [dict(sweeps=st[q].sweeps, ls_err=st[q].ls_err, tollerance=st[q].tollerance,
                      guess_used=st[q].guess_used, res_plain=st[q].res_plain,
-                     res_guess=st[q].res_guess, cc_extracted=st[q].cc_extracted)
+                     res_guess=st[q].res_guess, cc_extracted=st[q].cc_extracted,
+                     coarse_sweeps=st[q].coarse_sweeps, tol_inf=st[q].tol_inf)
                 for q in range(done.value)]
 
     def phi(self, dtype=np.float64):

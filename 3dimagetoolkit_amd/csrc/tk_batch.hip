@@ -244,9 +244,10 @@ int tk_batch_create(tk_ctx ctx, int n_vol, const int* dims3, tk_dtype dtype,
     TK_CHECK(dtype == TK_F32 || dtype == TK_F64, TK_ERR_DTYPE, "tk_batch_create: dtype");
     // the batch enqueues whole iterations without host waits (and captures them): the
     // automatic extraction reads its seed test back and dumps write files, per run
-    TK_CHECK(!p->auto_extract_conn_comp && p->dumpfrequency == 0, TK_ERR_UNSUPPORTED,
-             "tk_batch_create: auto_extract_conn_comp and dumpfrequency are per-run options "
-             "(tk_rsfe_*)");
+    TK_CHECK(!p->auto_extract_conn_comp && p->dumpfrequency == 0 && !p->verbosity,
+             TK_ERR_UNSUPPORTED,
+             "tk_batch_create: auto_extract_conn_comp, dumpfrequency and verbosity are per-run "
+             "options (tk_rsfe_*)");
     for (int v = 0; v < n_vol; ++v)
         TK_CHECK(dims3[3 * v] >= 3 && dims3[3 * v + 1] >= 3 && dims3[3 * v + 2] >= 3,
                  TK_ERR_SHAPE, "tk_batch_create: volume %d: every edge must be >= 3", v);

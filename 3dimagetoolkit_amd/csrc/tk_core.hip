@@ -253,6 +253,119 @@ __global__ void k_bw(const T* __restrict__ a, uint8_t* __restrict__ out, long n,
         out[i] = (a[i] < pivot) ? (uint8_t)(0 + neg) : (uint8_t)(1 - neg);
 }
 
+// ---- image3d elementwise operators (image3d_imp.hxx:187-345) -------------------------
+// a op= s with s already cast to T (static_cast<T>(addend)), in T arithmetic
+template <class T>
+__global__ void k_op_scalar(T* __restrict__ a, long n, T s, int op)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const T x = a[i];
+        a[i] = op == TK_OP_ADD ? x + s : op == TK_OP_SUB ? x - s : op == TK_OP_MUL ? x * s : x / s;
+    }
+}
+
+// a op= static_cast<T>(b) element by element
+template <class T, class S>
+__global__ void k_op_vol(T* __restrict__ a, const S* __restrict__ b, long n, int op)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const T x = a[i], y = (T)b[i];
+        a[i] = op == TK_OP_ADD ? x + y : op == TK_OP_SUB ? x - y : op == TK_OP_MUL ? x * y : x / y;
+    }
+}
+
+// per-block FP64 partials: MODE 0: sum a * static_cast<T>(b) (scalarprod,
+// image3d_imp.hxx:385-407); MODE 1: sum |a| (norm1, 1387-1398).  part[2 b] holds the sum.
+template <int MODE, class T, class S>
+__global__ __launch_bounds__(kRedThreads) void k_sum_partial(const T* __restrict__ a,
+                                                             const S* __restrict__ b, long n,
+                                                             double* __restrict__ part)
+{
+    double s1 = 0, s2 = 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        if constexpr (MODE == 0) s1 += (double)(a[i] * (T)b[i]);
+        else s1 += fabs((double)a[i]);
+    }
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = 0;
+    }
+}
+
+// norminf as image3d_imp.hxx:1427-1452: a running value starts at |x[0]| and takes |x[i]|
+// when x[i] exceeds it, so entries below it (negative ones among them) never count; the
+// result is max(|x[0]|, max_i x[i]) whatever the partition.  x = a - b (b may be null: x = a);
+// part[2 blk] = that maximum over the block's share, part[2 blk + 1] = the same for x = b.
+template <class T>
+__global__ __launch_bounds__(kRedThreads) void k_norminf_partial(const T* __restrict__ a,
+                                                                 const T* __restrict__ b, long n,
+                                                                 double* __restrict__ part)
+{
+    __shared__ double s1[kRedThreads], s2[kRedThreads];
+    // the difference is formed in T, as the temporary (phi - phiold) of the reference
+    const double b0 = b ? (double)b[0] : 0.0;
+    double m1 = fabs(b ? (double)(a[0] - b[0]) : (double)a[0]), m2 = fabs(b0);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const double y = b ? (double)b[i] : 0.0, x = b ? (double)(a[i] - b[i]) : (double)a[i];
+        if (m1 < x) m1 = fabs(x);
+        if (m2 < y) m2 = fabs(y);
+    }
+    s1[threadIdx.x] = m1;
+    s2[threadIdx.x] = m2;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_max_pairs(const double* __restrict__ part,
+                                                          int nb, double* __restrict__ out)
+{
+    __shared__ double s1[kRedThreads], s2[kRedThreads];
+    double m1 = part[0], m2 = part[1];
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        m1 = fmax(m1, part[2 * i]);
+        m2 = fmax(m2, part[2 * i + 1]);
+    }
+    s1[threadIdx.x] = m1;
+    s2[threadIdx.x] = m2;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1[0];
+        out[1] = s2[0];
+    }
+}
+
+// vector_abs (image3d_imp.hxx:892-943): sqrt of the sum of squares in T, 2 or 3 components
+template <class T>
+__global__ void k_vector_abs(const T* __restrict__ f0, const T* __restrict__ f1,
+                             const T* __restrict__ f2, T* __restrict__ res, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        T s = f0[i] * f0[i] + f1[i] * f1[i];
+        if (f2) s = s + f2[i] * f2[i];
+        res[i] = sqrt(s);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -781,6 +894,203 @@ int tk_im_to_black_and_white(tk_ctx ctx, tk_vol img, double threshold, int negat
                            (const float*)img->d, (uint8_t*)out->d, n, pivot, negative ? 1 : 0);
     }
     TK_LAUNCHED();
+    return TK_OK;
+}
+
+// ---- R1: image3d elementwise operators -------------------------------------------------
+int tk_vol_op_scalar(tk_ctx ctx, tk_vol a, int op, double s)
+{
+    CHECK_VOL(a);
+    CHECK_FP(a);
+    TK_CHECK(op >= TK_OP_ADD && op <= TK_OP_DIV, TK_ERR_ARG, "tk_vol_op_scalar: op");
+    const long n = a->n();
+    const unsigned nb = blocks_for(n, 256, 8192);
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL(k_op_scalar<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)a->d, n, s, op);
+    else
+        hipLaunchKernelGGL(k_op_scalar<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (float*)a->d, n, (float)s, op);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_vol_op_vol(tk_ctx ctx, tk_vol a, int op, tk_vol b)
+{
+    CHECK_VOL(a);
+    CHECK_VOL(b);
+    CHECK_FP(a);
+    CHECK_FP(b);
+    TK_CHECK(op >= TK_OP_ADD && op <= TK_OP_DIV, TK_ERR_ARG, "tk_vol_op_vol: op");
+    TK_CHECK(same_dims(a, b), TK_ERR_SHAPE, "tk_vol_op_vol: dimensions must agree");
+    const long n = a->n();
+    const unsigned nb = blocks_for(n, 256, 8192);
+#define TK_OPV(T, S)                                                                          \
+    hipLaunchKernelGGL((k_op_vol<T, S>), dim3(nb), dim3(256), 0, ctx->stream, (T*)a->d,       \
+                       (const S*)b->d, n, op)
+    if (a->dtype == TK_F64) {
+        if (b->dtype == TK_F64) TK_OPV(double, double);
+        else TK_OPV(double, float);
+    } else {
+        if (b->dtype == TK_F64) TK_OPV(float, double);
+        else TK_OPV(float, float);
+    }
+#undef TK_OPV
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+static int sum_final(tk_ctx ctx, int nb, double* out)
+{
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(1024), 0, ctx->stream, ctx->acc + 64, nb,
+                       ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->h_acc[0];
+    return TK_OK;
+}
+
+int tk_scalarprod(tk_ctx ctx, tk_vol a, tk_vol b, double* out)
+{
+    CHECK_VOL(a);
+    CHECK_VOL(b);
+    CHECK_FP(a);
+    CHECK_FP(b);
+    TK_CHECK(out, TK_ERR_ARG, "tk_scalarprod: null out");
+    TK_CHECK(same_dims(a, b), TK_ERR_SHAPE, "tk_scalarprod: dimensions must agree");
+    const long n = a->n();
+    const int nb = (int)blocks_for(n, kRedThreads, kRedBlocks);
+    double* part = ctx->acc + 64;
+#define TK_DOT(T, S)                                                                          \
+    hipLaunchKernelGGL((k_sum_partial<0, T, S>), dim3(nb), dim3(kRedThreads), 0, ctx->stream,  \
+                       (const T*)a->d, (const S*)b->d, n, part)
+    if (a->dtype == TK_F64) {
+        if (b->dtype == TK_F64) TK_DOT(double, double);
+        else TK_DOT(double, float);
+    } else {
+        if (b->dtype == TK_F64) TK_DOT(float, double);
+        else TK_DOT(float, float);
+    }
+#undef TK_DOT
+    TK_LAUNCHED();
+    return sum_final(ctx, nb, out);
+}
+
+int tk_norm1(tk_ctx ctx, tk_vol a, double* out)
+{
+    CHECK_VOL(a);
+    CHECK_FP(a);
+    TK_CHECK(out, TK_ERR_ARG, "tk_norm1: null out");
+    const long n = a->n();
+    const int nb = (int)blocks_for(n, kRedThreads, kRedBlocks);
+    double* part = ctx->acc + 64;
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL((k_sum_partial<1, double, double>), dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const double*)a->d, (const double*)nullptr, n, part);
+    else
+        hipLaunchKernelGGL((k_sum_partial<1, float, float>), dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const float*)a->d, (const float*)nullptr, n, part);
+    TK_LAUNCHED();
+    return sum_final(ctx, nb, out);
+}
+
+int tk_norminf_diff(tk_ctx ctx, tk_vol a, tk_vol b, double* out_diff, double* out_b)
+{
+    CHECK_VOL(a);
+    CHECK_FP(a);
+    TK_CHECK(out_diff, TK_ERR_ARG, "tk_norminf_diff: null out");
+    if (b) {
+        CHECK_VOL(b);
+        TK_CHECK(same_dims(a, b), TK_ERR_SHAPE, "tk_norminf_diff: dimensions must agree");
+        TK_CHECK(a->dtype == b->dtype, TK_ERR_DTYPE, "tk_norminf_diff: dtypes differ");
+    }
+    const long n = a->n();
+    const int nb = (int)blocks_for(n, kRedThreads, kRedBlocks);
+    double* part = ctx->acc + 64;
+    if (a->dtype == TK_F64)
+        hipLaunchKernelGGL(k_norminf_partial<double>, dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const double*)a->d,
+                           b ? (const double*)b->d : nullptr, n, part);
+    else
+        hipLaunchKernelGGL(k_norminf_partial<float>, dim3(nb), dim3(kRedThreads), 0,
+                           ctx->stream, (const float*)a->d, b ? (const float*)b->d : nullptr, n,
+                           part);
+    TK_LAUNCHED();
+    hipLaunchKernelGGL(k_max_pairs, dim3(1), dim3(kRedThreads), 0, ctx->stream, part, nb,
+                       ctx->acc);
+    TK_LAUNCHED();
+    TK_HIP(hipMemcpyAsync(ctx->h_acc, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    *out_diff = ctx->h_acc[0];
+    if (out_b) *out_b = b ? ctx->h_acc[1] : 0.0;
+    return TK_OK;
+}
+
+int tk_norminf(tk_ctx ctx, tk_vol a, double* out)
+{
+    return tk_norminf_diff(ctx, a, nullptr, out, nullptr);
+}
+
+int tk_vector_abs(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol res)
+{
+    CHECK_VOL(f0);
+    CHECK_VOL(f1);
+    CHECK_VOL(res);
+    CHECK_FP(res);
+    TK_CHECK(same_dims(f0, f1) && same_dims(f0, res) && (!f2 || same_dims(f0, f2)),
+             TK_ERR_SHAPE, "tk_vector_abs: dimensions of the components must agree");
+    TK_CHECK(f0->dtype == res->dtype && f1->dtype == res->dtype &&
+                 (!f2 || f2->dtype == res->dtype),
+             TK_ERR_DTYPE, "tk_vector_abs: components and result of one dtype");
+    const long n = res->n();
+    const unsigned nb = blocks_for(n, 256, 8192);
+    if (res->dtype == TK_F64)
+        hipLaunchKernelGGL(k_vector_abs<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)f0->d, (const double*)f1->d,
+                           f2 ? (const double*)f2->d : nullptr, (double*)res->d, n);
+    else
+        hipLaunchKernelGGL(k_vector_abs<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)f0->d, (const float*)f1->d,
+                           f2 ? (const float*)f2->d : nullptr, (float*)res->d, n);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+static int element_ptr(tk_vol v, int i, int j, int k, void** p)
+{
+    CHECK_VOL(v);
+    TK_CHECK(i >= 0 && i < v->nx && j >= 0 && j < v->ny && k >= 0 && k < v->nz, TK_ERR_ARG,
+             "element (%d, %d, %d) outside the volume", i, j, k);
+    const long off = ((long)i * v->ny + j) * v->nz + k;
+    *p = (char*)v->d + off * dtype_size(v->dtype);
+    return TK_OK;
+}
+
+int tk_vol_get(tk_vol v, int i, int j, int k, double* out)
+{
+    void* p = nullptr;
+    TK_TRY(element_ptr(v, i, j, k, &p));
+    TK_CHECK(out, TK_ERR_ARG, "tk_vol_get: null out");
+    union { double d; float f; uint8_t u; } x;
+    TK_HIP(hipMemcpyAsync(&x, p, dtype_size(v->dtype), hipMemcpyDeviceToHost, v->ctx->stream));
+    TK_HIP(hipStreamSynchronize(v->ctx->stream));
+    *out = v->dtype == TK_F64 ? x.d : v->dtype == TK_F32 ? (double)x.f : (double)x.u;
+    return TK_OK;
+}
+
+int tk_vol_set(tk_vol v, int i, int j, int k, double value)
+{
+    void* p = nullptr;
+    TK_TRY(element_ptr(v, i, j, k, &p));
+    union { double d; float f; uint8_t u; } x;
+    if (v->dtype == TK_F64) x.d = value;
+    else if (v->dtype == TK_F32) x.f = (float)value;
+    else x.u = (uint8_t)value;
+    TK_HIP(hipMemcpyAsync(p, &x, dtype_size(v->dtype), hipMemcpyHostToDevice, v->ctx->stream));
+    TK_HIP(hipStreamSynchronize(v->ctx->stream));
     return TK_OK;
 }
 

@@ -163,6 +163,9 @@ inline long boundary_count(int X, int Y, int Z)
 constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
 constexpr int kLexTZ = 32;  // tile depth in z
 constexpr int kLexBatch = 32;  // tile loads per lane in flight at once
+// the staging loop records which of a lane's kLexBatch loads fell outside the volume as one
+// bit each of a 32-bit mask
+static_assert(kLexBatch <= 32, "kLexBatch loads per lane must fit the 32-bit out-of-volume mask");
 
 // one Neumann update of a voxel on the boundary (mask m != 0): per axis (p+ - q) on the
 // low face, (p- + q) on the high face, (p+ + p-) inside; then the interior expression with

@@ -251,6 +251,13 @@ struct tk_rsfe_s {
     tk_unet net = nullptr;     // learned warm start (same context), or none
     tk_vol guess = nullptr;    // its guess
     tk_vol resid = nullptr;    // the residual field it reads
+    int ws_f = 1;              // the network runs on the run's grid coarsened ws_f times
+    tk_vol cresid = nullptr;   // ws_f > 1: the residual restricted to the network's grid
+    tk_vol cguess = nullptr;   // ws_f > 1: the network's correction on its grid
+    tk_vol cnet = nullptr;     // ws_f > 1: the coarse residual the network reads
+    tk_vol vaux = nullptr;     // verbosity: phi before the last sweep (aux, imp:1774-1777)
+    int it_coarse = 0;         // coarse sweeps of the current iteration
+    double it_tol_inf = -1;    // verbosity: norminf ratio of the current iteration
     tk_vol ccmask = nullptr;   // U8: the automatic extraction's black-and-white / component
     tk_vol ccneg = nullptr;    // U8: its negative black-and-white image (non-binary output)
     double* tol_dev = nullptr; // device: sum (phi-phiold)^2, sum phiold^2 of the last iteration
@@ -389,6 +396,7 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true, bool* ah
     tk_ctx ctx = s->ctx;
     const tk_rsfe_params& p = s->p;
     const bool ready = ahead && *ahead;
+    s->it_coarse = 0;
     if (ahead) *ahead = false;
     if (!ready) TK_TRY(rsfe_forcing<T>(s));
     // Poisson loop (rsfe_splitbregman_imp.hxx:1735-1797): bc value, then the sweeps from
@@ -401,7 +409,7 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true, bool* ah
         // The residual weights high frequencies (the Laplacian amplifies them), which a few
         // sweeps remove, while the sweep count is set by the smooth error: both candidates
         // get kJudge sweeps before their residuals are compared; all of them are counted.
-        constexpr int kJudge = 16;
+        const int kJudge = p.ws_judge_sweeps;
         const int pre = p.ws_presweeps > 0 ? p.ws_presweeps : 0;
         for (int round = 0; round < p.warm_start; ++round) {
             if (pre > 0) {
@@ -412,8 +420,30 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true, bool* ah
             }
             TK_TRY(poisson_set_boundary(ctx, s->phi, chosen_bc));
             TK_TRY(tk_poisson_residual_field(ctx, s->phi, s->rhs, s->resid));
-            TK_TRY(tk_unet_predict(s->net, s->resid, s->guess));
-            TK_TRY(launch_add<T>(ctx, s->guess, s->phi));
+            if (s->ws_f > 1) {
+                // learned coarse-grid correction: the residual averaged over ws_f^3 blocks,
+                // the network's correction on that grid, interpolated back onto phi
+                TK_TRY(tk_vol_restrict(ctx, s->resid, s->cresid));
+                if (p.ws_coarse_sweeps > 0) {
+                    // sweeps from zero on the coarse problem (a dt of 1e30 relaxes it with
+                    // omega 1; the sweep's fixed point does not depend on dt), then the
+                    // network's estimate of what they leave, read from their residual
+                    TK_TRY(tk_vol_fill(s->cguess, 0.0));
+                    TK_TRY(tk_poisson_dir_sweeps(ctx, s->cguess, s->cresid, 1e30, 0.0,
+                                                 TK_GS_REDBLACK, p.ws_coarse_sweeps));
+                    s->it_coarse += p.ws_coarse_sweeps;
+                    TK_TRY(tk_poisson_residual_field(ctx, s->cguess, s->cresid, s->cnet));
+                    TK_TRY(tk_unet_predict(s->net, s->cnet, s->cnet));
+                    TK_TRY(launch_add<T>(ctx, s->cguess, s->cnet));
+                } else {
+                    TK_TRY(tk_unet_predict(s->net, s->cresid, s->cguess));
+                }
+                TK_TRY(tk_vol_copy(s->guess, s->phi));
+                TK_TRY(tk_vol_prolong_add(ctx, s->cguess, s->guess, 1.0));
+            } else {
+                TK_TRY(tk_unet_predict(s->net, s->resid, s->guess));
+                TK_TRY(launch_add<T>(ctx, s->guess, s->phi));
+            }
             TK_TRY(poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, kJudge,
                                          0, 0, s->alt ? &s->alt : nullptr, nullptr, nullptr));
             TK_TRY(poisson_dir_solve_alt(ctx, s->guess, s->rhs, p.dt, chosen_bc, p.gs_order,
@@ -426,29 +456,66 @@ int rsfe_iteration(tk_rsfe s, tk_rsfe_iter_stats* st, bool sync = true, bool* ah
                 res_guess = rg;
             }
             judge += 2 * kJudge;
-            if (!(rg < rp)) break;
+            if (!(rg < rp)) {
+                // the plain candidate continues; with ws_continue the next round proposes
+                // again from it
+                if (!p.ws_continue) break;
+                if (p.ls_res_tol > 0 && rp <= p.ls_res_tol) break;
+                continue;
+            }
             TK_TRY(tk_vol_copy(s->phi, s->guess));
             ++used;
             // the ls_res_tol gate may already hold
             if (p.ls_res_tol > 0 && rg <= p.ls_res_tol) break;
         }
     }
+    // the fixed-count sweeps (ls_tol == 0) in two calls when the verbose record wants the
+    // relative update of the last sweep: aux = phi before it (imp:1774-1777, 1786-1793)
+    const bool last_err = p.verbosity && p.ls_tol == 0 && p.ls_res_tol <= 0 && p.ls_steps >= 1;
+    auto plain = [&](int steps, int* sw, double* err) {
+        if (p.neumann)
+            return poisson_neu_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, steps,
+                                         p.ls_tol, p.max_sweeps, sw, err);
+        return poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, steps,
+                                     p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, sw, err,
+                                     s->faces_set);
+    };
     int r;
-    if (p.neumann)
-        r = poisson_neu_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
-                                  p.ls_tol, p.max_sweeps, &sweeps, &ls_err);
-    else if (p.ls_res_tol > 0)
+    if (p.ls_res_tol > 0 && !p.neumann)
         r = poisson_solve_residual_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order,
                                        p.ls_res_tol, p.res_check_every > 0 ? p.res_check_every : 64,
                                        p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps, &ls_err);
-    else
-        r = poisson_dir_solve_alt(ctx, s->phi, s->rhs, p.dt, chosen_bc, p.gs_order, p.ls_steps,
-                                  p.ls_tol, p.max_sweeps, s->alt ? &s->alt : nullptr, &sweeps,
-                                  &ls_err, s->faces_set);
+    else if (last_err) {
+        int s1 = 0, s2 = 0;
+        double e1 = -1, e2 = -1;
+        r = p.ls_steps > 1 ? plain(p.ls_steps - 1, &s1, &e1) : TK_OK;
+        if (r == TK_OK) {
+            if (!s->vaux)
+                TK_TRY(tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], s->dtype,
+                                    &s->vaux));
+            TK_TRY(tk_vol_copy(s->vaux, s->phi));
+            r = plain(1, &s2, &e2);
+        }
+        sweeps = s1 + s2;
+        if (r == TK_OK) {
+            double dn = 0, an = 0;
+            TK_TRY(tk_norm_l2_diff(ctx, s->phi, s->vaux, &dn));
+            TK_TRY(tk_norm_l2(ctx, s->vaux, &an));
+            ls_err = dn / an;
+        }
+    } else
+        r = plain(p.ls_steps, &sweeps, &ls_err);
     if (r != TK_OK && r != TK_ERR_SWEEPS) return r;
     if (!p.neumann && chosen_bc >= p.a0 && chosen_bc <= p.b0) s->faces_set = true;
     int extracted = 0;
     TK_TRY(rsfe_update<T>(s, &extracted));
+    s->it_tol_inf = -1;
+    if (p.verbosity) {
+        // (phi - phiold).norminf() / phiold.norminf() after cut_phi (imp:1835-1844)
+        double dinf = 0, oinf = 0;
+        TK_TRY(tk_norminf_diff(ctx, s->phi, s->phiold, &dinf, &oinf));
+        s->it_tol_inf = dinf / oinf;
+    }
     if (!sync) return TK_OK;
     const bool next = ahead && speculate;
     TK_TRY(rsfe_finish(s, st, sweeps + judge, ls_err, used, res_plain, res_guess, extracted,
@@ -553,6 +620,8 @@ int rsfe_finish(tk_rsfe s, tk_rsfe_iter_stats* st, int sweeps, double ls_err, in
         st->res_plain = res_plain;
         st->res_guess = res_guess;
         st->cc_extracted = extracted;
+        st->coarse_sweeps = s->it_coarse;
+        st->tol_inf = s->it_tol_inf;
     }
     return TK_OK;
 }
@@ -584,6 +653,10 @@ void free_state(tk_rsfe s)
     if (s->alt) (void)hipFree(s->alt);
     if (s->guess) tk_vol_free(s->guess);
     if (s->resid) tk_vol_free(s->resid);
+    if (s->cresid) tk_vol_free(s->cresid);
+    if (s->cguess) tk_vol_free(s->cguess);
+    if (s->cnet) tk_vol_free(s->cnet);
+    if (s->vaux) tk_vol_free(s->vaux);
     if (s->ccmask) tk_vol_free(s->ccmask);
     if (s->ccneg) tk_vol_free(s->ccneg);
     if (s->tol_dev) (void)hipFree(s->tol_dev);
@@ -693,6 +766,10 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     std::memset(p->outputname, 0, sizeof p->outputname);
     std::strcpy(p->outputname, "result");
     p->edge_detector_sigma = 0;
+    p->ws_judge_sweeps = 16;
+    p->ws_continue = 0;
+    p->ws_coarse_sweeps = 0;
+    p->verbosity = 0;
     return TK_OK;
 }
 
@@ -705,6 +782,11 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     TK_CHECK(u0->nx >= 3 && u0->ny >= 3 && u0->nz >= 3, TK_ERR_UNSUPPORTED,
              "tk_rsfe_create: 3-D volumes with every edge >= 3 only");
     TK_CHECK(p->warm_start >= 0 && p->warm_start <= 64, TK_ERR_ARG, "tk_rsfe_create: warm_start");
+    TK_CHECK(p->ws_judge_sweeps >= 1 && p->ws_judge_sweeps <= 4096 &&
+                 (p->ws_continue == 0 || p->ws_continue == 1),
+             TK_ERR_ARG, "tk_rsfe_create: ws_judge_sweeps in [1, 4096], ws_continue 0 or 1");
+    TK_CHECK(p->ws_coarse_sweeps >= 0 && (p->verbosity == 0 || p->verbosity == 1), TK_ERR_ARG,
+             "tk_rsfe_create: ws_coarse_sweeps >= 0, verbosity 0 or 1");
     TK_CHECK(p->gs_order == TK_GS_LEX || p->gs_order == TK_GS_REDBLACK, TK_ERR_ARG,
              "tk_rsfe_create: gs_order");
     TK_CHECK(p->acyclic == 0 || p->acyclic == 1, TK_ERR_ARG, "tk_rsfe_create: acyclic");
@@ -811,8 +893,9 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
     const tk_rsfe_params& p = s->p;
     // The next iteration's forcing passes go out before the wait for this one's tollerance
     // when this call runs another iteration unless the tollerance stops it, and tol <= 0
-    // (then only an exactly stationary phi stops the loop, the one case in which rhs is left
-    // holding the forcing of an iteration that did not run); not with per-kernel timing,
+    // (then only a tollerance of exactly 0, a stationary phi, or NaN, 0/0 with phi and
+    // phiold zero everywhere, stops the loop: the two cases in which rhs is left holding the
+    // forcing of an iteration that did not run); not with per-kernel timing,
     // whose read-out would wait for them.  Nothing is carried over to the next call.
     bool ahead = false;
     // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
@@ -1163,8 +1246,22 @@ int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
              "tk_rsfe_set_warm_start: the network must live on the run's context");
     int d[3];
     unet_dims(net, d);
-    TK_CHECK(d[0] == s->X && d[1] == s->Y && d[2] == s->Z, TK_ERR_SHAPE,
-             "tk_rsfe_set_warm_start: network dims differ from the run's");
+    const int f = d[0] > 0 ? s->X / d[0] : 0;
+    TK_CHECK(f >= 1 && s->X == f * d[0] && s->Y == f * d[1] && s->Z == f * d[2], TK_ERR_SHAPE,
+             "tk_rsfe_set_warm_start: the run's dims must be the network's times one integer "
+             "factor on every axis");
+    if (f > 1 && (!s->cresid || s->cresid->nx != d[0] || s->cresid->ny != d[1] ||
+                  s->cresid->nz != d[2])) {
+        if (s->cresid) tk_vol_free(s->cresid);
+        if (s->cguess) tk_vol_free(s->cguess);
+        if (s->cnet) tk_vol_free(s->cnet);
+        s->cresid = s->cguess = s->cnet = nullptr;
+        const double H[3] = {s->h[0] * f, s->h[1] * f, s->h[2] * f};
+        TK_TRY(tk_vol_alloc(s->ctx, d[0], d[1], d[2], H[0], H[1], H[2], s->dtype, &s->cresid));
+        TK_TRY(tk_vol_alloc(s->ctx, d[0], d[1], d[2], H[0], H[1], H[2], s->dtype, &s->cguess));
+        TK_TRY(tk_vol_alloc(s->ctx, d[0], d[1], d[2], H[0], H[1], H[2], s->dtype, &s->cnet));
+    }
+    s->ws_f = f;
     if (!s->guess)
         TK_TRY(tk_vol_alloc(s->ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], s->dtype,
                             &s->guess));

@@ -51,6 +51,66 @@ __global__ void k_unet_output(const float* __restrict__ out, long n, double s, T
         dst[v] = (T)((double)out[v] * s);
 }
 
+// coarse[c] = mean of the f x f x f fine voxels of coarse voxel c (fine voxels f*c .. f*c+f-1
+// per axis); one thread per coarse voxel, FP64 sum
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict_avg(const T* __restrict__ fine,
+                                                      T* __restrict__ coarse, int cx, int cy,
+                                                      int cz, int f)
+{
+    const long nc = (long)cx * cy * cz;
+    const int FY = cy * f, FZ = cz * f;
+    const double inv = 1.0 / ((double)f * f * f);
+    for (long c = blockIdx.x * 256L + threadIdx.x; c < nc; c += (long)gridDim.x * 256) {
+        const int k = (int)(c % cz), j = (int)((c / cz) % cy), i = (int)(c / ((long)cy * cz));
+        double s = 0;
+        for (int a = 0; a < f; ++a)
+            for (int b = 0; b < f; ++b) {
+                const T* row = fine + ((long)(i * f + a) * FY + (j * f + b)) * FZ + (long)k * f;
+                for (int e = 0; e < f; ++e) s += (double)row[e];
+            }
+        coarse[c] = (T)(s * inv);
+    }
+}
+
+// interpolation weight along one axis: fine index q of a cell-centred coarse grid of n
+// cells, f fine voxels per cell; constant beyond the first and last cell centres
+__device__ __forceinline__ void prolong_w(int q, int f, int n, int& c0, double& w)
+{
+    double x = (q + 0.5) / f - 0.5;
+    x = fmin(fmax(x, 0.0), (double)(n - 1));
+    c0 = min((int)x, max(n - 2, 0));
+    w = n > 1 ? x - c0 : 0.0;
+}
+
+// fine[v] += gain * trilinear(coarse) at fine voxel v
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add(const T* __restrict__ coarse,
+                                                     T* __restrict__ fine, int cx, int cy, int cz,
+                                                     int f, double gain)
+{
+    const int FX = cx * f, FY = cy * f, FZ = cz * f;
+    const long nf = (long)FX * FY * FZ;
+    const long sy = cz, sx = (long)cy * cz;
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < nf; v += (long)gridDim.x * 256) {
+        const int k = (int)(v % FZ), j = (int)((v / FZ) % FY), i = (int)(v / ((long)FY * FZ));
+        int a, b, c;
+        double wa, wb, wc;
+        prolong_w(i, f, cx, a, wa);
+        prolong_w(j, f, cy, b, wb);
+        prolong_w(k, f, cz, c, wc);
+        const int da = cx > 1 ? 1 : 0, db = cy > 1 ? 1 : 0, dc = cz > 1 ? 1 : 0;
+        const T* p = coarse + a * sx + b * sy + c;
+        auto at = [&](int x, int y, int z) { return (double)p[x * da * sx + y * db * sy + z * dc]; };
+        const double v00 = at(0, 0, 0) * (1 - wc) + at(0, 0, 1) * wc;
+        const double v01 = at(0, 1, 0) * (1 - wc) + at(0, 1, 1) * wc;
+        const double v10 = at(1, 0, 0) * (1 - wc) + at(1, 0, 1) * wc;
+        const double v11 = at(1, 1, 0) * (1 - wc) + at(1, 1, 1) * wc;
+        const double v0 = v00 * (1 - wb) + v01 * wb, v1 = v10 * (1 - wb) + v11 * wb;
+        fine[v] = (T)((double)fine[v] + gain * (v0 * (1 - wa) + v1 * wa));
+    }
+}
+
 // MSE: partial sums of (out - t)^2 per block; g[v] = bf16(2 (out - t) / n)
 template <class T>
 __global__ __launch_bounds__(256) void k_unet_loss(const float* __restrict__ out,
@@ -558,6 +618,57 @@ int tk_unet_get_output(tk_unet n, float* host)
     TK_CHECK(n && host, TK_ERR_ARG, "tk_unet_get_output: argument");
     TK_TRY(tk_ctx_sync(n->ctx));
     TK_HIP(hipMemcpy(host, n->out, 4 * (size_t)n->nlev[1], hipMemcpyDeviceToHost));
+    return TK_OK;
+}
+
+// ---- coarse-grid application of the network (no reference counterpart) -------------
+static int coarse_factor(tk_vol fine, tk_vol coarse, const char* what, int* f)
+{
+    TK_CHECK(fine && coarse && fine->d && coarse->d, TK_ERR_ARG, "%s: null volume", what);
+    TK_CHECK(fine->dtype == coarse->dtype && (fine->dtype == TK_F32 || fine->dtype == TK_F64),
+             TK_ERR_DTYPE, "%s: both volumes F32 or both F64", what);
+    TK_CHECK(coarse->nx > 0 && fine->nx % coarse->nx == 0, TK_ERR_SHAPE, "%s: dims", what);
+    const int q = fine->nx / coarse->nx;
+    TK_CHECK(q >= 1 && fine->ny == q * coarse->ny && fine->nz == q * coarse->nz, TK_ERR_SHAPE,
+             "%s: fine dims must be the same integer multiple of the coarse dims on every axis",
+             what);
+    *f = q;
+    return TK_OK;
+}
+
+int tk_vol_restrict(tk_ctx ctx, tk_vol fine, tk_vol coarse)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_vol_restrict: null context");
+    int f = 0;
+    TK_TRY(coarse_factor(fine, coarse, "tk_vol_restrict", &f));
+    const unsigned nb = nb_for(coarse->n());
+    if (fine->dtype == TK_F64)
+        hipLaunchKernelGGL(k_restrict_avg<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)fine->d, (double*)coarse->d, coarse->nx, coarse->ny,
+                           coarse->nz, f);
+    else
+        hipLaunchKernelGGL(k_restrict_avg<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)fine->d, (float*)coarse->d, coarse->nx, coarse->ny,
+                           coarse->nz, f);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
+int tk_vol_prolong_add(tk_ctx ctx, tk_vol coarse, tk_vol fine, double gain)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_vol_prolong_add: null context");
+    int f = 0;
+    TK_TRY(coarse_factor(fine, coarse, "tk_vol_prolong_add", &f));
+    const unsigned nb = nb_for(fine->n());
+    if (fine->dtype == TK_F64)
+        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const double*)coarse->d, (double*)fine->d, coarse->nx, coarse->ny,
+                           coarse->nz, f, gain);
+    else
+        hipLaunchKernelGGL(k_prolong_add<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float*)coarse->d, (float*)fine->d, coarse->nx, coarse->ny,
+                           coarse->nz, f, gain);
+    TK_LAUNCHED();
     return TK_OK;
 }
 

@@ -15,14 +15,19 @@ with the largest total time, from HIP events recorded around every launch on the
 context's stream in a second pass over K steps.  cpu_baseline times the CPU FP64
 oracle (OpenMP) on the same workload for a bounded number of iterations.
 """
-import argparse
-import importlib
-import json
 import os
-import sys
-import time
 
-import numpy as np
+# the CPU baseline's OpenMP threads stay on neighbouring cores; set before any library that
+# carries an OpenMP runtime is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
+import argparse  # noqa: E402
+import importlib  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -157,6 +162,121 @@ def bench_warmstart(tk, a):
     print(json.dumps(out), flush=True)
 
 
+def c1_parity(tk, ctx, u0, phi0, dt, order, sigma, golden):
+    """a fresh run of the mode to the 50 outer iterations of the golden fixture; phi against
+    the CPU FP64 oracle's: max-abs over the fixture's 65536 samples, mask Dice and voxels
+    differing over every voxel"""
+    vu = ctx.from_numpy(u0.astype(dt))
+    tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+    vi = ctx.from_numpy(phi0.astype(dt))
+    iters = int(json.loads(str(golden["meta"]))["maxiter"])
+    p = tk.default_params(sigma=sigma, maxiter=iters, tol=0.0, ls_steps=5, ls_tol=0.0,
+                          gs_order=order)
+    run = tk.Rsfe(ctx, vu, vi, p)
+    vi.free()
+    vu.free()
+    run.iterate(iters)
+    flat = run.phi(np.float64).reshape(-1)
+    run.close()
+    err = float(np.max(np.abs(flat[golden["sample_idx"]] - golden["phi_samples"])))
+    mask = flat >= 0.5
+    ref = np.unpackbits(golden["mask_bits"])[:flat.size].astype(bool)
+    inter = int(np.count_nonzero(mask & ref))
+    dice = 2.0 * inter / max(1, int(mask.sum()) + int(ref.sum()))
+    return {"iterations": iters, "max_abs_samples": err, "dice": round(dice, 7),
+            "mask_voxels_differing": int(np.count_nonzero(mask != ref))}
+
+
+def c1_mode(tk, ctx, u0, phi0, n, sigma, dtype_name, order, a, timing_pass):
+    """one timed mode of the 256^3 workload: W untimed iterations, then K timed; with
+    timing_pass a second pass of K iterations records HIP events around every launch"""
+    N = n ** 3
+    dt = np.float32 if dtype_name == "f32" else np.float64
+    w = 4 if dtype_name == "f32" else 8
+    vu = ctx.from_numpy(u0.astype(dt))
+    tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+    vi = ctx.from_numpy(phi0.astype(dt))
+    p = tk.default_params(sigma=sigma, maxiter=1 << 30, tol=0.0, ls_steps=a.ls_steps,
+                          ls_tol=a.ls_tol, gs_order=order)
+    t0 = time.perf_counter()
+    run = tk.Rsfe(ctx, vu, vi, p)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    vi.free()
+    vu.free()
+    if a.warmup:
+        run.iterate(a.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    stats = run.iterate(a.steps)
+    ctx.sync()
+    t_loop = time.perf_counter() - t0
+    sweeps = [s["sweeps"] for s in stats]
+    kernels = {}
+    if timing_pass:
+        ctx.set_timing(True)
+        ctx.kernel_times(reset=True)
+        run.iterate(a.steps)
+        ctx.sync()
+        kt = ctx.kernel_times(reset=True)
+        ctx.set_timing(False)
+        for tag, (ms, cnt) in kt.items():
+            if cnt and tag in BYTES_PER_VOXEL_W:
+                avg = ms / cnt
+                per_launch = BYTES_PER_VOXEL_W[tag] * w * N
+                if tag == "gs" and order == tk.TK_GS_LEX:
+                    # a lexicographic launch is one pipeline step over part of the tiles:
+                    # count the sweeps' bytes over all steps
+                    per_launch = 3 * w * N * a.steps * a.ls_steps / cnt
+                gbs = per_launch / (avg * 1e-3) / 1e9
+                kernels[tag] = {"total_ms": round(ms, 3), "launches": cnt,
+                                "avg_us": round(avg * 1e3, 2), "GBps": round(gbs, 1),
+                                "bytes_per_launch": int(per_launch)}
+    run.close()
+    # whole-loop models.  Fused passes of this implementation: conv 5 passes (25w) + S sweeps
+    # (3w each) + sb (13w) + div (4w) = N (42w + 3wS).  Reference data-flow model of the
+    # metric's definition: N (58w + 3wS) (every temporary of rsfe_splitbregman_imp.hxx
+    # :1547-1995 read and written once).
+    S = float(np.mean(sweeps))
+    loop_gbs = N * (42 * w + 3 * w * S) * a.steps / t_loop / 1e9
+    ref_model_gbs = N * (58 * w + 3 * w * S) * a.steps / t_loop / 1e9
+    return {"value": N * a.steps / t_loop, "ms_per_step": 1e3 * t_loop / a.steps,
+            "t_setup_s": round(t_setup, 4), "sweeps": sweeps,
+            "loop_model_GBps": round(loop_gbs, 1),
+            "loop_model_frac": round(loop_gbs / HBM_PEAK_GBS, 4),
+            "loop_ref_model_GBps": round(ref_model_gbs, 1),
+            "loop_ref_model_frac": round(ref_model_gbs / HBM_PEAK_GBS, 4),
+            "kernels": kernels}
+
+
+def cpu_baseline(n, sigma, u0, phi0, a):
+    """the CPU FP64 oracle (OpenMP, the project's own FFT in place of FFTW) on the same
+    workload: a.cpu_runs runs of a.cpu_iters outer iterations each, median reported"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    import ctypes as C
+    N = n ** 3
+    po = orc.default_params(sigma=sigma, ls_steps=a.ls_steps, ls_tol=0.0, tol=0.0)
+    u255 = orc.change_range(u0, 255.0, 0.0)
+    rates, fft_share = [], []
+    for _ in range(a.cpu_runs):
+        st = orc.OrStats()
+        phi = phi0.copy()
+        orc.lib().or_rsfe_time_iters(orc.dp(u255), C.c_int(n), C.c_int(n), C.c_int(n),
+                                     C.byref(po), C.c_int(a.cpu_iters), orc.dp(phi), C.byref(st))
+        rates.append(N * st.iters / st.t_loop)
+        fft_share.append(st.t_fft / st.t_loop)
+    med = float(np.median(rates))
+    return {"value": med, "unit": "voxel-iters/s", "cores": orc.lib().or_num_threads(),
+            "kind": "port",
+            "sample": f"{n}^3 FP64 CPU oracle (OpenMP, OMP_PROC_BIND="
+                      f"{os.environ.get('OMP_PROC_BIND', '')}), median of {a.cpu_runs} runs of "
+                      f"{a.cpu_iters} outer iterations at S={a.ls_steps}, set-up excluded; the "
+                      f"oracle's own mixed-radix FFT stands in for FFTW (absent), convolution "
+                      f"share {float(np.median(fft_share)):.2f} of the loop",
+            "runs": [round(r, 1) for r in rates]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,8 +287,13 @@ def main():
     ap.add_argument("--ls-steps", type=int, default=5)
     ap.add_argument("--ls-tol", type=float, default=0.0)
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the 50-iteration runs compared with the FP64 oracle's fixture")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="time the headline mode only (not the two lexicographic modes)")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
     ap.add_argument("--workload", choices=["c1", "batch", "warmstart"], default="c1",
@@ -196,119 +321,93 @@ def main():
         return
     n = a.size
     shape = (n, n, n)
-    N = n ** 3
     scale = max(1, n // 64)
     sigma = 6.0 * n / 256.0 if n != 64 else 3.0
-    dt = np.float32 if a.dtype == "f32" else np.float64
-    w = 4 if a.dtype == "f32" else 8
     order = tk.TK_GS_REDBLACK if a.dtype == "f32" else tk.TK_GS_LEX
     if a.gs != "default":
         order = tk.TK_GS_REDBLACK if a.gs == "rb" else tk.TK_GS_LEX
+    oname = "redblack" if order == tk.TK_GS_REDBLACK else "lex"
 
     u0, phi0 = tk.gen_standard(shape, SEED, scale, 2)
     ctx = tk.Context(0)
-    vu = ctx.from_numpy(u0.astype(dt))
-    tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
-    vi = ctx.from_numpy(phi0.astype(dt))
-    p = tk.default_params(sigma=sigma, maxiter=1 << 30, tol=0.0, ls_steps=a.ls_steps,
-                          ls_tol=a.ls_tol, gs_order=order)
-    t0 = time.perf_counter()
-    run = tk.Rsfe(ctx, vu, vi, p)
-    ctx.sync()
-    t_setup = time.perf_counter() - t0
-    if a.warmup:
-        run.iterate(a.warmup)
-    ctx.sync()
-    t0 = time.perf_counter()
-    stats = run.iterate(a.steps)
-    ctx.sync()
-    t_loop = time.perf_counter() - t0
-    sweeps = [s["sweeps"] for s in stats]
-    value = N * a.steps / t_loop
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "c1_256.npz")
+    if (not a.no_parity and n == 256 and a.ls_steps == 5 and a.ls_tol == 0.0
+            and os.path.exists(gpath)):
+        d = np.load(gpath)
+        golden = {k: d[k] for k in d.files}
+
+    head = c1_mode(tk, ctx, u0, phi0, n, sigma, a.dtype, order, a, not a.no_timing_pass)
+    modes = {f"{a.dtype}_{oname}": head}
+    if not a.only_headline:
+        for dn, o in (("f32", tk.TK_GS_LEX), ("f64", tk.TK_GS_LEX)):
+            key = f"{dn}_lex"
+            if key not in modes:
+                modes[key] = c1_mode(tk, ctx, u0, phi0, n, sigma, dn, o, a, False)
+    if golden is not None:
+        for key, m in modes.items():
+            dn, on = key.split("_")
+            m["parity_vs_fp64_oracle"] = c1_parity(
+                tk, ctx, u0, phi0, np.float32 if dn == "f32" else np.float64,
+                tk.TK_GS_REDBLACK if on == "redblack" else tk.TK_GS_LEX, sigma, golden)
+    ctx.close()
 
     roof = None
-    kernels = {}
-    if not a.no_timing_pass:
-        ctx.set_timing(True)
-        ctx.kernel_times(reset=True)
-        run.iterate(a.steps)
-        ctx.sync()
-        kt = ctx.kernel_times(reset=True)
-        ctx.set_timing(False)
-        for tag, (ms, cnt) in kt.items():
-            if cnt and tag in BYTES_PER_VOXEL_W:
-                avg = ms / cnt
-                per_launch = BYTES_PER_VOXEL_W[tag] * w * N
-                if tag == "gs" and order == tk.TK_GS_LEX:
-                    # a lexicographic launch is one pipeline step over part of the tiles:
-                    # count the sweeps' bytes over all steps
-                    per_launch = 3 * w * N * a.steps * a.ls_steps / cnt
-                gbs = per_launch / (avg * 1e-3) / 1e9
-                kernels[tag] = {"total_ms": round(ms, 3), "launches": cnt,
-                                "avg_us": round(avg * 1e3, 2), "GBps": round(gbs, 1),
-                                "bytes_per_launch": int(per_launch)}
-        if kernels:
-            dom = max(kernels, key=lambda t: kernels[t]["total_ms"])
-            k = kernels[dom]
-            key = (n, a.dtype, "rb" if order == tk.TK_GS_REDBLACK else "lex", dom)
-            traffic = MEASURED_TRAFFIC.get(key)
-            roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4),
-                    "traffic": traffic,
-                    "bytes_per_launch": kernels[dom]["bytes_per_launch"]}
-            if traffic is not None:
-                roof["traffic_source"] = TRAFFIC_SOURCE
-    # whole-loop models.  Fused passes of this implementation: conv 5 passes (25w) + S sweeps
-    # (3w each) + sb (13w) + div (4w) = N (42w + 3wS).  Reference data-flow model of the
-    # metric's definition: N (58w + 3wS) (every temporary of rsfe_splitbregman_imp.hxx
-    # :1547-1995 read and written once).
-    S = float(np.mean(sweeps))
-    model_bytes = N * (42 * w + 3 * w * S)
-    loop_gbs = model_bytes * a.steps / t_loop / 1e9
-    ref_model_gbs = N * (58 * w + 3 * w * S) * a.steps / t_loop / 1e9
-    run.close()
+    kernels = head["kernels"]
+    w = 4 if a.dtype == "f32" else 8
+    if kernels:
+        dom = max(kernels, key=lambda t: kernels[t]["total_ms"])
+        k = kernels[dom]
+        key = (n, a.dtype, "rb" if order == tk.TK_GS_REDBLACK else "lex", dom)
+        traffic = MEASURED_TRAFFIC.get(key)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": k["GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(k["GBps"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "bytes_per_launch": k["bytes_per_launch"]}
+        if traffic is not None:
+            roof["traffic_source"] = TRAFFIC_SOURCE
 
     cpu = None
-    if not a.no_cpu and a.cpu_iters > 0:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as orc
-        import ctypes as C
-        st = orc.OrStats()
-        po = orc.default_params(sigma=sigma, ls_steps=a.ls_steps, ls_tol=0.0, tol=0.0)
-        u255 = orc.change_range(u0, 255.0, 0.0)
-        phi = phi0.copy()
-        orc.lib().or_rsfe_time_iters(orc.dp(u255), C.c_int(n), C.c_int(n), C.c_int(n),
-                                     C.byref(po), C.c_int(a.cpu_iters), orc.dp(phi), C.byref(st))
-        cpu = {"value": N * st.iters / st.t_loop, "unit": "voxel-iters/s",
-               "cores": orc.lib().or_num_threads(), "kind": "port",
-               "sample": f"{n}^3 FP64 CPU oracle (OpenMP), {st.iters} outer iterations at "
-                         f"S={a.ls_steps}, set-up excluded; convolution share "
-                         f"{st.t_fft / st.t_loop:.2f} of the loop",
-               "t_loop_s": round(st.t_loop, 3)}
+    if not a.no_cpu and a.cpu_iters > 0 and a.cpu_runs > 0:
+        cpu = cpu_baseline(n, sigma, u0, phi0, a)
 
+    sweeps = head["sweeps"]
+    other = {}
+    for key, m in modes.items():
+        if m is head:
+            continue
+        other[key] = {"value": m["value"], "ms_per_step": round(m["ms_per_step"], 4),
+                      "loop_ref_model_frac": m["loop_ref_model_frac"]}
+        if "parity_vs_fp64_oracle" in m:
+            other[key]["parity_vs_fp64_oracle"] = m["parity_vs_fp64_oracle"]
+    config = {"workload": f"rsfe_splitbregman {n}^3 {a.dtype}, sigma {sigma:g} cyclic, "
+                          f"Gauss-Seidel {'red-black' if order else 'lexicographic'} "
+                          f"ls_steps {a.ls_steps} ls_tol {a.ls_tol:g}",
+              "voxels": n ** 3,
+              "sweeps_per_iter": sweeps[:4] + (["..."] if len(sweeps) > 4 else []),
+              "t_setup_s": head["t_setup_s"],
+              "loop_model_GBps": head["loop_model_GBps"],
+              "loop_model_frac": head["loop_model_frac"],
+              "loop_ref_model_GBps": head["loop_ref_model_GBps"],
+              "loop_ref_model_frac": head["loop_ref_model_frac"],
+              "kernels": kernels}
+    if "parity_vs_fp64_oracle" in head:
+        config["parity_vs_fp64_oracle"] = head["parity_vs_fp64_oracle"]
+    if other:
+        config["other_modes"] = other
     out = {
         "metric": "Split-Bregman voxel-iters/s @256^3",
-        "value": value,
+        "value": head["value"],
         "unit": "voxel-iters/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * t_loop / a.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64-reductions" if a.dtype == "f32" else "f64",
         "data": "synthetic (seeded ellipsoid phantom, rescaled to [0,255])",
-        "config": {"workload": f"rsfe_splitbregman {n}^3 {a.dtype}, sigma {sigma:g} cyclic, "
-                               f"Gauss-Seidel {'red-black' if order else 'lexicographic'} "
-                               f"ls_steps {a.ls_steps} ls_tol {a.ls_tol:g}",
-                   "voxels": N, "sweeps_per_iter": sweeps[:4] + (["..."] if len(sweeps) > 4 else []),
-                   "t_setup_s": round(t_setup, 4),
-                   "loop_model_GBps": round(loop_gbs, 1),
-                   "loop_model_frac": round(loop_gbs / HBM_PEAK_GBS, 4),
-                   "loop_ref_model_GBps": round(ref_model_gbs, 1),
-                   "loop_ref_model_frac": round(ref_model_gbs / HBM_PEAK_GBS, 4),
-                   "kernels": kernels},
+        "config": config,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

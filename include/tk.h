@@ -219,6 +219,37 @@ int tk_shrink(tk_ctx ctx, tk_vol b, tk_vol g, tk_vol d);
    spacing factor).  The two passes of the run loop. */
 int tk_sb_update(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol g, const tk_vol* b,
                  const tk_vol* q, tk_vol divdb, double a0, double b0, double* norms);
+/* ---- R1: image3d elementwise operators ------------------------------------------
+ * operator+= -= *= /= with a scalar (image3d_imp.hxx:187-198, 229-240, 266-277, 304-315): the
+ * scalar is cast to the volume's type first (static_cast<T>), then every voxel is updated in
+ * that type.  With an image (201-225, 243-267, 280-304, 318-342): b's voxel is cast to a's
+ * type; dims must agree, else TK_ERR_SHAPE and a unchanged (the reference prints its WARNING
+ * and does nothing; the C++ layer include/tk/image3d.hxx does the same).  Binary + - * /
+ * (349-378) copy the left operand and apply these. */
+enum tk_op { TK_OP_ADD = 0, TK_OP_SUB = 1, TK_OP_MUL = 2, TK_OP_DIV = 3 };
+int tk_vol_op_scalar(tk_ctx ctx, tk_vol a, int op, double s);
+int tk_vol_op_vol(tk_ctx ctx, tk_vol a, int op, tk_vol b);
+/* element access, image3d::operator() (image3d.hxx:252, 262): one voxel read or written
+   (synchronous); (i, j, k) outside the volume: TK_ERR_ARG */
+int tk_vol_get(tk_vol v, int i, int j, int k, double* out);
+int tk_vol_set(tk_vol v, int i, int j, int k, double value);
+/* im3d::scalarprod (image3d_imp.hxx:385-407): sum a * static_cast<T_a>(b), FP64 accumulation
+   in a fixed order (the reference's OpenMP reduction order varies with the thread count);
+   scalarprod_L2 (411-426) multiplies by hx hy hz in the C++ layer */
+int tk_scalarprod(tk_ctx ctx, tk_vol a, tk_vol b, double* out);
+/* image3d::norm1 (1387-1398): sum |a|, FP64 accumulation; normL1 (1403-1406) times hx hy hz */
+int tk_norm1(tk_ctx ctx, tk_vol a, double* out);
+/* image3d::norminf (1427-1452) with its quirk: a running value starts at |a(0,0,0)| and takes
+   |a_i| only when a_i exceeds it, so the result is max(|a_0|, max_i a_i): large negative
+   entries are ignored */
+int tk_norminf(tk_ctx ctx, tk_vol a, double* out);
+/* norminf(a - b) without a temporary, and norminf(b) in the same pass (out_b may be NULL; b
+   NULL: norminf(a)); the ratio (phi - phiold).norminf() / phiold.norminf() of the verbose
+   iteration record (rsfe_splitbregman_imp.hxx:1835-1844) */
+int tk_norminf_diff(tk_ctx ctx, tk_vol a, tk_vol b, double* out_diff, double* out_b);
+/* im3d::vector_abs (image3d_imp.hxx:892-943): res = sqrt(f0^2 + f1^2 + f2^2) in the volumes'
+   type, f2 NULL for two components; every volume of one dims and dtype (res pre-sized) */
+int tk_vector_abs(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol res);
 /* image3d::normL2 (image3d_imp.hxx:1419-1423) with FP64 accumulation */
 int tk_norm_l2(tk_ctx ctx, tk_vol a, double* out);
 /* ||a - b||_L2 without a temporary (rsfe_splitbregman_imp.hxx:1783,1832) */
@@ -288,6 +319,21 @@ typedef struct tk_rsfe_params {
     double edge_detector_sigma; /* > 0: the edge detector (GRAD) reads the image convolved by
                                    the run's cv with a Gaussian of this sigma, radius 3
                                    (imp:214-227); 0: the image itself (default) */
+    int ws_judge_sweeps;      /* R15: sweeps each candidate gets before the residuals are
+                                 compared (default 16) */
+    int ws_continue;          /* R15: 0 (default): a round whose candidate loses ends the
+                                 rounds; 1: the rounds go on from the plain candidate (up to
+                                 warm_start of them, or until ls_res_tol holds) */
+    int ws_coarse_sweeps;     /* R15, network on a coarsened grid (tk_rsfe_set_warm_start):
+                                 > 0: the coarse correction starts as that many sweeps from
+                                 zero on the coarse problem, and the network predicts the rest
+                                 from the coarse residual they leave (smooth, where the network
+                                 is accurate); 0 (default): the network reads the restricted
+                                 residual itself */
+    int verbosity;            /* 1: the reference's verbose record (imp:1786-1793, 1835-1844)
+                                 in tk_rsfe_iter_stats: ls_err of the last sweep also when
+                                 ls_tol == 0 (one extra copy of phi per iteration) and
+                                 tol_inf; 0 (default; timing runs) */
 } tk_rsfe_params;
 
 typedef struct tk_rsfe_iter_stats {
@@ -298,6 +344,9 @@ typedef struct tk_rsfe_iter_stats {
     double res_plain;         /* R15: first round: residual of the previous iterate after 16 sweeps */
     double res_guess;         /* R15: first round: residual of the candidate after 16 sweeps */
     int cc_extracted;         /* 1: the automatic extraction replaced phi in this iteration */
+    int coarse_sweeps;        /* R15: sweeps on the coarse grid (ws_coarse_sweeps per round) */
+    double tol_inf;           /* verbosity: (phi - phiold).norminf() / phiold.norminf()
+                                 (imp:1835-1844, the norminf quirk included); -1 otherwise */
 } tk_rsfe_iter_stats;
 
 int tk_rsfe_default_params(tk_rsfe_params* p);
@@ -311,19 +360,30 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
    stats may be NULL, else has room for n_iters entries */
 int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_done);
 tk_vol tk_rsfe_phi(tk_rsfe s);
-/* the right-hand side of the last outer iteration's Poisson step (valid until the next;
- * with tol <= 0, a tk_rsfe_iterate call that stops on an exactly stationary phi leaves in it
- * the forcing of the iteration it did not run, enqueued while the tollerance was read back) */
+/* the right-hand side of the last outer iteration's Poisson step (valid until the next).
+ * With tol <= 0 the loop test `tollerance > tol` fails only for a tollerance of exactly 0
+ * (a stationary phi) or NaN (0/0: phi and phiold both zero everywhere).  A tk_rsfe_iterate
+ * call that stops on either, with another iteration still due in the call, leaves in rhs
+ * the forcing of the iteration it did not run, enqueued while the tollerance was read back;
+ * phi, the statistics and the iteration count are those of the iterations that ran. */
 tk_vol tk_rsfe_rhs(tk_rsfe s);
 /* R15: attach the learned warm start (created on the run's context for its dims).  The
    Poisson problem is linear, so the network that maps a right-hand side to its solution
    also maps the current residual r = rhs + lap phi to the correction.  The network is
    accurate on smooth residuals (those the sweeps leave) and poor on rough ones, so each
    round first runs ws_presweeps sweeps, then candidate = phi + network(r); candidate and phi
-   each get 16 sweeps and the one with the lower relative residual continues; a round whose
-   candidate loses ends the rounds, then the Poisson step continues as configured.  All
+   each get ws_judge_sweeps sweeps and the one with the lower relative residual continues; a
+   round whose candidate loses ends the rounds (or, with ws_continue, the next round starts
+   from the plain candidate), then the Poisson step continues as configured.  All
    sweeps are counted in tk_rsfe_iter_stats.sweeps. */
 int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net);
+/* (continued) The network may also be created for the run's dims divided by one integer
+   factor f on every axis (e.g. 64^3 for a 256^3 run, f = 4): it then serves as a learned
+   coarse-grid correction.  Each round restricts the residual to the network's grid
+   (tk_vol_restrict), predicts the correction there and adds its trilinear interpolation to
+   phi (tk_vol_prolong_add, gain 1: the network's out_scale carries the factor f^2 by which
+   the correction of the coarse problem exceeds that of a unit-spacing one).  The sweeps that
+   follow remove the interpolation's rough part. */
 /* the run's filter (FP32 runs autotune its per-axis methods at creation) */
 tk_filter tk_rsfe_filter(tk_rsfe s);
 int tk_rsfe_destroy(tk_rsfe s);
@@ -358,6 +418,13 @@ int tk_nn_upsample2_bwd(tk_ctx ctx, const void* dy, int ds, int dof, int C, int 
                         int nz, void* dx, int xs, int xo);
 int tk_nn_add_slice(tk_ctx ctx, void* a, int as, int ao, const void* b, int bs, int bo, int C,
                     long n);
+/* coarse = the mean of each f x f x f block of fine voxels (f = fine dims / coarse dims, one
+   integer on every axis; coarse voxel c covers fine voxels f c .. f c + f - 1); F32 or F64,
+   both volumes the same */
+int tk_vol_restrict(tk_ctx ctx, tk_vol fine, tk_vol coarse);
+/* fine += gain * the trilinear interpolation of coarse at the fine voxel centres (coarse
+   values at their block centres, constant beyond the outermost centres) */
+int tk_vol_prolong_add(tk_ctx ctx, tk_vol coarse, tk_vol fine, double gain);
 int tk_nn_f32_to_bf16(tk_ctx ctx, const float* s, long n, void* d);
 int tk_nn_bf16_to_f32(tk_ctx ctx, const void* s, long n, float* d);
 

@@ -968,6 +968,15 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
                   double hy, double hz, const or_params* p, double* phi_out,
                   int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
 {
+    return or_rsfe_apply_ckpt(myim, init, X, Y, Z, hx, hy, hz, p, phi_out, sweeps_per_iter,
+                              tol_per_iter, st, 0, nullptr, nullptr);
+}
+
+int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int Z, double hx,
+                       double hy, double hz, const or_params* p, double* phi_out,
+                       int* sweeps_per_iter, double* tol_per_iter, or_stats* st, int n_ckpt,
+                       const int* ckpt_iters, double* ckpt_phi)
+{
     double t_start = now_s();
     g_t_fft = 0;
     const long n = (long)X * Y * Z;
@@ -1070,6 +1079,9 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
             std::fill(divdb.begin(), divdb.end(), 0.);
             for (int a = 0; a < space_dim; ++a) std::fill(b[a].begin(), b[a].end(), 0.);
             ++iters;
+            for (int c = 0; c < n_ckpt; ++c)
+                if (ckpt_iters[c] == t)
+                    std::memcpy(ckpt_phi + (long)c * n, phi.data(), sizeof(double) * n);
             continue;
         }
         /* imp:1868-1936: d = grad(phi); b += d; d = shrink(b, g); b -= d; d -= b; div */
@@ -1088,6 +1100,8 @@ int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, d
         }
         or_div(d[0].data(), d[1].data(), d[2].data(), X, Y, Z, hx, hy, hz, divdb.data());
         ++iters;
+        for (int c = 0; c < n_ckpt; ++c)
+            if (ckpt_iters[c] == t) std::memcpy(ckpt_phi + (long)c * n, phi.data(), sizeof(double) * n);
     }
     double t_end = now_s();
     std::memcpy(phi_out, phi.data(), sizeof(double) * n);

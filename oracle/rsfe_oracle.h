@@ -141,6 +141,13 @@ int or_rsfe_apply(const double* u0, const double* init, int nx, int ny, int nz,
                   double hx, double hy, double hz, const or_params* p, double* phi_out,
                   int* sweeps_per_iter, double* tol_per_iter, or_stats* st);
 
+/* the same loop; phi after outer iteration ckpt_iters[c] (1-based) is copied to
+   ckpt_phi + c * nx*ny*nz for each of the n_ckpt checkpoints */
+int or_rsfe_apply_ckpt(const double* u0, const double* init, int nx, int ny, int nz,
+                       double hx, double hy, double hz, const or_params* p, double* phi_out,
+                       int* sweeps_per_iter, double* tol_per_iter, or_stats* st, int n_ckpt,
+                       const int* ckpt_iters, double* ckpt_phi);
+
 /* time n outer iterations with a fixed sweep count per iteration (CPU baseline) */
 int or_rsfe_time_iters(const double* u0, int nx, int ny, int nz, const or_params* p,
                        int n_iters, double* phi_inout, or_stats* st);

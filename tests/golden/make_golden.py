@@ -24,7 +24,21 @@ tests/test_oracle_pins.py.  Inputs come from the project's seeded generator.
   c3_batch_it10.npz -- the same 64 runs stopped after 10 outer iterations (the state the
                  FP32 loop is compared with before rounding-level differences can grow).
 
-Usage: python tests/golden/make_golden.py [c0|s5|c1|c3|c3_10|all]
+  c3_batch_pert.npz -- the same 64 runs (30 iterations) with the input raised by 1e-12 at
+                 every 7th voxel before the rescale: per volume the bit-packed mask and its
+                 Dice against the unperturbed run of c3_batch.npz.  It measures how far the
+                 reference arithmetic itself carries a last-bit-sized input change on each
+                 volume's trajectory.
+  c2_ct.npz   -- CT-like 512x512x384 (tk_gen_ct, seed 20240603, 12 vessels; the FP32 image
+                 widened to FP64 and rescaled to [0,255]), phi from the cube contour
+                 (apply(myim)), sigma 4, ls_steps 5, ls_tol 0, 50 outer iterations.  At
+                 iterations 5 and 10: the bit-packed mask phi >= 0.5, phi at 65536 seeded
+                 samples, the FP64 sum and sum of squares of phi; the per-iteration
+                 tollerance; and, for a second run whose input is raised by 1e-12 at every
+                 7th voxel, max|dphi| and the mask Dice against the first run at iterations
+                 5, 10, 20, 30, 40 and 50 (the reference arithmetic's own sensitivity).
+
+Usage: python tests/golden/make_golden.py [c0|s5|c1|c3|c3_10|c3_pert|c2|all]
 """
 import importlib
 import json
@@ -113,6 +127,107 @@ def run_c3(maxiter=30, name="c3_batch.npz"):
     print("c3", meta)
 
 
+def perturbed(x, eps=1e-12):
+    """every 7th voxel raised by eps (a min-max rescale keeps it; scaling the whole image
+    would cancel)"""
+    x = np.array(x, np.float64)
+    x.reshape(-1)[::7] += eps
+    return x
+
+
+def dice(a, b):
+    s = int(a.sum()) + int(b.sum())
+    return 1.0 if s == 0 else 2.0 * int((a & b).sum()) / s
+
+
+def run_c3_pert(maxiter=30):
+    d = np.load(os.path.join(HERE, "c3_batch.npz"))
+    dims, nell = d["dims"], d["nell"]
+    p = orc.default_params(sigma=3.0, maxiter=maxiter, tol=0.0, ls_steps=5, ls_tol=0.0)
+    out = {}
+    dices = np.zeros(64)
+    t0 = time.time()
+    for v in range(64):
+        u0, phi0 = c3_volume(v, dims[v], nell[v])
+        phi, _, _, _ = orc.rsfe_apply(orc.change_range(perturbed(u0), 255.0, 0.0), phi0, p)
+        m = (phi >= 0.5).reshape(-1)
+        ref = np.unpackbits(d[f"mask_{v}"])[:m.size].astype(bool)
+        dices[v] = dice(m, ref)
+        out[f"mask_{v}"] = np.packbits(m)
+        print(f"c3_pert volume {v} dims {dims[v].tolist()} Dice vs unperturbed {dices[v]:.6f}",
+              flush=True)
+    meta = dict(seed=C3_SEED, n_vol=64, sigma=3.0, maxiter=maxiter, ls_steps=5, ls_tol=0.0,
+                perturbation="input + 1e-12 at every 7th voxel before change_range(255, 0)",
+                seconds=time.time() - t0, threads=orc.lib().or_num_threads())
+    np.savez_compressed(os.path.join(HERE, "c3_batch_pert.npz"), dice=dices,
+                        meta=json.dumps(meta), **out)
+    print("c3_pert", meta, "Dice min", dices.min(), "below 0.99:", np.nonzero(dices < 0.99)[0])
+
+
+C2_SEED = 20240603
+C2_CKPT = [5, 10, 20, 30, 40, 50]
+
+
+def run_c2(maxiter=50):
+    shape = (512, 512, 384)
+    img, seed = tk.gen_ct(shape, C2_SEED, 12)
+    x = img.astype(np.float32).astype(np.float64)
+    del img
+    phi0, _ = orc.init_phi(None, shape)
+    p = orc.default_params(sigma=4.0, maxiter=maxiter, tol=0.0, ls_steps=5, ls_tol=0.0)
+    idx = None
+    res = {}
+    t0 = time.time()
+    phi, sweeps, tols, st, ck = orc.rsfe_apply_ckpt(orc.change_range(x, 255.0, 0.0), phi0, p,
+                                                    C2_CKPT)
+    t_run = time.time() - t0
+    print(f"c2 oracle run: {t_run:.0f} s, tollerance {tols[:3].tolist()} ... {tols[-1]}",
+          flush=True)
+    del phi
+    n = ck[0].size
+    idx = np.sort(np.random.default_rng(2).choice(n, C1_SAMPLES, replace=False))
+    for q, it in enumerate(C2_CKPT[:2]):
+        flat = ck[q].reshape(-1)
+        res[f"mask_bits_{it}"] = np.packbits(flat >= 0.5)
+        res[f"phi_samples_{it}"] = flat[idx]
+        res[f"phi_sum_{it}"] = np.float64(flat.sum())
+        res[f"phi_sumsq_{it}"] = np.float64((flat * flat).sum())
+    masks = [(c.reshape(-1) >= 0.5) for c in ck]
+    ref_last = ck[-1].reshape(-1).copy()
+    del ck
+    t1 = time.time()
+    _, _, tols_p, _, ckp = orc.rsfe_apply_ckpt(orc.change_range(perturbed(x), 255.0, 0.0),
+                                               phi0, p, C2_CKPT)
+    t_pert = time.time() - t1
+    pert_err, pert_dice = [], []
+    for q, it in enumerate(C2_CKPT):
+        fp = ckp[q].reshape(-1)
+        if q == len(C2_CKPT) - 1:
+            e = float(np.max(np.abs(fp - ref_last)))
+        else:
+            e = float("nan")
+        pert_dice.append(dice(fp >= 0.5, masks[q]))
+        pert_err.append(e)
+    # max-abs at every checkpoint needs both fields: recompute from the stored samples
+    for q, it in enumerate(C2_CKPT[:2]):
+        pert_err[q] = float(np.max(np.abs(ckp[q].reshape(-1)[idx] - res[f"phi_samples_{it}"])))
+    print("c2 perturbed run: Dice", pert_dice, "max|dphi| (samples at 5/10, full at 50)",
+          pert_err, flush=True)
+    meta = dict(seed=C2_SEED, shape=list(shape), sigma=4.0, maxiter=maxiter, ls_steps=5,
+                ls_tol=0.0, init="cube (apply(myim))", input="tk_gen_ct FP32 -> FP64 -> "
+                "change_range(255, 0)", iters=int(st.iters), seconds=t_run,
+                seconds_perturbed=t_pert, threads=orc.lib().or_num_threads(),
+                samples=C1_SAMPLES, sample_rng="numpy default_rng(2).choice(N, 65536, "
+                "replace=False), sorted", checkpoints=C2_CKPT,
+                perturbation="input + 1e-12 at every 7th voxel before change_range(255, 0)",
+                pert_dice=pert_dice, pert_max_abs=pert_err,
+                pert_max_abs_note="iterations 5 and 10 over the samples, 50 over every voxel")
+    np.savez_compressed(os.path.join(HERE, "c2_ct.npz"), sample_idx=idx.astype(np.int64),
+                        sweeps=sweeps, tols=tols, tols_perturbed=tols_p,
+                        pert_dice=np.array(pert_dice), meta=json.dumps(meta), **res)
+    print("c2", meta)
+
+
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "all"
     if which in ("s5", "all"):
@@ -125,3 +240,7 @@ if __name__ == "__main__":
         run_c3()
     if which in ("c3_10", "all"):
         run_c3(10, "c3_batch_it10.npz")
+    if which in ("c3_pert", "all"):
+        run_c3_pert()
+    if which in ("c2", "all"):
+        run_c2()

@@ -289,3 +289,20 @@ def rsfe_forcing(u0, phi, divdb, params, h=(1.0, 1.0, 1.0)):
     lib().or_rsfe_forcing(dp(u0), dp(phi), dp(divdb), *[C.c_int(s) for s in u0.shape],
                           *[C.c_double(x) for x in h], C.byref(params), dp(rhs))
     return rhs
+
+
+def rsfe_apply_ckpt(u0, init, params, ckpts, h=(1.0, 1.0, 1.0)):
+    """rsfe_apply that also returns phi after each outer iteration listed in ckpts (1-based)"""
+    u0 = np.ascontiguousarray(u0, np.float64)
+    init = np.ascontiguousarray(init, np.float64)
+    phi = np.zeros_like(u0)
+    sw = np.zeros(max(params.maxiter, 1), np.int32)
+    tl = np.zeros(max(params.maxiter, 1), np.float64)
+    st = OrStats()
+    ck = np.ascontiguousarray(ckpts, np.int32)
+    out = np.zeros((len(ck),) + u0.shape, np.float64)
+    lib().or_rsfe_apply_ckpt.restype = C.c_int
+    n = lib().or_rsfe_apply_ckpt(dp(u0), dp(init), *[C.c_int(s) for s in u0.shape],
+                                 *[C.c_double(x) for x in h], C.byref(params), dp(phi), ip(sw),
+                                 dp(tl), C.byref(st), C.c_int(len(ck)), ip(ck), dp(out))
+    return phi, sw[:n].copy(), tl[:n].copy(), st, out

@@ -250,3 +250,40 @@ def test_iterate_ahead_equals_one_call_per_iteration(tk, ctx, dtype, order):
     assert np.array_equal(phi_a, phi_b)
     assert [s["sweeps"] for s in st_a] == [s["sweeps"] for s in st_b]
     assert [s["tollerance"] for s in st_a] == [s["tollerance"] for s in st_b]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("level", [0.25, 0.0])
+def test_iterate_ahead_loop_stops_on_its_tollerance(tk, orc, ctx, dtype, level):
+    """a loop that stops while the next iteration's convolution passes are already enqueued
+    (tol 0, iterate(5)).  a0 == b0 == level: initialize_phi_with_init gives phi == level
+    everywhere (x /= max / (b0 - a0) divides by +inf) and cut_phi clamps every Poisson step
+    back to it, so phi is exactly stationary.  The tollerance ||phi - phiold|| / ||phiold|| is
+    0 for level 0.25 and 0 / 0 = NaN for level 0; either fails `tollerance > tol`
+    (rsfe_splitbregman_imp.hxx:1659), so the reference runs one iteration.  Same count, phi
+    and tollerance as the oracle; a later call runs nothing."""
+    shape = (24, 20, 28)
+    u0, phi0 = tk.gen_standard(shape, 5, 1, 2)
+    u255 = orc.change_range(u0, 255.0, 0.0)
+    p = tk.default_params(sigma=2.0, maxiter=50, tol=0.0, ls_steps=5, ls_tol=0.0, a0=level,
+                          b0=level, gs_order=tk.TK_GS_LEX)
+    vu = ctx.from_numpy(u255.astype(dtype))
+    vi = ctx.from_numpy(phi0.astype(dtype))
+    r = tk.Rsfe(ctx, vu, vi, p)
+    st = r.iterate(5)
+    phi = r.phi()
+    again = r.iterate(3)
+    phi_again = r.phi()
+    r.close()
+    ref, sweeps, tols, ost = orc.rsfe_apply(u255, phi0, orc.default_params(
+        sigma=2.0, maxiter=50, tol=0.0, ls_steps=5, ls_tol=0.0, a0=level, b0=level))
+    say(f"stationary phi = {level} ({np.dtype(dtype).name}): iterations {len(st)} (oracle "
+        f"{ost.iters}), tollerance {st[0]['tollerance']} (oracle {tols[0]})")
+    assert len(st) == ost.iters == 1
+    assert np.all(phi == level) and np.array_equal(phi, ref)
+    if level == 0.0:
+        assert np.isnan(st[0]["tollerance"]) and np.isnan(tols[0])
+    else:
+        assert st[0]["tollerance"] == 0.0 == tols[0]
+    assert st[0]["sweeps"] == sweeps[0] == 5
+    assert again == [] and np.array_equal(phi_again, phi)
