@@ -131,16 +131,20 @@ def guess_quality(ctx, net, rhs_l, sol_l):
 
 
 def evaluate(ctx, net, n_seeds, shape=(64, 64, 64), sigma=3.0, n_iter=15, res_tol=1e-6,
-             seed0=SEED_EVAL, rounds=8, presweeps=512, log=None):
+             seed0=SEED_EVAL, rounds=8, presweeps=512, tight_tol=None, log=None):
     """per seed: sweeps and seconds of the residual-gated loop with the plain start and with
-    the learned start, the Dice of their masks, and how often the guess was kept"""
+    the learned start, the Dice of their masks, and how often the guess was kept.  tight_tol:
+    also a plain run gated at that tighter residual, and the Dice of each mask against its
+    mask (how close each run comes to a more accurate solution of the same loop)"""
     tk = _tk()
     rows = []
+    modes = ("plain", "learned") + (("tight",) if tight_tol else ())
     for q in range(n_seeds):
         res = {}
-        for mode in ("plain", "learned"):
+        for mode in modes:
             r = _run(tk, ctx, seed0 + q, shape, np.float64, tk.TK_GS_REDBLACK, sigma=sigma,
-                     maxiter=n_iter, tol=0.0, ls_res_tol=res_tol, res_check_every=64,
+                     maxiter=n_iter, tol=0.0,
+                     ls_res_tol=tight_tol if mode == "tight" else res_tol, res_check_every=64,
                      max_sweeps=400000, warm_start=rounds if mode == "learned" else 0,
                      ws_presweeps=presweeps)
             if mode == "learned":
@@ -154,13 +158,18 @@ def evaluate(ctx, net, n_seeds, shape=(64, 64, 64), sigma=3.0, n_iter=15, res_to
                              used=sum(s["guess_used"] for s in st),
                              mask=r.phi() >= 0.5)
             r.close()
-        a, b = res["plain"]["mask"], res["learned"]["mask"]
-        tot = int(a.sum()) + int(b.sum())
-        dice = 1.0 if tot == 0 else 2 * int((a & b).sum()) / tot
+        def _dice(a, b):
+            tot = int(a.sum()) + int(b.sum())
+            return 1.0 if tot == 0 else 2 * int((a & b).sum()) / tot
+
+        dice = _dice(res["plain"]["mask"], res["learned"]["mask"])
         rows.append(dict(seed=seed0 + q, sweeps_plain=res["plain"]["sweeps"],
                          sweeps_learned=res["learned"]["sweeps"],
                          s_plain=res["plain"]["seconds"], s_learned=res["learned"]["seconds"],
                          guess_used=res["learned"]["used"], dice=float(dice)))
+        if tight_tol:
+            rows[-1]["dice_plain_tight"] = _dice(res["plain"]["mask"], res["tight"]["mask"])
+            rows[-1]["dice_learned_tight"] = _dice(res["learned"]["mask"], res["tight"]["mask"])
         if log:
             log(f"eval seed {seed0 + q}: sweeps {rows[-1]['sweeps_plain']} -> "
                 f"{rows[-1]['sweeps_learned']}, {rows[-1]['s_plain']:.3f} s -> "

@@ -1,7 +1,8 @@
 """The learned Poisson warm start end to end, with the network trained on the GPU by the
 project's own kernels (3dimagetoolkit_amd/weights/warmstart_64.npz, written by
 `python 3dimagetoolkit_amd/warmstart.py`; its metadata holds the training record and the
-50-seed evaluation).
+50-seed evaluation).  At 256^3 the same network serves as a learned coarse-grid correction
+(weights/warmstart_256c4.npz: its parameters with out_scale x16 for the coarse spacing 4).
 
 On seeds the network has not seen: 64^3 ellipsoid phantoms, FP64 Split-Bregman (red-black
 sweeps), every Poisson step gated on a relative residual of 1e-6, 15 outer iterations; once
@@ -10,7 +11,6 @@ from the reference's start (previous iterate) and once with learned correction r
 problem to the same residual, so accuracy is unchanged and only the sweep count drops.
 """
 import importlib
-import json
 
 import numpy as np
 import pytest
@@ -30,17 +30,27 @@ def ws():
     return importlib.import_module("3dimagetoolkit_amd.warmstart")
 
 
-def test_weights_record(ws):
+def test_weights_files(ws):
+    """the shipped files hold what the runs load: the 64^3 network, and the 256^3 coarse
+    correction made from it (same parameters, out_scale x f^2).  Performance claims are
+    measured by the tests below, not read from the files' records."""
     d = np.load(ws.WEIGHTS)
-    meta = json.loads(str(d["meta"]))
-    say("training record:", json.dumps(meta))
+    say("training record:", str(d["meta"]))
     assert tuple(d["dims"]) == (64, 64, 64)
-    assert meta["evaluation"]["seeds"] == 50 and meta["evaluation"]["time_ratio"] < 0.5
+    c = np.load(ws.WEIGHTS_256)
+    f = int(c["coarsen"])
+    assert f == 4 and tuple(c["dims"]) == (64, 64, 64) and int(c["coarse_sweeps"]) > 0
+    assert np.array_equal(c["params"], d["params"])
+    assert float(c["out_scale"]) == float(d["out_scale"]) * f * f
 
 
 def test_learned_start_halves_the_poisson_work(tk, ctx, ws):
+    """per seed, the learned run's mask agrees with the plain run's (Dice >= 0.999), or, on
+    the seeds whose small masks move with the Poisson tolerance itself, it is at least as
+    close (within 0.02 Dice) to a plain run gated ten times tighter (1e-7) as the plain 1e-6
+    run is; the wall time is measured"""
     net = ws.load(ctx)
-    rows = ws.evaluate(ctx, net, SEEDS, seed0=SEED0, log=say)
+    rows = ws.evaluate(ctx, net, SEEDS, seed0=SEED0, tight_tol=1e-7, log=say)
     net.close()
     sp = sum(r["sweeps_plain"] for r in rows)
     sl = sum(r["sweeps_learned"] for r in rows)
@@ -51,8 +61,11 @@ def test_learned_start_halves_the_poisson_work(tk, ctx, ws):
         f"time {tp:.2f} -> {tl:.2f} s ({tl / tp:.3f}x), Dice min {dices.min():.6f} "
         f"median {np.median(dices):.6f}, rounds kept {sum(r['guess_used'] for r in rows)}")
     assert all(r["guess_used"] > 0 for r in rows)
-    assert sl / sp < 0.6 and tl / tp < 0.7
-    assert np.median(dices) >= 0.999
+    for r in rows:
+        say(f"seed {r['seed']}: Dice learned/plain {r['dice']:.6f}, against the 1e-7 run: "
+            f"plain {r['dice_plain_tight']:.6f}, learned {r['dice_learned_tight']:.6f}")
+        assert r["dice"] >= 0.999 or r["dice_learned_tight"] >= r["dice_plain_tight"] - 0.02
+    assert sl / sp < 0.5 and tl / tp <= 0.5
 
 
 def test_mask_sensitivity_to_the_solver_tolerance(tk, ctx, ws):
@@ -71,3 +84,27 @@ def test_mask_sensitivity_to_the_solver_tolerance(tk, ctx, ws):
     d = 1.0 if tot == 0 else 2 * int((a & b).sum()) / tot
     say(f"seed 50017, plain start, residual 1e-6 vs 1e-7: Dice {d:.6f}, mask voxels "
         f"{int(a.sum())} / {int(b.sum())}")
+
+
+C4_SEEDS = (90011, 90012)  # C1 generator (two ellipsoids), unseen: training used 30000+
+
+
+def test_c4_256_learned_coarse_correction(tk, ctx, ws):
+    """C4 at 256^3: the C1 generator (sigma 6), FP64 Split-Bregman with red-black sweeps and
+    every Poisson step gated on a relative residual of 1e-6, 3 outer iterations, plain start
+    against the learned coarse-grid correction.  Per seed and per outer iteration the masks
+    agree (Dice >= 0.999; under a converged Poisson step this model's level set collapses by
+    the second or third iteration, so the later masks can be empty on both sides and phi
+    itself is compared too), and the wall time is measured on the device-synchronised loop."""
+    net, coarse = ws.load256(ctx)
+    rows = ws.evaluate_256(ctx, net, C4_SEEDS, n_iter=3, coarse=coarse, log=say)
+    net.close()
+    tp = sum(r["s_plain"] for r in rows)
+    tl = sum(r["s_learned"] for r in rows)
+    say(f"C4 256^3, {len(rows)} unseen seeds: {tp:.2f} s -> {tl:.2f} s ({tl / tp:.4f}x)")
+    for r in rows:
+        assert min(r["dice"]) >= 0.999, r
+        assert max(r["max_abs"]) < 0.05, r
+        assert r["s_learned"] / r["s_plain"] <= 0.5, r
+        assert sum(r["rounds_used"]) > 0
+    assert tl / tp <= 0.5
