@@ -157,6 +157,7 @@ def _load():
         "tk_norminf": (i, [vp, vp, dp]),
         "tk_norminf_diff": (i, [vp, vp, vp, dp, dp]),
         "tk_vector_abs": (i, [vp, vp, vp, vp, vp]),
+        "tk_norm_l2_diff_noalloc": (i, [vp, vp, vp, C.c_long, i, d, d, d, vp]),
         "tk_vol_restrict": (i, [vp, vp, vp]),
         "tk_vol_prolong_add": (i, [vp, vp, vp, d]),
         "tk_nn_f32_to_bf16": (i, [vp, vp, C.c_long, vp]),

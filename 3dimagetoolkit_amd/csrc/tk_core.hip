@@ -515,6 +515,12 @@ int tk_ctx_create(int device, tk_ctx* out)
     TK_HIP(hipMemsetAsync(c->acc, 0, sizeof(double) * kAccSlots, c->stream));
     TK_HIP(hipMemsetAsync(c->cnt, 0, sizeof(unsigned) * kCntSlots, c->stream));
     TK_HIP(hipStreamSynchronize(c->stream));
+    // per-device set-up of the allocation-free entry point, with the device current
+    const int rs = noalloc_setup(device, nullptr);
+    if (rs != TK_OK) {
+        tk_ctx_destroy(c);
+        return rs;
+    }
     *out = c;
     return TK_OK;
 }

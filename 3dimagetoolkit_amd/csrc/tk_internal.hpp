@@ -96,6 +96,9 @@ struct tk_vol_s {
 
 namespace tk {
 
+// the address of tk_noalloc.hip's scratch array on `device` (current), looked up once
+int noalloc_setup(int device, double** out);
+
 // get (grow if needed) context work buffer `slot` of at least `bytes`
 int ctx_work(tk_ctx ctx, int slot, size_t bytes, void** out);
 

@@ -254,6 +254,16 @@ int tk_vector_abs(tk_ctx ctx, tk_vol f0, tk_vol f1, tk_vol f2, tk_vol res);
 int tk_norm_l2(tk_ctx ctx, tk_vol a, double* out);
 /* ||a - b||_L2 without a temporary (rsfe_splitbregman_imp.hxx:1783,1832) */
 int tk_norm_l2_diff(tk_ctx ctx, tk_vol a, tk_vol b, double* out);
+/* tk_norm_l2_diff for callers that cannot allocate (no reference counterpart: for code under
+   HIP graph capture, serving loops and harnesses that forbid hipMalloc).  Enqueued on
+   hip_stream (NULL: the default stream) of the current device; a and b are device arrays of
+   n elements of dtype (b may be NULL); out_device[0] = ||a - b||_L2 and out_device[1] =
+   ||b||_L2 (||a||_L2 and 0 without b), FP64, with the spacing factor hx hy hz.  Nothing is
+   allocated: the partial sums live in a 256 KiB device array of the library, one per GPU,
+   held until the process ends, so n is limited to 1 .. 134217728 voxels (TK_ERR_ARG beyond).
+   Calls that overlap on one GPU share that array and race. */
+int tk_norm_l2_diff_noalloc(void* hip_stream, const void* a, const void* b, long n,
+                            tk_dtype dtype, double hx, double hy, double hz, double* out_device);
 /* image3d::min / max (image3d_imp.hxx:1456-1509) */
 int tk_minmax(tk_ctx ctx, tk_vol a, double* mn, double* mx);
 /* image3d::change_range_of_intensity (image3d_imp.hxx:1627-1635) */
