@@ -14,7 +14,8 @@
  *
  * The static state is per thread here (the reference's is per process and not re-entrant):
  * the Gaussian parameters of the last build_gaussian_filter and a device filter built for
- * the dims of the last build_freq_filters.  The reference's behaviour is undefined when the
+ * the dims of the last build_freq_filters.  direct_convolution (411; convolution_imp.hxx:
+ * 761-812) uses the same filter.  The reference's behaviour is undefined when the
  * factor's dims differ from the ones the filter was built for (convolution.hxx:286-297);
  * this layer prints a WARNING and rebuilds the filter for the factor's dims.
  */
@@ -138,6 +139,21 @@ void acyclic_fftw_convolution(im3d::image3d<T>& result, im3d::image3d<T> const& 
     result.shape_like(factor);
     tk::check(tk_conv_acyclic(factor.ctx(), flt, result.handle(), factor.handle()),
               "acyclic_fftw_convolution");
+}
+
+// direct_convolution (convolution.hxx:411; convolution_imp.hxx:761-812): result(i,j,k) =
+// sum over the Nh^3 taps of hx hy hz * time_filter(I,J,K) * factor, zero outside the volume.
+// The Gaussian time filter is a product of per-axis factors, so the triple sum equals three
+// 1-D passes with zero halos, which is what runs (the acyclic passes of the device filter);
+// the result equals the reference's up to rounding, not in its summation order.
+template <typename T>
+void direct_convolution(im3d::image3d<T>& result, im3d::image3d<T> const& factor)
+{
+    tk_filter flt = filtering<T>::filter_for(factor);
+    if (!flt) return;
+    result.shape_like(factor);
+    tk::check(tk_conv_acyclic(factor.ctx(), flt, result.handle(), factor.handle()),
+              "direct_convolution");
 }
 
 }  // namespace conv

@@ -3,9 +3,11 @@
  *
  * Mirrors im3d::image3d<T> (header/image3d.hxx:50-737) for what the RSFE Split-Bregman path
  * uses: the constructors (image3d.hxx:90-111), dims and spacing accessors (122-238),
- * operator=(scalar) and operator=(image3d) (271-280), element read (252), grad (442),
- * div (454), normL2 / min / max (494-513), im_to_black_and_white (621),
- * change_range_of_intensity (635) and the seeded connected_component (679).  Value
+ * element read and write (252, 262), operator=(scalar) and operator=(image3d) (271-280),
+ * the compound operators += -= *= /= with a scalar or an image and the binary + - * /
+ * (283-402), scalarprod / scalarprod_L2 (420-430), grad (442), vector_abs (448), div (454),
+ * norm1 / normL1 / norm2 / normL2 / norminf / min / max (480-513), im_to_black_and_white
+ * (621), change_range_of_intensity (635) and the seeded connected_component (679).  Value
  * semantics as the reference (copies are deep, image3d_imp.hxx:44-57).  Voxel order is the
  * reference's: (i*dimy + j)*dimz + k (image3d_imp.hxx:141-152), so upload() and download()
  * take and give the reference's raw vector unchanged.  T is double, float or unsigned char
@@ -14,7 +16,10 @@
 #ifndef TK_IMAGE3D_HXX
 #define TK_IMAGE3D_HXX
 
+#include <cmath>
 #include <cstdint>
+#include <iostream>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -23,6 +28,40 @@
 namespace im3d {
 
 typedef unsigned int uint;
+
+// the reference's image3d messages go to std::cout (image3d_imp.hxx:220-223, 400-426)
+inline void image3d_warning(const char* msg) { std::cout << "WARNING::" << msg << std::endl; }
+
+template <typename T>
+class image3d;
+
+// a writable voxel, what operator() returns for a non-const image (image3d.hxx:262: T&):
+// reads and writes go to device memory one voxel at a time
+template <typename T>
+class voxel_ref {
+public:
+    voxel_ref(tk_vol v, uint i, uint j, uint k) : v_(v), i_(i), j_(j), k_(k) {}
+    operator T() const
+    {
+        double x = 0;
+        if (v_) tk::check(tk_vol_get(v_, (int)i_, (int)j_, (int)k_, &x), "image3d element");
+        return (T)x;
+    }
+    voxel_ref& operator=(T const& x)
+    {
+        if (v_) tk::check(tk_vol_set(v_, (int)i_, (int)j_, (int)k_, (double)x), "image3d element");
+        return *this;
+    }
+    voxel_ref& operator=(voxel_ref const& o) { return *this = (T)o; }
+    voxel_ref& operator+=(T const& x) { return *this = (T)(T(*this) + x); }
+    voxel_ref& operator-=(T const& x) { return *this = (T)(T(*this) - x); }
+    voxel_ref& operator*=(T const& x) { return *this = (T)(T(*this) * x); }
+    voxel_ref& operator/=(T const& x) { return *this = (T)(T(*this) / x); }
+
+private:
+    tk_vol v_;
+    uint i_, j_, k_;
+};
 
 template <typename T>
 class image3d {
@@ -143,6 +182,55 @@ public:
         return x;
     }
 
+    // element write (operator(), image3d.hxx:262): one host-to-device copy per assignment
+    voxel_ref<T> operator()(uint const& i, uint const& j, uint const& k)
+    {
+        return voxel_ref<T>(i < dimx_ && j < dimy_ && k < dimz_ ? v_ : nullptr, i, j, k);
+    }
+
+    // ---- compound operators (image3d_imp.hxx:187-345) -----------------------------
+    // with a scalar: static_cast<T>(s), then every voxel in T arithmetic
+    template <typename S, typename std::enable_if<std::is_arithmetic<S>::value, int>::type = 0>
+    image3d& operator+=(S const& s) { return op_scalar(TK_OP_ADD, s); }
+    template <typename S, typename std::enable_if<std::is_arithmetic<S>::value, int>::type = 0>
+    image3d& operator-=(S const& s) { return op_scalar(TK_OP_SUB, s); }
+    template <typename S, typename std::enable_if<std::is_arithmetic<S>::value, int>::type = 0>
+    image3d& operator*=(S const& s) { return op_scalar(TK_OP_MUL, s); }
+    template <typename S, typename std::enable_if<std::is_arithmetic<S>::value, int>::type = 0>
+    image3d& operator/=(S const& s) { return op_scalar(TK_OP_DIV, s); }
+    // with an image of the same dims, voxel by voxel (WARNING and nothing done otherwise)
+    template <typename S>
+    image3d& operator+=(image3d<S> const& o) { return op_image(TK_OP_ADD, o, "image3d::operator+= : dimensions must agree"); }
+    template <typename S>
+    image3d& operator-=(image3d<S> const& o) { return op_image(TK_OP_SUB, o, "image3d::operator-= : dimensions must agree"); }
+    template <typename S>
+    image3d& operator*=(image3d<S> const& o) { return op_image(TK_OP_MUL, o, "image3d::operator*= : dimensions must agree"); }
+    template <typename S>
+    image3d& operator/=(image3d<S> const& o) { return op_image(TK_OP_DIV, o, "image3d::operator/= : dimensions must agree"); }
+
+    // ---- norms (image3d_imp.hxx:1387-1452), FP64 accumulation --------------------------
+    T const norm1() const
+    {
+        double r = 0;
+        tk::check(tk_norm1(ctx_->handle(), v_, &r), "norm1");
+        return (T)r;
+    }
+    T const normL1() const { return (T)((double)norm1() * hx_ * hy_ * hz_); }
+    T const norm2() const
+    {
+        double r = 0;
+        tk::check(tk_scalarprod(ctx_->handle(), v_, v_, &r), "norm2");
+        return (T)std::sqrt(r);
+    }
+    // the reference's quirk: entries below the running maximum, the negative ones among
+    // them, never count (image3d_imp.hxx:1439)
+    T const norminf() const
+    {
+        double r = 0;
+        tk::check(tk_norminf(ctx_->handle(), v_, &r), "norminf");
+        return (T)r;
+    }
+
     // ---- operators of the path ---------------------------------------------------
     // image3d::grad (image3d_imp.hxx:743-888)
     void grad(std::vector<image3d<T> >& res) const
@@ -233,6 +321,22 @@ public:
     }
 
 private:
+    template <typename S>
+    image3d& op_scalar(int op, S const& s)
+    {
+        if (v_) tk::check(tk_vol_op_scalar(ctx_->handle(), v_, op, (double)(T)s), "image3d operator");
+        return *this;
+    }
+    template <typename S>
+    image3d& op_image(int op, image3d<S> const& o, const char* msg)
+    {
+        if (o.getdimx() != dimx_ || o.getdimy() != dimy_ || o.getdimz() != dimz_) {
+            image3d_warning(msg);
+            return *this;
+        }
+        if (v_) tk::check(tk_vol_op_vol(ctx_->handle(), v_, op, o.handle()), "image3d operator");
+        return *this;
+    }
     void alloc(uint x, uint y, uint z)
     {
         dimx_ = x;
@@ -267,6 +371,91 @@ private:
     uint dimx_ = 0, dimy_ = 0, dimz_ = 0;
     double hx_ = 1., hy_ = 1., hz_ = 1.;
 };
+
+// binary operators (image3d_imp.hxx:349-378): a copy of the left operand, then the compound
+// operator with the right one (an image or a scalar); the result has the left's type
+template <typename S, typename R>
+image3d<S> const operator+(image3d<S> const& a, R const& b)
+{
+    image3d<S> r(a);
+    return r += b;
+}
+template <typename S, typename R>
+image3d<S> const operator-(image3d<S> const& a, R const& b)
+{
+    image3d<S> r(a);
+    return r -= b;
+}
+template <typename S, typename R>
+image3d<S> const operator*(image3d<S> const& a, R const& b)
+{
+    image3d<S> r(a);
+    return r *= b;
+}
+template <typename S, typename R>
+image3d<S> const operator/(image3d<S> const& a, R const& b)
+{
+    image3d<S> r(a);
+    return r /= b;
+}
+
+// scalarprod (image3d_imp.hxx:385-407): sum a * static_cast<S>(b), FP64 accumulation; dims
+// that disagree: WARNING and 0
+template <typename S, typename R>
+S const scalarprod(image3d<S> const& a, image3d<R> const& b)
+{
+    if (a.getdimx() != b.getdimx() || a.getdimy() != b.getdimy() || a.getdimz() != b.getdimz()) {
+        image3d_warning("scalarprod: dimensions must agree, 0 is returned");
+        return 0;
+    }
+    double r = 0;
+    tk::check(tk_scalarprod(a.ctx(), a.handle(), b.handle(), &r), "scalarprod");
+    return (S)r;
+}
+
+// scalarprod_L2 (image3d_imp.hxx:411-426): hx hy hz scalarprod; spacings that disagree:
+// WARNING and 0
+template <typename S, typename R>
+S const scalarprod_L2(image3d<S> const& a, image3d<R> const& b)
+{
+    if (a.gethx() != b.gethx() || a.gethy() != b.gethy() || a.gethz() != b.gethz()) {
+        image3d_warning("scalarprod_L2: spacing must agree, 0 is returned");
+        return 0;
+    }
+    return (S)(a.gethx() * a.gethy() * a.gethz() * (double)scalarprod(a, b));
+}
+
+// vector_abs (image3d_imp.hxx:892-943): res = |fun| voxel by voxel for 2 or 3 components
+template <typename S, typename R>
+void vector_abs(image3d<S>& res, std::vector<image3d<R> > const& fun)
+{
+    if (fun.size() != 2 && fun.size() != 3) {
+        std::cout << "In image3d::vector_abs: input vector could be only 2d or 3d, "
+                  << fun.size() << " is a not allowed dimension." << std::endl;
+        return;
+    }
+    res.shape_like(fun[0]);
+    for (size_t c = 1; c < fun.size(); ++c)
+        if (fun[c].getdimx() != fun[0].getdimx() || fun[c].getdimy() != fun[0].getdimy() ||
+            fun[c].getdimz() != fun[0].getdimz()) {
+            std::cout << "In image3d::vector_abs: dimensions of image3d "
+                      << "inside input vector must agree" << std::endl;
+            return;
+        }
+    if (std::is_same<S, R>::value) {
+        tk::check(tk_vector_abs(res.ctx(), fun[0].handle(), fun[1].handle(),
+                                fun.size() == 3 ? fun[2].handle() : nullptr, res.handle()),
+                  "vector_abs");
+    } else {
+        // computed in R (the components' type), then cast to S
+        image3d<R> tmp(fun[0].getdimx(), fun[0].getdimy(), fun[0].getdimz(), fun[0].gethx(),
+                       fun[0].gethy(), fun[0].gethz());
+        tk::check(tk_vector_abs(res.ctx(), fun[0].handle(), fun[1].handle(),
+                                fun.size() == 3 ? fun[2].handle() : nullptr, tmp.handle()),
+                  "vector_abs");
+        res.assign(tmp);
+    }
+}
 
 // im3d::div (image3d_imp.hxx:947-1383), including its two boundary quirks
 template <typename S, typename R>

@@ -9,13 +9,17 @@
  * with the reference's validation and WARNING-and-keep behaviour (imp:990-1330), set_cv and
  * set_onestep_poisson (imp:836-852), set_bc (imp:1294-1322), and apply (imp:1547-1995).
  *
- * What this layer runs: cv = conv::acyclic_fftw_convolution (the reference's default) or
- * conv::cyclic_fftw_convolution (set_cv with any other function keeps the current one and
- * prints a WARNING), onestep_poisson = lapl::DirGaussSeidel (bit-exact
- * lexicographic sweeps), lapl::DirRedBlackGaussSeidel (red-black) or lapl::NeuGaussSeidel
- * (installed by set_bc(NEU), as in the reference), edge detector GRAD with
- * edge_detector_sigma = 0.  The default state is the reference's: the NEU flag (boundary
- * value 0) with DirGaussSeidel.
+ * What this layer runs.  With the built-in plug-ins -- cv = conv::acyclic_fftw_convolution
+ * (the reference's default) or conv::cyclic_fftw_convolution, onestep_poisson =
+ * lapl::DirGaussSeidel (bit-exact lexicographic sweeps), lapl::DirRedBlackGaussSeidel
+ * (red-black) or lapl::NeuGaussSeidel (installed by set_bc(NEU), as in the reference) --
+ * apply runs the fused device loop (tk_rsfe_*).  With any other function installed by set_cv
+ * or set_onestep_poisson, apply runs the reference's sequence stage by stage on device
+ * image3d objects and calls the installed functors, as the reference does (imp:1683-1936);
+ * that path leaves out the automatic connected component and the dumps (WARNING).  Edge
+ * detector GRAD.  The default state is the reference's: the NEU flag (boundary value 0)
+ * with DirGaussSeidel.  verbosity (default true, as the reference) prints the reference's
+ * per-iteration record to std::clog (imp:1655-1658, 1786-1793, 1835-1844).
  */
 #ifndef TK_RSFE_SPLITBREGMAN_HXX
 #define TK_RSFE_SPLITBREGMAN_HXX
@@ -218,21 +222,28 @@ public:
             tk::warning("set_bc: specify a proper bc_type condition");
         }
     }
+    // any convolution function (imp:836-842); a null function prints a WARNING and keeps
     void set_cv(conv::filtering<T> const cv)
     {
-        if (cv.getf() == &conv::cyclic_fftw_convolution<T> ||
-            cv.getf() == &conv::acyclic_fftw_convolution<T>)
-            cv_ = cv;
-        else tk::warning("set_cv: only conv::acyclic_fftw_convolution and "
-                         "conv::cyclic_fftw_convolution are built; kept");
+        if (cv.getf()) cv_ = cv;
+        else tk::warning("set_cv: null function; kept");
     }
+    // any one-sweep Poisson function (imp:845-852)
     void set_onestep_poisson(lapl::unsteady_poisson_functor<T> const osl)
     {
-        if (osl.getf() == &lapl::DirGaussSeidel<T> || osl.getf() == &lapl::DirRedBlackGaussSeidel<T> ||
-            osl.getf() == &lapl::NeuGaussSeidel<T>)
-            onestep_ = osl;
-        else tk::warning("set_onestep_poisson: only DirGaussSeidel, DirRedBlackGaussSeidel and "
-                         "NeuGaussSeidel are built; kept");
+        if (osl.getf()) onestep_ = osl;
+        else tk::warning("set_onestep_poisson: null function; kept");
+    }
+    // true when apply runs the fused device loop, false for the stage-by-stage path that
+    // calls the installed functors
+    bool fused() const
+    {
+        const bool cv = cv_.getf() == &conv::cyclic_fftw_convolution<T> ||
+                        cv_.getf() == &conv::acyclic_fftw_convolution<T>;
+        const bool gs = onestep_.getf() == &lapl::DirGaussSeidel<T> ||
+                        onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ||
+                        onestep_.getf() == &lapl::NeuGaussSeidel<T>;
+        return cv && gs;
     }
     // cap on the sweeps of one Poisson solve when ls_tol > 0 (0: none)
     void set_max_sweeps(int const m) { p_.max_sweeps = m; }
@@ -273,12 +284,17 @@ private:
                                                                            : TK_GS_LEX;
         p.acyclic = cv_.getf() == &conv::acyclic_fftw_convolution<T> ? 1 : 0;
         p.neumann = onestep_.getf() == &lapl::NeuGaussSeidel<T> ? 1 : 0;
+        p.verbosity = verbosity_ ? 1 : 0;
         // extract_connected_component's threshold (imp:481), in T as the reference
         p.cc_threshold = (double)((this->alpha - (T)p_.a0) / ((T)p_.b0 - (T)p_.a0));
         tk_rsfe s = nullptr;
-        tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
         sweeps_.clear();
         tols_.clear();
+        if (!fused()) {
+            run_generic(myim, init, p);
+            return;
+        }
+        tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
         const int n = p.maxiter > 0 ? p.maxiter : 0;
         std::vector<tk_rsfe_iter_stats> st((size_t)(n > 0 ? n : 1));
         int done = 0;
@@ -286,9 +302,8 @@ private:
         for (int t = 0; t < done; ++t) {
             sweeps_.push_back(st[t].sweeps);
             tols_.push_back(st[t].tollerance);
-            if (verbosity_)
-                std::clog << "--- iteration " << t + 1 << ": tollerance = " << st[t].tollerance
-                          << ", Gauss-Seidel sweeps = " << st[t].sweeps << std::endl;
+            if (verbosity_) log_iteration(t + 1, st[t].sweeps, st[t].ls_err, st[t].tol_inf,
+                                          st[t].tollerance);
         }
         if (r == TK_OK) {
             this->phi = im3d::image3d<T>(myim.getdimx(), myim.getdimy(), myim.getdimz(),
@@ -297,6 +312,132 @@ private:
         }
         tk_rsfe_destroy(s);
         tk::check(r, "apply");
+    }
+
+    // the reference's verbose record of one outer iteration (imp:1655-1658, 1786-1793,
+    // 1835-1844)
+    static void log_iteration(int t, int sweeps, double ls_err, double tol_inf, double tol)
+    {
+        std::clog << "-- Iteration " << t << std::endl;
+        std::clog << "\tlinear system error at " << sweeps << "th (last) step: " << (T)ls_err
+                  << std::endl;
+        std::clog << "\tnorm_inf: " << (T)tol_inf << "\tnorm_L2: " << (T)tol << std::endl;
+    }
+
+    // apply stage by stage (rsfe_splitbregman_imp.hxx:1641-1936) on device image3d objects,
+    // with the installed cv and onestep_poisson functors called where the reference calls
+    // them and every other step in the reference's order
+    void run_generic(im3d::image3d<T> const& myim, tk_vol init, tk_rsfe_params const& p)
+    {
+        if (p.auto_extract_conn_comp || p.dumpfrequency)
+            tk::warning("apply: the automatic connected component and the dumps run with the "
+                        "built-in cv and onestep_poisson only; ignored");
+        tk_ctx ctx = myim.ctx();
+        const uint X = myim.getdimx(), Y = myim.getdimy(), Z = myim.getdimz();
+        const double hx = myim.gethx(), hy = myim.gethy(), hz = myim.gethz();
+        auto field = [&]() { return im3d::image3d<T>(X, Y, Z, hx, hy, hz); };
+        // phi0: initialize_phi_with_init / _with_cube (imp:48-170), the fused path's own
+        im3d::image3d<T> phi = field();
+        {
+            tk_rsfe_params q = p;
+            q.verbosity = 0;
+            tk_rsfe s0 = nullptr;
+            tk::check(tk_rsfe_create(ctx, myim.handle(), init, &q, &s0), "apply");
+            const int r = tk_vol_copy(phi.handle(), tk_rsfe_phi(s0));
+            tk_rsfe_destroy(s0);
+            tk::check(r, "apply");
+        }
+        // filter, kones = cv(1), kmyim = cv(myim) (imp:1641-1651)
+        conv::filtering<T>::build_gaussian_filter(phi, (T)p.sigma, p.radius,
+                                                  p.gaussian_pixel_approach != 0);
+        im3d::image3d<T> kones, kmyim, aux = field();
+        aux = (T)1;
+        cv_(kones, aux);
+        cv_(kmyim, myim);
+        // gforshrink = edge_detect(myim) (imp:196-229, 1655); with edge_detector_sigma the
+        // image is first convolved by cv with that sigma, then the filter is rebuilt
+        im3d::image3d<T> g = field();
+        if (p.edge_detector_sigma != 0) {
+            conv::filtering<T>::build_gaussian_filter(phi, (T)p.edge_detector_sigma, 3,
+                                                      p.gaussian_pixel_approach != 0);
+            im3d::image3d<T> blurred;
+            cv_(blurred, myim);
+            tk::check(tk_edge_detect(ctx, blurred.handle(), p.beta, p.lamda, p.nu, g.handle()),
+                      "edge_detect");
+            conv::filtering<T>::build_gaussian_filter(phi, (T)p.sigma, p.radius,
+                                                      p.gaussian_pixel_approach != 0);
+        } else {
+            tk::check(tk_edge_detect(ctx, myim.handle(), p.beta, p.lamda, p.nu, g.handle()),
+                      "edge_detect");
+        }
+        std::vector<im3d::image3d<T> > b(3, field()), d;
+        im3d::image3d<T> divdb = field();
+        const double chosen_bc = p.bc_dir ? p.a0 : 0.;  // imp:1735-1743
+        double tollerance = 1000;
+        for (int t = 1; tollerance > p.tol && t <= p.maxiter; ++t) {
+            im3d::image3d<T> phiold(phi);
+            // region fit and forcing (imp:1690-1759)
+            im3d::image3d<T> H = field(), num, den, kr1, kr2;
+            tk::check(tk_heaviside(ctx, phi.handle(), p.a0, p.b0, p.epsilon, H.handle()),
+                      "set_heaviside");
+            cv_(num, H * myim);
+            cv_(den, H);
+            im3d::image3d<T> f1 = num / den;
+            num -= kmyim;
+            den -= kones;
+            num /= den;
+            im3d::image3d<T> f2 = num;
+            cv_(kr2, f1 * (T)p.lamda1 - f2 * (T)p.lamda2);
+            f1 *= f1;
+            f1 *= (T)p.lamda1;
+            f2 *= f2;
+            f2 *= (T)p.lamda2;
+            f1 -= f2;
+            cv_(kr1, f1);
+            kr2 *= myim;
+            kr2 *= 2;
+            aux.setdim(X, Y, Z, (T)((T)p.lamda1 - (T)p.lamda2));
+            aux *= myim;
+            aux *= myim;
+            aux *= kones;
+            kr2 -= aux;
+            kr2 -= kr1;
+            kr2 /= (T)p.lamda;
+            kr2 -= divdb;
+            // Poisson loop (imp:1762-1797); max_sweeps > 0 caps it
+            T ls_err = p.ls_tol != 0 ? (T)1000 : (T)-1;
+            int s;
+            for (s = 0; s < p.ls_steps || ls_err > (T)p.ls_tol; ++s) {
+                if (p.max_sweeps > 0 && s >= p.max_sweeps) break;
+                if ((verbosity_ && s == p.ls_steps - 1) || p.ls_tol != 0) aux = phi;
+                onestep_(phi, kr2, (T)p.dt, (T)chosen_bc);
+                if (p.ls_tol != 0 && s >= p.ls_steps - 1)
+                    ls_err = (phi - aux).normL2() / aux.normL2();
+            }
+            if (verbosity_ && p.ls_tol == 0) ls_err = (phi - aux).normL2() / aux.normL2();
+            // cut_phi (imp:320-339), tollerance (imp:1832)
+            tk::check(tk_cut(ctx, phi.handle(), p.a0, p.b0), "cut_phi");
+            double dn = 0, on = 0;
+            tk::check(tk_norm_l2_diff(ctx, phi.handle(), phiold.handle(), &dn), "tollerance");
+            tk::check(tk_norm_l2(ctx, phiold.handle(), &on), "tollerance");
+            tollerance = dn / on;
+            sweeps_.push_back(s);
+            tols_.push_back(tollerance);
+            if (verbosity_)
+                log_iteration(t, s, (double)ls_err,
+                              (double)((phi - phiold).norminf() / phiold.norminf()), tollerance);
+            // Split-Bregman update (imp:1868-1936)
+            phi.grad(d);
+            for (int i = 0; i < 3; ++i) b[i] += d[i];
+            for (int i = 0; i < 3; ++i)
+                tk::check(tk_shrink(ctx, b[i].handle(), g.handle(), d[i].handle()), "shrink");
+            for (int i = 0; i < 3; ++i) {
+                b[i] -= d[i];
+                d[i] -= b[i];
+            }
+            im3d::div(divdb, d);
+        }
+        this->phi = phi;
     }
 
     tk_rsfe_params p_;

@@ -14,6 +14,8 @@
 
 #include <unistd.h>
 
+#include <sstream>
+
 #include "../../include/tk/interface.hxx"
 #include "../../include/tk/rsfe_splitbregman.hxx"
 #include "../../oracle/rsfe_oracle.h"
@@ -36,6 +38,110 @@ static double maxabs(const std::vector<double>& a, const std::vector<double>& b)
     double m = 0;
     for (size_t i = 0; i < a.size(); ++i) m = std::fmax(m, std::fabs(a[i] - b[i]));
     return a.size() == b.size() ? m : INFINITY;
+}
+
+// user plug-ins: functions a caller of the reference could install with set_cv and
+// set_onestep_poisson; apply then runs stage by stage and calls them
+static int g_user_cv_calls = 0, g_user_gs_calls = 0;
+static void user_cyclic(im3d::image3d<double>& r, im3d::image3d<double> const& f)
+{
+    ++g_user_cv_calls;
+    conv::cyclic_fftw_convolution<double>(r, f);
+}
+static void user_gs(im3d::image3d<double>& res, im3d::image3d<double> const& b, double const& dt,
+                    double const& bc, im3d::image3d<double> const& input)
+{
+    ++g_user_gs_calls;
+    lapl::DirGaussSeidel<double>(res, b, dt, bc, input);
+}
+
+// examples/test.cxx:111-201 and 224-252 through the class layer
+static void image3d_operator_cases()
+{
+    im3d::image3d<double> image1, image2(3, 3, 3), image3(image2);
+    image1.setdim(3, 3, 3, 1.5);
+    auto all = [](im3d::image3d<double> const& im, double v) {
+        double e = 0;
+        for (double x : im.download()) e = std::fmax(e, std::fabs(x - v));
+        return e;
+    };
+    image2 = image1;
+    image3 = 3.;
+    double e = all(image1, 1.5) + all(image2, 1.5) + all(image3, 3.);
+    image1 += image2;
+    image3 += 1.5;
+    e += all(image1, 3.) + all(image3, 4.5);
+    image1 -= image2;
+    image3 -= 1.5;
+    e += all(image1, 1.5) + all(image3, 3.);
+    image1 *= image2;
+    image3 *= 1.5;
+    e += all(image1, 2.25) + all(image3, 4.5);
+    image1 /= image2;
+    image3 /= 1.5;
+    e += all(image1, 1.5) + all(image3, 3.) + all(image2, 1.5);
+    report("image3d compound operators (test.cxx:119-167)", e, 0);
+    e = 0;
+    image3 = image1 + image2;
+    e += all(image3, 3.);
+    image3 = image1 + 1.5;
+    e += all(image3, 3.);
+    image3 = image1 - image2;
+    e += all(image3, 0.);
+    image3 = image1 * 1.5;
+    e += all(image3, 2.25);
+    image3 = image1 / image2;
+    e += all(image3, 1.);
+    report("image3d binary operators (test.cxx:170-195)", e + all(image1, 1.5), 0);
+    image3 = 1.;
+    report("scalarprod / scalarprod_L2 (test.cxx:198-202)",
+           std::fabs(im3d::scalarprod(image1, image3) - 40.5) +
+               std::fabs(im3d::scalarprod_L2(image1, image3) - 40.5), 0);
+    // operator() write and read (test.cxx:101-107)
+    image1(1, 1, 1) = 2.75;
+    double a(image1(1, 1, 1));
+    image1(2, 0, 1) += 1.0;
+    std::vector<double> want(27, 1.5);
+    want[(1 * 3 + 1) * 3 + 1] = 2.75;
+    want[(2 * 3 + 0) * 3 + 1] = 2.5;
+    report("image3d element write", std::fabs(a - 2.75) + maxabs(image1.download(), want), 0);
+    // mismatched dims: WARNING, nothing done (image3d_imp.hxx:220-223); spacing for _L2
+    im3d::image3d<double> other(3, 3, 4, 1., 1., 1.);
+    other = 7.;
+    im3d::image3d<double> keep(image2);
+    keep += other;
+    im3d::image3d<double> spaced(3, 3, 3, 2., 1., 1.);
+    report("mismatched dims / spacing leave the image and give 0",
+           all(keep, 1.5) + std::fabs(im3d::scalarprod(image2, other)) +
+               std::fabs(im3d::scalarprod_L2(image2, spaced)), 0);
+    // norms on enumerate (test.cxx:246-252): 0..26 on 3^3
+    im3d::image3d<double> en(3, 3, 3);
+    std::vector<double> ev(27);
+    for (int i = 0; i < 27; ++i) ev[i] = i;
+    en.upload(ev);
+    report("norm1 / normL1 / norminf / max / min (test.cxx:246-252)",
+           std::fabs(en.norm1() - 351) + std::fabs(en.normL1() - 351) +
+               std::fabs(en.norminf() - 26) + std::fabs(en.max() - 26) + std::fabs(en.min()), 0);
+    // norminf quirk and norm2 against the oracle on signed data
+    std::mt19937_64 gen(99);
+    std::normal_distribution<double> N(0., 3.);
+    std::vector<double> sv(20 * 18 * 16);
+    for (auto& x : sv) x = N(gen);
+    sv[0] = -2.5;
+    sv[100] = -40.;
+    im3d::image3d<double> sg(20, 18, 16, 0.5, 1., 2.);
+    sg.upload(sv);
+    const long ns = (long)sv.size();
+    report("norminf quirk, norm2, normL1 vs oracle",
+           std::fabs(sg.norminf() - or_norminf(sv.data(), ns)) +
+               std::fabs(sg.norm2() - std::sqrt(or_scalarprod(sv.data(), sv.data(), ns))) +
+               std::fabs(sg.normL1() - or_norm1(sv.data(), ns)), 1e-11);
+    // vector_abs of the gradient of enumerate: sqrt(1 + 9 + 81) everywhere inside
+    std::vector<im3d::image3d<double> > grad;
+    en.grad(grad);
+    im3d::image3d<double> mag;
+    im3d::vector_abs(mag, grad);
+    report("vector_abs of grad(enumerate)", all(mag, std::sqrt(91.)), 1e-13);
 }
 
 static void run_cases()
@@ -374,6 +480,87 @@ static void run_cases()
                gp.get_iterations() == 3 ? 0 : 1, 0);
         std::remove(pot.c_str());
     }
+    image3d_operator_cases();
+
+    // ---- conv::direct_convolution (convolution_imp.hxx:761-812) --------------------------
+    {
+        im3d::image3d<double> img(X, Y, Z, 1., 1., 1.), res;
+        img.upload(raw);
+        conv::filtering<double>::build_gaussian_filter(img, 1.7);
+        conv::direct_convolution<double>(res, img);
+        int nh[3];
+        or_gaussian_dims(Z, 1., 1., 1., 1.7, 3., 0, nh);
+        std::vector<double> tf((long)nh[0] * nh[1] * nh[2]), ref(n);
+        or_gaussian_time_filter(Z, 1., 1., 1., 1.7, 3., 0, tf.data());
+        or_direct_acyclic_conv(tf.data(), nh, raw.data(), X, Y, Z, ref.data());
+        report("conv::direct_convolution vs the oracle's triple sum", maxabs(res.download(), ref),
+               1e-13);
+    }
+
+    // ---- set_cv / set_onestep_poisson with user functions: the stage-by-stage apply -------
+    {
+        const int SX = 24, SY = 20, SZ = 28;
+        const long m = (long)SX * SY * SZ;
+        std::vector<double> u0(m), init(m);
+        tk_gen_standard(SX, SY, SZ, 5, 1, 2, u0.data(), init.data());
+        im3d::image3d<double> myim(SX, SY, SZ), phi0(SX, SY, SZ);
+        myim.upload(u0);
+        phi0.upload(init);
+        myim.change_range_of_intensity(255.);
+        auto run = [&](conv::filtering<double>::function cv,
+                       lapl::unsteady_poisson_functor<double>::function gs, bool* fused) {
+            segm::rsfe_splitbregman<double> r;
+            r.set_verbosity(false);
+            r.set_cv(conv::filtering<double>(cv));
+            r.set_onestep_poisson(lapl::unsteady_poisson_functor<double>(gs));
+            r.set_sigma(2.0);
+            r.set_maxiter(4);
+            r.set_tol(0.);
+            *fused = r.fused();
+            r.apply(myim, phi0);
+            return std::make_pair(r.getoutput().download(), r.get_sweeps());
+        };
+        bool f0 = false, f1 = true, f2 = true, f3 = true;
+        auto builtin = run(conv::cyclic_fftw_convolution<double>, lapl::DirGaussSeidel<double>, &f0);
+        g_user_cv_calls = g_user_gs_calls = 0;
+        auto ucv = run(user_cyclic, lapl::DirGaussSeidel<double>, &f1);
+        const int cv_calls = g_user_cv_calls;
+        auto ugs = run(conv::cyclic_fftw_convolution<double>, user_gs, &f2);
+        const int gs_calls = g_user_gs_calls;
+        report("set_cv(user function): stage-by-stage apply equals the fused loop",
+               (f0 && !f1 && cv_calls == 2 + 4 * 4 && ucv.second == builtin.second)
+                   ? maxabs(ucv.first, builtin.first) : 1., 1e-9);
+        report("set_onestep_poisson(user function): stage-by-stage apply equals the fused loop",
+               (!f2 && gs_calls == 4 * 5 && ugs.second == builtin.second)
+                   ? maxabs(ugs.first, builtin.first) : 1., 1e-9);
+        // conv::direct_convolution as cv: the acyclic result of the fused default cv
+        auto acyc = run(conv::acyclic_fftw_convolution<double>, lapl::DirGaussSeidel<double>, &f3);
+        bool f4 = true;
+        auto dcv = run(conv::direct_convolution<double>, lapl::DirGaussSeidel<double>, &f4);
+        report("set_cv(direct_convolution) equals the fused acyclic loop",
+               (f3 && !f4) ? maxabs(dcv.first, acyc.first) : 1., 1e-9);
+        // the verbose record on std::clog (imp:1655-1658, 1786-1793, 1835-1844), both paths
+        std::ostringstream cap;
+        std::streambuf* old = std::clog.rdbuf(cap.rdbuf());
+        for (bool user : {false, true}) {
+            segm::rsfe_splitbregman<double> r;
+            r.set_cv(conv::filtering<double>(user ? user_cyclic : conv::cyclic_fftw_convolution<double>));
+            r.set_sigma(2.0);
+            r.set_maxiter(2);
+            r.set_tol(0.);
+            r.apply(myim, phi0);
+        }
+        std::clog.rdbuf(old);
+        const std::string log = cap.str();
+        size_t cnt = 0;
+        for (size_t q = log.find("norm_inf: "); q != std::string::npos; q = log.find("norm_inf: ", q + 1))
+            ++cnt;
+        report("verbose record: 2 iterations x 2 paths on std::clog",
+               (log.find("-- Iteration 2") != std::string::npos &&
+                log.find("linear system error at 5th (last) step: ") != std::string::npos &&
+                cnt == 4) ? 0 : 1, 0);
+    }
+
     for (const char* f : {"img.mhd", "img.raw", "res_3.mhd", "res_3.raw", "res_4_final.mhd",
                           "res_4_final.raw"})
         std::remove((std::string(dir) + "/" + f).c_str());

@@ -310,31 +310,66 @@ def test_c3_batch_f32_lex_10_iterations_vs_golden(tk, ctx):
     assert dices.min() >= 0.999
 
 
+def c3_masks(tk, ctx, dims, nell, dtype, order, n_iter=30):
+    b = c3_batch(tk, ctx, dims, nell, list(range(64)), dtype=dtype, gs_order=order)
+    b.iterate(n_iter, use_graph=True)
+    out = [(b.phi(v) >= 0.5).reshape(-1) for v in range(64)]
+    b.close()
+    return out
+
+
+def c3_dices(a, b):
+    return np.array([dice(x, y) for x, y in zip(a, b)])
+
+
+def c3_golden_masks(d):
+    return [np.unpackbits(d[f"mask_{v}"])[:int(np.prod(d["dims"][v]))].astype(bool)
+            for v in range(64)]
+
+
+def say_dices(name, dc):
+    low = [(int(v), round(float(dc[v]), 5)) for v in np.argsort(dc)[:4]]
+    say(f"C3 {name}: Dice min {dc.min():.6f} mean {dc.mean():.6f}, lowest (volume, Dice) {low}")
+
+
+def test_c3_batch_f64_lex_vs_golden(tk, ctx):
+    """the 64 uneven volumes in FP64 with the reference's (lexicographic) sweeps after 30
+    outer iterations: every mask equals the FP64 oracle's"""
+    d = np.load(os.path.join(GOLDEN, "c3_batch.npz"))
+    dc = c3_dices(c3_masks(tk, ctx, d["dims"], d["nell"], np.float64, tk.TK_GS_LEX),
+                  c3_golden_masks(d))
+    say_dices("FP64 lex vs FP64 oracle", dc)
+    assert dc.min() >= 0.99999
+
+
 @pytest.mark.parametrize("order", ["lex", "redblack"])
 def test_c3_batch_f32_vs_golden(tk, ctx, order):
-    """per-volume Dice of the FP32 batch against the FP64 oracle.  Lexicographic sweeps
-    follow the reference's iteration, so what remains is precision; red-black sweeps (5
-    per outer iteration, far from converged) follow another iteration, stated."""
+    """per-volume Dice of the FP32 batch against the FP64 oracle, split into its two causes.
+    Precision: the FP32 batch against the FP64 batch of the same sweep order; every volume
+    but one agrees to >= 0.995, and volume 14 (160x96x128, 3 ellipsoids) to ~0.954 in either
+    order.  The FP64 oracle itself does not move on that volume when its input is rounded to
+    FP32 or raised by 1e-7 at every 7th voxel (tests/golden/c3_sensitivity.json), so the
+    deviation comes from FP32 arithmetic inside the loop, not from an unstable trajectory.
+    Sweep order: lexicographic FP64 equals the oracle (test above); red-black sweeps (5 per
+    outer iteration, far from converged) follow another iteration, in FP64 as well: volume 41
+    ends at Dice 0.814 and volume 60 at 0.905 whatever the precision."""
     d = np.load(os.path.join(GOLDEN, "c3_batch.npz"))
     dims, nell = d["dims"], d["nell"]
-    vols = list(range(64))
-    b = c3_batch(tk, ctx, dims, nell, vols,
-                 gs_order=tk.TK_GS_LEX if order == "lex" else tk.TK_GS_REDBLACK)
-    b.iterate(30, use_graph=True)
-    dices = []
-    for v in vols:
-        n = int(np.prod(dims[v]))
-        ref = np.unpackbits(d[f"mask_{v}"])[:n].astype(bool)
-        got = (b.phi(v) >= 0.5).reshape(-1)
-        dices.append(dice(got, ref))
-    b.close()
-    dices = np.array(dices)
-    say(f"C3 64 volumes FP32 {order} vs FP64 oracle: Dice min {dices.min():.5f} "
-        f"mean {dices.mean():.5f}, below 0.99: {int((dices < 0.99).sum())}, "
-        f"below 0.999: {int((dices < 0.999).sum())}")
+    gs = tk.TK_GS_LEX if order == "lex" else tk.TK_GS_REDBLACK
+    m32 = c3_masks(tk, ctx, dims, nell, np.float32, gs)
+    m64 = c3_masks(tk, ctx, dims, nell, np.float64, gs)
+    ref = c3_golden_masks(d)
+    prec, vs_ref, order_part = c3_dices(m32, m64), c3_dices(m32, ref), c3_dices(m64, ref)
+    say_dices(f"FP32 {order} vs FP64 {order} (precision)", prec)
+    say_dices(f"FP64 {order} vs FP64 oracle (sweep order)", order_part)
+    say_dices(f"FP32 {order} vs FP64 oracle", vs_ref)
+    sens = json.load(open(os.path.join(GOLDEN, "c3_sensitivity.json")))
+    assert min(min(v for k, v in r.items() if k.startswith(("input", "every"))) for r in
+               sens.values()) == 1.0
+    assert prec.min() >= 0.95 and int((prec < 0.995).sum()) <= 1 and prec.mean() >= 0.999
     if order == "lex":
-        # single volumes can drift like the CT trajectory does (rounding-level differences
-        # grow along an unstable level-set evolution); the bulk agrees to 1e-3
-        assert dices.mean() >= 0.999 and int((dices < 0.99).sum()) <= 2
+        assert order_part.min() >= 0.99999
+        assert vs_ref.min() >= 0.95 and int((vs_ref < 0.995).sum()) <= 1
     else:
-        assert dices.mean() >= 0.95
+        assert order_part.min() >= 0.80 and order_part.mean() >= 0.975
+        assert vs_ref.min() >= 0.80 and vs_ref.mean() >= 0.975
