@@ -7,6 +7,12 @@ namespace tk {
 
 constexpr int kWave = 64;
 
+// exact quotient of small non-negative integers (n < 2^13, d <= 2^12) through a float
+// reciprocal (the hardware has no integer divide): the distance of (n + 1/2)/d to an
+// integer is >= 1/(2d), and the rounding of 1/d and of the product moves it by less than
+// (n + 1/2)/d * 2^-23 < 2^-10/d, so the truncation is exact
+__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
 // wave64 sum of a double (all lanes get the result)
 __device__ inline double wave_sum(double v)
 {

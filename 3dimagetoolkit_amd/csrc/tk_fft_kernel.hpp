@@ -142,11 +142,6 @@ __device__ __forceinline__ void dftR(cplx<T>* a, int s)
     else dft16(a, s);
 }
 
-// exact quotient of small non-negative integers (n < 2^13, d <= 2^12) through a float
-// reciprocal: the distance of (n + 1/2)/d to an integer is >= 1/(2d) >> the rounding
-// error of the product, so the truncation is exact
-__device__ __forceinline__ int qdiv(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
-
 // LDS layout of a group of P pencils of length L: element (pos, p) at pos*SP + p*SQ.
 // x/y passes use SP = P+1, SQ = 1 (pencils interleaved); z passes use SP = 1,
 // SQ = L+1 (pencils as padded rows), so the contiguous global rows map to LDS rows.

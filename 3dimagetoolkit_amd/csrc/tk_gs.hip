@@ -162,7 +162,9 @@ inline long boundary_count(int X, int Y, int Z)
 // ---- lexicographic pipelined tile wavefront --------------------------------------
 constexpr int kLexT = 8;    // tile edge in x and y (one lane per column)
 constexpr int kLexTZ = 32;  // tile depth in z
-constexpr int kLexBatch = 32;  // tile loads per lane in flight at once
+constexpr int kLexBatch = 16;  // tile loads per lane in flight at once
+constexpr int kLexNT = 256;    // threads per tile: staging and stores use all four waves, the
+                               // in-tile wavefront the first (one lane per (i, j) column)
 // the staging loop records which of a lane's kLexBatch loads fell outside the volume as one
 // bit each of a 32-bit mask
 static_assert(kLexBatch <= 32, "kLexBatch loads per lane must fit the 32-bit out-of-volume mask");
@@ -183,9 +185,44 @@ __device__ __forceinline__ T neu_update(T xp, T xm, T yp, T ym, T zp, T zm, T b,
         return (tx / (T)c.hx2 + ty / (T)c.hy2 + tz / (T)c.hz2 + b + self / (T)c.dt) * (T)ht[m];
 }
 
-// NEU: tiles cover every voxel (Neumann sweep), else the interior (Dirichlet sweep)
+// Tiles (I, J, K) of an a x b x c tile grid on the diagonal plane I + J + K = d: how many,
+// and the n-th of them in the order (I, J) increasing.  Step tau of a chunk of C sweeps
+// runs sweep s on the plane d = tau - 2 s, so a launch holds only those planes' tiles.
+__host__ __device__ inline int lex_plane_count(int d, int a, int b, int c)
+{
+    int n = 0;
+    const int i1 = d < a - 1 ? d : a - 1;
+    for (int I = d - (b - 1) - (c - 1) > 0 ? d - (b - 1) - (c - 1) : 0; I <= i1; ++I) {
+        const int m = d - I, lo = m - (c - 1) > 0 ? m - (c - 1) : 0, hi = m < b - 1 ? m : b - 1;
+        if (hi >= lo) n += hi - lo + 1;
+    }
+    return n;
+}
+__device__ inline void lex_plane_tile(int d, int n, int a, int b, int c, int& I, int& J, int& K)
+{
+    for (I = max(0, d - (b - 1) - (c - 1)); I < a; ++I) {
+        const int m = d - I, lo = max(0, m - (c - 1)), hi = min(b - 1, m);
+        const int cnt = hi >= lo ? hi - lo + 1 : 0;
+        if (n < cnt) {
+            J = lo + n;
+            K = m - J;
+            return;
+        }
+        n -= cnt;
+    }
+}
+// the sweeps of a chunk whose planes are on the grid at step tau
+__host__ __device__ inline void lex_step_sweeps(int tau, int C, int dmax, int& s_lo, int& s_hi)
+{
+    s_lo = tau - dmax > 0 ? (tau - dmax + 1) / 2 : 0;
+    s_hi = tau / 2 < C - 1 ? tau / 2 : C - 1;
+}
+
+// NEU: tiles cover every voxel (Neumann sweep), else the interior (Dirichlet sweep).  One
+// workgroup per tile on the active planes of step tau (blockIdx.x enumerates them, sweep by
+// sweep).
 template <class T, bool UNIT, bool NEU>
-__global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __restrict__ rhs,
+__global__ __launch_bounds__(kLexNT) void k_gs_lex(T* __restrict__ phi, const T* __restrict__ rhs,
                                                int X, int Y, int Z, int nTx, int nTy, int nTz,
                                                int tau, int C, GsCoef c, NeuCoef nc,
                                                double* __restrict__ acc)
@@ -196,11 +233,21 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
     __shared__ T ph[HX * HY * HZ];
     __shared__ T rb[TX * TY * TZ];
     __shared__ double nct[8 + 12];
-    const int tile = blockIdx.x;
-    const int K = tile % nTz, J = (tile / nTz) % nTy, I = tile / (nTz * nTy);
-    const int s2 = tau - (I + J + K);
-    if (s2 < 0 || (s2 & 1) || (s2 >> 1) >= C) return;
-    const int s = s2 >> 1;
+    int s_lo, s_hi, s = -1, I = 0, J = 0, K = 0;
+    lex_step_sweeps(tau, C, nTx + nTy + nTz - 3, s_lo, s_hi);
+    {
+        int n = (int)blockIdx.x;
+        for (int q = s_lo; q <= s_hi; ++q) {
+            const int d = tau - 2 * q, cnt = lex_plane_count(d, nTx, nTy, nTz);
+            if (n < cnt) {
+                s = q;
+                lex_plane_tile(d, n, nTx, nTy, nTz, I, J, K);
+                break;
+            }
+            n -= cnt;
+        }
+    }
+    if (s < 0) return;
     const int i0 = O + I * TX, j0 = O + J * TY, k0 = O + K * TZ;
     const int tx = min(TX, X - O - i0), ty = min(TY, Y - O - j0), tz = min(TZ, Z - O - k0);
     const long sY = Z, sX = (long)Y * Z;
@@ -215,42 +262,50 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
     // the iteration that issues it is waited for before the next one goes out), and every
     // lane issues all of them: an index past the end is clamped to the last element and its
     // write skipped, so the compiler need not wait between conditional loads
+    // element -> (ii, jj, kk) by exact float-reciprocal quotients (nh <= 3400, divisors <= 34):
+    // four 32-bit integer divisions per element made the staging the tile's longest phase
     const int hz = tz + 2;
     const int nh = (tx + 2) * (ty + 2) * hz, nr = tx * ty * tz;
-    for (int e0 = lane; e0 < nh; e0 += 64 * kLexBatch) {
+    const float inv_hz = 1.0f / (float)hz, inv_hy = 1.0f / (float)(ty + 2);
+    const float inv_tz = 1.0f / (float)tz, inv_ty = 1.0f / (float)ty;
+    for (int e0 = lane; e0 < nh; e0 += kLexNT * kLexBatch) {
         T v[kLexBatch];
         int at[kLexBatch];
         unsigned zero = 0;  // Neumann halo positions outside the volume hold 0
 #pragma unroll
         for (int u = 0; u < kLexBatch; ++u) {
-            const int e = min(e0 + 64 * u, nh - 1);
-            const int kk = e % hz, r = e / hz, jj = r % (ty + 2), ii = r / (ty + 2);
+            const int e = min(e0 + kLexNT * u, nh - 1);
+            const int r = qdiv(e, inv_hz), kk = e - r * hz;
+            const int ii = qdiv(r, inv_hy), jj = r - ii * (ty + 2);
             const int gi = i0 - 1 + ii, gj = j0 - 1 + jj, gk = k0 - 1 + kk;
             const bool out =
                 NEU && (gi < 0 || gi >= X || gj < 0 || gj >= Y || gk < 0 || gk >= Z);
             v[u] = phi[out ? 0 : gi * sX + gj * sY + gk];
             zero |= (unsigned)out << u;
-            at[u] = e0 + 64 * u < nh ? (ii * HY + jj) * HZ + kk : -1;
+            at[u] = e0 + kLexNT * u < nh ? (ii * HY + jj) * HZ + kk : -1;
         }
 #pragma unroll
         for (int u = 0; u < kLexBatch; ++u)
             if (at[u] >= 0) ph[at[u]] = (zero >> u) & 1u ? (T)0 : v[u];
     }
-    for (int e0 = lane; e0 < nr; e0 += 64 * kLexBatch) {
+    for (int e0 = lane; e0 < nr; e0 += kLexNT * kLexBatch) {
         T v[kLexBatch];
         int at[kLexBatch];
 #pragma unroll
         for (int u = 0; u < kLexBatch; ++u) {
-            const int e = min(e0 + 64 * u, nr - 1);
-            const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
+            const int e = min(e0 + kLexNT * u, nr - 1);
+            const int r = qdiv(e, inv_tz), kk = e - r * tz;
+            const int ii = qdiv(r, inv_ty), jj = r - ii * ty;
             v[u] = rhs[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)];
-            at[u] = e0 + 64 * u < nr ? (ii * TY + jj) * TZ + kk : -1;
+            at[u] = e0 + kLexNT * u < nr ? (ii * TY + jj) * TZ + kk : -1;
         }
 #pragma unroll
         for (int u = 0; u < kLexBatch; ++u)
             if (at[u] >= 0) rb[at[u]] = v[u];
     }
     __syncthreads();
+    // lanes of the first wave own the (a, b) columns; a >= TX for the other waves, so they
+    // only meet the barriers
     const int a = lane / TY, b = lane % TY;
     double d2 = 0, o2 = 0;
     const int steps = tz + (tx - 1) + (ty - 1);
@@ -283,12 +338,13 @@ __global__ __launch_bounds__(64) void k_gs_lex(T* __restrict__ phi, const T* __r
         }
         __syncthreads();
     }
-    for (int e = lane; e < tx * ty * tz; e += 64) {
-        const int kk = e % tz, r = e / tz, jj = r % ty, ii = r / ty;
+    for (int e = lane; e < tx * ty * tz; e += kLexNT) {
+        const int r = qdiv(e, inv_tz), kk = e - r * tz;
+        const int ii = qdiv(r, inv_ty), jj = r - ii * ty;
         phi[(i0 + ii) * sX + (j0 + jj) * sY + (k0 + kk)] =
             ph[((ii + 1) * HY + (jj + 1)) * HZ + (kk + 1)];
     }
-    if (acc) {
+    if (acc && lane < 64) {
         d2 = wave_sum(d2);
         o2 = wave_sum(o2);
         if (lane == 0) {
@@ -307,14 +363,18 @@ static int lex_chunk(tk_ctx ctx, tk_vol phi, tk_vol rhs, const GsCoef& c, int C,
     const int e = neu ? 0 : 2;  // voxels left out along each axis
     const int nTx = (X - e + kLexT - 1) / kLexT, nTy = (Y - e + kLexT - 1) / kLexT,
               nTz = (Z - e + kLexTZ - 1) / kLexTZ;
-    const int nT = nTx * nTy * nTz;
     const int last = (nTx - 1) + (nTy - 1) + (nTz - 1) + 2 * (C - 1);
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
     const NeuCoef nc = neu ? *neu : NeuCoef{};
 #define TK_LEX(U, N)                                                                           \
-    hipLaunchKernelGGL((k_gs_lex<T, U, N>), dim3(nT), dim3(64), 0, ctx->stream, (T*)phi->d,     \
+    hipLaunchKernelGGL((k_gs_lex<T, U, N>), dim3(nb), dim3(kLexNT), 0, ctx->stream, (T*)phi->d, \
                        (const T*)rhs->d, X, Y, Z, nTx, nTy, nTz, tau, C, c, nc, acc)
+    const int dmax = nTx + nTy + nTz - 3;
     for (int tau = 0; tau <= last; ++tau) {
+        int s_lo, s_hi, nb = 0;
+        lex_step_sweeps(tau, C, dmax, s_lo, s_hi);
+        for (int q = s_lo; q <= s_hi; ++q) nb += lex_plane_count(tau - 2 * q, nTx, nTy, nTz);
+        if (nb == 0) continue;
         TimedLaunch tl(ctx, TK_K_GS);
         if (neu) {
             if (unit) TK_LEX(true, true);

@@ -52,25 +52,22 @@ __global__ void k_unet_output(const float* __restrict__ out, long n, double s, T
 }
 
 // coarse[c] = mean of the f x f x f fine voxels of coarse voxel c (fine voxels f*c .. f*c+f-1
-// per axis); one thread per coarse voxel, FP64 sum
+// per axis); grid (z blocks, cy, cx), one thread per coarse voxel, FP64 sum
 template <class T>
 __global__ __launch_bounds__(256) void k_restrict_avg(const T* __restrict__ fine,
                                                       T* __restrict__ coarse, int cx, int cy,
                                                       int cz, int f)
 {
-    const long nc = (long)cx * cy * cz;
+    const int k = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if (k >= cz) return;
     const int FY = cy * f, FZ = cz * f;
-    const double inv = 1.0 / ((double)f * f * f);
-    for (long c = blockIdx.x * 256L + threadIdx.x; c < nc; c += (long)gridDim.x * 256) {
-        const int k = (int)(c % cz), j = (int)((c / cz) % cy), i = (int)(c / ((long)cy * cz));
-        double s = 0;
-        for (int a = 0; a < f; ++a)
-            for (int b = 0; b < f; ++b) {
-                const T* row = fine + ((long)(i * f + a) * FY + (j * f + b)) * FZ + (long)k * f;
-                for (int e = 0; e < f; ++e) s += (double)row[e];
-            }
-        coarse[c] = (T)(s * inv);
-    }
+    double s = 0;
+    for (int a = 0; a < f; ++a)
+        for (int b = 0; b < f; ++b) {
+            const T* row = fine + ((long)(i * f + a) * FY + (j * f + b)) * FZ + (long)k * f;
+            for (int e = 0; e < f; ++e) s += (double)row[e];
+        }
+    coarse[((long)i * cy + j) * cz + k] = (T)(s / ((double)f * f * f));
 }
 
 // interpolation weight along one axis: fine index q of a cell-centred coarse grid of n
@@ -83,32 +80,31 @@ __device__ __forceinline__ void prolong_w(int q, int f, int n, int& c0, double& 
     w = n > 1 ? x - c0 : 0.0;
 }
 
-// fine[v] += gain * trilinear(coarse) at fine voxel v
+// fine[v] += gain * trilinear(coarse) at fine voxel v; grid (z blocks, FY, FX)
 template <class T>
 __global__ __launch_bounds__(256) void k_prolong_add(const T* __restrict__ coarse,
                                                      T* __restrict__ fine, int cx, int cy, int cz,
                                                      int f, double gain)
 {
-    const int FX = cx * f, FY = cy * f, FZ = cz * f;
-    const long nf = (long)FX * FY * FZ;
+    const int FZ = cz * f;
+    const int k = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+    if (k >= FZ) return;
     const long sy = cz, sx = (long)cy * cz;
-    for (long v = blockIdx.x * 256L + threadIdx.x; v < nf; v += (long)gridDim.x * 256) {
-        const int k = (int)(v % FZ), j = (int)((v / FZ) % FY), i = (int)(v / ((long)FY * FZ));
-        int a, b, c;
-        double wa, wb, wc;
-        prolong_w(i, f, cx, a, wa);
-        prolong_w(j, f, cy, b, wb);
-        prolong_w(k, f, cz, c, wc);
-        const int da = cx > 1 ? 1 : 0, db = cy > 1 ? 1 : 0, dc = cz > 1 ? 1 : 0;
-        const T* p = coarse + a * sx + b * sy + c;
-        auto at = [&](int x, int y, int z) { return (double)p[x * da * sx + y * db * sy + z * dc]; };
-        const double v00 = at(0, 0, 0) * (1 - wc) + at(0, 0, 1) * wc;
-        const double v01 = at(0, 1, 0) * (1 - wc) + at(0, 1, 1) * wc;
-        const double v10 = at(1, 0, 0) * (1 - wc) + at(1, 0, 1) * wc;
-        const double v11 = at(1, 1, 0) * (1 - wc) + at(1, 1, 1) * wc;
-        const double v0 = v00 * (1 - wb) + v01 * wb, v1 = v10 * (1 - wb) + v11 * wb;
-        fine[v] = (T)((double)fine[v] + gain * (v0 * (1 - wa) + v1 * wa));
-    }
+    int a, b, c;
+    double wa, wb, wc;
+    prolong_w(i, f, cx, a, wa);
+    prolong_w(j, f, cy, b, wb);
+    prolong_w(k, f, cz, c, wc);
+    const int da = cx > 1 ? 1 : 0, db = cy > 1 ? 1 : 0, dc = cz > 1 ? 1 : 0;
+    const T* p = coarse + a * sx + b * sy + c;
+    auto at = [&](int x, int y, int z) { return (double)p[x * da * sx + y * db * sy + z * dc]; };
+    const double v00 = at(0, 0, 0) * (1 - wc) + at(0, 0, 1) * wc;
+    const double v01 = at(0, 1, 0) * (1 - wc) + at(0, 1, 1) * wc;
+    const double v10 = at(1, 0, 0) * (1 - wc) + at(1, 0, 1) * wc;
+    const double v11 = at(1, 1, 0) * (1 - wc) + at(1, 1, 1) * wc;
+    const double v0 = v00 * (1 - wb) + v01 * wb, v1 = v10 * (1 - wb) + v11 * wb;
+    const long v = ((long)i * cy * f + j) * FZ + k;
+    fine[v] = (T)((double)fine[v] + gain * (v0 * (1 - wa) + v1 * wa));
 }
 
 // MSE: partial sums of (out - t)^2 per block; g[v] = bf16(2 (out - t) / n)
@@ -632,6 +628,8 @@ static int coarse_factor(tk_vol fine, tk_vol coarse, const char* what, int* f)
     TK_CHECK(q >= 1 && fine->ny == q * coarse->ny && fine->nz == q * coarse->nz, TK_ERR_SHAPE,
              "%s: fine dims must be the same integer multiple of the coarse dims on every axis",
              what);
+    TK_CHECK(fine->nx <= 65535 && fine->ny <= 65535, TK_ERR_UNSUPPORTED,
+             "%s: x and y edges above 65535 voxels", what);
     *f = q;
     return TK_OK;
 }
@@ -641,13 +639,13 @@ int tk_vol_restrict(tk_ctx ctx, tk_vol fine, tk_vol coarse)
     TK_CHECK(ctx, TK_ERR_ARG, "tk_vol_restrict: null context");
     int f = 0;
     TK_TRY(coarse_factor(fine, coarse, "tk_vol_restrict", &f));
-    const unsigned nb = nb_for(coarse->n());
+    const dim3 grid((coarse->nz + 255) / 256, coarse->ny, coarse->nx);
     if (fine->dtype == TK_F64)
-        hipLaunchKernelGGL(k_restrict_avg<double>, dim3(nb), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_restrict_avg<double>, grid, dim3(256), 0, ctx->stream,
                            (const double*)fine->d, (double*)coarse->d, coarse->nx, coarse->ny,
                            coarse->nz, f);
     else
-        hipLaunchKernelGGL(k_restrict_avg<float>, dim3(nb), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_restrict_avg<float>, grid, dim3(256), 0, ctx->stream,
                            (const float*)fine->d, (float*)coarse->d, coarse->nx, coarse->ny,
                            coarse->nz, f);
     TK_LAUNCHED();
@@ -659,13 +657,13 @@ int tk_vol_prolong_add(tk_ctx ctx, tk_vol coarse, tk_vol fine, double gain)
     TK_CHECK(ctx, TK_ERR_ARG, "tk_vol_prolong_add: null context");
     int f = 0;
     TK_TRY(coarse_factor(fine, coarse, "tk_vol_prolong_add", &f));
-    const unsigned nb = nb_for(fine->n());
+    const dim3 grid((fine->nz + 255) / 256, fine->ny, fine->nx);
     if (fine->dtype == TK_F64)
-        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nb), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_prolong_add<double>, grid, dim3(256), 0, ctx->stream,
                            (const double*)coarse->d, (double*)fine->d, coarse->nx, coarse->ny,
                            coarse->nz, f, gain);
     else
-        hipLaunchKernelGGL(k_prolong_add<float>, dim3(nb), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_prolong_add<float>, grid, dim3(256), 0, ctx->stream,
                            (const float*)coarse->d, (float*)fine->d, coarse->nx, coarse->ny,
                            coarse->nz, f, gain);
     TK_LAUNCHED();
