@@ -38,7 +38,8 @@ def _run(cmd):
 
 def build_lib(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp"))
+    srcs = sorted((f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp")),
+                  key=lambda f: -os.path.getsize(os.path.join(CSRC, f)))
     cmds = []
     objs = []
     for s in srcs:
@@ -60,7 +61,8 @@ def build_lib(jobs=8):
         if any(ln.strip() for ln in lines):
             sys.stderr.write("\n".join(lines) + "\n")
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    # -z defs: an instance a translation unit expects from another one must exist
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

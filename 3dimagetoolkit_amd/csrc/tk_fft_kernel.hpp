@@ -270,7 +270,8 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
     for (int e = threadIdx.x; e < 2 * S * y.P; e += NT) {
         const int p = e & (y.P - 1), t = e >> y.lp;
         const int pos = t < S ? t - S : L + (t - S);
-        int src = pos % L;
+        // one wrap when S <= L (no integer division); shorter pencils wrap repeatedly
+        int src = S <= L ? (pos < 0 ? pos + L : pos - L) : pos % L;
         if (src < 0) src += L;
         buf[y.off + pos * y.SP + p * y.SQ] =
             y.zero ? cplx<T>{0, 0} : buf[y.off + src * y.SP + p * y.SQ];
@@ -283,7 +284,7 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
     // z pass (pencils SQ apart, positions adjacent): a wave's 16- or 8-pencil lane groups
     // take blocks 4 or 2 apart inside each run of 8 blocks, so their reads fall 32 or 16
     // dwords apart in the LDS banks (a bijection on the blocks when L/B is a multiple of 8)
-    const int G = 32 >> y.lp;
+    const int G = 32 >> y.lp, lg = 5 - y.lp;
     const bool swz = y.SP == 1 && (G == 2 || G == 4) && (L / B) % 8 == 0;
     cplx<T> out[MAXI][B];
     int dst[MAXI];
@@ -293,7 +294,10 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
         dst[q] = -1;
         if (it < items) {
             const int p = it & (y.P - 1), br = it >> y.lp;
-            const int b = swz ? (br % G) * (8 / G) + (br / G) % (8 / G) + (br & ~7) : br;
+            // G = 2^lg: (br % G) * (8 / G) + (br / G) % (8 / G) + (br & ~7) with shifts
+            const int b = swz ? ((br & (G - 1)) << (3 - lg)) + ((br >> lg) & ((8 >> lg) - 1)) +
+                                    (br & ~7)
+                              : br;
             const int base = y.off + (b * B - S) * y.SP + p * y.SQ;
             cplx<T> x[W];
 #pragma unroll

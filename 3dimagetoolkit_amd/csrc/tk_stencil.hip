@@ -153,13 +153,36 @@ __global__ __launch_bounds__(256) void k_sb_update(T* __restrict__ phi, const T*
 // of a line; the x and y neighbours are whole rows, the z neighbours a (V+2)-window.
 // Arithmetic and operand order are those of the per-voxel kernels above.
 
+// x / (2h), the reference's difference quotient.  With P2 (2h a power of two, unit spacing
+// included) the product with the reciprocal, itself exact, is the same number (both round
+// the exact quotient once) for one instruction instead of a division sequence.
+template <class T, bool P2>
+__device__ __forceinline__ T q2h(T x, double h)
+{
+    if constexpr (P2) return x * (T)(0.5 / h);
+    else return x / (T)(2. * h);
+}
+
+// 2h is a power of two well inside the range of either type
+__host__ __device__ inline bool pow2_2h(double h)
+{
+    int e = 0;
+    const double m = frexp(2. * h, &e);
+    return m == 0.5 && e > -100 && e < 100;
+}
+
+__host__ __device__ inline bool pow2_2h3(double hx, double hy, double hz)
+{
+    return pow2_2h(hx) && pow2_2h(hy) && pow2_2h(hz);
+}
+
 // derivative along z of element e of a window c[0..V+1] (c[1+e] = f(k0+e))
-template <class T, int V>
+template <class T, int V, bool P2>
 __device__ __forceinline__ T dz_win(const T* c, int e, int k, int Z, double h)
 {
-    if (k == 0) return (4 * c[2] - 3 * c[1] - c[3]) / (T)(2. * h);
-    if (k == Z - 1) return (3 * c[e + 1] - 4 * c[e] + c[e - 1]) / (T)(2. * h);
-    return (c[e + 2] - c[e]) / (T)(2. * h);
+    if (k == 0) return q2h<T, P2>(4 * c[2] - 3 * c[1] - c[3], h);
+    if (k == Z - 1) return q2h<T, P2>(3 * c[e + 1] - 4 * c[e] + c[e - 1], h);
+    return q2h<T, P2>(c[e + 2] - c[e], h);
 }
 
 // The rows of f that d/dx (or d/dy) of the row at index `idx` reads, along an axis of
@@ -190,22 +213,22 @@ __device__ __forceinline__ DRows<T> drow_load(const T* __restrict__ f, long v0, 
 
 // central -> (f(+1) - f(-1)) / 2h; first -> (4 f(1) - 3 f(0) - f(2)) / 2h; last ->
 // (3 f(n-1) - 4 f(n-2) + f(n-3)) / 2h, each value through clampf
-template <class T, class Fn>
+template <class T, bool P2, class Fn>
 __device__ __forceinline__ void drow_eval(const DRows<T>& d, double h, Fn clampf, T* out)
 {
     constexpr int V = Vec16<T>::n;
     const Row<T>&a = d.r[0], &b = d.r[1], &c = d.r[2];
     if (d.c == 1) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) out[e] = (clampf(a.v[e]) - clampf(b.v[e])) / (T)(2. * h);
+        for (int e = 0; e < V; ++e) out[e] = q2h<T, P2>(clampf(a.v[e]) - clampf(b.v[e]), h);
     } else if (d.c == 0) {
 #pragma unroll
         for (int e = 0; e < V; ++e)
-            out[e] = (4 * clampf(b.v[e]) - 3 * clampf(a.v[e]) - clampf(c.v[e])) / (T)(2. * h);
+            out[e] = q2h<T, P2>(4 * clampf(b.v[e]) - 3 * clampf(a.v[e]) - clampf(c.v[e]), h);
     } else {
 #pragma unroll
         for (int e = 0; e < V; ++e)
-            out[e] = (3 * clampf(a.v[e]) - 4 * clampf(b.v[e]) + clampf(c.v[e])) / (T)(2. * h);
+            out[e] = q2h<T, P2>(3 * clampf(a.v[e]) - 4 * clampf(b.v[e]) + clampf(c.v[e]), h);
     }
 }
 
@@ -218,7 +241,7 @@ __device__ __forceinline__ void drow_eval(const DRows<T>& d, double h, Fn clampf
             for (long v0 = (long)r_ * (Z) + k0, once_ = 1; once_; once_ = 0)
 
 // the Split-Bregman update on the rows of one thread (sums of the tolerance into s1, s2)
-template <class T>
+template <class T, bool P2>
 __device__ __forceinline__ void sb_rows(T* __restrict__ phi, const T* __restrict__ phiold,
                                         T* __restrict__ b0, T* __restrict__ b1, T* __restrict__ b2,
                                         const T* __restrict__ g, T* __restrict__ q0,
@@ -243,15 +266,15 @@ __device__ __forceinline__ void sb_rows(T* __restrict__ phi, const T* __restrict
 #pragma unroll
         for (int a = 0; a < 3; ++a) bb[a] = ldrow(bp[a] + v0);
         T d[3][V];
-        drow_eval(rx, hx, cl, d[0]);
-        drow_eval(ry, hy, cl, d[1]);
+        drow_eval<T, P2>(rx, hx, cl, d[0]);
+        drow_eval<T, P2>(ry, hy, cl, d[1]);
         T c[V + 2];
         c[0] = k0 > 0 ? cl(zl) : (T)0;
         c[V + 1] = k0 + V < Z ? cl(zr) : (T)0;
 #pragma unroll
         for (int e = 0; e < V; ++e) c[e + 1] = cl(cc.v[e]);
 #pragma unroll
-        for (int e = 0; e < V; ++e) d[2][e] = dz_win<T, V>(c, e, k0 + e, Z, hz);
+        for (int e = 0; e < V; ++e) d[2][e] = dz_win<T, V, P2>(c, e, k0 + e, Z, hz);
         Row<T> pc;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -294,9 +317,14 @@ __global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const 
     // blocks renumbered so that rows j-1, j+1 of a block's edge rows are read on one XCD
     const unsigned w = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     double s1 = 0, s2 = 0;
-    sb_rows<T>(phi, phiold, b0, b1, b2, g, q0, q1, q2, X, Y, Z, hx, hy, hz, lo, hi, s1, s2,
-               threadIdx.x, threadIdx.y, blockDim.x, blockDim.y, w % gridDim.x, w / gridDim.x,
-               gridDim.x, gridDim.y);
+    if (pow2_2h3(hx, hy, hz))
+        sb_rows<T, true>(phi, phiold, b0, b1, b2, g, q0, q1, q2, X, Y, Z, hx, hy, hz, lo, hi, s1,
+                         s2, threadIdx.x, threadIdx.y, blockDim.x, blockDim.y, w % gridDim.x,
+                         w / gridDim.x, gridDim.x, gridDim.y);
+    else
+        sb_rows<T, false>(phi, phiold, b0, b1, b2, g, q0, q1, q2, X, Y, Z, hx, hy, hz, lo, hi, s1,
+                          s2, threadIdx.x, threadIdx.y, blockDim.x, blockDim.y, w % gridDim.x,
+                          w / gridDim.x, gridDim.x, gridDim.y);
     block_sum2(s1, s2);
     if (flat_tid() == 0) {
         const int b = blockIdx.x + gridDim.x * blockIdx.y;
@@ -305,7 +333,7 @@ __global__ __launch_bounds__(256) void k_sb_update_v(T* __restrict__ phi, const 
     }
 }
 
-template <class T>
+template <class T, bool P2>
 __device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __restrict__ f1,
                                          const T* __restrict__ f2, T* __restrict__ out, int X,
                                          int Y, int Z, double hx, double hy, double hz, int tx,
@@ -322,7 +350,7 @@ __device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __re
         const T zl = k0 > 0 ? f2[v0 - 1] : (T)0, zr = k0 + V < Z ? f2[v0 + V] : (T)0;
         const int ci = rx.c, cj = ry.c;
         T tx[V], ty[V];
-        drow_eval(rx, hx, id, tx);
+        drow_eval<T, P2>(rx, hx, id, tx);
         if (ci == 2 && cj == 1) {
             // face i = X-1 (j, k inside): "/ 2. * (hx)" (image3d_imp.hxx:1092-1095); the
             // rows X-1, X-2, X-3 are those of the last-plane derivative
@@ -330,10 +358,11 @@ __device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __re
             for (int e = 0; e < V; ++e) {
                 const int k = k0 + e;
                 if (k >= 1 && k <= Z - 2)
-                    tx[e] = (3 * rx.r[0].v[e] - 4 * rx.r[1].v[e] + rx.r[2].v[e]) / (T)2. * (T)hx;
+                    // "/ 2." as a product with 0.5: the same number for every operand
+                    tx[e] = (3 * rx.r[0].v[e] - 4 * rx.r[1].v[e] + rx.r[2].v[e]) * (T)0.5 * (T)hx;
             }
         }
-        drow_eval(ry, hy, id, ty);
+        drow_eval<T, P2>(ry, hy, id, ty);
         T c[V + 2];
         c[0] = zl;
         c[V + 1] = zr;
@@ -346,9 +375,9 @@ __device__ __forceinline__ void div_rows(const T* __restrict__ f0, const T* __re
             T tz;
             if (ci == 1 && cj == 0 && k == Z - 1)
                 // edge (i inside, 0, Z-1): Z-2 used twice (image3d_imp.hxx:1192-1194)
-                tz = (3 * c[e + 1] - 4 * c[e] + c[e]) / (T)(2. * hz);
+                tz = q2h<T, P2>(3 * c[e + 1] - 4 * c[e] + c[e], hz);
             else
-                tz = dz_win<T, V>(c, e, k, Z, hz);
+                tz = dz_win<T, V, P2>(c, e, k, Z, hz);
             o.v[e] = tx[e] + ty[e] + tz;
         }
         strow(out + v0, o);
@@ -361,8 +390,14 @@ __global__ __launch_bounds__(256) void k_div_v(const T* __restrict__ f0, const T
                                                int Y, int Z, double hx, double hy, double hz)
 {
     const unsigned w = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    div_rows<T>(f0, f1, f2, out, X, Y, Z, hx, hy, hz, threadIdx.x, threadIdx.y, blockDim.x,
-                blockDim.y, w % gridDim.x, w / gridDim.x, gridDim.x, gridDim.y);
+    if (pow2_2h3(hx, hy, hz))
+        div_rows<T, true>(f0, f1, f2, out, X, Y, Z, hx, hy, hz, threadIdx.x, threadIdx.y,
+                          blockDim.x, blockDim.y, w % gridDim.x, w / gridDim.x, gridDim.x,
+                          gridDim.y);
+    else
+        div_rows<T, false>(f0, f1, f2, out, X, Y, Z, hx, hy, hz, threadIdx.x, threadIdx.y,
+                           blockDim.x, blockDim.y, w % gridDim.x, w / gridDim.x, gridDim.x,
+                           gridDim.y);
 }
 
 // ---- many volumes per launch (the batch's staged path) --------------------------------
@@ -400,11 +435,18 @@ __global__ __launch_bounds__(256) void k_sb_update_batch(const SbJob* __restrict
     if (w >= gx * gy) return;
     double s1 = 0, s2 = 0;
     const int ty = threadIdx.x / bdx;
-    if (ty < bdy)
-        sb_rows<T>((T*)jb.phi, (const T*)jb.phiold, (T*)jb.b[0], (T*)jb.b[1], (T*)jb.b[2],
-                   (const T*)jb.g, (T*)jb.q[0], (T*)jb.q[1], (T*)jb.q[2], jb.X, jb.Y, jb.Z,
-                   jb.hx, jb.hy, jb.hz, (T)jb.lo, (T)jb.hi, s1, s2, threadIdx.x % bdx, ty, bdx,
-                   bdy, w % gx, w / gx, gx, gy);
+    if (ty < bdy) {
+        if (pow2_2h3(jb.hx, jb.hy, jb.hz))
+            sb_rows<T, true>((T*)jb.phi, (const T*)jb.phiold, (T*)jb.b[0], (T*)jb.b[1],
+                             (T*)jb.b[2], (const T*)jb.g, (T*)jb.q[0], (T*)jb.q[1], (T*)jb.q[2],
+                             jb.X, jb.Y, jb.Z, jb.hx, jb.hy, jb.hz, (T)jb.lo, (T)jb.hi, s1, s2,
+                             threadIdx.x % bdx, ty, bdx, bdy, w % gx, w / gx, gx, gy);
+        else
+            sb_rows<T, false>((T*)jb.phi, (const T*)jb.phiold, (T*)jb.b[0], (T*)jb.b[1],
+                              (T*)jb.b[2], (const T*)jb.g, (T*)jb.q[0], (T*)jb.q[1], (T*)jb.q[2],
+                              jb.X, jb.Y, jb.Z, jb.hx, jb.hy, jb.hz, (T)jb.lo, (T)jb.hi, s1, s2,
+                              threadIdx.x % bdx, ty, bdx, bdy, w % gx, w / gx, gx, gy);
+    }
     block_sum2(s1, s2);
     if (threadIdx.x == 0) {
         jb.part[2 * w] = s1;
@@ -423,10 +465,16 @@ __global__ __launch_bounds__(256) void k_div_batch(const DivJob* __restrict__ jo
     const unsigned w = xcd_remap(blockIdx.x - (unsigned)jb.first, (unsigned)jb.nblk);
     if (w >= gx * gy) return;
     const int ty = threadIdx.x / bdx;
-    if (ty < bdy)
-        div_rows<T>((const T*)jb.q[0], (const T*)jb.q[1], (const T*)jb.q[2], (T*)jb.out, jb.X,
-                    jb.Y, jb.Z, jb.hx, jb.hy, jb.hz, threadIdx.x % bdx, ty, bdx, bdy, w % gx,
-                    w / gx, gx, gy);
+    if (ty < bdy) {
+        if (pow2_2h3(jb.hx, jb.hy, jb.hz))
+            div_rows<T, true>((const T*)jb.q[0], (const T*)jb.q[1], (const T*)jb.q[2],
+                              (T*)jb.out, jb.X, jb.Y, jb.Z, jb.hx, jb.hy, jb.hz,
+                              threadIdx.x % bdx, ty, bdx, bdy, w % gx, w / gx, gx, gy);
+        else
+            div_rows<T, false>((const T*)jb.q[0], (const T*)jb.q[1], (const T*)jb.q[2],
+                               (T*)jb.out, jb.X, jb.Y, jb.Z, jb.hx, jb.hy, jb.hz,
+                               threadIdx.x % bdx, ty, bdx, bdy, w % gx, w / gx, gx, gy);
+    }
 }
 
 // each volume's tolerance sums from its block partials, one block per volume, in the order
