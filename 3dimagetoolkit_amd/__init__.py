@@ -25,7 +25,7 @@ TK_ERR_DTYPE, TK_ERR_UNSUPPORTED, TK_ERR_SWEEPS, TK_ERR_IO = -5, -6, -7, -8
 TK_F32, TK_F64, TK_U8 = 1, 2, 3
 TK_GS_LEX, TK_GS_REDBLACK = 0, 1
 # tk_kernel_tag order
-KERNEL_TAGS = ("conv_x1", "conv_y", "conv_z2", "conv_x2", "gs", "sb", "div", "other")
+KERNEL_TAGS = ("conv_x1", "conv_y", "conv_z2", "conv_x2", "gs", "sb", "div", "other", "gs2")
 
 _NP2TK = {np.dtype(np.float32): TK_F32, np.dtype(np.float64): TK_F64, np.dtype(np.uint8): TK_U8}
 _TK2NP = {v: k for k, v in _NP2TK.items()}
@@ -115,6 +115,7 @@ def _load():
         "tk_poisson_neu_solve": (i, [vp, vp, vp, d, d, i, i, d, i, ip, dp]),
         "tk_poisson_residual": (i, [vp, vp, vp, dp]),
         "tk_poisson_rb_autotune": (i, [vp, vp, ip, dp]),
+        "tk_poisson_rb_set_pair": (i, [vp, i]),
         "tk_poisson_solve_residual": (i, [vp, vp, vp, d, d, i, d, i, i, ip, dp]),
         "tk_poisson_residual_field": (i, [vp, vp, vp, vp]),
         "tk_rsfe_rhs": (vp, [vp]),
@@ -460,11 +461,17 @@ def cc_seeded(ctx, img, seed, threshold, full, out):
 
 
 def poisson_rb_autotune(ctx, rhs):
-    """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16))."""
+    """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16, ms8 and
+    ms16 of the two-sweep launch, 0 where it does not apply))."""
     rows = C.c_int()
-    ms = (C.c_double * 2)()
+    ms = (C.c_double * 4)()
     _check(lib.tk_poisson_rb_autotune(ctx.h, rhs.h, C.byref(rows), ms))
-    return rows.value, (ms[0], ms[1])
+    return rows.value, (ms[0], ms[1], ms[2], ms[3])
+
+
+def poisson_rb_set_pair(ctx, enable):
+    """fixed red-black sweep counts: two sweeps per launch (True, the default) or one"""
+    _check(lib.tk_poisson_rb_set_pair(ctx.h, int(bool(enable))))
 
 
 def poisson_rb_set_tile(ctx, rhs, rows):

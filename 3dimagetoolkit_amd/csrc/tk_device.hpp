@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace tk {
 
 constexpr int kWave = 64;
@@ -133,6 +135,25 @@ __device__ __forceinline__ void strow(T* __restrict__ p, const Row<T>& r)
     VT x;
     __builtin_memcpy(&x, r.v, 16);
     *(VT*)p = x;
+}
+
+// f(integral_constant<int, 0>), .., f(integral_constant<int, N-1>): a loop whose index is a
+// compile-time constant in the body, so that arrays indexed by it stay in registers
+template <int U, int N>
+struct StaticUnroll {
+    template <class F>
+    __device__ __forceinline__ static void run(F& f)
+    {
+        if constexpr (U < N) {
+            f(std::integral_constant<int, U>{});
+            StaticUnroll<U + 1, N>::run(f);
+        }
+    }
+};
+template <int N, class F>
+__device__ __forceinline__ void static_unroll(F&& f)
+{
+    StaticUnroll<0, N>::run(f);
 }
 
 // Visit an X*Y*Z volume in 16-byte rows (Z % V == 0): threads along k in steps of V.

@@ -527,192 +527,183 @@ __global__ __launch_bounds__(256) void k_gs_rb_pp(const T* __restrict__ src, T* 
 }
 
 // The same sweep for rows of 16 bytes (nz a multiple of V): the tile's k columns are
-// aligned to multiples of 64, so every plane row is moved with 16-byte accesses; the rhs
-// rows are staged in LDS with the old planes, and the planes two steps ahead are fetched
-// into registers (two alternating sets) while the current one is computed.  TJ (rows per
-// tile) is 8 (256 threads) or 16 (512 threads, the same work per thread), picked per volume
-// shape and type by timing both.
+// aligned to multiples of 64, so every plane row is moved with 16-byte accesses.  TJ (rows
+// per tile) is 8 or 16, picked per volume shape and type by timing both.
 constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
 
-// one workgroup's tile (bl: tile column in k, row block in j, chunk in i; nblk: tiles of
-// the launch, for the gate's arrival count)
-template <class T, bool UNIT, int TJ, int NT>
-__device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restrict__ dst,
-                                           const T* __restrict__ rhs, int X, int Y, int Z, int CI,
-                                           const GsCoef& c, int s, int slot, int gated,
-                                           const RbGate& g, uint3 bl, unsigned nblk)
+// Red-black sweeps with the tile's columns in registers.  Thread u owns one 16-byte row
+// segment (row r, quad c4) of the tile and its ring, and keeps in registers the planes of
+// its own column that the half-sweeps need along x; only the neighbours along y and z go
+// through LDS.  NS sweeps (1 or 2) are 2 NS half-sweep stages (0: red of sweep 1, 1: black
+// of sweep 1, 2: red of sweep 2, 3: black of sweep 2); stage s updates plane q - s at
+// step q, over rows [s + 1, OJ - 1 - s) and columns [4 - (L - s), 67 + (L - s)] of the
+// extended tile (L = 2 NS - 1, the last stage, whose range is the tile's own TJ rows and 64
+// columns), from the output of stage s - 1 (the old field for stage 0), so the rings
+// recompute what the neighbouring tiles compute from the same values, bit for bit.  One
+// barrier per step: stage s reads its input plane's y and z neighbours from the LDS slot
+// that stage s - 1 filled one step earlier (stage 0: the old plane written at the start of
+// the step), and writes its own output to the other slot.  Every voxel goes through the
+// updates of NS one-sweep launches, in the same order with the same operands.
+template <class T, bool UNIT, int NS, int TJ, int NT>
+__device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __restrict__ dst,
+                                            const T* __restrict__ rhs, int X, int Y, int Z, int CI,
+                                            const GsCoef& c, int s_idx, int slot, int gated,
+                                            const RbGate& g, uint3 bl, unsigned nblk)
 {
     if (gated && *(volatile unsigned*)g.stop) return;
     constexpr int V = Vec16<T>::n;
     constexpr int W = kRbW, WV = W / V;
-    constexpr int OJ = TJ + 4;  // old planes: rows j0-2 .. j0+TJ+1
-    constexpr int RJ = TJ + 2;  // red and rhs planes: rows j0-1 .. j0+TJ
-    constexpr int NO = (OJ * WV + NT - 1) / NT, NB = (RJ * WV + NT - 1) / NT;
-    __shared__ __align__(16) T O[4][OJ * W];
-    __shared__ __align__(16) T Rp[3][RJ * W];
-    __shared__ __align__(16) T Bp[3][RJ * W];
+    constexpr int NST = 2 * NS, LS = NST - 1;
+    constexpr int OJ = TJ + 2 * NST;  // rows j0-NST .. j0+TJ+NST-1
+    static_assert(OJ * WV <= NT, "one row segment per thread");
+    __shared__ __align__(16) T P[NST][2][OJ * W];  // each stage's input plane, two slots
     const int kb = bl.x * kRbTK, kbase = kb - 4;
     const int j0 = 1 + bl.y * TJ;
     const int ia = 1 + bl.z * CI;
     const int ib = min(ia + CI, X - 1);
     const long sX = (long)Y * Z;
     const int tid = threadIdx.x;
-    Row<T> poA[NO], pbA[NB], poB[NO], pbB[NB];
-    // Every lane issues its n loads on every path (a position outside the volume reads a
-    // clamped row, and the deposit writes zeros for it instead): with a fixed number of
-    // loads per step, and nothing touching their registers before the deposit, the
-    // compiler waits only for the set two steps old, not for every load in flight.
-    auto inside = [&](int rows, int row0, int n) {
+    const bool act = tid < OJ * WV;
+    const int r = act ? tid / WV : 0, c4 = act ? tid - r * WV : 0;
+    const int j = j0 - NST + r, k0 = kbase + c4 * V;
+    // every lane loads on every step (a clamped row where its segment is outside the
+    // volume), so the wait before a use counts only the loads issued after it.  Values
+    // outside the volume are never updated and are neighbours only of boundary voxels,
+    // which are not updated either, so what the clamped rows hold does not matter.
+    const long off = (long)min(max(j, 0), Y - 1) * Z + min(max(k0, 0), Z - V);
+    const int li = r * W + c4 * V;
+    auto ld_plane = [&](const T* f, int p) {
+        return ldrow(f + (long)min(max(p, 0), X - 1) * sX + off);
+    };
+    // Per stage, what does not change along the march: whether this segment's row is in
+    // the stage's rows and inside the volume's interior in j (rowok), and which of its V
+    // voxels lie in the stage's columns and in the interior in k (colm, bit u); the colour
+    // of voxel u on plane p is (p + j + k0 + u) & 1.
+    unsigned colm[NST];
+    bool rowok[NST];
+#pragma unroll
+    for (int s = 0; s < NST; ++s) {
+        const int cA = 4 - (LS - s), cB = 67 + (LS - s);
+        rowok[s] = act && r >= s + 1 && r < OJ - 1 - s && j >= 1 && j <= Y - 2;
         unsigned m = 0;
-        for (int q = 0; q < n; ++q) {
-            const int e = tid + q * NT;
-            const int r = e / WV, c4 = e - r * WV;
-            const int j = row0 + r, k = kbase + c4 * V;
-            if (e < rows * WV && j >= 0 && j < Y && k >= 0 && k < Z) m |= 1u << q;
-        }
-        return m;
-    };
-    const unsigned in_o = inside(OJ, j0 - 2, NO), in_b = inside(RJ, j0 - 1, NB);
-    auto fetch = [&](const T* base, int row0, Row<T>* out, int n) {
 #pragma unroll
-        for (int q = 0; q < n; ++q) {
-            const int e = tid + q * NT;
-            const int r = e / WV, c4 = e - r * WV;
-            const int jc = min(max(row0 + r, 0), Y - 1), kc = min(max(kbase + c4 * V, 0), Z - V);
-            out[q] = ldrow(base + (long)jc * Z + kc);
+        for (int u = 0; u < V; ++u) {
+            const int col = c4 * V + u, k = k0 + u;
+            if (col >= cA && col <= cB && k >= 1 && k <= Z - 2) m |= 1u << u;
         }
+        colm[s] = m;
+    }
+    // y and z neighbours of one stage, read from its LDS plane before any stage of the step
+    // runs (the planes a step reads were written in earlier steps)
+    struct Nbr {
+        Row<T> yp, ym;
+        T zl, zr;
     };
-    auto deposit = [&](T* lds, int rows, const Row<T>* in, int n, unsigned m) {
-#pragma unroll
-        for (int q = 0; q < n; ++q) {
-            const int e = tid + q * NT;
-            if (e < rows * WV) strow(lds + e * V, (m >> q) & 1u ? in[q] : Row<T>{});
-        }
+    auto nbr = [&](const T* pl) {
+        Nbr n;
+        n.yp = ldrow(pl + li + W);
+        n.ym = ldrow(pl + li - W);
+        n.zl = pl[li - 1];
+        n.zr = pl[li + V];
+        return n;
     };
-    // Each lane works on one 16-byte row of V voxels: the centre row, its four x/y
-    // neighbour rows and the rhs row are whole-row LDS reads (conflict-free), the two z
-    // neighbours outside the row are scalar reads; every voxel's update is computed and
-    // the colour selects it.
-    auto red = [&](int p) {
-        T* r = Rp[p % 3];
-        const T* o = O[p & 3];
-        const T* om = O[(p + 3) & 3];
-        const T* op = O[(p + 1) & 3];
-        const T* b = Bp[p % 3];
-        const bool inner_p = p >= 1 && p <= X - 2;
-        for (int e = tid; e < RJ * WV; e += NT) {
-            const int rr = e / WV, c4 = e - rr * WV;
-            const int j = j0 - 1 + rr;
-            const bool jin = inner_p && j >= 1 && j <= Y - 2;
-            const int oi = (rr + 1) * W + c4 * V;
-            const Row<T> ctr = ldrow(o + oi);
-            Row<T> out = ctr;
-            if (jin) {
-                const Row<T> xp = ldrow(op + oi), xm = ldrow(om + oi);
-                const Row<T> yp = ldrow(o + oi + W), ym = ldrow(o + oi - W);
-                const Row<T> bq = ldrow(b + rr * W + c4 * V);
-                const T zl = o[oi - 1], zr = o[oi + V];
-#pragma unroll
-                for (int u = 0; u < V; ++u) {
-                    const int col = c4 * V + u, k = kbase + col;
-                    const T zm = u > 0 ? ctr.v[u - 1] : zl;
-                    const T zp = u < V - 1 ? ctr.v[u + 1] : zr;
-                    const T v = gs_update<T, UNIT>(xp.v[u], xm.v[u], yp.v[u], ym.v[u], zp, zm,
-                                                   bq.v[u], ctr.v[u], c);
-                    if (col >= 3 && col <= kRbTK + 4 && k >= 1 && k <= Z - 2 &&
-                        ((p + j + k) & 1) == 0)
-                        out.v[u] = v;
-                }
-            }
-            strow(r + rr * W + c4 * V, out);
-        }
-    };
-    double d2 = 0, o2 = 0;
-    auto black = [&](int p) {
-        const T* r = Rp[p % 3];
-        const T* rm = Rp[(p + 2) % 3];
-        const T* rp = Rp[(p + 1) % 3];
-        const T* o = O[p & 3];
-        const T* b = Bp[p % 3];
-        constexpr int TV = kRbTK / V;
-        for (int e = tid; e < TJ * TV; e += NT) {
-            const int jj = e / TV, c4 = e - jj * TV;
-            const int j = j0 + jj, k0 = kb + c4 * V;
-            if (j > Y - 2 || k0 >= Z) continue;
-            const int ri = (jj + 1) * W + 4 + c4 * V;
-            const Row<T> ctr = ldrow(r + ri);
-            const Row<T> xp = ldrow(rp + ri), xm = ldrow(rm + ri);
-            const Row<T> yp = ldrow(r + ri + W), ym = ldrow(r + ri - W);
-            const Row<T> bq = ldrow(b + ri);
-            const T zl = r[ri - 1], zr = r[ri + V];
-            Row<T> out = ctr;
+    // stage s on plane p: centre ctr, x neighbours xm (p - 1) and xp (p + 1), y and z
+    // neighbours n, rhs b
+    auto half = [&](int s, int p, const Row<T>& xm, const Row<T>& ctr, const Row<T>& xp,
+                    const Nbr& n, const Row<T>& b) {
+        Row<T> o = ctr;
+        if (rowok[s] && p >= 1 && p <= X - 2) {
+            // voxels of the stage's colour: bit u set where (p + j + k0 + u) & 1 == s & 1
+            const unsigned par = ((p + j + k0 + s) & 1) ? 0xAu : 0x5u;
+            const unsigned m = colm[s] & par;
 #pragma unroll
             for (int u = 0; u < V; ++u) {
-                const int k = k0 + u;
-                const T zm = u > 0 ? ctr.v[u - 1] : zl;
-                const T zp = u < V - 1 ? ctr.v[u + 1] : zr;
-                const T v = gs_update<T, UNIT>(xp.v[u], xm.v[u], yp.v[u], ym.v[u], zp, zm,
-                                               bq.v[u], ctr.v[u], c);
-                if (k >= 1 && k <= Z - 2 && ((p + j + k) & 1) == 1) out.v[u] = v;
+                const T zm = u > 0 ? ctr.v[u - 1] : n.zl;
+                const T zp = u < V - 1 ? ctr.v[u + 1] : n.zr;
+                const T v = gs_update<T, UNIT>(xp.v[u], xm.v[u], n.yp.v[u], n.ym.v[u], zp, zm,
+                                               b.v[u], ctr.v[u], c);
+                if ((m >> u) & 1u) o.v[u] = v;
             }
-            if (gated) {
-                const Row<T> old = ldrow(o + (jj + 2) * W + 4 + c4 * V);
+        }
+        return o;
+    };
+    const int qa = ia - LS, qe = ib + LS;  // steps; stage s runs on planes q - s
+    // Registers, as rings of K slots with plane p in slot (p - qa) mod K; the march is
+    // unrolled by K steps, so every slot index is a constant and no value is copied from
+    // one register to another (a copy would wait for the load that fills it):
+    //   old planes q-1 .. q+1 in use and q+2 .. q+1+KL in flight (loaded KL steps ahead),
+    //   rhs planes q-2NS+1 .. q in use (stage s takes plane q-s) and q+1 .. q+KB in flight,
+    //   the last three outputs of every stage but the last.
+    constexpr int K = 6, KL = K - 3, KB = K - NST;
+    static_assert(KB >= 1, "rhs ring");
+    Row<T> ro[K], rb[K], ra[NST > 1 ? NST - 1 : 1][K];
+    // prologue: old planes qa-1 .. qa+1+KL, rhs planes qa-NST+1 .. qa+KB
 #pragma unroll
-                for (int u = 0; u < V; ++u) {
-                    const int k = k0 + u;
-                    if (k >= 1 && k <= Z - 2) {
-                        const double d = (double)out.v[u] - (double)old.v[u];
-                        d2 += d * d;
-                        o2 += (double)old.v[u] * (double)old.v[u];
+    for (int d = -1; d <= 1 + KL; ++d) ro[(d + K) % K] = ld_plane(src, qa + d);
+#pragma unroll
+    for (int d = 1 - NST; d <= KB; ++d) rb[(d + K) % K] = ld_plane(rhs, qa + d);
+#pragma unroll
+    for (int s = 0; s < NST - 1; ++s)
+#pragma unroll
+        for (int m = 0; m < K; ++m) ra[s][m] = Row<T>{};
+    double d2 = 0, o2 = 0;
+    const int owned_c0 = 4 / V, owned_c1 = (4 + kRbTK) / V;
+    // step q = q0 + u (q0 - qa a multiple of K)
+    auto step = [&](auto u_c, int q) {
+        constexpr int u = decltype(u_c)::value;
+        auto sl = [](int d) { return ((u + d) % K + K) % K; };  // slot of plane q + d
+        if (act) strow(&P[0][q & 1][li], ro[sl(0)]);
+        __syncthreads();
+        Nbr nb[NST];
+        static_unroll<NST>([&](auto s_c) {
+            constexpr int s = decltype(s_c)::value;
+            nb[s] = nbr(P[s][(q - s) & 1]);
+        });
+        // stage 0 inputs: old planes q-1, q, q+1
+        Row<T> x = ro[sl(-1)], y = ro[sl(0)], z = ro[sl(1)];
+        static_unroll<NST>([&](auto s_c) {
+            constexpr int s = decltype(s_c)::value;
+            const int p = q - s;
+            const bool run = p >= ia - (LS - s) && p < ib + (LS - s) && p >= 0 && p <= X - 1;
+            Row<T> out = y;
+            if (run) out = half(s, p, x, y, z, nb[s], rb[sl(-s)]);
+            if constexpr (s < LS) {
+                // plane p of this stage; the next stage (plane p - 1) takes p - 2, p - 1, p
+                if (act && run) strow(&P[s + 1][p & 1][li], out);
+                ra[s][sl(-s)] = out;
+                x = ra[s][sl(-s - 2)];
+                y = ra[s][sl(-s - 1)];
+                z = out;
+            } else {
+                if (run && p >= ia && p < ib && r >= NST && r < NST + TJ && j <= Y - 2 &&
+                    c4 >= owned_c0 && c4 < owned_c1 && k0 < Z) {
+                    strow(dst + (long)p * sX + (long)j * Z + k0, out);
+                    if (NS == 1 && gated) {
+                        const Row<T>& old = ro[sl(-1)];  // the old value of plane q - 1
+#pragma unroll
+                        for (int e = 0; e < V; ++e) {
+                            const int k = k0 + e;
+                            if (k >= 1 && k <= Z - 2) {
+                                const double d = (double)out.v[e] - (double)old.v[e];
+                                d2 += d * d;
+                                o2 += (double)old.v[e] * (double)old.v[e];
+                            }
+                        }
                     }
                 }
             }
-            strow(dst + (long)p * sX + (long)j * Z + k0, out);
-        }
+        });
+        // refill: old plane q+2+KL into the slot of q-1, rhs plane q+1+KB into the slot of
+        // q+1-NST (the planes this step was the last to use)
+        ro[sl(2 + KL)] = ld_plane(src, q + 2 + KL);
+        rb[sl(1 + KB)] = ld_plane(rhs, q + 1 + KB);
     };
-    // prologue: old planes ia-2 .. ia+1 and rhs planes ia-1, ia into LDS
-    for (int p = ia - 2; p <= ia + 1; ++p)
-        if (p >= 0 && p <= X - 1) {
-            fetch(src + (long)p * sX, j0 - 2, poA, NO);
-            deposit(O[p & 3], OJ, poA, NO, in_o);
-        }
-    for (int p = ia - 1; p <= ia; ++p) {
-        fetch(rhs + (long)p * sX, j0 - 1, pbA, NB);
-        deposit(Bp[p % 3], RJ, pbA, NB, in_b);
-    }
-    // set A holds old plane ia+2 and rhs plane ia+1, set B old ia+3 and rhs ia+2 (plane
-    // indices past X-1 are clamped: those sets are fetched but never deposited)
-    const auto old_plane = [&](int p) { return src + (long)min(p, X - 1) * sX; };
-    const auto rhs_plane = [&](int p) { return rhs + (long)min(p, X - 1) * sX; };
-    fetch(old_plane(ia + 2), j0 - 2, poA, NO);
-    fetch(rhs_plane(ia + 1), j0 - 1, pbA, NB);
-    fetch(old_plane(ia + 3), j0 - 2, poB, NO);
-    fetch(rhs_plane(ia + 2), j0 - 1, pbB, NB);
-    __syncthreads();
-    red(ia - 1);
-    red(ia);
-    __syncthreads();
-    // step p deposits old plane p+2 and rhs plane p+1 (fetched two steps earlier) and
-    // refills the same register set with old plane p+4 and rhs plane p+3
-    auto step = [&](int p, Row<T>* po, Row<T>* pb) {
-        // slots refilled here were last read two planes ago (separated by barriers)
-        if (p + 2 <= X - 1) deposit(O[(p + 2) & 3], OJ, po, NO, in_o);
-        deposit(Bp[(p + 1) % 3], RJ, pb, NB, in_b);
-        fetch(old_plane(p + 4), j0 - 2, po, NO);
-        fetch(rhs_plane(p + 3), j0 - 1, pb, NB);
-        __syncthreads();
-        red(p + 1);
-        __syncthreads();
-        black(p);
-    };
-    // both steps of the pair run on every trip, so each deposit has the other set's
-    // loads behind it on every path into it
-    int p = ia;
-    for (; p + 1 < ib; p += 2) {
-        step(p, poA, pbA);
-        step(p + 1, poB, pbB);
-    }
-    if (p < ib) step(p, poA, pbA);
+    int q = qa;
+    for (; q + K <= qe; q += K)
+        static_unroll<K>([&](auto u_c) { step(u_c, q + decltype(u_c)::value); });
+    static_unroll<K - 1>([&](auto u_c) {
+        if (q + decltype(u_c)::value < qe) step(u_c, q + decltype(u_c)::value);
+    });
     if (!gated) return;
     block_sum2(d2, o2);
     if (tid == 0) {
@@ -723,24 +714,30 @@ __device__ __forceinline__ void rb_pv_tile(const T* __restrict__ src, T* __restr
         if (ticket == nblk - 1) {
             const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
             double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
-            if (s > 0) so += g.bnd_o2;
+            if (s_idx > 0) so += g.bnd_o2;
             const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
             g.err[slot] = err;
             g.arrive[slot] = 0;
-            if (s >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s + 1));
+            if (s_idx >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s_idx + 1));
         }
     }
 }
 
-template <class T, bool UNIT, int TJ, int NT>
-__global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* __restrict__ dst,
-                                                  const T* __restrict__ rhs, int X, int Y, int Z,
-                                                  int CI, GsCoef c, int s, int slot, int gated,
-                                                  RbGate g)
+// threads of a column-march workgroup: one per row segment of the extended tile
+template <class T, int NS, int TJ>
+constexpr int rbc_threads()
 {
-    // tiles sharing halo rows run on one XCD
-    rb_pv_tile<T, UNIT, TJ, NT>(src, dst, rhs, X, Y, Z, CI, c, s, slot, gated, g, xcd_block3(),
-                                gridDim.x * gridDim.y * gridDim.z);
+    return ((TJ + 4 * NS) * (kRbW / Vec16<T>::n) + 63) / 64 * 64;
+}
+
+template <class T, bool UNIT, int NS, int TJ>
+__global__ __launch_bounds__((rbc_threads<T, NS, TJ>())) void k_gs_rbc(
+    const T* __restrict__ src, T* __restrict__ dst, const T* __restrict__ rhs, int X, int Y, int Z,
+    int CI, GsCoef c, int s, int slot, int gated, RbGate g)
+{
+    rb_col_tile<T, UNIT, NS, TJ, rbc_threads<T, NS, TJ>()>(src, dst, rhs, X, Y, Z, CI, c, s, slot,
+                                                           gated, g, xcd_block3(),
+                                                           gridDim.x * gridDim.y * gridDim.z);
 }
 
 // Many volumes, one sweep each, in one launch (the batch's fixed-sweep path): the job table
@@ -748,9 +745,9 @@ __global__ __launch_bounds__(NT) void k_gs_rb_pv(const T* __restrict__ src, T* _
 // one, so the XCD-aware order inside a volume is that of its own launch; padding blocks
 // exit at once.  The sweep of each voxel does not depend on the tiling, so every volume
 // gets the values of its own one-volume sweep.
-template <class T, bool UNIT, int TJ, int NT>
-__global__ __launch_bounds__(NT) void k_gs_rb_pv_batch(const RbJob* __restrict__ jobs, int njobs,
-                                                        GsCoef c)
+template <class T, bool UNIT, int TJ>
+__global__ __launch_bounds__((rbc_threads<T, 1, TJ>())) void k_gs_rb_pv_batch(
+    const RbJob* __restrict__ jobs, int njobs, GsCoef c)
 {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
@@ -767,8 +764,9 @@ __global__ __launch_bounds__(NT) void k_gs_rb_pv_batch(const RbJob* __restrict__
     w /= jb.gx;
     bl.y = w % jb.gy;
     bl.z = w / jb.gy;
-    rb_pv_tile<T, UNIT, TJ, NT>((const T*)jb.src, (T*)jb.dst, (const T*)jb.rhs, jb.X, jb.Y, jb.Z,
-                                jb.CI, c, 0, 0, 0, RbGate{}, bl, n);
+    rb_col_tile<T, UNIT, 1, TJ, rbc_threads<T, 1, TJ>()>((const T*)jb.src, (T*)jb.dst,
+                                                         (const T*)jb.rhs, jb.X, jb.Y, jb.Z,
+                                                         jb.CI, c, 0, 0, 0, RbGate{}, bl, n);
 }
 
 __global__ void k_zero_d(double* p, int n)
@@ -785,32 +783,91 @@ inline long long rb_key(int X, int Y, int Z, int w)
     return (((long long)X * 65536 + Y) * 65536 + Z) * 16 + w;
 }
 
-// aligned red-black sweep with TJ-row tiles
+// LDS bytes of a column-march workgroup: two slots per stage of (TJ + 4 NS) rows
+template <class T, int NS, int TJ>
+constexpr long rbc_lds_bytes()
+{
+    return (long)sizeof(T) * 2L * 2 * NS * (TJ + 4L * NS) * kRbW;
+}
+
+// NS aligned red-black sweeps src -> dst with TJ-row tiles
+template <class T, int NS, int TJ>
+static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
+                      int slot, int gated, const RbGate& g, bool unit)
+{
+    constexpr int NT = rbc_threads<T, NS, TJ>();
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    // chunk length in x chosen so the grid is one round of as many workgroups per CU as the
+    // kernel's registers and LDS allow (asked once per instance); a chunk recomputes
+    // 2 NS - 1 planes on each side, so chunks keep at least 8 planes
+    static int resident = 0;
+    if (resident <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gs_rbc<T, true, NS, TJ>, NT, 0) !=
+                hipSuccess ||
+            nb < 1)
+            nb = 1;
+        resident = nb;
+    }
+    const long per_cu = resident;
+    const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + TJ - 1) / TJ;
+    long chunks = ((long)ctx->num_cu * per_cu + (long)kt * jt / 2) / ((long)kt * jt);
+    if (chunks < 1) chunks = 1;
+    if (chunks > (X - 2) / 8) chunks = std::max(1, (X - 2) / 8);
+    const int CI = (int)((X - 2 + chunks - 1) / chunks);
+    const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
+    if (unit)
+        hipLaunchKernelGGL((k_gs_rbc<T, true, NS, TJ>), grid, dim3(NT), 0, ctx->stream,
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                           gated, g);
+    else
+        hipLaunchKernelGGL((k_gs_rbc<T, false, NS, TJ>), grid, dim3(NT), 0, ctx->stream,
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
+                           gated, g);
+    TK_LAUNCHED();
+    return TK_OK;
+}
+
 template <class T, int TJ>
 static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
                         int s, int slot, int gated, const RbGate& g, bool unit)
 {
-    constexpr int NT = TJ == 16 ? 512 : 256;
-    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
-    // chunk length in x chosen so the grid is one round of as many workgroups per CU as
-    // the LDS holds (160 KB per CU)
-    const long lds = (long)sizeof(T) * (4L * (TJ + 4) + 6L * (TJ + 2)) * kRbW + 256;
-    const long per_cu = std::max(1L, 163840L / lds);
-    const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + TJ - 1) / TJ;
-    long chunks = ((long)ctx->num_cu * per_cu + (long)kt * jt / 2) / ((long)kt * jt);
-    if (chunks < 1) chunks = 1;
-    if (chunks > X - 2) chunks = X - 2;
-    const int CI = (int)((X - 2 + chunks - 1) / chunks);
-    const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
-    if (unit)
-        hipLaunchKernelGGL((k_gs_rb_pv<T, true, TJ, NT>), grid, dim3(NT), 0, ctx->stream,
-                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                           gated, g);
-    else
-        hipLaunchKernelGGL((k_gs_rb_pv<T, false, TJ, NT>), grid, dim3(NT), 0, ctx->stream,
-                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                           gated, g);
-    TK_LAUNCHED();
+    return launch_rbc<T, 1, TJ>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
+}
+
+template <int TJ>
+static int launch_rb2_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
+                         bool unit)
+{
+    return launch_rbc<float, 2, TJ>(ctx, src, dst, rhs, c, 0, 0, 0, RbGate{}, unit);
+}
+
+// the autotuner's key for the two-sweep launch of a shape (next to the one-sweep key)
+inline long long rb2_key(int X, int Y, int Z) { return rb_key(X, Y, Z, 15); }
+
+// two sweeps src -> dst in one launch where the kernel serves the volume (FP32, z a multiple
+// of 4, at least 3 interior planes), else nothing: *done = false
+template <class T>
+static int rb_sweep_pair(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
+                         bool* done)
+{
+    *done = false;
+    if constexpr (std::is_same<T, float>::value) {
+        const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+        if (!ctx->rb_pair || Z % Vec16<float>::n != 0 || X < 5 || Y < 3) return TK_OK;
+        const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+        TimedLaunch tl(ctx, TK_K_GS2);
+        // tile rows of the two-sweep launch for this shape: 8 or 16, or 0 where the
+        // autotuner found it slower than two one-sweep launches; untuned shapes run one
+        // sweep per launch
+        const auto it = ctx->rb_rows.find(rb2_key(X, Y, Z));
+        if (it == ctx->rb_rows.end() || it->second == 0) return TK_OK;
+        if (it->second == 16)
+            TK_TRY(launch_rb2_pv<16>(ctx, src, dst, rhs, c, unit));
+        else
+            TK_TRY(launch_rb2_pv<8>(ctx, src, dst, rhs, c, unit));
+        *done = true;
+    }
     return TK_OK;
 }
 
@@ -824,8 +881,12 @@ template <class T>
 static int rb_batch_table_t(tk_ctx ctx, const RbBatchVol* vols, int n, void** table, int* nblocks)
 {
     constexpr int TJ = kRbBatchTJ;
-    const long lds = (long)sizeof(T) * (4L * (TJ + 4) + 6L * (TJ + 2)) * kRbW + 256;
-    const long per_cu = std::max(1L, 163840L / lds);
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_gs_rb_pv_batch<T, true, TJ>,
+                                                     rbc_threads<T, 1, TJ>(), 0) != hipSuccess ||
+        resident < 1)
+        resident = 1;
+    const long per_cu = resident;
     long planes = 0;  // tile-planes of the whole batch
     for (int v = 0; v < n; ++v) {
         const int kt = (vols[v].Z + kRbTK - 1) / kRbTK, jt = (vols[v].Y - 2 + TJ - 1) / TJ;
@@ -934,6 +995,30 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
         return TK_OK;
     };
     rc = body();
+    // the two-sweep launch (FP32): the same two tile heights
+    double ms2[2] = {0, 0};
+    if (rc == TK_OK && std::is_same<T, float>::value && X >= 5 && Y >= 3) {
+        rc = [&]() -> int {
+            for (int v = 0; v < 2; ++v) {
+                auto launch = [&]() {
+                    return v == 0 ? launch_rb2_pv<8>(ctx, a, b, rhs, c, unit)
+                                  : launch_rb2_pv<16>(ctx, a, b, rhs, c, unit);
+                };
+                TK_TRY(launch());
+                TK_HIP(hipEventRecord(e0, ctx->stream));
+                for (int r = 0; r < 3; ++r) TK_TRY(launch());
+                TK_HIP(hipEventRecord(e1, ctx->stream));
+                TK_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                TK_HIP(hipEventElapsedTime(&t, e0, e1));
+                ms2[v] = t / 3.0;
+            }
+            // kept only where one launch of two sweeps beats two launches of one
+            const double best1 = std::min(ms[0], ms[1]), best2 = std::min(ms2[0], ms2[1]);
+            ctx->rb_rows[rb2_key(X, Y, Z)] = best2 < 2 * best1 ? (ms2[1] < ms2[0] ? 16 : 8) : 0;
+            return TK_OK;
+        }();
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (a) (void)hipFree(a);
@@ -945,6 +1030,8 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
     if (ms_out) {
         ms_out[0] = ms[0];
         ms_out[1] = ms[1];
+        ms_out[2] = ms2[0];
+        ms_out[3] = ms2[1];
     }
     return TK_OK;
 }
@@ -998,11 +1085,22 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
         int sweeps = 0;
         double err = -1;
         int r = TK_OK;
+        int launches = 0;  // the result is in buf[launches & 1]
         if (ls_tol == 0) {
             if (!faces_set) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
             RbGate g{};
-            for (int q = 0; q < ls_steps; ++q)
-                TK_TRY(rb_sweep<T>(ctx, buf[q & 1], buf[(q + 1) & 1], rhs, c, q, 0, 0, g));
+            for (int q = 0; q < ls_steps; ++launches) {
+                void* from = buf[launches & 1];
+                void* to = buf[(launches + 1) & 1];
+                bool pair = false;
+                if (ls_steps - q >= 2) TK_TRY(rb_sweep_pair<T>(ctx, from, to, rhs, c, &pair));
+                if (pair) {
+                    q += 2;
+                } else {
+                    TK_TRY(rb_sweep<T>(ctx, from, to, rhs, c, q, 0, 0, g));
+                    q += 1;
+                }
+            }
             sweeps = ls_steps;
         } else {
             if (max_sweeps <= 0) max_sweeps = 1 << 30;
@@ -1047,8 +1145,9 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
             }
             if (r != TK_OK)
                 set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", sweeps);
+            launches = sweeps;
         }
-        if (sweeps & 1) {
+        if (launches & 1) {
             // the result is in the scratch volume: hand it to phi
             if (alt) {
                 *alt = phi->d;
@@ -1220,22 +1319,23 @@ int rb_batch_sweep(tk_ctx ctx, tk_dtype dt, const void* table, int n, int nblock
     like.hz = hz;
     const GsCoef c = gs_coef(&like, dt_step);
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
-    constexpr int TJ = kRbBatchTJ, NT = 256;
+    constexpr int TJ = kRbBatchTJ;
+    constexpr int NT64 = rbc_threads<double, 1, TJ>(), NT32 = rbc_threads<float, 1, TJ>();
     TimedLaunch tl(ctx, TK_K_GS);
     const RbJob* jobs = (const RbJob*)table;
     if (dt == TK_F64) {
         if (unit)
-            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, true, TJ, NT>), dim3(nblocks), dim3(NT), 0,
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, true, TJ>), dim3(nblocks), dim3(NT64), 0,
                                ctx->stream, jobs, n, c);
         else
-            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, false, TJ, NT>), dim3(nblocks), dim3(NT),
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<double, false, TJ>), dim3(nblocks), dim3(NT64),
                                0, ctx->stream, jobs, n, c);
     } else {
         if (unit)
-            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, true, TJ, NT>), dim3(nblocks), dim3(NT), 0,
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, true, TJ>), dim3(nblocks), dim3(NT32), 0,
                                ctx->stream, jobs, n, c);
         else
-            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, false, TJ, NT>), dim3(nblocks), dim3(NT),
+            hipLaunchKernelGGL((k_gs_rb_pv_batch<float, false, TJ>), dim3(nblocks), dim3(NT32),
                                0, ctx->stream, jobs, n, c);
     }
     TK_LAUNCHED();
@@ -1411,6 +1511,13 @@ int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms)
                                 : rb_autotune<float>(ctx, rhs, tile_rows, ms);
 }
 
+int tk_poisson_rb_set_pair(tk_ctx ctx, int enable)
+{
+    TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_rb_set_pair: null context");
+    ctx->rb_pair = enable ? 1 : 0;
+    return TK_OK;
+}
+
 int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows)
 {
     TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_rb_set_tile: null context");
@@ -1418,6 +1525,7 @@ int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows)
     TK_CHECK(tile_rows == 8 || tile_rows == 16, TK_ERR_ARG,
              "tk_poisson_rb_set_tile: tile_rows must be 8 or 16");
     ctx->rb_rows[rb_key(rhs->nx, rhs->ny, rhs->nz, (int)dtype_size(rhs->dtype))] = tile_rows;
+    if (rhs->dtype == TK_F32) ctx->rb_rows[rb2_key(rhs->nx, rhs->ny, rhs->nz)] = tile_rows;
     return TK_OK;
 }
 

@@ -81,6 +81,7 @@ struct tk_ctx_s {
     long tag_n[TK_K_COUNT] = {0};
     // red-black sweep tile rows (8 or 16) picked by timing, per volume shape and type
     std::map<long long, int> rb_rows;
+    int rb_pair = 1;  // fixed red-black sweep counts run two sweeps per launch where possible
 };
 
 struct tk_vol_s {

@@ -71,7 +71,8 @@ enum tk_kernel_tag {
     TK_K_SB = 5,      /* cut + norms + grad/shrink/Bregman update */
     TK_K_DIV = 6,     /* divergence */
     TK_K_OTHER = 7,
-    TK_K_COUNT = 8
+    TK_K_GS2 = 8,     /* two red-black sweeps in one launch */
+    TK_K_COUNT = 9
 };
 int tk_ctx_set_timing(tk_ctx ctx, int enable);
 /* totals since the last call with reset != 0: milliseconds and launches per tag */
@@ -194,10 +195,20 @@ int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, dou
 /* autotuner of the red-black sweep (no reference counterpart: the reference has one fixed
    lexicographic loop): times the 8-row and 16-row tile variants on rhs's shape and type,
    keeps the faster for every later red-black sweep of that shape on ctx, returns the rows
-   in *tile_rows and the times in ms[0..1] (either may be null).  Both variants give
-   identical results.  Runs synchronously; tk_rsfe_create calls it for FP32 runs. */
+   in *tile_rows and the times in ms[0..1] (either may be null); for FP32 it also times the
+   two-sweep launch with both tile heights (ms[2..3], 0 otherwise; ms holds 4 doubles) and
+   keeps the faster, or one launch per sweep where that is faster than the pairs.  All
+   variants give identical results.  Runs synchronously; tk_rsfe_create calls it for FP32
+   runs. */
 int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms);
-/* set the red-black tile rows (8 or 16) for rhs's shape and type on ctx */
+/* fixed red-black sweep counts (ls_tol == 0) run two sweeps per launch where the volume
+   allows (FP32, z a multiple of 4; the result is bitwise that of one launch per sweep) and
+   the shape's tile rows for the two-sweep launch were set (the autotuner, or
+   tk_poisson_rb_set_tile): enable 1 (the default) or 0 for one launch per sweep, on every
+   later solve on ctx */
+int tk_poisson_rb_set_pair(tk_ctx ctx, int enable);
+/* set the red-black tile rows (8 or 16) for rhs's shape and type on ctx (the one-sweep and,
+   for FP32, the two-sweep launch) */
 int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows);
 
 /* ---- R6, R10-R13: pointwise and stencil operators ----------------------------- */

@@ -312,11 +312,38 @@ def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rows, rng):
     assert np.array_equal(vp.numpy(), ref)
 
 
+@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("shape,sweeps,h", [((16, 18, 20), 2, 1.0), ((40, 37, 136), 5, 1.0),
+                                            ((9, 70, 64), 4, 1.0), ((70, 45, 200), 3, 0.7),
+                                            ((128, 96, 256), 5, 1.0)])
+def test_gs_redblack_pair_launch_bitwise(tk, ctx, shape, sweeps, h, rows, rng):
+    """two sweeps per launch (the default for fixed counts) against one launch per sweep:
+    bitwise equal, at tile edges in j and k, chunk edges in i, odd counts (a pair and a
+    single) and general spacing"""
+    say("gs rb pair", shape, sweeps, h, "tile rows", rows)
+    phi = rng.random(shape).astype(np.float32)
+    rhs = rng.standard_normal(shape).astype(np.float32)
+    out = []
+    for pair in (False, True):
+        vp, vr = ctx.from_numpy(phi, h=(h, 1.0, h)), ctx.from_numpy(rhs, h=(h, 1.0, h))
+        tk.poisson_rb_set_tile(ctx, vr, rows)
+        tk.poisson_rb_set_pair(ctx, pair)
+        tk.poisson_sweeps(ctx, vp, vr, sweeps, 1.0, 0.0, tk.TK_GS_REDBLACK)
+        out.append(vp.numpy())
+    tk.poisson_rb_set_pair(ctx, True)
+    assert np.array_equal(out[0], out[1])
+    if h == 1.0 and np.prod(shape) < 1e6:
+        ref = phi
+        for _ in range(sweeps):
+            ref = rb_sweep_f32(ref, rhs)
+        assert np.array_equal(out[1], ref)
+
+
 def test_gs_redblack_autotune(tk, ctx, rng):
     shape = (64, 66, 128)
     rhs = ctx.from_numpy(rng.standard_normal(shape).astype(np.float32))
     rows, ms = tk.poisson_rb_autotune(ctx, rhs)
-    say("rb autotune ms (8 rows, 16 rows):", ms, "->", rows)
+    say("rb autotune ms (8 rows, 16 rows; two-sweep launch 8, 16):", ms, "->", rows)
     assert rows in (8, 16) and min(ms) > 0
     assert rows == (16 if ms[1] < ms[0] else 8)
     phi = rng.random(shape).astype(np.float32)
