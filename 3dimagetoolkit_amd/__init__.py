@@ -25,7 +25,8 @@ TK_ERR_DTYPE, TK_ERR_UNSUPPORTED, TK_ERR_SWEEPS, TK_ERR_IO = -5, -6, -7, -8
 TK_F32, TK_F64, TK_U8 = 1, 2, 3
 TK_GS_LEX, TK_GS_REDBLACK = 0, 1
 # tk_kernel_tag order
-KERNEL_TAGS = ("conv_x1", "conv_y", "conv_z2", "conv_x2", "gs", "sb", "div", "other", "gs2")
+KERNEL_TAGS = ("conv_x1", "conv_y", "conv_z2", "conv_x2", "gs", "sb", "div", "other", "gs2",
+               "sbdiv")
 
 _NP2TK = {np.dtype(np.float32): TK_F32, np.dtype(np.float64): TK_F64, np.dtype(np.uint8): TK_U8}
 _TK2NP = {v: k for k, v in _NP2TK.items()}
@@ -85,6 +86,7 @@ def _load():
         "tk_ctx_stream": (vp, [vp]),
         "tk_ctx_set_timing": (i, [vp, i]),
         "tk_ctx_kernel_times": (i, [vp, dp, C.POINTER(C.c_long), i]),
+        "tk_ctx_set_fused_update": (i, [vp, i]),
         "tk_vol_alloc": (i, [vp, i, i, i, d, d, d, i, C.POINTER(vp)]),
         "tk_vol_wrap": (i, [vp, vp, i, i, i, d, d, d, i, C.POINTER(vp)]),
         "tk_vol_free": (i, [vp]),
@@ -270,6 +272,10 @@ class Context:
 
     def set_timing(self, on):
         _check(lib.tk_ctx_set_timing(self.h, int(bool(on))))
+
+    def set_fused_update(self, on):
+        """update and div of single-volume runs in one pass (True, the default) or two"""
+        _check(lib.tk_ctx_set_fused_update(self.h, int(bool(on))))
 
     def kernel_times(self, reset=True):
         """{tag: (total ms, launches)} from HIP events on this context's stream."""
@@ -515,6 +521,12 @@ class Rsfe:
     def phi(self, dtype=np.float64):
         out = np.empty(self.shape, dtype)
         _check(lib.tk_vol_download(lib.tk_rsfe_phi(self.h), _ptr(out), _NP2TK[np.dtype(dtype)]))
+        return out
+
+    def rhs(self, dtype=np.float64):
+        """the forcing term of the last iteration (it holds that iteration's divdb)"""
+        out = np.empty(self.shape, dtype)
+        _check(lib.tk_vol_download(lib.tk_rsfe_rhs(self.h), _ptr(out), _NP2TK[np.dtype(dtype)]))
         return out
 
     def close(self):

@@ -557,6 +557,13 @@ int tk_ctx_set_timing(tk_ctx c, int enable)
     return TK_OK;
 }
 
+int tk_ctx_set_fused_update(tk_ctx c, int enable)
+{
+    TK_CHECK(c, TK_ERR_ARG, "tk_ctx_set_fused_update: null ctx");
+    c->sb_fused = enable != 0;
+    return TK_OK;
+}
+
 int tk_ctx_kernel_times(tk_ctx c, double* ms, long* n, int reset)
 {
     TK_CHECK(c, TK_ERR_ARG, "tk_ctx_kernel_times: null ctx");

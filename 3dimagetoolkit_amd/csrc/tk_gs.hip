@@ -799,7 +799,9 @@ static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const 
     const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
     // chunk length in x chosen so the grid is one round of as many workgroups per CU as the
     // kernel's registers and LDS allow (asked once per instance); a chunk recomputes
-    // 2 NS - 1 planes on each side, so chunks keep at least 8 planes
+    // 2 NS - 1 planes on each side for the first stage, so chunks keep at least 2 planes
+    // for one sweep (small volumes then spread over more workgroups: a 64^3 sweep is a few
+    // march steps long) and 8 for two
     static int resident = 0;
     if (resident <= 0) {
         int nb = 0;
@@ -813,7 +815,8 @@ static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const 
     const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + TJ - 1) / TJ;
     long chunks = ((long)ctx->num_cu * per_cu + (long)kt * jt / 2) / ((long)kt * jt);
     if (chunks < 1) chunks = 1;
-    if (chunks > (X - 2) / 8) chunks = std::max(1, (X - 2) / 8);
+    constexpr int min_ci = NS == 1 ? 2 : 8;
+    if (chunks > (X - 2) / min_ci) chunks = std::max(1, (X - 2) / min_ci);
     const int CI = (int)((X - 2 + chunks - 1) / chunks);
     const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
     if (unit)
@@ -856,12 +859,12 @@ static int rb_sweep_pair(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, con
         const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
         if (!ctx->rb_pair || Z % Vec16<float>::n != 0 || X < 5 || Y < 3) return TK_OK;
         const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
-        TimedLaunch tl(ctx, TK_K_GS2);
         // tile rows of the two-sweep launch for this shape: 8 or 16, or 0 where the
         // autotuner found it slower than two one-sweep launches; untuned shapes run one
         // sweep per launch
         const auto it = ctx->rb_rows.find(rb2_key(X, Y, Z));
         if (it == ctx->rb_rows.end() || it->second == 0) return TK_OK;
+        TimedLaunch tl(ctx, TK_K_GS2);
         if (it->second == 16)
             TK_TRY(launch_rb2_pv<16>(ctx, src, dst, rhs, c, unit));
         else

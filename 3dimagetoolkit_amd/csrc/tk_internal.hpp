@@ -82,6 +82,7 @@ struct tk_ctx_s {
     // red-black sweep tile rows (8 or 16) picked by timing, per volume shape and type
     std::map<long long, int> rb_rows;
     int rb_pair = 1;  // fixed red-black sweep counts run two sweeps per launch where possible
+    int sb_fused = 1;  // single-volume runs: update and div in one pass
 };
 
 struct tk_vol_s {

@@ -40,6 +40,9 @@ int launch_edge(tk_ctx, tk_vol, double, double, double, tk_vol);
 template <class T>
 int launch_sb_update(tk_ctx, tk_vol, tk_vol, tk_vol*, tk_vol, tk_vol*, double, double, int*,
                      double*);
+template <class T>
+int launch_sb_div(tk_ctx, tk_vol, tk_vol, tk_vol, tk_vol*, tk_vol*, tk_vol, double, double, int*,
+                  double*);
 
 template <class T>
 __device__ __forceinline__ T heaviside(T p, T center, T eps)
@@ -246,6 +249,7 @@ struct tk_rsfe_s {
     tk_vol u0 = nullptr, phi = nullptr, phiold = nullptr, rhs = nullptr, kones = nullptr,
            kmyim = nullptr, g = nullptr, divdb = nullptr;
     tk_vol b[3] = {nullptr, nullptr, nullptr}, q[3] = {nullptr, nullptr, nullptr};
+    bool in_batch = false;     // a batch's tables hold b and q: no fused update (it swaps them)
     void* C = nullptr;
     void* alt = nullptr;       // red-black ping-pong partner of phi (this run's own)
     tk_unet net = nullptr;     // learned warm start (same context), or none
@@ -557,10 +561,21 @@ int rsfe_update(tk_rsfe s, int* extracted_out)
         TK_LAUNCHED();
         for (int a = 0; a < 3; ++a) TK_TRY(tk_vol_fill(s->b[a], 0.));
         TK_TRY(tk_vol_fill(s->divdb, 0.));
+    } else if (ctx->sb_fused && !s->in_batch && s->Z % (16 / (int)sizeof(T)) == 0 &&
+               s->X >= 3 && s->Y >= 3 && s->Z >= 3) {
+        // cut + norms + grad/shrink/Bregman and div in one pass: b -> q's buffers (q itself
+        // never reaches memory), then the two swap so that b holds the new values
+        {
+            TimedLaunch tl(ctx, TK_K_SBDIV);
+            TK_TRY(launch_sb_div<T>(ctx, s->phi, s->phiold, s->g, s->b, s->q, s->divdb, lo, hi,
+                                    &nb, part));
+        }
+        for (int a = 0; a < 3; ++a) std::swap(s->b[a]->d, s->q[a]->d);
+        hipLaunchKernelGGL(k_sum_pairs_rsfe, dim3(1), dim3(1024), 0, ctx->stream, part, nb,
+                           s->tol_dev);
+        TK_LAUNCHED();
     } else {
-        // cut + norms + grad/shrink/Bregman, then div.  (A single pass recomputing q of the
-        // six neighbours for div measured 281 us at 256^3 FP32 against 155 + 70 us for the
-        // two passes: 140 registers per lane, three waves per SIMD.)
+        // cut + norms + grad/shrink/Bregman, then div
         {
             TimedLaunch tl(ctx, TK_K_SB);
             TK_TRY(launch_sb_update<T>(ctx, s->phi, s->phiold, s->b, s->g, s->q, lo, hi, &nb,
@@ -1127,6 +1142,7 @@ int rsfe_batch_update_build(tk_rsfe* runs, int n, RsfeBatchUpdate** out)
     int first = 0;
     for (int v = 0; v < n; ++v) {
         tk_rsfe s = runs[v];
+        s->in_batch = true;
         SbJob& j = sb[(size_t)v];
         j.phiold = s->phiold->d;
         j.g = s->g->d;

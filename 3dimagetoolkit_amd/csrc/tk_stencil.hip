@@ -577,6 +577,357 @@ int stencil_batch_grid(tk_dtype dt, int X, int Y, int Z, int* gy, int* nparts)
     return TK_OK;
 }
 
+// ---- Split-Bregman update and div in one pass -------------------------------------------
+// The update's q = d - b is read only by div (imp:1906-1932: d is resized away after it), so
+// one pass can form q and its divergence without q going through memory.  div at a voxel
+// needs q_x of the planes before and after, q_y of the rows before and after and q_z of the
+// columns before and after, and q at a voxel needs phi around it; a workgroup owns a tile of
+// TJ rows x 64 columns over a chunk of planes and marches in x.  Thread u owns one 16-byte
+// row segment (row r, quad c4) of the tile's frame: rows j0-2 .. j0+TJ+1 and columns
+// kb-4 .. kb+67.  At step q it forms q on plane q for the frame rows j0-1 .. j0+TJ (every
+// column), with phi of its own column along x from registers and phi's y and z neighbours
+// from the plane staged in LDS; q_x stays in the thread's registers, q_y and q_z go to LDS;
+// then div on plane q-1 for the tile's own voxels (plane 0 waits one more step for q_x of
+// plane 2, its one-sided derivative).  The ring rows, the halo columns and the planes next
+// to a chunk are recomputed from the same inputs by the neighbouring workgroups, which
+// agree bit for bit; since those read b where its owner writes the new b, the pass reads
+// b_in and writes b_out (the caller swaps them).  Every value is the one the separate
+// update and div passes compute, with the same expressions in the same order; only the
+// tolerance sums are added in another order.
+constexpr int kSdTK = 64, kSdW = kSdTK + 8;
+
+template <class T, int TJ>
+constexpr int sd_threads()
+{
+    return ((TJ + 4) * (kSdW / Vec16<T>::n) + 63) / 64 * 64;
+}
+
+template <class T>
+struct SdArgs {
+    T* phi;
+    const T* phiold;
+    const T* g;
+    const T* bin[3];
+    T* bout[3];
+    T* divdb;
+    int X, Y, Z, CI;
+    double hx, hy, hz;
+    T lo, hi;
+    double* part;  // per-workgroup pairs (sum (phi-phiold)^2, sum phiold^2)
+};
+
+// three waves per SIMD: the march is latency-bound, and the registers of three waves hold
+// its rings
+template <class T, bool P2, int TJ>
+__global__ __launch_bounds__((sd_threads<T, TJ>())) __attribute__((amdgpu_waves_per_eu(3)))
+void k_sb_div(SdArgs<T> a)
+{
+    constexpr int V = Vec16<T>::n, W = kSdW, WV = W / V;
+    constexpr int RJ = TJ + 4;
+    constexpr int K = 4;  // ring period of the per-plane registers
+    __shared__ __align__(16) T Ph[2][RJ * W];  // phi of the plane being updated (raw)
+    __shared__ __align__(16) T Qy[2][RJ * W];  // q_y, q_z of the plane before
+    __shared__ __align__(16) T Qz[2][RJ * W];
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const uint3 bl = xcd_block3();  // tiles sharing rings run on one XCD
+    const int kb = bl.x * kSdTK, kbase = kb - 4;
+    // owned rows [jo0, jo1); the frame starts at j0, the last tile's frame ends at Y so that
+    // its own rows keep two rows of frame on each side inside the volume
+    const int jo0 = bl.y * TJ, jo1 = min(jo0 + TJ, Y);
+    const int j0 = max(0, min(jo0, Y - TJ));
+    const int ia = bl.z * a.CI, ib = min(ia + a.CI, X);  // owned planes (the host keeps the
+                                                          // first and last chunks >= 2 planes)
+    const int tid = threadIdx.x;
+    const bool act = tid < RJ * WV;
+    const int r = act ? tid / WV : 0, c4 = act ? tid - r * WV : 0;
+    const int j = j0 - 2 + r, k0 = kbase + c4 * V;
+    const bool inside = act && j >= 0 && j < Y && k0 >= 0 && k0 < Z;
+    const bool qunit = inside && r >= 1 && r <= RJ - 2;
+    const bool own = inside && j >= jo0 && j < jo1 && c4 >= 4 / V && c4 < (4 + kSdTK) / V;
+    const long sX = (long)Y * Z;
+    // every lane loads on every step (clamped rows outside the volume, never used)
+    const long off = (long)min(max(j, 0), Y - 1) * Z + min(max(k0, 0), Z - V);
+    const int li = r * W + c4 * V;
+    const int cj = plane_class(j, Y);
+    auto ld = [&](const T* f, int p) { return ldrow(f + (long)min(max(p, 0), X - 1) * sX + off); };
+    auto cl = [&](T x) { return clampv(x, a.lo, a.hi); };
+    auto id = [](T x) { return x; };
+    // y rows of a staged plane for row class cj (central: r+1, r-1; first: r, r+1, r+2;
+    // last: r, r-1, r-2), as the derivative helpers take them
+    auto yrows = [&](const T* pl) {
+        DRows<T> d;
+        d.c = cj;
+        if (cj == 1) {
+            d.r[0] = ldrow(pl + li + W);
+            d.r[1] = ldrow(pl + li - W);
+        } else if (cj == 0) {
+            d.r[0] = ldrow(pl + li);
+            d.r[1] = ldrow(pl + li + W);
+            d.r[2] = ldrow(pl + li + 2 * W);
+        } else {
+            d.r[0] = ldrow(pl + li);
+            d.r[1] = ldrow(pl + li - W);
+            d.r[2] = ldrow(pl + li - 2 * W);
+        }
+        return d;
+    };
+    const int qs = ia - 1, qe = ib;  // steps qs .. qe: update of plane q, div of plane q-1
+    // rings (plane p in slot (p - qs) mod K; the march is unrolled by K, so slots are
+    // constants): phi planes q-1 .. q+1 in use and q+2 in flight; b_in, g, phiold of plane
+    // q in use and q+1 in flight; q_x of planes q-3 .. q
+    Row<T> ph[K], rb0[K], rb1[K], rb2[K], rg[K], rpo[K], qx[K];
+#pragma unroll
+    for (int d = -1; d <= 2; ++d) ph[(d + K) % K] = ld(a.phi, qs + d);
+#pragma unroll
+    for (int d = 0; d <= 1; ++d) {
+        rb0[d] = ld(a.bin[0], qs + d);
+        rb1[d] = ld(a.bin[1], qs + d);
+        rb2[d] = ld(a.bin[2], qs + d);
+        rg[d] = ld(a.g, qs + d);
+        rpo[d] = ld(a.phiold, qs + d);
+    }
+#pragma unroll
+    for (int m = 0; m < K; ++m) qx[m] = Row<T>{};
+    Row<T> ty0{}, tz0{};  // plane 0's y and z terms, kept one step for its x term
+    double s1 = 0, s2 = 0;
+    // (inlined into every unrolled step: the rings must stay registers)
+    auto step = [&](auto u_c, int q) __attribute__((always_inline)) {
+        constexpr int u = decltype(u_c)::value;
+        auto sl = [](int d) { return ((u + d) % K + K) % K; };  // slot of plane q + d
+        if (act) strow(&Ph[q & 1][li], ph[sl(0)]);
+        __syncthreads();
+        // ---- update of plane q: cut, tolerance sums, grad, b += d, shrink, b -= d, d -= b
+        if (q >= 0 && q <= X - 1 && qunit) {
+            const T* pl = Ph[q & 1];
+            // x rows: the ring's planes q+1, q-1 inside; the first and last planes read their
+            // one-sided rows from memory (the ring's slots are only ever indexed by
+            // constants, so it stays in registers)
+            DRows<T> rx;
+            rx.c = plane_class(q, X);
+            rx.r[0] = ph[sl(1)];
+            rx.r[1] = ph[sl(-1)];
+            if (rx.c == 0) {
+                rx.r[0] = ld(a.phi, 0);
+                rx.r[1] = ld(a.phi, 1);
+                rx.r[2] = ld(a.phi, 2);
+            } else if (rx.c == 2) {
+                rx.r[0] = ld(a.phi, X - 1);
+                rx.r[1] = ld(a.phi, X - 2);
+                rx.r[2] = ld(a.phi, X - 3);
+            }
+            const DRows<T> ry = yrows(pl);
+            const Row<T> cc = ph[sl(0)];
+            const T zl = k0 > 0 ? pl[li - 1] : (T)0, zr = k0 + V < Z ? pl[li + V] : (T)0;
+            T d[3][V];
+            drow_eval<T, P2>(rx, a.hx, cl, d[0]);
+            drow_eval<T, P2>(ry, a.hy, cl, d[1]);
+            T c[V + 2];
+            c[0] = k0 > 0 ? cl(zl) : (T)0;
+            c[V + 1] = k0 + V < Z ? cl(zr) : (T)0;
+#pragma unroll
+            for (int e = 0; e < V; ++e) c[e + 1] = cl(cc.v[e]);
+#pragma unroll
+            for (int e = 0; e < V; ++e) d[2][e] = dz_win<T, V, P2>(c, e, k0 + e, Z, a.hz);
+            const bool mine = own && q >= ia && q < ib;
+            const long v0 = (long)q * sX + (long)j * Z + k0;
+            if (mine) {
+                const Row<T> po = rpo[sl(0)];
+                Row<T> pc;
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    pc.v[e] = c[e + 1];
+                    const double dd = (double)pc.v[e] - (double)po.v[e];
+                    s1 += dd * dd;
+                    s2 += (double)po.v[e] * (double)po.v[e];
+                }
+                strow(a.phi + v0, pc);
+            }
+            const Row<T> gv = rg[sl(0)];
+            Row<T> bb[3] = {rb0[sl(0)], rb1[sl(0)], rb2[sl(0)]};
+            Row<T> qq[3];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    T b = bb[ax].v[e];
+                    b += d[ax][e];
+                    const T m = fabs(b) - gv.v[e];
+                    const T z = 0;
+                    T dn = (T)((int)(b > 0) - (int)(b < 0)) * ((m < z) ? z : m);
+                    b -= dn;
+                    dn -= b;
+                    bb[ax].v[e] = b;
+                    qq[ax].v[e] = dn;
+                }
+                if (mine) strow(a.bout[ax] + v0, bb[ax]);
+            }
+            qx[sl(0)] = qq[0];
+            strow(&Qy[q & 1][li], qq[1]);
+            strow(&Qz[q & 1][li], qq[2]);
+        }
+        // ---- div of plane p = q - 1 (image3d_imp.hxx:947-1383, both quirks)
+        const int p = q - 1;
+        if (own && p >= ia && p < ib) {
+            const T* py = Qy[p & 1];
+            const T* pz = Qz[p & 1];
+            const DRows<T> ry = yrows(py);
+            T ty[V], tz[V];
+            drow_eval<T, P2>(ry, a.hy, id, ty);
+            const int ci = plane_class(p, X);
+            T c[V + 2];
+            c[0] = k0 > 0 ? pz[li - 1] : (T)0;
+            c[V + 1] = k0 + V < Z ? pz[li + V] : (T)0;
+            const Row<T> zc = ldrow(pz + li);
+#pragma unroll
+            for (int e = 0; e < V; ++e) c[e + 1] = zc.v[e];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const int k = k0 + e;
+                if (ci == 1 && cj == 0 && k == Z - 1)
+                    // edge (i inside, 0, Z-1): Z-2 used twice (image3d_imp.hxx:1192-1194)
+                    tz[e] = q2h<T, P2>(3 * c[e + 1] - 4 * c[e] + c[e], a.hz);
+                else
+                    tz[e] = dz_win<T, V, P2>(c, e, k, Z, a.hz);
+            }
+            if (p == 0) {
+                // x term needs q_x of plane 2, formed at the next step
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    ty0.v[e] = ty[e];
+                    tz0.v[e] = tz[e];
+                }
+            } else {
+                // x rows: planes p+1, p-1 inside; p = X-1: X-1, X-2, X-3 (the ring holds q_x of
+                // planes q-3 .. q; each case reads its own constant slots)
+                DRows<T> rx;
+                rx.c = ci;
+                if (ci == 1) {
+                    rx.r[0] = qx[sl(0)];
+                    rx.r[1] = qx[sl(-2)];
+                } else {
+                    const Row<T> x1 = qx[sl(-1)], x2 = qx[sl(-2)], x3 = qx[sl(-3)];
+                    rx.r[0] = x1;
+                    rx.r[1] = x2;
+                    rx.r[2] = x3;
+                }
+                T tx[V];
+                drow_eval<T, P2>(rx, a.hx, id, tx);
+                if (ci == 2 && cj == 1) {
+                    // face i = X-1 (j, k inside): "/ 2. * (hx)" (image3d_imp.hxx:1092-1095),
+                    // "/ 2." as a product with 0.5: the same number for every operand
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        const int k = k0 + e;
+                        if (k >= 1 && k <= Z - 2)
+                            tx[e] = (3 * rx.r[0].v[e] - 4 * rx.r[1].v[e] + rx.r[2].v[e]) *
+                                    (T)0.5 * (T)a.hx;
+                    }
+                }
+                Row<T> o;
+#pragma unroll
+                for (int e = 0; e < V; ++e) o.v[e] = tx[e] + ty[e] + tz[e];
+                strow(a.divdb + (long)p * sX + (long)j * Z + k0, o);
+            }
+        }
+        // plane 0's x term: q_x of planes 0, 1, 2 (the step that formed plane 2)
+        if (own && q == 2 && ia == 0) {
+            DRows<T> rx;
+            rx.c = 0;
+            rx.r[0] = qx[sl(-2)];
+            rx.r[1] = qx[sl(-1)];
+            rx.r[2] = qx[sl(0)];
+            T tx[V];
+            drow_eval<T, P2>(rx, a.hx, id, tx);
+            Row<T> o;
+#pragma unroll
+            for (int e = 0; e < V; ++e) o.v[e] = tx[e] + ty0.v[e] + tz0.v[e];
+            strow(a.divdb + (long)j * Z + k0, o);
+        }
+        // refill the slots this step was the last to use (phi q-1, the fields of q)
+        ph[sl(3)] = ld(a.phi, q + 3);
+        rb0[sl(2)] = ld(a.bin[0], q + 2);
+        rb1[sl(2)] = ld(a.bin[1], q + 2);
+        rb2[sl(2)] = ld(a.bin[2], q + 2);
+        rg[sl(2)] = ld(a.g, q + 2);
+        rpo[sl(2)] = ld(a.phiold, q + 2);
+    };
+    int q = qs;
+    for (; q + K <= qe + 1; q += K)
+        static_unroll<K>([&](auto u_c) { step(u_c, q + decltype(u_c)::value); });
+    static_unroll<K - 1>([&](auto u_c) {
+        if (q + decltype(u_c)::value <= qe) step(u_c, q + decltype(u_c)::value);
+    });
+    block_sum2(s1, s2);
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        a.part[2 * b] = s1;
+        a.part[2 * b + 1] = s2;
+    }
+}
+
+// the update and div of one volume in one pass (b_in -> b_out); *nblocks: the partial sums
+// written to part.  Needs 16-byte rows (nz % V == 0), every edge >= 3.
+template <class T>
+int launch_sb_div(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol g, tk_vol* b_in, tk_vol* b_out,
+                  tk_vol divdb, double a0, double b0, int* nblocks, double* part)
+{
+    // 8-row tiles: 16 rows measured 213 us and 4 rows 226 us against 184 us at 256^3 FP32
+    constexpr int TJ = 8;
+    constexpr int NT = sd_threads<T, TJ>();
+    const int X = phi->nx, Y = phi->ny, Z = phi->nz;
+    static int resident = 0;
+    if (resident <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sb_div<T, true, TJ>, NT, 0) !=
+                hipSuccess ||
+            nb < 1)
+            nb = 1;
+        resident = nb;
+    }
+    const int kt = (Z + kSdTK - 1) / kSdTK, jt = (Y + TJ - 1) / TJ;
+    // one round of resident workgroups; chunks of at least 4 planes (each recomputes one
+    // plane on each side), and no chunk of one plane at the end
+    long chunks = ((long)ctx->num_cu * resident + (long)kt * jt / 2) / ((long)kt * jt);
+    chunks = std::max(1L, std::min(chunks, (long)X / 4));
+    chunks = std::min(chunks, (long)kMaxPart / ((long)kt * jt));
+    int CI = (int)((X + chunks - 1) / chunks);
+    if (X % CI == 1) ++CI;
+    const int nz = (X + CI - 1) / CI;
+    TK_CHECK((long)kt * jt * nz <= kMaxPart, TK_ERR_UNSUPPORTED,
+             "update and div: %ld workgroups exceed the partial-sum slots", (long)kt * jt * nz);
+    SdArgs<T> args;
+    args.phi = (T*)phi->d;
+    args.phiold = (const T*)phiold->d;
+    args.g = (const T*)g->d;
+    for (int c = 0; c < 3; ++c) {
+        args.bin[c] = (const T*)b_in[c]->d;
+        args.bout[c] = (T*)b_out[c]->d;
+    }
+    args.divdb = (T*)divdb->d;
+    args.X = X;
+    args.Y = Y;
+    args.Z = Z;
+    args.CI = CI;
+    args.hx = phi->hx;
+    args.hy = phi->hy;
+    args.hz = phi->hz;
+    args.lo = (T)a0;
+    args.hi = (T)b0;
+    args.part = part;
+    const dim3 grid(kt, jt, nz);
+    if (pow2_2h3(phi->hx, phi->hy, phi->hz))
+        hipLaunchKernelGGL((k_sb_div<T, true, TJ>), grid, dim3(NT), 0, ctx->stream, args);
+    else
+        hipLaunchKernelGGL((k_sb_div<T, false, TJ>), grid, dim3(NT), 0, ctx->stream, args);
+    TK_LAUNCHED();
+    *nblocks = (int)(grid.x * grid.y * grid.z);
+    return TK_OK;
+}
+template int launch_sb_div<float>(tk_ctx, tk_vol, tk_vol, tk_vol, tk_vol*, tk_vol*, tk_vol, double,
+                                  double, int*, double*);
+template int launch_sb_div<double>(tk_ctx, tk_vol, tk_vol, tk_vol, tk_vol*, tk_vol*, tk_vol,
+                                   double, double, int*, double*);
+
 int launch_sb_batch(tk_ctx ctx, tk_dtype dt, const SbJob* jobs, int n, unsigned nblocks)
 {
     TimedLaunch tl(ctx, TK_K_SB);

@@ -44,6 +44,9 @@ BYTES_PER_VOXEL_W = {
     "gs": 3,        # one red-black sweep: read phi, rhs; write phi
     "sb": 13,       # read phi, phiold, b(3), g; write phi, b(3), q(3)
     "div": 4,       # read q(3), write divdb
+    "sbdiv": 11,    # update and div in one pass: read phi, phiold, b(3), g; write phi, b(3),
+                    # divdb (q never reaches memory)
+    "gs2": 6,       # two red-black sweeps in one launch: 2 x (read phi, rhs; write phi)
 }
 
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this bench's
@@ -161,6 +164,72 @@ def bench_warmstart(tk, a):
                       "sweeps_plain": sp, "sweeps_learned": sl, "sweep_ratio": sl / sp,
                       "seconds_plain": tp, "seconds_learned": tl,
                       "dice_median": float(np.median(dices)), "dice_min": float(min(dices))},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
+def bench_c0(tk, a):
+    """Config 0, time to converged mask: 64^3 two-ellipsoid phantom (seed of the committed
+    oracle fixture), rescaled to [0,255], sigma 3, FP64, every Poisson step gated on the
+    reference's relative-update rule at ls_tol 1e-8, 30 outer iterations; a step is one
+    whole run (set-up included).  Modes: lexicographic sweeps (the reference's order, the
+    parity mode) and red-black sweeps.  The masks are compared with the CPU FP64 oracle's
+    fixture."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "c0_64.npz"))
+    meta = json.loads(str(d["meta"]))
+    ref_mask = d["phi"] >= 0.5
+    shape = tuple(meta["shape"])
+    u0, phi0 = tk.gen_standard(shape, meta["seed"], 1, 2)
+    ctx = tk.Context(0)
+
+    def run(order, n_iter):
+        ctx.sync()
+        t0 = time.perf_counter()
+        vu = ctx.from_numpy(u0)
+        tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+        vi = ctx.from_numpy(phi0)
+        p = tk.default_params(sigma=meta["sigma"], maxiter=meta["maxiter"], tol=0.0,
+                              ls_steps=meta["ls_steps"], ls_tol=meta["ls_tol"], gs_order=order)
+        r = tk.Rsfe(ctx, vu, vi, p)
+        st = r.iterate(n_iter)
+        phi = r.phi()
+        ctx.sync()
+        t = time.perf_counter() - t0
+        r.close()
+        vu.free()
+        vi.free()
+        return t, phi, [s["sweeps"] for s in st]
+
+    modes = {}
+    for name, order in (("f64_lex", tk.TK_GS_LEX), ("f64_redblack", tk.TK_GS_REDBLACK)):
+        if a.warmup:
+            run(order, 1)
+        ts, sw = [], None
+        for _ in range(max(1, a.steps)):
+            t, phi, sw = run(order, meta["maxiter"])
+            ts.append(t)
+        mask = phi >= 0.5
+        inter = int(np.count_nonzero(mask & ref_mask))
+        dice = 2.0 * inter / max(1, int(mask.sum()) + int(ref_mask.sum()))
+        modes[name] = {"seconds": float(np.median(ts)), "runs": [round(x, 3) for x in ts],
+                       "sweeps_total": int(sum(sw)), "sweeps_first": sw[:3],
+                       "sweeps_equal_oracle": sw == [int(x) for x in d["sweeps"]],
+                       "max_abs_vs_oracle": float(np.max(np.abs(phi - d["phi"]))),
+                       "dice_vs_oracle": round(dice, 7),
+                       "mask_voxels_differing": int(np.count_nonzero(mask != ref_mask))}
+    ctx.close()
+    head = modes["f64_lex"]
+    out = {"metric": "time to converged mask, 64^3 FP64, Poisson gate ls_tol 1e-8 (config 0)",
+           "value": head["seconds"], "unit": "s", "n_gpus": 1, "steps": max(1, a.steps),
+           "warmup": a.warmup, "ms_per_step": 1e3 * head["seconds"], "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded two-ellipsoid phantom, rescaled to [0,255])",
+           "config": {"workload": "rsfe_splitbregman 64^3 f64, sigma 3 cyclic, 30 outer "
+                                  "iterations, relative-update gate 1e-8 (ls_steps 5); set-up "
+                                  "and host transfers included",
+                      "modes": modes,
+                      "oracle_seconds_recorded": meta.get("seconds"),
+                      "oracle_threads_recorded": meta.get("threads")},
            "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -337,9 +406,10 @@ def main():
                     help="time the headline mode only (not the two lexicographic modes)")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
-    ap.add_argument("--workload", choices=["c1", "batch", "warmstart"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "batch", "warmstart", "c0"], default="c1",
                     help="c1: one 256^3 volume (default); batch: 64 uneven volumes; "
-                         "warmstart: learned vs plain Poisson start (64^3, residual 1e-6)")
+                         "warmstart: learned vs plain Poisson start (64^3, residual 1e-6); "
+                         "c0: time to converged mask, 64^3 FP64 gated at 1e-8")
     ap.add_argument("--no-graph", action="store_true", help="batch: eager launches")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 512 MiB volume copy (k_copy16): the known-size kernel "
@@ -359,6 +429,9 @@ def main():
         return
     if a.workload == "warmstart":
         bench_warmstart(tk, a)
+        return
+    if a.workload == "c0":
+        bench_c0(tk, a)
         return
     n = a.size
     shape = (n, n, n)

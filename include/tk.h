@@ -72,9 +72,14 @@ enum tk_kernel_tag {
     TK_K_DIV = 6,     /* divergence */
     TK_K_OTHER = 7,
     TK_K_GS2 = 8,     /* two red-black sweeps in one launch */
-    TK_K_COUNT = 9
+    TK_K_SBDIV = 9,   /* cut + norms + grad/shrink/Bregman update and divergence in one pass */
+    TK_K_COUNT = 10
 };
 int tk_ctx_set_timing(tk_ctx ctx, int enable);
+/* single-volume Split-Bregman runs: the update and div in one pass (enable 1, the default)
+   or in two (0), on every later iteration of runs on ctx; the two give bitwise the same
+   phi, b and divdb (the tolerance sums are added in another order) */
+int tk_ctx_set_fused_update(tk_ctx ctx, int enable);
 /* totals since the last call with reset != 0: milliseconds and launches per tag */
 int tk_ctx_kernel_times(tk_ctx ctx, double ms[TK_K_COUNT], long launches[TK_K_COUNT], int reset);
 

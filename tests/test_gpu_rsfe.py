@@ -225,6 +225,34 @@ def test_c0_s5_f32_redblack_vs_golden(tk, ctx):
     assert d >= 0.98
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,h", [((24, 20, 28), 1.0), ((33, 21, 64), 1.0), ((9, 70, 68), 0.7),
+                                     ((96, 40, 136), 1.0)])
+def test_fused_update_and_div_equal_two_passes(tk, ctx, dtype, shape, h):
+    """the update and div in one pass (tiles with recomputed rings, the first and last planes,
+    rows and columns, general spacing) against the two passes: phi and rhs (which holds the
+    divergence of the iteration before) bitwise equal after several iterations; the
+    tolerance, whose sums are added in another order, to rounding"""
+    say("fused update", shape, h, dtype.__name__)
+    u0, phi0 = tk.gen_standard(shape, 7, 1, 2)
+    out = []
+    for fused in (False, True):
+        ctx.set_fused_update(fused)
+        vu = ctx.from_numpy(u0.astype(dtype), h=(h, 1.0, h))
+        tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+        vi = ctx.from_numpy(phi0.astype(dtype), h=(h, 1.0, h))
+        p = tk.default_params(sigma=2.0, maxiter=6, tol=0.0, ls_steps=5, ls_tol=0.0,
+                              gs_order=tk.TK_GS_LEX)
+        r = tk.Rsfe(ctx, vu, vi, p)
+        st = r.iterate(6)
+        out.append((r.phi(), r.rhs(), [x["tollerance"] for x in st]))
+        r.close()
+    ctx.set_fused_update(True)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.allclose(out[0][2], out[1][2], rtol=1e-9 if dtype == np.float32 else 1e-12, atol=0)
+
+
 @pytest.mark.parametrize("dtype,order", [(np.float32, "redblack"), (np.float64, "lex")])
 def test_iterate_ahead_equals_one_call_per_iteration(tk, ctx, dtype, order):
     """tk_rsfe_iterate(n) enqueues each next iteration's convolution passes before it waits
