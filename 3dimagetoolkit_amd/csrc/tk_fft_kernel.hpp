@@ -264,7 +264,7 @@ __device__ __forceinline__ T fmat(T a, T b, T c)
 template <class T, int EPT, int NT, int S>
 __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay& y)
 {
-    constexpr int B = 4, W = B + 2 * S, MAXI = (EPT + B - 1) / B;
+    constexpr int B = 4, MAXI = (EPT + B - 1) / B;
     const int L = y.L;
     // halo: positions -S..-1 and L..L+S-1 hold x((pos mod L)), or zeros (acyclic)
     for (int e = threadIdx.x; e < 2 * S * y.P; e += NT) {
@@ -299,19 +299,41 @@ __device__ __forceinline__ void direct_win(cplx<T>* buf, const T* wh, const Lay&
                                     (br & ~7)
                               : br;
             const int base = y.off + (b * B - S) * y.SP + p * y.SQ;
-            cplx<T> x[W];
-#pragma unroll
-            for (int t = 0; t < W; ++t) x[t] = buf[base + t * y.SP];
+            // Output u = w(0) x(u+S) + sum_{n=1..S} w(n) (x(u+S-n) + x(u+S+n)), n ascending.
+            // For each n the B outputs take the pairs from a left window L(u) = x(u+S-n) and
+            // a right window R(u) = x(u+S+n); going to n+1 both slide outwards by one
+            // position, so a step reads two values from LDS and only 2B inputs are live
+            // (not the B + 2S of the whole window).
+            cplx<T> L[B], R[B], a[B];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                cplx<T> a = {w[0] * x[u + S].x, w[0] * x[u + S].y};
-#pragma unroll
-                for (int n = 1; n <= S; ++n) {
-                    a.x = fmat(w[n], x[u + S - n].x + x[u + S + n].x, a.x);
-                    a.y = fmat(w[n], x[u + S - n].y + x[u + S + n].y, a.y);
-                }
-                out[q][u] = a;
+                L[u] = buf[base + (S - 1 + u) * y.SP];
+                R[u] = buf[base + (S + 1 + u) * y.SP];
             }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                // x(u+S): inside L or R for every u but the two ends of B = 4
+                const cplx<T> xc = u + 1 < B ? L[u + 1] : R[u - 1];
+                a[u] = {w[0] * xc.x, w[0] * xc.y};
+            }
+#pragma unroll
+            for (int n = 1; n <= S; ++n) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    a[u].x = fmat(w[n], L[u].x + R[u].x, a[u].x);
+                    a[u].y = fmat(w[n], L[u].y + R[u].y, a[u].y);
+                }
+                if (n < S) {
+#pragma unroll
+                    for (int u = B - 1; u > 0; --u) L[u] = L[u - 1];
+                    L[0] = buf[base + (S - n - 1) * y.SP];
+#pragma unroll
+                    for (int u = 0; u < B - 1; ++u) R[u] = R[u + 1];
+                    R[B - 1] = buf[base + (S + n + B) * y.SP];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) out[q][u] = a[u];
             dst[q] = y.off + b * B * y.SP + p * y.SQ;
         }
     }
@@ -821,6 +843,7 @@ struct ConvPlan {
     AxisDev ax;
     AxisGeom g;
     bool mf = false, quad = false;
+    bool small = false;  // FP32 register window in groups of 2048 elements on 256 threads
     int S = 0;
     unsigned nb = 0;
     size_t lds = 0;
@@ -842,7 +865,13 @@ int conv_plan(const tk_filter_s* f, int axis, bool acyclic, ConvPlan* out)
     // the group shape decides the tiling)
     p.mf = std::is_same<T, float>::value && ax.direct == 2 && dims4 &&
            direct_width_supported(ax.nh) && mfma_window_fits(axis, f->dims);
-    const int elems = (p.mf ? kMfmaEpt : EPT) * NT;
+    // the FP32 register window of the z axis runs groups of 2048 elements on 256 threads,
+    // like the matrix-core window (six workgroups per CU at different phases of their load,
+    // arithmetic and store: the z pass with the region fit 107 -> 100 us at 256^3); x and y
+    // keep groups of 4096 on 512 threads (x 76 -> 87 us with the smaller groups)
+    p.small = std::is_same<T, float>::value && axis == 2 &&
+              (ax.direct == 1 || (acyclic && direct_width_supported(ax.nh)));
+    const int elems = (p.mf || p.small ? kMfmaEpt : EPT) * NT;
     p.g = make_geom(axis, f->dims[0], f->dims[1], f->dims[2], ax.L, elems);
     TK_CHECK(ax.L * p.g.P <= EPT * NT, TK_ERR_UNSUPPORTED, "conv: axis length %d exceeds %d",
              ax.L, EPT * NT);
@@ -865,6 +894,11 @@ int conv_plan(const tk_filter_s* f, int axis, bool acyclic, ConvPlan* out)
             if ((plan).mf && S_ == 9) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 9, true), NT_);    \
             else if ((plan).mf && S_ == 12) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 12, true), NT_); \
             else if ((plan).mf && S_ == 18) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 18, true), NT_); \
+            else if ((plan).small && S_ == 9) KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 9, false), NT_); \
+            else if ((plan).small && S_ == 12)                                                 \
+                KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 12, false), NT_);                       \
+            else if ((plan).small && S_ == 18)                                                 \
+                KERNEL_LAUNCH((T, kMfmaEpt, NT_, true, 18, false), NT_);                       \
             else if (S_ == 9) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 9, false), 2 * NT_);   \
             else if (S_ == 12) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 12, false), 2 * NT_); \
             else if (S_ == 18) KERNEL_LAUNCH((T, EPT_ / 2, 2 * NT_, true, 18, false), 2 * NT_); \

@@ -1002,7 +1002,7 @@ int build_stage(std::vector<BatchGroup>& out, tk_rsfe* runs, int n, int axis, in
         TK_TRY(conv_plan<T>(s->flt, axis, s->p.acyclic != 0, &pl));
         size_t q = 0;
         while (q < out.size() && !(out[q].like.S == pl.S && out[q].like.quad == pl.quad &&
-                                   out[q].like.mf == pl.mf))
+                                   out[q].like.mf == pl.mf && out[q].like.small == pl.small))
             ++q;
         if (q == out.size()) {
             out.emplace_back();
