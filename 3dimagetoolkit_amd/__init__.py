@@ -476,7 +476,8 @@ def poisson_rb_autotune(ctx, rhs):
 
 
 def poisson_rb_set_pair(ctx, enable):
-    """fixed red-black sweep counts: two sweeps per launch (True, the default) or one"""
+    """red-black sweeps: two per launch where the autotuner found that faster (True, the
+    default) or always one; gated solves stop at the same sweep with the same field"""
     _check(lib.tk_poisson_rb_set_pair(ctx.h, int(bool(enable))))
 
 

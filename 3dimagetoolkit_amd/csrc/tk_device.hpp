@@ -23,6 +23,60 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// wave64 sum of a double through DPP row moves (a few cycles each, where the shuffles above
+// go through the LDS crossbar): per row of 16 lanes, quad sums (quad_perm xor 1, xor 2),
+// then row rotations by 4 and 8; lanes 0, 16, 32, 48 then hold the row sums, added as
+// (r0 + r1) + (r2 + r3).  The order is fixed, so equal inputs give equal sums in every
+// wave.  All 64 lanes must be active; the result is uniform.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline double wave_sum_dpp(double v)
+{
+    v += dpp_mov_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov_f64<0x124>(v);  // row_ror:4
+    v += dpp_mov_f64<0x128>(v);  // row_ror:8
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// block sum of two doubles with the DPP wave sum (every wave full); result valid in the
+// thread with threadIdx.x == 0 of a 1-D block of at most 1024 threads
+__device__ inline void block_sum2_dpp(double& a, double& b)
+{
+    __shared__ double sa[16], sb[16];
+    a = wave_sum_dpp(a);
+    b = wave_sum_dpp(b);
+    const int t = threadIdx.x, w = t / kWave, nw = blockDim.x / kWave;
+    __syncthreads();
+    if ((t & (kWave - 1)) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double x = 0, y = 0;
+        for (int i = 0; i < nw; ++i) {
+            x += sa[i];
+            y += sb[i];
+        }
+        a = x;
+        b = y;
+    }
+}
+
 __device__ inline float wave_min(float v)
 {
 #pragma unroll

@@ -403,7 +403,8 @@ struct RbJob {
 };
 
 struct RbGate {
-    double* acc;        // pair per sweep slot: sum (new-old)^2, sum old^2 (interior)
+    double* acc;        // pair per sweep slot: sum (new-old)^2, sum old^2 (interior); the
+                        // column kernels use acc[0..1] only: the faces' terms of sweep 0
     double* err;        // per slot: relative update error
     unsigned* stop;     // cnt[0]
     unsigned* arrive;   // cnt + 1
@@ -411,7 +412,100 @@ struct RbGate {
     double ls_tol;
     double h3;          // hx*hy*hz
     double bnd_o2;      // boundary contribution to sum old^2 for sweeps >= 1
+    // column kernels: per-block sum pairs of sweep s at part + (s & 3) * 2 * kRbGateBlk,
+    // and the previous launch of the batch (its first sweep, sweep count and block count;
+    // prev_n = 0: none), whose stop test the launch runs before its own sweeps
+    double* part;
+    int base;           // sweep index of error slot 0
+    int prev_first, prev_n;
+    unsigned prev_nblk;
 };
+
+// The update norm sums of sweep sp: the per-block pairs its launch stored (nblk blocks),
+// added in one fixed order (lane-strided sums, then the DPP wave sum), so every workgroup
+// that tests a sweep gets the same values.  Called by a whole wave; uniform result.
+__device__ double2 rb_gate_sums(const RbGate& g, int sp, unsigned nblk)
+{
+    const double2* pp = (const double2*)(g.part + (size_t)(sp & 3) * (2 * kRbGateBlk));
+    const unsigned l = threadIdx.x & 63;
+    double sd = 0, so = 0;
+    // the first 256 pairs in straight-line code: four loads per lane issued together, with
+    // clamped indices (a loop, or loads under a branch, would make the wave wait first for
+    // every load in flight, the march's prologue included)
+    {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = pp[min(64u * u + l, nblk - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (64u * u + l < nblk) {
+                sd += v[u].x;
+                so += v[u].y;
+            }
+    }
+    for (unsigned b = 256 + l; b < nblk; b += 64) {
+        sd += pp[b].x;
+        so += pp[b].y;
+    }
+    return double2{wave_sum_dpp(sd), wave_sum_dpp(so)};
+}
+
+// The stop test of sweep sp from its sums (plus the faces' terms: those of sweep 0 in
+// acc[0..1], the constant face value's for later sweeps); the writer's lane 0 records the
+// error and, on a stop, the sweep count.
+__device__ bool rb_gate_decide(const RbGate& g, int sp, double2 s, bool writer)
+{
+    double sd = s.x, so = s.y;
+    if (sp == 0) {
+        sd += g.acc[0];
+        so += g.acc[1];
+    } else {
+        so += g.bnd_o2;
+    }
+    const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
+    const bool stop = sp >= g.ls_steps - 1 && !(err > g.ls_tol);
+    if (writer && (threadIdx.x & 63) == 0) {
+        g.err[sp - g.base] = err;
+        if (stop) *(volatile unsigned*)g.stop = (unsigned)(sp + 1);
+    }
+    return stop;
+}
+
+// Whether a gated column launch runs: the stop flag is clear and no sweep of the previous
+// launch met the stop condition (tested in sweep order).  Wave 0 decides, the workgroup
+// learns it through LDS; block 0 is the writer.  The flag is read with a vector load, so it
+// is in flight together with the sums' loads (and the march's prologue loads issued
+// before): the flag that matters was set by an earlier launch; one set by this launch's
+// writer only repeats the decision that every workgroup reaches by itself.
+__device__ bool rb_gate_live(const RbGate& g, bool writer)
+{
+    __shared__ int live;
+    if (threadIdx.x < 64) {
+        const unsigned stopped = __hip_atomic_load(g.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool go = true;
+        if (g.prev_n > 0) {
+            const double2 s0 = rb_gate_sums(g, g.prev_first, g.prev_nblk);
+            go = !rb_gate_decide(g, g.prev_first, s0, writer && stopped == 0);
+            if (go && g.prev_n > 1) {
+                const double2 s1 = rb_gate_sums(g, g.prev_first + 1, g.prev_nblk);
+                go = !rb_gate_decide(g, g.prev_first + 1, s1, writer && stopped == 0);
+            }
+        }
+        if (threadIdx.x == 0) live = go && stopped == 0 ? 1 : 0;
+    }
+    __syncthreads();
+    return live != 0;
+}
+
+// the stop test of the batch's last launch (its successor would have run it)
+__global__ __launch_bounds__(64) void k_rb_gate_final(RbGate g)
+{
+    if (*(volatile const unsigned*)g.stop) return;
+    for (int t = 0; t < g.prev_n; ++t)
+        if (rb_gate_decide(g, g.prev_first + t, rb_gate_sums(g, g.prev_first + t, g.prev_nblk),
+                           true))
+            return;
+}
 
 // Fused red-black sweep, one launch per sweep, dst = sweep(src) (ping-pong buffers).
 // A workgroup owns a TJ x TK tile of (j, k) and marches over CI planes in i.  For each
@@ -547,10 +641,9 @@ constexpr int kRbW = kRbTK + 8;   // LDS row: k from 64*bx - 4 to 64*bx + 67
 template <class T, bool UNIT, int NS, int TJ, int NT>
 __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __restrict__ dst,
                                             const T* __restrict__ rhs, int X, int Y, int Z, int CI,
-                                            const GsCoef& c, int s_idx, int slot, int gated,
-                                            const RbGate& g, uint3 bl, unsigned nblk)
+                                            const GsCoef& c, int s_idx, int gated,
+                                            const RbGate& g, uint3 bl, unsigned fb)
 {
-    if (gated && *(volatile unsigned*)g.stop) return;
     constexpr int V = Vec16<T>::n;
     constexpr int W = kRbW, WV = W / V;
     constexpr int NST = 2 * NS, LS = NST - 1;
@@ -646,7 +739,24 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
     for (int s = 0; s < NST - 1; ++s)
 #pragma unroll
         for (int m = 0; m < K; ++m) ra[s][m] = Row<T>{};
-    double d2 = 0, o2 = 0;
+    // gated: the stop test of the previous launch, while the prologue's loads are in flight
+    if (gated && !rb_gate_live(g, fb == 0)) return;
+    // update norm sums of each sweep over the voxels this workgroup owns
+    double d2[NS], o2[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) d2[n] = o2[n] = 0;
+    auto acc_norm = [&](int n, const Row<T>& nw, const Row<T>& old) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const int k = k0 + e;
+            if (k >= 1 && k <= Z - 2) {
+                const double d = (double)nw.v[e] - (double)old.v[e];
+                d2[n] += d * d;
+                o2[n] += (double)old.v[e] * (double)old.v[e];
+            }
+        }
+    };
+    const bool own_seg = act && r >= NST && r < NST + TJ && j <= Y - 2;
     const int owned_c0 = 4 / V, owned_c1 = (4 + kRbTK) / V;
     // step q = q0 + u (q0 - qa a multiple of K)
     auto step = [&](auto u_c, int q) {
@@ -667,6 +777,8 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
             const bool run = p >= ia - (LS - s) && p < ib + (LS - s) && p >= 0 && p <= X - 1;
             Row<T> out = y;
             if (run) out = half(s, p, x, y, z, nb[s], rb[sl(-s)]);
+            const bool own = run && own_seg && p >= ia && p < ib && c4 >= owned_c0 &&
+                             c4 < owned_c1 && k0 < Z;
             if constexpr (s < LS) {
                 // plane p of this stage; the next stage (plane p - 1) takes p - 2, p - 1, p
                 if (act && run) strow(&P[s + 1][p & 1][li], out);
@@ -674,22 +786,14 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
                 x = ra[s][sl(-s - 2)];
                 y = ra[s][sl(-s - 1)];
                 z = out;
+                // stage 1 of two sweeps ends the first: its old values are plane q - 1
+                if (s == 1 && gated && own) acc_norm(0, out, ro[sl(-1)]);
             } else {
-                if (run && p >= ia && p < ib && r >= NST && r < NST + TJ && j <= Y - 2 &&
-                    c4 >= owned_c0 && c4 < owned_c1 && k0 < Z) {
+                if (own) {
                     strow(dst + (long)p * sX + (long)j * Z + k0, out);
-                    if (NS == 1 && gated) {
-                        const Row<T>& old = ro[sl(-1)];  // the old value of plane q - 1
-#pragma unroll
-                        for (int e = 0; e < V; ++e) {
-                            const int k = k0 + e;
-                            if (k >= 1 && k <= Z - 2) {
-                                const double d = (double)out.v[e] - (double)old.v[e];
-                                d2 += d * d;
-                                o2 += (double)old.v[e] * (double)old.v[e];
-                            }
-                        }
-                    }
+                    // old values of plane q - LS: the field (one sweep) or the first
+                    // sweep's output (two)
+                    if (gated) acc_norm(NS - 1, out, NS == 1 ? ro[sl(-1)] : ra[NS - 1][sl(-3)]);
                 }
             }
         });
@@ -705,20 +809,14 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
         if (q + decltype(u_c)::value < qe) step(u_c, q + decltype(u_c)::value);
     });
     if (!gated) return;
-    block_sum2(d2, o2);
-    if (tid == 0) {
-        atomicAdd(&g.acc[2 * slot], d2);
-        atomicAdd(&g.acc[2 * slot + 1], o2);
-        __threadfence();
-        const unsigned ticket = atomicAdd(&g.arrive[slot], 1u);
-        if (ticket == nblk - 1) {
-            const double sd = atomicAdd(&g.acc[2 * slot], 0.0);
-            double so = atomicAdd(&g.acc[2 * slot + 1], 0.0);
-            if (s_idx > 0) so += g.bnd_o2;
-            const double err = sqrt(g.h3 * sd) / sqrt(g.h3 * so);
-            g.err[slot] = err;
-            g.arrive[slot] = 0;
-            if (s_idx >= g.ls_steps - 1 && !(err > g.ls_tol)) atomicExch(g.stop, (unsigned)(s_idx + 1));
+    // this block's sums of each sweep, for the stop test that the next launch runs
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        block_sum2_dpp(d2[n], o2[n]);
+        if (tid == 0) {
+            double* pp = g.part + (size_t)((s_idx + n) & 3) * (2 * kRbGateBlk) + 2 * (size_t)fb;
+            pp[0] = d2[n];
+            pp[1] = o2[n];
         }
     }
 }
@@ -733,11 +831,11 @@ constexpr int rbc_threads()
 template <class T, bool UNIT, int NS, int TJ>
 __global__ __launch_bounds__((rbc_threads<T, NS, TJ>())) void k_gs_rbc(
     const T* __restrict__ src, T* __restrict__ dst, const T* __restrict__ rhs, int X, int Y, int Z,
-    int CI, GsCoef c, int s, int slot, int gated, RbGate g)
+    int CI, GsCoef c, int s, int gated, RbGate g)
 {
-    rb_col_tile<T, UNIT, NS, TJ, rbc_threads<T, NS, TJ>()>(src, dst, rhs, X, Y, Z, CI, c, s, slot,
-                                                           gated, g, xcd_block3(),
-                                                           gridDim.x * gridDim.y * gridDim.z);
+    rb_col_tile<T, UNIT, NS, TJ, rbc_threads<T, NS, TJ>()>(
+        src, dst, rhs, X, Y, Z, CI, c, s, gated, g, xcd_block3(),
+        blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // Many volumes, one sweep each, in one launch (the batch's fixed-sweep path): the job table
@@ -766,7 +864,7 @@ __global__ __launch_bounds__((rbc_threads<T, 1, TJ>())) void k_gs_rb_pv_batch(
     bl.z = w / jb.gy;
     rb_col_tile<T, UNIT, 1, TJ, rbc_threads<T, 1, TJ>()>((const T*)jb.src, (T*)jb.dst,
                                                          (const T*)jb.rhs, jb.X, jb.Y, jb.Z,
-                                                         jb.CI, c, 0, 0, 0, RbGate{}, bl, n);
+                                                         jb.CI, c, 0, 0, RbGate{}, bl, 0);
 }
 
 __global__ void k_zero_d(double* p, int n)
@@ -790,22 +888,19 @@ constexpr long rbc_lds_bytes()
     return (long)sizeof(T) * 2L * 2 * NS * (TJ + 4L * NS) * kRbW;
 }
 
-// NS aligned red-black sweeps src -> dst with TJ-row tiles
+// grid of NS aligned red-black sweeps with TJ-row tiles: the chunk length in x is chosen so
+// the grid is one round of as many workgroups per CU as the kernel's registers and LDS
+// allow (asked once per instance); a chunk recomputes 2 NS - 1 planes on each side for the
+// first stage, so chunks keep at least 2 planes for one sweep (small volumes then spread
+// over more workgroups: a 64^3 sweep is a few march steps long) and 8 for two
 template <class T, int NS, int TJ>
-static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
-                      int slot, int gated, const RbGate& g, bool unit)
+static void rbc_grid(tk_ctx ctx, int X, int Y, int Z, dim3* grid, int* CI)
 {
-    constexpr int NT = rbc_threads<T, NS, TJ>();
-    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
-    // chunk length in x chosen so the grid is one round of as many workgroups per CU as the
-    // kernel's registers and LDS allow (asked once per instance); a chunk recomputes
-    // 2 NS - 1 planes on each side for the first stage, so chunks keep at least 2 planes
-    // for one sweep (small volumes then spread over more workgroups: a 64^3 sweep is a few
-    // march steps long) and 8 for two
     static int resident = 0;
     if (resident <= 0) {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gs_rbc<T, true, NS, TJ>, NT, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gs_rbc<T, true, NS, TJ>,
+                                                         rbc_threads<T, NS, TJ>(), 0) !=
                 hipSuccess ||
             nb < 1)
             nb = 1;
@@ -815,63 +910,97 @@ static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const 
     const int kt = (Z + kRbTK - 1) / kRbTK, jt = (Y - 2 + TJ - 1) / TJ;
     long chunks = ((long)ctx->num_cu * per_cu + (long)kt * jt / 2) / ((long)kt * jt);
     if (chunks < 1) chunks = 1;
-    constexpr int min_ci = NS == 1 ? 2 : 8;
+    constexpr int min_ci = 2;
     if (chunks > (X - 2) / min_ci) chunks = std::max(1, (X - 2) / min_ci);
-    const int CI = (int)((X - 2 + chunks - 1) / chunks);
-    const dim3 grid(kt, jt, (X - 2 + CI - 1) / CI);
+    *CI = (int)((X - 2 + chunks - 1) / chunks);
+    *grid = dim3(kt, jt, (X - 2 + *CI - 1) / *CI);
+}
+
+// NS aligned red-black sweeps src -> dst (sweep indices s, s + 1) with TJ-row tiles
+template <class T, int NS, int TJ>
+static int launch_rbc(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
+                      int gated, const RbGate& g, bool unit)
+{
+    constexpr int NT = rbc_threads<T, NS, TJ>();
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    dim3 grid;
+    int CI = 0;
+    rbc_grid<T, NS, TJ>(ctx, X, Y, Z, &grid, &CI);
     if (unit)
         hipLaunchKernelGGL((k_gs_rbc<T, true, NS, TJ>), grid, dim3(NT), 0, ctx->stream,
-                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                           gated, g);
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, gated, g);
     else
         hipLaunchKernelGGL((k_gs_rbc<T, false, NS, TJ>), grid, dim3(NT), 0, ctx->stream,
-                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, slot,
-                           gated, g);
+                           (const T*)src, (T*)dst, (const T*)rhs->d, X, Y, Z, CI, c, s, gated, g);
     TK_LAUNCHED();
     return TK_OK;
 }
 
-template <class T, int TJ>
-static int launch_rb_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
-                        int s, int slot, int gated, const RbGate& g, bool unit)
-{
-    return launch_rbc<T, 1, TJ>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
-}
-
-template <int TJ>
-static int launch_rb2_pv(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
-                         bool unit)
-{
-    return launch_rbc<float, 2, TJ>(ctx, src, dst, rhs, c, 0, 0, 0, RbGate{}, unit);
-}
-
-// the autotuner's key for the two-sweep launch of a shape (next to the one-sweep key)
-inline long long rb2_key(int X, int Y, int Z) { return rb_key(X, Y, Z, 15); }
-
-// two sweeps src -> dst in one launch where the kernel serves the volume (FP32, z a multiple
-// of 4, at least 3 interior planes), else nothing: *done = false
+// the autotuner's key for the two-sweep launch of a shape and type (next to the one-sweep
+// key, whose last field is the element size)
 template <class T>
-static int rb_sweep_pair(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c,
-                         bool* done)
+inline long long rb2_key(int X, int Y, int Z)
 {
-    *done = false;
-    if constexpr (std::is_same<T, float>::value) {
-        const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
-        if (!ctx->rb_pair || Z % Vec16<float>::n != 0 || X < 5 || Y < 3) return TK_OK;
-        const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
-        // tile rows of the two-sweep launch for this shape: 8 or 16, or 0 where the
-        // autotuner found it slower than two one-sweep launches; untuned shapes run one
-        // sweep per launch
-        const auto it = ctx->rb_rows.find(rb2_key(X, Y, Z));
-        if (it == ctx->rb_rows.end() || it->second == 0) return TK_OK;
-        TimedLaunch tl(ctx, TK_K_GS2);
-        if (it->second == 16)
-            TK_TRY(launch_rb2_pv<16>(ctx, src, dst, rhs, c, unit));
-        else
-            TK_TRY(launch_rb2_pv<8>(ctx, src, dst, rhs, c, unit));
-        *done = true;
+    return rb_key(X, Y, Z, sizeof(T) == 4 ? 15 : 14);
+}
+
+// tile rows of the column kernel for n (1 or 2) sweeps per launch on rhs's shape: 8 or 16,
+// or 0 where it does not serve: rows not 16-byte aligned; for two sweeps also fewer than 3
+// interior planes, two-sweep launches switched off, an untuned shape, or one where the
+// autotuner found them slower than two one-sweep launches
+template <class T>
+static int rbc_rows(tk_ctx ctx, tk_vol rhs, int n)
+{
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    if (Z % Vec16<T>::n != 0) return 0;
+    if (n == 1) {
+        const auto it = ctx->rb_rows.find(rb_key(X, Y, Z, (int)sizeof(T)));
+        return it != ctx->rb_rows.end() && it->second == 16 ? 16 : 8;
     }
-    return TK_OK;
+    if (!ctx->rb_pair || X < 5 || Y < 3) return 0;
+    const auto it = ctx->rb_rows.find(rb2_key<T>(X, Y, Z));
+    return it == ctx->rb_rows.end() ? 0 : it->second;
+}
+
+// FP64 two-sweep tiles: 6 rows, so the 14-row extended tile is 504 threads (8 waves, up to
+// 256 registers each; the 8-row tile's 9 waves would allow 168, and its rings spill)
+constexpr int kRb2TJ64 = 6;
+
+// blocks of the column launch of n sweeps with `rows` tile rows
+template <class T>
+static unsigned rbc_blocks(tk_ctx ctx, tk_vol rhs, int n, int rows)
+{
+    dim3 grid;
+    int CI = 0;
+    const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
+    if (n == 2) {
+        if constexpr (sizeof(T) == 8) rbc_grid<T, 2, kRb2TJ64>(ctx, X, Y, Z, &grid, &CI);
+        else if (rows == 16) rbc_grid<T, 2, 16>(ctx, X, Y, Z, &grid, &CI);
+        else rbc_grid<T, 2, 8>(ctx, X, Y, Z, &grid, &CI);
+    } else {
+        if (rows == 16) rbc_grid<T, 1, 16>(ctx, X, Y, Z, &grid, &CI);
+        else rbc_grid<T, 1, 8>(ctx, X, Y, Z, &grid, &CI);
+    }
+    return grid.x * grid.y * grid.z;
+}
+
+// n (1 or 2) sweeps src -> dst with the column kernel of `rows` tile rows
+template <class T>
+static int rbc_launch(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int n,
+                      int rows, int s, int gated, const RbGate& g)
+{
+    const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
+    if (n == 2) {
+        TimedLaunch tl(ctx, TK_K_GS2);
+        if constexpr (sizeof(T) == 8)
+            return launch_rbc<T, 2, kRb2TJ64>(ctx, src, dst, rhs, c, s, gated, g, unit);
+        else
+            return rows == 16 ? launch_rbc<T, 2, 16>(ctx, src, dst, rhs, c, s, gated, g, unit)
+                              : launch_rbc<T, 2, 8>(ctx, src, dst, rhs, c, s, gated, g, unit);
+    }
+    TimedLaunch tl(ctx, TK_K_GS);
+    return rows == 16 ? launch_rbc<T, 1, 16>(ctx, src, dst, rhs, c, s, gated, g, unit)
+                      : launch_rbc<T, 1, 8>(ctx, src, dst, rhs, c, s, gated, g, unit);
 }
 
 // the batched sweep: 8-row tiles of 256 threads (the per-volume autotuner's choice at
@@ -932,20 +1061,15 @@ static int rb_batch_table_t(tk_ctx ctx, const RbBatchVol* vols, int n, void** ta
     return TK_OK;
 }
 
+// one red-black sweep with the per-voxel kernel (rows not 16-byte aligned, or a gated grid
+// with more blocks than the column kernels' stop test holds)
 template <class T>
-static int rb_sweep(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
-                    int slot, int gated, const RbGate& g)
+static int rb_sweep_pp(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const GsCoef& c, int s,
+                       int slot, int gated, const RbGate& g)
 {
     const int X = rhs->nx, Y = rhs->ny, Z = rhs->nz;
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1. && c.dt == 1.;
     TimedLaunch tl(ctx, TK_K_GS);
-    constexpr int V = Vec16<T>::n;
-    if (Z % V == 0) {
-        const auto it = ctx->rb_rows.find(rb_key(X, Y, Z, (int)sizeof(T)));
-        if (it != ctx->rb_rows.end() && it->second == 16)
-            return launch_rb_pv<T, 16>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
-        return launch_rb_pv<T, 8>(ctx, src, dst, rhs, c, s, slot, gated, g, unit);
-    }
     const dim3 grid((Z - 2 + kRbTK - 1) / kRbTK, (Y - 2 + kRbTJ - 1) / kRbTJ,
                     (X - 2 + kRbCI - 1) / kRbCI);
     if (unit)
@@ -983,8 +1107,8 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
         TK_HIP(hipEventCreate(&e1));
         for (int v = 0; v < 2; ++v) {
             auto launch = [&]() {
-                return v == 0 ? launch_rb_pv<T, 8>(ctx, a, b, rhs, c, 0, 0, 0, g, unit)
-                              : launch_rb_pv<T, 16>(ctx, a, b, rhs, c, 0, 0, 0, g, unit);
+                return v == 0 ? launch_rbc<T, 1, 8>(ctx, a, b, rhs, c, 0, 0, g, unit)
+                              : launch_rbc<T, 1, 16>(ctx, a, b, rhs, c, 0, 0, g, unit);
             };
             TK_TRY(launch());
             TK_HIP(hipEventRecord(e0, ctx->stream));
@@ -998,14 +1122,17 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
         return TK_OK;
     };
     rc = body();
-    // the two-sweep launch (FP32): the same two tile heights
+    // the two-sweep launch: the same two tile heights (FP64: one, of 6 rows, in ms2[0])
     double ms2[2] = {0, 0};
-    if (rc == TK_OK && std::is_same<T, float>::value && X >= 5 && Y >= 3) {
+    if (rc == TK_OK && X >= 5 && Y >= 3) {
         rc = [&]() -> int {
-            for (int v = 0; v < 2; ++v) {
+            for (int v = 0; v < (sizeof(T) == 8 ? 1 : 2); ++v) {
                 auto launch = [&]() {
-                    return v == 0 ? launch_rb2_pv<8>(ctx, a, b, rhs, c, unit)
-                                  : launch_rb2_pv<16>(ctx, a, b, rhs, c, unit);
+                    if constexpr (sizeof(T) == 8)
+                        return launch_rbc<T, 2, kRb2TJ64>(ctx, a, b, rhs, c, 0, 0, g, unit);
+                    else
+                        return v == 0 ? launch_rbc<T, 2, 8>(ctx, a, b, rhs, c, 0, 0, g, unit)
+                                      : launch_rbc<T, 2, 16>(ctx, a, b, rhs, c, 0, 0, g, unit);
                 };
                 TK_TRY(launch());
                 TK_HIP(hipEventRecord(e0, ctx->stream));
@@ -1017,8 +1144,10 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
                 ms2[v] = t / 3.0;
             }
             // kept only where one launch of two sweeps beats two launches of one
-            const double best1 = std::min(ms[0], ms[1]), best2 = std::min(ms2[0], ms2[1]);
-            ctx->rb_rows[rb2_key(X, Y, Z)] = best2 < 2 * best1 ? (ms2[1] < ms2[0] ? 16 : 8) : 0;
+            const double best1 = std::min(ms[0], ms[1]);
+            const double best2 = sizeof(T) == 8 ? ms2[0] : std::min(ms2[0], ms2[1]);
+            ctx->rb_rows[rb2_key<T>(X, Y, Z)] =
+                best2 < 2 * best1 ? (sizeof(T) == 4 && ms2[1] < ms2[0] ? 16 : 8) : 0;
             return TK_OK;
         }();
     }
@@ -1089,18 +1218,20 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
         double err = -1;
         int r = TK_OK;
         int launches = 0;  // the result is in buf[launches & 1]
+        // column kernels: tile rows of the one- and two-sweep launches (0: not served)
+        const int rows1 = rbc_rows<T>(ctx, rhs, 1), rows2 = rows1 ? rbc_rows<T>(ctx, rhs, 2) : 0;
         if (ls_tol == 0) {
             if (!faces_set) TK_TRY(set_boundary<T>(ctx, phi, bc, nullptr));
             RbGate g{};
             for (int q = 0; q < ls_steps; ++launches) {
                 void* from = buf[launches & 1];
                 void* to = buf[(launches + 1) & 1];
-                bool pair = false;
-                if (ls_steps - q >= 2) TK_TRY(rb_sweep_pair<T>(ctx, from, to, rhs, c, &pair));
-                if (pair) {
+                if (rows2 && ls_steps - q >= 2) {
+                    TK_TRY(rbc_launch<T>(ctx, from, to, rhs, c, 2, rows2, q, 0, g));
                     q += 2;
                 } else {
-                    TK_TRY(rb_sweep<T>(ctx, from, to, rhs, c, q, 0, 0, g));
+                    if (rows1) TK_TRY(rbc_launch<T>(ctx, from, to, rhs, c, 1, rows1, q, 0, g));
+                    else TK_TRY(rb_sweep_pp<T>(ctx, from, to, rhs, c, q, 0, 0, g));
                     q += 1;
                 }
             }
@@ -1109,7 +1240,7 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
             if (max_sweeps <= 0) max_sweeps = 1 << 30;
             // batches of B sweeps; the device stops itself, the host polls between batches
             const int B = 256;
-            RbGate g;
+            RbGate g{};
             g.acc = ctx->acc + kAccSweep;
             g.err = ctx->acc + kAccErr;  // B error slots
             g.stop = ctx->cnt;
@@ -1118,16 +1249,50 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
             g.ls_tol = ls_tol;
             g.h3 = h3;
             g.bnd_o2 = bnd_o2;
+            g.part = ctx->acc + kAccGate;
+            // Column kernels where their grids fit the per-block sums of the stop test: each
+            // launch first tests the sweeps of the launch before it (all its workgroups add
+            // the same sums in the same order), and one more kernel tests the batch's last
+            // launch.  The per-voxel kernel tests its own sweep (atomic sums).
+            int r1 = rows1, r2 = rows2;
+            const unsigned nb1 = r1 ? rbc_blocks<T>(ctx, rhs, 1, r1) : 0;
+            const unsigned nb2 = r2 ? rbc_blocks<T>(ctx, rhs, 2, r2) : 0;
+            if (nb1 > (unsigned)kRbGateBlk) r1 = r2 = 0;
+            if (nb2 > (unsigned)kRbGateBlk) r2 = 0;
+            struct Launch {
+                int first, n, src;  // first sweep, sweeps, source buffer
+            };
+            std::vector<Launch> ls;
+            ls.reserve(B);
             hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(256), 0, ctx->stream, ctx->cnt, B + 1);
-            int done = 0;
+            int done = 0, cur = 0;  // cur: the buffer that holds the latest sweep's output
             r = TK_ERR_SWEEPS;
             while (done < max_sweeps) {
                 const int nb = std::min(B, max_sweeps - done);
                 hipLaunchKernelGGL(k_zero_d, dim3(4), dim3(256), 0, ctx->stream, g.acc, 2 * B);
                 if (done == 0) TK_TRY(set_boundary<T>(ctx, phi, bc, g.acc));
-                for (int q = 0; q < nb; ++q) {
+                g.base = done;
+                g.prev_n = 0;
+                ls.clear();
+                for (int q = 0; q < nb;) {
                     const int sw = done + q;
-                    TK_TRY(rb_sweep<T>(ctx, buf[sw & 1], buf[(sw + 1) & 1], rhs, c, sw, q, 1, g));
+                    const int n = r2 && nb - q >= 2 ? 2 : 1;
+                    ls.push_back({sw, n, cur});
+                    if (r1) {
+                        TK_TRY(rbc_launch<T>(ctx, buf[cur], buf[cur ^ 1], rhs, c, n,
+                                             n == 2 ? r2 : r1, sw, 1, g));
+                        g.prev_first = sw;
+                        g.prev_n = n;
+                        g.prev_nblk = n == 2 ? nb2 : nb1;
+                    } else {
+                        TK_TRY(rb_sweep_pp<T>(ctx, buf[cur], buf[cur ^ 1], rhs, c, sw, q, 1, g));
+                    }
+                    cur ^= 1;
+                    q += n;
+                }
+                if (r1) {
+                    hipLaunchKernelGGL(k_rb_gate_final, dim3(1), dim3(64), 0, ctx->stream, g);
+                    TK_LAUNCHED();
                 }
                 unsigned stop = 0;
                 TK_HIP(hipMemcpyAsync(ctx->h_acc, g.err, sizeof(double) * B, hipMemcpyDeviceToHost,
@@ -1140,6 +1305,17 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
                     sweeps = (int)stop;
                     err = ctx->h_acc[sweeps - 1 - done];
                     r = TK_OK;
+                    // the launches after the stop sweep's ran nothing; where that sweep was
+                    // the first of two in one launch, run it again by itself over the
+                    // second's output (its source is untouched)
+                    const int sp = sweeps - 1;
+                    size_t i = 0;
+                    while (i + 1 < ls.size() && sp >= ls[i].first + ls[i].n) ++i;
+                    const Launch& L = ls[i];
+                    if (L.n == 2 && sp == L.first)
+                        TK_TRY(rbc_launch<T>(ctx, buf[L.src], buf[L.src ^ 1], rhs, c, 1, r1, sp, 0,
+                                             g));
+                    cur = L.src ^ 1;
                     break;
                 }
                 done += nb;
@@ -1148,7 +1324,7 @@ static int solve(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, double bc, int o
             }
             if (r != TK_OK)
                 set_error("tk_poisson_dir_solve: %d sweeps without reaching ls_tol", sweeps);
-            launches = sweeps;
+            launches = cur;
         }
         if (launches & 1) {
             // the result is in the scratch volume: hand it to phi
@@ -1528,7 +1704,8 @@ int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows)
     TK_CHECK(tile_rows == 8 || tile_rows == 16, TK_ERR_ARG,
              "tk_poisson_rb_set_tile: tile_rows must be 8 or 16");
     ctx->rb_rows[rb_key(rhs->nx, rhs->ny, rhs->nz, (int)dtype_size(rhs->dtype))] = tile_rows;
-    if (rhs->dtype == TK_F32) ctx->rb_rows[rb2_key(rhs->nx, rhs->ny, rhs->nz)] = tile_rows;
+    ctx->rb_rows[rhs->dtype == TK_F32 ? rb2_key<float>(rhs->nx, rhs->ny, rhs->nz)
+                                      : rb2_key<double>(rhs->nx, rhs->ny, rhs->nz)] = tile_rows;
     return TK_OK;
 }
 

@@ -46,15 +46,19 @@ inline size_t dtype_size(tk_dtype t) { return t == TK_F64 ? 8 : (t == TK_F32 ? 4
 
 // Device scratch owned by a context: FP64 accumulators, counters and flags used by
 // the reductions and the device-side Poisson stop test.  Sizes are fixed.
-constexpr int kAccSlots = 32768;  // doubles
-constexpr int kCntSlots = 4096;   // unsigned
+constexpr int kAccSlots = 163840;  // doubles
+constexpr int kCntSlots = 4096;    // unsigned
 // layout of ctx->acc: [0, 64) results; [kAccPart, +2*kMaxPart) per-block partial pairs;
-// [kAccErr, +512) per-sweep relative errors; [kAccSweep, +4096) per-sweep sum pairs
+// [kAccErr, +512) per-sweep relative errors; [kAccSweep, +4096) per-sweep sum pairs;
+// [kAccGate, +4*2*kRbGateBlk) per-block sum pairs of the red-black sweeps' stop test, four
+// sweeps deep (a launch reads those of the one or two sweeps before it and writes its own)
 constexpr int kAccPart = 64;
 constexpr int kMaxPart = 8192;
 constexpr int kAccErr = kAccPart + 2 * kMaxPart;
 constexpr int kAccSweep = kAccErr + 512;
-static_assert(kAccSweep + 4096 <= kAccSlots, "acc layout");
+constexpr int kAccGate = kAccSweep + 4096;
+constexpr int kRbGateBlk = 16384;
+static_assert(kAccGate + 4 * 2 * kRbGateBlk <= kAccSlots, "acc layout");
 
 }  // namespace tk
 

@@ -277,12 +277,12 @@ def test_gs_neumann_solve_matches_oracle_loop(tk, orc, ctx, ls_steps, ls_tol, rn
     assert abs(e - e_ref) <= 1e-10 * e_ref
 
 
-def rb_sweep_f32(phi, rhs):
-    """red (i+j+k even) then black, float32 throughout, operand order of the kernel."""
+def rb_sweep(phi, rhs, dt=np.float32):
+    """red (i+j+k even) then black, in dt throughout, operand order of the kernel."""
     X, Y, Z = phi.shape
-    p = phi.astype(np.float32).copy()
-    b = rhs.astype(np.float32)
-    ht = np.float32(1.0 / 7.0)
+    p = phi.astype(dt).copy()
+    b = rhs.astype(dt)
+    ht = dt(1.0 / 7.0)
     i, j, k = np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij")
     par = (i + j + k) % 2
     p[0] = 0; p[-1] = 0; p[:, 0] = 0; p[:, -1] = 0; p[:, :, 0] = 0; p[:, :, -1] = 0
@@ -295,6 +295,10 @@ def rb_sweep_f32(phi, rhs):
         inner[m] = v[m]
         p[s, s, s] = inner
     return p
+
+
+def rb_sweep_f32(phi, rhs):
+    return rb_sweep(phi, rhs, np.float32)
 
 
 @pytest.mark.parametrize("rows", [8, 16])
@@ -337,6 +341,72 @@ def test_gs_redblack_pair_launch_bitwise(tk, ctx, shape, sweeps, h, rows, rng):
         for _ in range(sweeps):
             ref = rb_sweep_f32(ref, rhs)
         assert np.array_equal(out[1], ref)
+
+
+@pytest.mark.parametrize("shape,sweeps", [((16, 18, 20), 2), ((40, 37, 136), 5), ((9, 70, 64), 4),
+                                          ((64, 64, 64), 3)])
+def test_gs_redblack_pair_launch_bitwise_f64(tk, ctx, shape, sweeps, rng):
+    """FP64 two-sweep launches (6-row tiles) against one launch per sweep and numpy"""
+    say("gs rb pair f64", shape, sweeps)
+    phi = rng.random(shape)
+    rhs = rng.standard_normal(shape)
+    out = []
+    for pair in (False, True):
+        vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+        tk.poisson_rb_set_tile(ctx, vr, 8)
+        tk.poisson_rb_set_pair(ctx, pair)
+        tk.poisson_sweeps(ctx, vp, vr, sweeps, 1.0, 0.0, tk.TK_GS_REDBLACK)
+        out.append(vp.numpy())
+    tk.poisson_rb_set_pair(ctx, True)
+    assert np.array_equal(out[0], out[1])
+    ref = phi
+    for _ in range(sweeps):
+        ref = rb_sweep(ref, rhs, np.float64)
+    assert np.array_equal(out[1], ref)
+
+
+def rb_gate_replay(phi, rhs, dt, ls_steps, tol, cap=100000):
+    """sweeps until the relative update (whole volume, faces included) is <= tol after at
+    least ls_steps sweeps; returns the field, the sweep count and the last error"""
+    q = phi
+    s = 0
+    while True:
+        prev = q.astype(np.float64)
+        q = rb_sweep(q, rhs, dt)
+        s += 1
+        d = q.astype(np.float64) - prev
+        err = np.sqrt((d * d).sum()) / np.sqrt((prev * prev).sum())
+        if (s >= ls_steps and not err > tol) or s >= cap:
+            return q, s, err
+
+
+@pytest.mark.parametrize("dtype,shape,tols", [
+    (np.float32, (20, 22, 24), (3e-3, 1e-3, 3e-4, 1e-4)),
+    (np.float64, (18, 20, 22), (1e-3, 3e-4, 1e-4, 3e-5)),
+    (np.float64, (24, 26, 32), (2e-6,)),  # more than one batch of 256 sweeps
+])
+def test_gs_redblack_gate_pairs_match_replay(tk, ctx, dtype, shape, tols, rng):
+    """gated solves with two sweeps per launch and with one: the same sweep count as the
+    numpy replay, the field bitwise, the error to rounding; the tolerances put the stop on
+    the first and on the second sweep of a launch"""
+    phi = rng.random(shape).astype(dtype)
+    rhs = rng.standard_normal(shape).astype(dtype)
+    parities = set()
+    for tol in tols:
+        q, s, err = rb_gate_replay(phi, rhs, dtype, 5, tol)
+        parities.add(s % 2)
+        for pair in (True, False):
+            say("gs rb gate", np.dtype(dtype).name, shape, "tol", tol, "pair", pair, "sweeps", s)
+            vp, vr = ctx.from_numpy(phi), ctx.from_numpy(rhs)
+            tk.poisson_rb_set_tile(ctx, vr, 8)
+            tk.poisson_rb_set_pair(ctx, pair)
+            sg, eg = tk.poisson_solve(ctx, vp, vr, 5, tol, order=tk.TK_GS_REDBLACK)
+            assert sg == s
+            assert np.array_equal(vp.numpy(), q)
+            assert abs(eg - err) < 1e-6 * err
+    tk.poisson_rb_set_pair(ctx, True)
+    if len(tols) > 1:
+        assert parities == {0, 1}
 
 
 def test_gs_redblack_autotune(tk, ctx, rng):
