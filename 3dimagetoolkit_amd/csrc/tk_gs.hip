@@ -664,10 +664,15 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
     // outside the volume are never updated and are neighbours only of boundary voxels,
     // which are not updated either, so what the clamped rows hold does not matter.
     const long off = (long)min(max(j, 0), Y - 1) * Z + min(max(k0, 0), Z - V);
+    // rhs is used on the first stage's rows 1 .. OJ-2; the two outer rows read a row that
+    // another lane reads (no extra bytes from memory, the load count stays fixed)
+    const int jr = j0 - NST + min(max(r, 1), OJ - 2);
+    const long offr = (long)min(max(jr, 0), Y - 1) * Z + min(max(k0, 0), Z - V);
     const int li = r * W + c4 * V;
     auto ld_plane = [&](const T* f, int p) {
         return ldrow(f + (long)min(max(p, 0), X - 1) * sX + off);
     };
+    auto ld_rhs = [&](int p) { return ldrow(rhs + (long)min(max(p, 0), X - 1) * sX + offr); };
     // Per stage, what does not change along the march: whether this segment's row is in
     // the stage's rows and inside the volume's interior in j (rowok), and which of its V
     // voxels lie in the stage's columns and in the interior in k (colm, bit u); the colour
@@ -734,7 +739,7 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
 #pragma unroll
     for (int d = -1; d <= 1 + KL; ++d) ro[(d + K) % K] = ld_plane(src, qa + d);
 #pragma unroll
-    for (int d = 1 - NST; d <= KB; ++d) rb[(d + K) % K] = ld_plane(rhs, qa + d);
+    for (int d = 1 - NST; d <= KB; ++d) rb[(d + K) % K] = ld_rhs(qa + d);
 #pragma unroll
     for (int s = 0; s < NST - 1; ++s)
 #pragma unroll
@@ -800,7 +805,7 @@ __device__ __forceinline__ void rb_col_tile(const T* __restrict__ src, T* __rest
         // refill: old plane q+2+KL into the slot of q-1, rhs plane q+1+KB into the slot of
         // q+1-NST (the planes this step was the last to use)
         ro[sl(2 + KL)] = ld_plane(src, q + 2 + KL);
-        rb[sl(1 + KB)] = ld_plane(rhs, q + 1 + KB);
+        rb[sl(1 + KB)] = ld_rhs(q + 1 + KB);
     };
     int q = qa;
     for (; q + K <= qe; q += K)

@@ -645,11 +645,22 @@ void k_sb_div(SdArgs<T> a)
     const bool qunit = inside && r >= 1 && r <= RJ - 2;
     const bool own = inside && j >= jo0 && j < jo1 && c4 >= 4 / V && c4 < (4 + kSdTK) / V;
     const long sX = (long)Y * Z;
-    // every lane loads on every step (clamped rows outside the volume, never used)
+    // every lane loads on every step (clamped rows outside the volume, never used); b and g
+    // are used on the rows where q is formed and phiold on the tile's own voxels, so lanes
+    // outside those read a row that another lane of the workgroup reads (no extra bytes
+    // from memory, and the load count stays fixed)
     const long off = (long)min(max(j, 0), Y - 1) * Z + min(max(k0, 0), Z - V);
+    const int jq = min(max(j, max(j0 - 1, 0)), min(j0 + TJ, Y - 1));
+    const long offq = (long)jq * Z + min(max(k0, 0), Z - V);
+    const int jw = min(max(j, jo0), jo1 - 1);
+    const int kw = kb + min(max(c4 * V - 4, 0), kSdTK - V);
+    const long offw = (long)jw * Z + min(kw, Z - V);
     const int li = r * W + c4 * V;
     const int cj = plane_class(j, Y);
-    auto ld = [&](const T* f, int p) { return ldrow(f + (long)min(max(p, 0), X - 1) * sX + off); };
+    auto ldo = [&](const T* f, int p, long o) {
+        return ldrow(f + (long)min(max(p, 0), X - 1) * sX + o);
+    };
+    auto ld = [&](const T* f, int p) { return ldo(f, p, off); };
     auto cl = [&](T x) { return clampv(x, a.lo, a.hi); };
     auto id = [](T x) { return x; };
     // y rows of a staged plane for row class cj (central: r+1, r-1; first: r, r+1, r+2;
@@ -680,11 +691,11 @@ void k_sb_div(SdArgs<T> a)
     for (int d = -1; d <= 2; ++d) ph[(d + K) % K] = ld(a.phi, qs + d);
 #pragma unroll
     for (int d = 0; d <= 1; ++d) {
-        rb0[d] = ld(a.bin[0], qs + d);
-        rb1[d] = ld(a.bin[1], qs + d);
-        rb2[d] = ld(a.bin[2], qs + d);
-        rg[d] = ld(a.g, qs + d);
-        rpo[d] = ld(a.phiold, qs + d);
+        rb0[d] = ldo(a.bin[0], qs + d, offq);
+        rb1[d] = ldo(a.bin[1], qs + d, offq);
+        rb2[d] = ldo(a.bin[2], qs + d, offq);
+        rg[d] = ldo(a.g, qs + d, offq);
+        rpo[d] = ldo(a.phiold, qs + d, offw);
     }
 #pragma unroll
     for (int m = 0; m < K; ++m) qx[m] = Row<T>{};
@@ -845,11 +856,11 @@ void k_sb_div(SdArgs<T> a)
         }
         // refill the slots this step was the last to use (phi q-1, the fields of q)
         ph[sl(3)] = ld(a.phi, q + 3);
-        rb0[sl(2)] = ld(a.bin[0], q + 2);
-        rb1[sl(2)] = ld(a.bin[1], q + 2);
-        rb2[sl(2)] = ld(a.bin[2], q + 2);
-        rg[sl(2)] = ld(a.g, q + 2);
-        rpo[sl(2)] = ld(a.phiold, q + 2);
+        rb0[sl(2)] = ldo(a.bin[0], q + 2, offq);
+        rb1[sl(2)] = ldo(a.bin[1], q + 2, offq);
+        rb2[sl(2)] = ldo(a.bin[2], q + 2, offq);
+        rg[sl(2)] = ldo(a.g, q + 2, offq);
+        rpo[sl(2)] = ldo(a.phiold, q + 2, offw);
     };
     int q = qs;
     for (; q + K <= qe + 1; q += K)

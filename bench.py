@@ -85,6 +85,21 @@ def copy_probe(tk, steps):
     print(json.dumps({"probe": "k_copy16 512 MiB", "bytes_per_launch": nbytes,
                       "us_per_launch": round(t * 1e6, 1), "GBps": round(nbytes / t / 1e9, 1)}),
           flush=True)
+    import ctypes as C
+    # a second shape for the counters' scale: a read-only reduction (sum of squares, 512 MiB
+    # read) and a write-only fill (512 MiB written)
+    out = C.c_double()
+    for name, fn, nb in (("k_sumsq_partial 512 MiB read", lambda: tk.lib.tk_norm_l2(ctx.h, a.h, C.byref(out)), 512 ** 3 * 4),
+                         ("k_fill 512 MiB write", lambda: tk.lib.tk_vol_fill(b.h, 2.0), 512 ** 3 * 4)):
+        tk._check(fn())
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tk._check(fn())
+        ctx.sync()
+        t = (time.perf_counter() - t0) / steps
+        print(json.dumps({"probe": name, "bytes_per_launch": nb, "us_per_launch": round(t * 1e6, 1),
+                          "GBps": round(nb / t / 1e9, 1)}), flush=True)
 
 
 BATCH_SEED = 20240604
