@@ -23,6 +23,25 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// FP32 quotient n / d from the hardware reciprocal and one Newton correction of the
+// quotient (Markstein): within 1 ulp of the IEEE quotient, in 6 instructions instead of
+// the IEEE sequence's 10 (for the vector-ALU-bound convolution passes).  For d normal or
+// zero; a zero or infinite operand makes the correction's residual NaN, and the product
+// n * (1/d) is then the IEEE quotient (+-inf, 0 or NaN).  FP64 keeps the IEEE division.
+__device__ __forceinline__ float fdiv_nr(float n, float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float q = n * r;
+    const float e = fmaf(-d, q, n);
+    return e == e ? fmaf(e, r, q) : q;
+}
+template <class T>
+__device__ __forceinline__ T div_nr(T n, T d)
+{
+    if constexpr (std::is_same<T, float>::value) return fdiv_nr(n, d);
+    else return n / d;
+}
+
 // wave64 sum of a double through DPP row moves (a few cycles each, where the shuffles above
 // go through the LDS crossbar): per row of 16 lanes, quad sums (quad_perm xor 1, xor 2),
 // then row rotations by 4 and 8; lanes 0, 16, 32, 48 then hold the row sums, added as

@@ -50,7 +50,7 @@ __device__ __forceinline__ T heaviside(T p, T center, T eps)
     if constexpr (std::is_same<T, double>::value)
         return 0.5 * (1. + 2. / M_PI * atan((p - center) / eps));
     else
-        return 0.5f * (1.f + (float)(2. / M_PI) * atanf((p - center) / eps));
+        return 0.5f * (1.f + (float)(2. / M_PI) * atanf(fdiv_nr(p - center, eps)));
 }
 
 template <class T>
@@ -91,23 +91,10 @@ struct MdRegionFit {
 #pragma unroll
         for (int e = 0; e < 4; ++e) c[e] = fit(c[e], km.v[e], ko.v[e]);
     }
-    // FP64: IEEE quotients (the reference's arithmetic).  FP32: the hardware reciprocal and
-    // one Newton correction of the quotient (Markstein), within 1 ulp of the IEEE quotient,
-    // in 6 instructions instead of the IEEE sequence's 10 (this pass is bound by its vector
-    // ALUs).  The divisors, den = K*H in (0, 1] and den - K*1 in [-1, 0], are normal numbers
-    // or zero; where the correction's residual is NaN (a zero or infinite operand) the plain
-    // product n * (1/d) is the IEEE quotient (+-inf, 0 or NaN).
-    static __device__ __forceinline__ T quot(T n, T d)
-    {
-        if constexpr (std::is_same<T, float>::value) {
-            const float r = __builtin_amdgcn_rcpf(d);
-            const float q = n * r;
-            const float e = fmaf(-d, q, n);
-            return e == e ? fmaf(e, r, q) : q;
-        } else {
-            return n / d;
-        }
-    }
+    // FP64: IEEE quotients (the reference's arithmetic); FP32: div_nr (this pass is bound by
+    // its vector ALUs; the divisors den = K*H in (0, 1] and den - K*1 in [-1, 0] are normal
+    // numbers or zero)
+    static __device__ __forceinline__ T quot(T n, T d) { return div_nr(n, d); }
     __device__ cplx<T> fit(cplx<T> c, T km, T ko) const
     {
         const T num = c.x, den = c.y;
@@ -158,7 +145,7 @@ struct StRhs {
         aux *= ko;
         r -= aux;
         r -= c.y;
-        r /= lam;
+        r = div_nr(r, lam);
         r -= dd;
         return r;
     }
