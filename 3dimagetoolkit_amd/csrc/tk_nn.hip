@@ -114,6 +114,177 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
     }
 }
 
+// Forward convolution with ci = CI (16, 32 or 64) and co a multiple of 16, the input staged
+// in LDS.  A workgroup of NW waves owns BY rows (y) x 16 voxels (z) x 16 output channels
+// (one of co / 16 slices) and marches in x over a chunk of CX planes.  The input planes x-1, x, x+1 with a one-voxel halo in y and z
+// ((BY + 2) x 18 voxels, each voxel's CI channels at a stride of CI + 8 bf16, so that eight
+// lanes reading 16 bytes of eight consecutive voxels hit disjoint banks) sit in three slots
+// of a ring of four; plane x+2 is loaded into registers before the step's products and
+// written to the fourth slot after them, so one barrier per step suffices.  The weights are
+// B fragments held in registers for the whole march.  v_mfma_f32_16x16x32_bf16 (gfx950): K =
+// 32 is half a tap (CI = 64), one tap (CI = 32) or two taps (CI = 16); operand layout for lane
+// l (element e = 0..7): A[m = l % 16][k = 8 (l / 16) + e], B[k = 8 (l / 16) + e][n = l % 16],
+// D[m = 4 (l / 16) + q][n = l % 16] (q = 0..3).  A wave computes MT = BY / NW rows; the A
+// fragments that two rows share (row t with dy = +1 is row t + 1 with dy = 0) are one LDS
+// address, read once.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int kMarchBY = 8, kMarchNW = 4;  // output rows per workgroup, waves per workgroup
+
+template <int CI, int BY, int NW>
+struct MarchCfg {
+    static constexpr int CIP = CI + 8;              // LDS voxel stride (bf16)
+    static constexpr int RY = BY + 2, RZ = 18;      // a plane slot: rows x columns of voxels
+    static constexpr int SLOT = RY * RZ * CIP;      // bf16 per slot
+    static constexpr int CH = CI / 8;               // 16-byte parts per voxel
+    static constexpr int NCH = RY * RZ * CH;        // 16-byte parts per plane
+    static constexpr int NT = 64 * NW;              // threads
+    static constexpr int PF = (NCH + NT - 1) / NT;  // parts per thread
+    static constexpr int KS = (27 * CI + 31) / 32;  // K steps of 32
+    static constexpr int MT = BY / NW;              // rows per wave
+    static constexpr int H = CI >= 32 ? CI / 32 : 1;  // K steps per tap (CI >= 32)
+};
+
+template <int CI, int BY, int NW>
+constexpr long march_lds_bytes()
+{
+    return 2L * 4 * MarchCfg<CI, BY, NW>::SLOT;
+}
+
+// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers
+template <int CI, int BY, int NW, bool RELU>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : 2)))
+void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_t* __restrict__ w,
+                   const float* __restrict__ bias, uint16_t* __restrict__ y, int ys, int yo,
+                   int X, int Y, int Z, int CX, int nsl)
+{
+    using C = MarchCfg<CI, BY, NW>;
+    static_assert(BY % NW == 0 && C::CH * 8 == CI, "tile");
+    __shared__ __align__(16) uint16_t sm[4 * C::SLOT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lg = lane >> 4, m = lane & 15;
+    const int z0 = blockIdx.x * 16, y0 = blockIdx.y * BY;
+    // output channels 16 sl .. 16 sl + 15: weight rows, bias and output slice
+    const int sl = blockIdx.z % nsl, chunk = blockIdx.z / nsl;
+    w += (long)16 * sl * (27 * CI);
+    if (bias) bias += 16 * sl;
+    yo += 16 * sl;
+    const int xa = chunk * CX, xb = min(xa + CX, X);
+    // weights w[n][k] (kpad = 27 CI) as B fragments: step s holds k = 32 s + 8 lg + e for
+    // output channel n = m, zero past 27 CI
+    v8bf bw[C::KS];
+#pragma unroll
+    for (int s = 0; s < C::KS; ++s) {
+        const int k = 32 * s + 8 * lg;
+        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+        if (k < 27 * CI) raw = *(const uint4*)(w + (long)m * (27 * CI) + k);
+        bw[s] = __builtin_bit_cast(v8bf, raw);
+    }
+    const float bo = bias ? bias[m] : 0.f;
+    // 16-byte part u of this thread in a plane: voxel r = row * RZ + col of the slot, part c
+    int poff[C::PF], soff[C::PF], gy[C::PF], gz[C::PF];
+#pragma unroll
+    for (int u = 0; u < C::PF; ++u) {
+        const int q = min(tid + C::NT * u, C::NCH - 1);
+        const int r = q / C::CH, c = q - r * C::CH;
+        const int ry = r / C::RZ, rz = r - ry * C::RZ;
+        gy[u] = y0 - 1 + ry;
+        gz[u] = z0 - 1 + rz;
+        soff[u] = r * C::CIP + 8 * c;
+        poff[u] = (min(max(gy[u], 0), Y - 1) * Z + min(max(gz[u], 0), Z - 1)) * xs + xo + 8 * c;
+    }
+    // every part is loaded from a clamped address; those outside the volume become zeros
+    // when written to LDS, after the step's products (so no wait is spent on them before)
+    auto plane_ld = [&](int p, uint4 (&b)[C::PF]) {
+        const long pb = (long)min(max(p, 0), X - 1) * Y * Z * xs;
+#pragma unroll
+        for (int u = 0; u < C::PF; ++u) b[u] = *(const uint4*)(x + pb + poff[u]);
+    };
+    auto plane_st = [&](int p, const uint4 (&b)[C::PF]) {
+        uint16_t* sl = sm + (p & 3) * C::SLOT;
+#pragma unroll
+        for (int u = 0; u < C::PF; ++u) {
+            if (tid + C::NT * u >= C::NCH) continue;
+            const bool in = p >= 0 && p < X && gy[u] >= 0 && gy[u] < Y && gz[u] >= 0 && gz[u] < Z;
+            *(uint4*)(sl + soff[u]) = in ? b[u] : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    {
+        uint4 b0[C::PF], b1[C::PF], b2[C::PF];
+        plane_ld(xa - 1, b0);
+        plane_ld(xa, b1);
+        plane_ld(xa + 1, b2);
+        plane_st(xa - 1, b0);
+        plane_st(xa, b1);
+        plane_st(xa + 1, b2);
+    }
+    __syncthreads();
+    for (int xc = xa; xc < xb; ++xc) {
+        uint4 pf[C::PF];
+        plane_ld(xc + 2, pf);
+        v4f acc[C::MT];
+#pragma unroll
+        for (int t = 0; t < C::MT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+        if constexpr (CI >= 32) {
+            // step s = H tap + h (channels 32 h .. 32 h + 31 of the tap): for each (dx, dz, h)
+            // the MT + 2 rows the three dy taps read, once
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                    for (int h = 0; h < C::H; ++h) {
+                        const uint16_t* a0 = sm + ((xc + dx) & 3) * C::SLOT +
+                                             (wave * C::MT * C::RZ + 1 + dz + m) * C::CIP + 32 * h +
+                                             8 * lg;
+                        v8bf a[C::MT + 2];
+#pragma unroll
+                        for (int r = 0; r < C::MT + 2; ++r)
+                            a[r] = __builtin_bit_cast(v8bf,
+                                                      *(const uint4*)(a0 + r * C::RZ * C::CIP));
+#pragma unroll
+                        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                            for (int t = 0; t < C::MT; ++t)
+                                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    a[t + dy + 1],
+                                    bw[((dx + 1) * 9 + (dy + 1) * 3 + dz + 1) * C::H + h], acc[t],
+                                    0, 0, 0);
+                    }
+        } else {
+#pragma unroll
+            for (int s = 0; s < C::KS; ++s) {
+                // two taps per step, 2 s for lanes 0-31 and 2 s + 1 for lanes 32-63 (the one
+                // past the last has zero weights and reads tap 26's values)
+                const int tp = min(2 * s + (lg >> 1), 26);
+                const int dx = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dz = tp % 3 - 1;
+                const uint16_t* a0 = sm + ((xc + dx) & 3) * C::SLOT +
+                                     ((1 + dy + wave * C::MT) * C::RZ + 1 + dz + m) * C::CIP +
+                                     8 * (lg & 1);
+#pragma unroll
+                for (int t = 0; t < C::MT; ++t) {
+                    const uint4 a = *(const uint4*)(a0 + t * C::RZ * C::CIP);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(v8bf, a), bw[s], acc[t], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < C::MT; ++t) {
+            const int yy = y0 + wave * C::MT + t;
+            if (yy >= Y) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int z = z0 + 4 * lg + q;
+                float v = acc[t][q] + bo;
+                if (RELU) v = fmaxf(v, 0.f);
+                y[(((long)xc * Y + yy) * Z + z) * ys + yo + m] = f2bf(v);
+            }
+        }
+        if (xc + 1 < xb) plane_st(xc + 2, pf);
+        __syncthreads();
+    }
+}
+
 // weight gradient of one chunk of voxels:
 // part[chunk][o][k] = sum_{v in chunk} dy[v][o] x[v + tap(k)][c(k)]
 __global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t* __restrict__ x, int xs,
@@ -351,6 +522,36 @@ int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const vo
     const unsigned nb = (unsigned)((tiles + 3) / 4);
     const uint16_t* xp = (const uint16_t*)x;
     const uint16_t* wp = (const uint16_t*)w;
+    // ci = 16, 32 or 64, co a multiple of 16, z a multiple of 16, 16-byte channel parts, BF16
+    // output: the LDS-staged march, one 16-channel output slice per workgroup, with chunks of
+    // x planes for about two workgroups per CU
+    if (co % 16 == 0 && (ci == 16 || ci == 32 || ci == 64) && !out_f32 && nz % 16 == 0 &&
+        xs % 8 == 0 && xo % 8 == 0 && kpad == 27 * ci && (uintptr_t)x % 16 == 0 &&
+        (uintptr_t)w % 16 == 0 && (uintptr_t)y % 2 == 0) {
+        constexpr int BY = kMarchBY, NW = kMarchNW;
+        const int nsl = co / 16;
+        const long nyz = (long)(nz / 16) * ((ny + BY - 1) / BY) * nsl;
+        int cx = (int)std::max(2L, ((long)nx * nyz + 2L * ctx->num_cu - 1) / (2L * ctx->num_cu));
+        cx = std::min(cx, nx);
+        const dim3 grid(nz / 16, (ny + BY - 1) / BY, (unsigned)(((nx + cx - 1) / cx) * nsl));
+        uint16_t* yp = (uint16_t*)y;
+#define TK_MARCH(CI_, R)                                                                       \
+    hipLaunchKernelGGL((k_conv3_march<CI_, BY, NW, R>), grid, dim3(64 * NW), 0, ctx->stream, xp, \
+                       xs, xo, wp, bias, yp, ys, yo, nx, ny, nz, cx, nsl)
+        if (ci == 64) {
+            if (relu) TK_MARCH(64, true);
+            else TK_MARCH(64, false);
+        } else if (ci == 32) {
+            if (relu) TK_MARCH(32, true);
+            else TK_MARCH(32, false);
+        } else {
+            if (relu) TK_MARCH(16, true);
+            else TK_MARCH(16, false);
+        }
+#undef TK_MARCH
+        TK_LAUNCHED();
+        return TK_OK;
+    }
     const bool ci16 = ci % 16 == 0 && xs % 4 == 0 && xo % 4 == 0 && kpad == 27 * ci &&
                       (uintptr_t)x % 8 == 0 && (uintptr_t)w % 8 == 0;
 #define TK_FWD(R, F, V)                                                                        \

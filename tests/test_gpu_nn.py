@@ -74,7 +74,21 @@ def kpad_of(ci):
                           ((3, 4, 16), 1, 16, True, False, 3, 8),
                           ((5, 3, 9), 32, 16, False, False, 8, 16),
                           ((4, 4, 24), 16, 1, False, True, 0, 0),
-                          ((2, 3, 40), 64, 64, True, False, 0, 0)])
+                          ((2, 3, 40), 64, 64, True, False, 0, 0),
+                          # co = 16, ci = 16 or 32, z a multiple of 16: the LDS-staged march
+                          # (partial row tiles, channel slices, one-plane volumes, no relu)
+                          ((6, 11, 32), 32, 16, True, False, 0, 0),
+                          ((9, 8, 16), 16, 16, True, False, 16, 16),
+                          ((5, 17, 48), 32, 16, False, False, 32, 0),
+                          ((1, 3, 16), 16, 16, True, False, 0, 0),
+                          ((3, 9, 32), 16, 16, False, False, 0, 32),
+                          ((64, 64, 64), 32, 16, True, False, 0, 0),
+                          # several 16-channel output slices, ci = 64
+                          ((4, 9, 32), 64, 32, True, False, 0, 0),
+                          ((6, 8, 16), 32, 32, True, False, 32, 32),
+                          ((3, 5, 16), 16, 32, False, False, 0, 0),
+                          ((5, 7, 16), 64, 64, True, False, 64, 0),
+                          ((32, 32, 32), 64, 32, True, False, 0, 0)])
 def test_conv3d_fwd(tk, ctx, rng, shape, ci, co, relu, out_f32, xs_extra, ys_extra):
     say("conv3d fwd", shape, ci, co, relu, out_f32)
     X, Y, Z = shape
