@@ -127,6 +127,49 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
 // D[m = 4 (l / 16) + q][n = l % 16] (q = 0..3).  A wave computes MT = BY / NW rows; the A
 // fragments that two rows share (row t with dy = +1 is row t + 1 with dy = 0) are one LDS
 // address, read once.
+// Forward convolution of a one-channel input to co <= 16 channels (the network's first
+// layer): one thread per output voxel, the 27 x co weights in LDS as FP32, the 27 inputs read
+// through the cache (zero outside the volume), FP32 sums over the taps in order.
+template <bool RELU>
+__global__ __launch_bounds__(256) void k_conv3_c1(const uint16_t* __restrict__ x, int xs, int xo,
+                                                  const uint16_t* __restrict__ w, int kpad,
+                                                  const float* __restrict__ bias,
+                                                  uint16_t* __restrict__ y, int ys, int yo,
+                                                  int co, int X, int Y, int Z)
+{
+    __shared__ float ws[27][16];
+    for (int e = threadIdx.x; e < 27 * 16; e += 256) {
+        const int t = e >> 4, o = e & 15;
+        ws[t][o] = o < co ? bf2f(w[(long)o * kpad + t]) : 0.f;
+    }
+    __syncthreads();
+    const long n = (long)X * Y * Z;
+    for (long v = blockIdx.x * 256L + threadIdx.x; v < n; v += (long)gridDim.x * 256) {
+        const int k = (int)(v % Z);
+        const long r = v / Z;
+        const int j = (int)(r % Y), i = (int)(r / Y);
+        float acc[16];
+#pragma unroll
+        for (int o = 0; o < 16; ++o) acc[o] = 0.f;
+#pragma unroll 3
+        for (int t = 0; t < 27; ++t) {
+            const int ii = i + t / 9 - 1, jj = j + (t / 3) % 3 - 1, kk = k + t % 3 - 1;
+            float xv = 0.f;
+            if (ii >= 0 && ii < X && jj >= 0 && jj < Y && kk >= 0 && kk < Z)
+                xv = bf2f(x[(((long)ii * Y + jj) * Z + kk) * xs + xo]);
+#pragma unroll
+            for (int o = 0; o < 16; ++o) acc[o] += xv * ws[t][o];
+        }
+#pragma unroll
+        for (int o = 0; o < 16; ++o) {
+            if (o >= co) break;
+            float vv = acc[o] + (bias ? bias[o] : 0.f);
+            if (RELU) vv = fmaxf(vv, 0.f);
+            y[v * ys + yo + o] = f2bf(vv);
+        }
+    }
+}
+
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int kMarchBY = 8, kMarchNW = 4;  // output rows per workgroup, waves per workgroup
 
@@ -150,11 +193,13 @@ constexpr long march_lds_bytes()
     return 2L * 4 * MarchCfg<CI, BY, NW>::SLOT;
 }
 
-// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers
-template <int CI, int BY, int NW, bool RELU>
+// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers.
+// Output channels past co (the last slice of a co that is not a multiple of 16) have zero
+// weights and are not written; OUT_F32 writes FP32 instead of BF16.
+template <int CI, int BY, int NW, bool RELU, bool OUT_F32>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : 2)))
 void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_t* __restrict__ w,
-                   const float* __restrict__ bias, uint16_t* __restrict__ y, int ys, int yo,
+                   const float* __restrict__ bias, void* __restrict__ y, int ys, int yo, int co,
                    int X, int Y, int Z, int CX, int nsl)
 {
     using C = MarchCfg<CI, BY, NW>;
@@ -163,11 +208,13 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lg = lane >> 4, m = lane & 15;
     const int z0 = blockIdx.x * 16, y0 = blockIdx.y * BY;
-    // output channels 16 sl .. 16 sl + 15: weight rows, bias and output slice
+    // output channels 16 sl .. 16 sl + 15: weight rows, bias and output slice; this lane's
+    // channel n = 16 sl + m exists if n < co
     const int sl = blockIdx.z % nsl, chunk = blockIdx.z / nsl;
     w += (long)16 * sl * (27 * CI);
     if (bias) bias += 16 * sl;
     yo += 16 * sl;
+    const bool nok = 16 * sl + m < co;
     const int xa = chunk * CX, xb = min(xa + CX, X);
     // weights w[n][k] (kpad = 27 CI) as B fragments: step s holds k = 32 s + 8 lg + e for
     // output channel n = m, zero past 27 CI
@@ -176,10 +223,10 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
     for (int s = 0; s < C::KS; ++s) {
         const int k = 32 * s + 8 * lg;
         uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-        if (k < 27 * CI) raw = *(const uint4*)(w + (long)m * (27 * CI) + k);
+        if (k < 27 * CI && nok) raw = *(const uint4*)(w + (long)m * (27 * CI) + k);
         bw[s] = __builtin_bit_cast(v8bf, raw);
     }
-    const float bo = bias ? bias[m] : 0.f;
+    const float bo = bias && nok ? bias[m] : 0.f;
     // 16-byte part u of this thread in a plane: voxel r = row * RZ + col of the slot, part c
     int poff[C::PF], soff[C::PF], gy[C::PF], gz[C::PF];
 #pragma unroll
@@ -271,13 +318,15 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
 #pragma unroll
         for (int t = 0; t < C::MT; ++t) {
             const int yy = y0 + wave * C::MT + t;
-            if (yy >= Y) continue;
+            if (yy >= Y || !nok) continue;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int z = z0 + 4 * lg + q;
                 float v = acc[t][q] + bo;
                 if (RELU) v = fmaxf(v, 0.f);
-                y[(((long)xc * Y + yy) * Z + z) * ys + yo + m] = f2bf(v);
+                const long e = (((long)xc * Y + yy) * Z + z) * ys + yo + m;
+                if (OUT_F32) ((float*)y)[e] = v;
+                else ((uint16_t*)y)[e] = f2bf(v);
             }
         }
         if (xc + 1 < xb) plane_st(xc + 2, pf);
@@ -420,6 +469,36 @@ __global__ void k_avgpool2(const uint16_t* __restrict__ x, int xs, int xo, int C
     }
 }
 
+// the same with 16-byte parts of 8 channels (C / 8 = G a power of two; strides, offsets and
+// pointers multiples of 8 channels): one workgroup per output row (i, j), threads over
+// (k, part); the 8 children are added in the same order as above
+__global__ __launch_bounds__(256) void k_avgpool2_v8(const uint16_t* __restrict__ x, int xs,
+                                                     int xo, int lgG, int X, int Y, int Z,
+                                                     uint16_t* __restrict__ y, int ys, int yo)
+{
+    const int Y2 = Y / 2, Z2 = Z / 2, G = 1 << lgG;
+    const int row = blockIdx.x, i = row / Y2, j = row - i * Y2;
+    for (int e = threadIdx.x; e < Z2 << lgG; e += 256) {
+        const int k = e >> lgG, g = e & (G - 1);
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int a = 0; a < 8; ++a) {
+            const int ii = 2 * i + (a >> 2), jj = 2 * j + ((a >> 1) & 1), kk = 2 * k + (a & 1);
+            const uint4 q = *(const uint4*)(x + (((long)ii * Y + jj) * Z + kk) * xs + xo + 8 * g);
+            const uint32_t u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                s[2 * c] += bf2f((uint16_t)(u[c] & 0xFFFFu));
+                s[2 * c + 1] += bf2f((uint16_t)(u[c] >> 16));
+            }
+        }
+        uint32_t o[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            o[c] = (uint32_t)f2bf(s[2 * c] * 0.125f) | ((uint32_t)f2bf(s[2 * c + 1] * 0.125f) << 16);
+        *(uint4*)(y + ((long)row * Z2 + k) * ys + yo + 8 * g) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
 // its backward: dx[child][c] (+)= dy[parent][c] / 8
 __global__ void k_avgpool2_bwd(const uint16_t* __restrict__ dy, int ds, int dof, int C, int X,
                                int Y, int Z, uint16_t* __restrict__ dx, int xs, int xo, int accum)
@@ -453,6 +532,21 @@ __global__ void k_upsample2(const uint16_t* __restrict__ x, int xs, int xo, int 
         const int j = (int)(r % Y), i = (int)(r / Y);
         const long p = ((long)(i / 2) * Y2 + j / 2) * Z2 + k / 2;
         y[v * ys + yo + c] = x[p * xs + xo + c];
+    }
+}
+
+// the same with 16-byte parts of 8 channels (as k_avgpool2_v8): one workgroup per output row
+__global__ __launch_bounds__(256) void k_upsample2_v8(const uint16_t* __restrict__ x, int xs,
+                                                      int xo, int lgG, int X, int Y, int Z,
+                                                      uint16_t* __restrict__ y, int ys, int yo)
+{
+    const int Y2 = Y / 2, Z2 = Z / 2, G = 1 << lgG;
+    const int row = blockIdx.x, i = row / Y, j = row - i * Y;
+    const long pr = ((long)(i >> 1) * Y2 + (j >> 1)) * Z2;
+    for (int e = threadIdx.x; e < Z << lgG; e += 256) {
+        const int k = e >> lgG, g = e & (G - 1);
+        *(uint4*)(y + ((long)row * Z + k) * ys + yo + 8 * g) =
+            *(const uint4*)(x + (pr + (k >> 1)) * xs + xo + 8 * g);
     }
 }
 
@@ -506,6 +600,19 @@ using namespace tk;
 
 namespace {
 inline unsigned nblk(long n) { return (unsigned)std::min<long>((n + 255) / 256, 16384); }
+// log2 of the 16-byte parts per voxel when C is 8 times a power of two and every stride,
+// offset and pointer allows 16-byte accesses, else -1
+inline int v8_parts_lg(int C, const void* x, int xs, int xo, const void* y, int ys, int yo)
+{
+    if (C % 8 != 0 || xs % 8 || xo % 8 || ys % 8 || yo % 8 || (uintptr_t)x % 16 ||
+        (uintptr_t)y % 16)
+        return -1;
+    const int G = C / 8;
+    if (G & (G - 1)) return -1;
+    int lg = 0;
+    while ((1 << lg) < G) ++lg;
+    return lg;
+}
 }  // namespace
 
 extern "C" {
@@ -522,32 +629,44 @@ int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const vo
     const unsigned nb = (unsigned)((tiles + 3) / 4);
     const uint16_t* xp = (const uint16_t*)x;
     const uint16_t* wp = (const uint16_t*)w;
-    // ci = 16, 32 or 64, co a multiple of 16, z a multiple of 16, 16-byte channel parts, BF16
-    // output: the LDS-staged march, one 16-channel output slice per workgroup, with chunks of
-    // x planes for about two workgroups per CU
-    if (co % 16 == 0 && (ci == 16 || ci == 32 || ci == 64) && !out_f32 && nz % 16 == 0 &&
-        xs % 8 == 0 && xo % 8 == 0 && kpad == 27 * ci && (uintptr_t)x % 16 == 0 &&
-        (uintptr_t)w % 16 == 0 && (uintptr_t)y % 2 == 0) {
+    // one input channel (the network's first layer), co <= 16, BF16 output
+    if (ci == 1 && co <= 16 && !out_f32 && kpad >= 27) {
+        const long n = (long)nx * ny * nz;
+        if (relu)
+            hipLaunchKernelGGL(k_conv3_c1<true>, dim3(nblk(n)), dim3(256), 0, ctx->stream, xp, xs,
+                               xo, wp, kpad, bias, (uint16_t*)y, ys, yo, co, nx, ny, nz);
+        else
+            hipLaunchKernelGGL(k_conv3_c1<false>, dim3(nblk(n)), dim3(256), 0, ctx->stream, xp,
+                               xs, xo, wp, kpad, bias, (uint16_t*)y, ys, yo, co, nx, ny, nz);
+        TK_LAUNCHED();
+        return TK_OK;
+    }
+    // ci = 16, 32 or 64, z a multiple of 16, 16-byte channel parts: the LDS-staged march,
+    // one 16-channel output slice per workgroup (the last one partial where co is not a
+    // multiple of 16), with chunks of x planes for about two workgroups per CU
+    if ((ci == 16 || ci == 32 || ci == 64) && nz % 16 == 0 && xs % 8 == 0 && xo % 8 == 0 &&
+        kpad == 27 * ci && (uintptr_t)x % 16 == 0 && (uintptr_t)w % 16 == 0 &&
+        (uintptr_t)y % (out_f32 ? 4 : 2) == 0) {
         constexpr int BY = kMarchBY, NW = kMarchNW;
-        const int nsl = co / 16;
+        const int nsl = (co + 15) / 16;
         const long nyz = (long)(nz / 16) * ((ny + BY - 1) / BY) * nsl;
         int cx = (int)std::max(2L, ((long)nx * nyz + 2L * ctx->num_cu - 1) / (2L * ctx->num_cu));
         cx = std::min(cx, nx);
         const dim3 grid(nz / 16, (ny + BY - 1) / BY, (unsigned)(((nx + cx - 1) / cx) * nsl));
-        uint16_t* yp = (uint16_t*)y;
-#define TK_MARCH(CI_, R)                                                                       \
-    hipLaunchKernelGGL((k_conv3_march<CI_, BY, NW, R>), grid, dim3(64 * NW), 0, ctx->stream, xp, \
-                       xs, xo, wp, bias, yp, ys, yo, nx, ny, nz, cx, nsl)
-        if (ci == 64) {
-            if (relu) TK_MARCH(64, true);
-            else TK_MARCH(64, false);
-        } else if (ci == 32) {
-            if (relu) TK_MARCH(32, true);
-            else TK_MARCH(32, false);
-        } else {
-            if (relu) TK_MARCH(16, true);
-            else TK_MARCH(16, false);
-        }
+#define TK_MARCH(CI_, R, F)                                                                    \
+    hipLaunchKernelGGL((k_conv3_march<CI_, BY, NW, R, F>), grid, dim3(64 * NW), 0, ctx->stream,  \
+                       xp, xs, xo, wp, bias, y, ys, yo, co, nx, ny, nz, cx, nsl)
+#define TK_MARCH_CI(CI_)                                                                       \
+    do {                                                                                       \
+        if (relu && out_f32) TK_MARCH(CI_, true, true);                                        \
+        else if (relu) TK_MARCH(CI_, true, false);                                             \
+        else if (out_f32) TK_MARCH(CI_, false, true);                                          \
+        else TK_MARCH(CI_, false, false);                                                      \
+    } while (0)
+        if (ci == 64) TK_MARCH_CI(64);
+        else if (ci == 32) TK_MARCH_CI(32);
+        else TK_MARCH_CI(16);
+#undef TK_MARCH_CI
 #undef TK_MARCH
         TK_LAUNCHED();
         return TK_OK;
@@ -628,9 +747,15 @@ int tk_nn_avgpool2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, int
 {
     TK_CHECK(ctx && x && y && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
              "tk_nn_avgpool2: argument");
-    hipLaunchKernelGGL(k_avgpool2, dim3(nblk((long)nx * ny * nz / 8 * C)), dim3(256), 0,
-                       ctx->stream, (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y, ys,
-                       yo);
+    const int lg = v8_parts_lg(C, x, xs, xo, y, ys, yo);
+    if (lg >= 0)
+        hipLaunchKernelGGL(k_avgpool2_v8, dim3((unsigned)((long)nx / 2 * (ny / 2))), dim3(256), 0,
+                           ctx->stream, (const uint16_t*)x, xs, xo, lg, nx, ny, nz, (uint16_t*)y,
+                           ys, yo);
+    else
+        hipLaunchKernelGGL(k_avgpool2, dim3(nblk((long)nx * ny * nz / 8 * C)), dim3(256), 0,
+                           ctx->stream, (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y,
+                           ys, yo);
     TK_LAUNCHED();
     return TK_OK;
 }
@@ -652,8 +777,15 @@ int tk_nn_upsample2(tk_ctx ctx, const void* x, int xs, int xo, int C, int nx, in
 {
     TK_CHECK(ctx && x && y && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, TK_ERR_ARG,
              "tk_nn_upsample2: argument");
-    hipLaunchKernelGGL(k_upsample2, dim3(nblk((long)nx * ny * nz * C)), dim3(256), 0, ctx->stream,
-                       (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y, ys, yo);
+    const int lg = v8_parts_lg(C, x, xs, xo, y, ys, yo);
+    if (lg >= 0)
+        hipLaunchKernelGGL(k_upsample2_v8, dim3((unsigned)((long)nx * ny)), dim3(256), 0,
+                           ctx->stream, (const uint16_t*)x, xs, xo, lg, nx, ny, nz, (uint16_t*)y,
+                           ys, yo);
+    else
+        hipLaunchKernelGGL(k_upsample2, dim3(nblk((long)nx * ny * nz * C)), dim3(256), 0,
+                           ctx->stream, (const uint16_t*)x, xs, xo, C, nx, ny, nz, (uint16_t*)y,
+                           ys, yo);
     TK_LAUNCHED();
     return TK_OK;
 }

@@ -88,7 +88,11 @@ def kpad_of(ci):
                           ((6, 8, 16), 32, 32, True, False, 32, 32),
                           ((3, 5, 16), 16, 32, False, False, 0, 0),
                           ((5, 7, 16), 64, 64, True, False, 64, 0),
-                          ((32, 32, 32), 64, 32, True, False, 0, 0)])
+                          ((32, 32, 32), 64, 32, True, False, 0, 0),
+                          # a partial last slice, one FP32 output channel (the output layer)
+                          ((3, 4, 16), 16, 20, True, False, 0, 0),
+                          ((4, 6, 32), 16, 1, False, True, 0, 0),
+                          ((3, 5, 16), 32, 1, True, True, 0, 0)])
 def test_conv3d_fwd(tk, ctx, rng, shape, ci, co, relu, out_f32, xs_extra, ys_extra):
     say("conv3d fwd", shape, ci, co, relu, out_f32)
     X, Y, Z = shape
@@ -170,17 +174,26 @@ def test_conv3d_wgrad_and_dgrad(tk, ctx, rng, shape, ci, co):
     assert np.max(np.abs(dx - refx)) <= 1e-5 * np.max(np.abs(refx))
 
 
-def test_pool_upsample_relu_add(tk, ctx, rng):
-    X, Y, Z, C = 4, 6, 8, 16
+@pytest.mark.parametrize("X,Y,Z,C,extra,xo", [(4, 6, 8, 16, 4, 2), (4, 6, 8, 16, 16, 8),
+                                                (6, 4, 10, 32, 0, 0)])
+def test_pool_upsample_relu_add(tk, ctx, rng, X, Y, Z, C, extra, xo):
+    """pool and upsample with 2-byte channels (offset 2) and with 16-byte parts of 8 channels
+    (strides and offsets multiples of 8): the two pools agree bit for bit"""
     n = X * Y * Z
-    x = to_bf16(rng.standard_normal((n, C + 4)))
+    x = to_bf16(rng.standard_normal((n, C + extra)))
     bx = Buf(tk, ctx, x)
     by = Buf(tk, ctx, nbytes=2 * (n // 8) * C)
-    tk._check(tk.lib.tk_nn_avgpool2(ctx.h, bx.p, C + 4, 2, C, X, Y, Z, by.p, C, 0))
-    xf = from_bf16(x).astype(np.float64).reshape(X, Y, Z, C + 4)[..., 2:2 + C]
+    tk._check(tk.lib.tk_nn_avgpool2(ctx.h, bx.p, C + extra, xo, C, X, Y, Z, by.p, C, 0))
+    xf = from_bf16(x).astype(np.float64).reshape(X, Y, Z, C + extra)[..., xo:xo + C]
     ref = xf.reshape(X // 2, 2, Y // 2, 2, Z // 2, 2, C).mean(axis=(1, 3, 5)).reshape(-1, C)
     got = from_bf16(by.get(np.uint16, (n // 8, C)))
     assert np.max(np.abs(got - ref)) <= 8e-3 * np.max(np.abs(ref))
+    # the same channels at a stride and offset that only 2-byte accesses serve
+    x2 = np.zeros((n, C + 6), np.uint16)
+    x2[:, 3:3 + C] = x.reshape(n, C + extra)[:, xo:xo + C]
+    bx2, by2 = Buf(tk, ctx, x2), Buf(tk, ctx, nbytes=2 * (n // 8) * C)
+    tk._check(tk.lib.tk_nn_avgpool2(ctx.h, bx2.p, C + 6, 3, C, X, Y, Z, by2.p, C, 0))
+    assert np.array_equal(by2.get(np.uint16, (n // 8, C)), by.get(np.uint16, (n // 8, C)))
     # upsample of the pooled field, and its backward (sum of the 8 children)
     bu = Buf(tk, ctx, nbytes=2 * n * C)
     tk._check(tk.lib.tk_nn_upsample2(ctx.h, by.p, C, 0, C, X, Y, Z, bu.p, C, 0))
