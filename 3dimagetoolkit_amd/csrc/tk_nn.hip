@@ -114,11 +114,12 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
     }
 }
 
-// Forward convolution with ci = CI (16, 32 or 64) and co a multiple of 16, the input staged
-// in LDS.  A workgroup of NW waves owns BY rows (y) x 16 voxels (z) x 16 output channels
-// (one of co / 16 slices) and marches in x over a chunk of CX planes.  The input planes x-1, x, x+1 with a one-voxel halo in y and z
-// ((BY + 2) x 18 voxels, each voxel's CI channels at a stride of CI + 8 bf16, so that eight
-// lanes reading 16 bytes of eight consecutive voxels hit disjoint banks) sit in three slots
+// Forward convolution with ci = CI (16, 32 or 64) and any co, the input staged in LDS.  A
+// workgroup of NW waves owns BY rows (y) x 16 voxels (z) x 16 output channels (one of
+// ceil(co / 16) slices) and marches in x over a chunk of CX planes.  The input planes x-1, x,
+// x+1 with a one-voxel halo in y and z ((BY + 2) x 18 voxels, stored part-major: the
+// 8-channel part c of every voxel of the slot contiguous, so that eight lanes reading part c
+// of eight consecutive voxels read 128 contiguous bytes, every bank once) sit in three slots
 // of a ring of four; plane x+2 is loaded into registers before the step's products and
 // written to the fourth slot after them, so one barrier per step suffices.  The weights are
 // B fragments held in registers for the whole march.  v_mfma_f32_16x16x32_bf16 (gfx950): K =
@@ -128,16 +129,17 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
 // fragments that two rows share (row t with dy = +1 is row t + 1 with dy = 0) are one LDS
 // address, read once.
 // Forward convolution of a one-channel input to co <= 16 channels (the network's first
-// layer): one thread per output voxel, the 27 x co weights in LDS as FP32, the 27 inputs read
-// through the cache (zero outside the volume), FP32 sums over the taps in order.
-template <bool RELU>
+// layer): one thread per output voxel, the 27 inputs read through the cache (zero outside
+// the volume), FP32 sums over the taps in order; the 27 x 16 weights in LDS as FP32, read as
+// four 16-byte broadcasts per tap.
+template <bool RELU, bool V16>
 __global__ __launch_bounds__(256) void k_conv3_c1(const uint16_t* __restrict__ x, int xs, int xo,
                                                   const uint16_t* __restrict__ w, int kpad,
                                                   const float* __restrict__ bias,
                                                   uint16_t* __restrict__ y, int ys, int yo,
                                                   int co, int X, int Y, int Z)
 {
-    __shared__ float ws[27][16];
+    __shared__ __align__(16) float ws[27][16];
     for (int e = threadIdx.x; e < 27 * 16; e += 256) {
         const int t = e >> 4, o = e & 15;
         ws[t][o] = o < co ? bf2f(w[(long)o * kpad + t]) : 0.f;
@@ -158,26 +160,48 @@ __global__ __launch_bounds__(256) void k_conv3_c1(const uint16_t* __restrict__ x
             if (ii >= 0 && ii < X && jj >= 0 && jj < Y && kk >= 0 && kk < Z)
                 xv = bf2f(x[(((long)ii * Y + jj) * Z + kk) * xs + xo]);
 #pragma unroll
-            for (int o = 0; o < 16; ++o) acc[o] += xv * ws[t][o];
+            for (int q = 0; q < 4; ++q) {
+                const float4 wq = *(const float4*)&ws[t][4 * q];
+                acc[4 * q] += xv * wq.x;
+                acc[4 * q + 1] += xv * wq.y;
+                acc[4 * q + 2] += xv * wq.z;
+                acc[4 * q + 3] += xv * wq.w;
+            }
         }
+        uint16_t o16[16];
 #pragma unroll
         for (int o = 0; o < 16; ++o) {
-            if (o >= co) break;
-            float vv = acc[o] + (bias ? bias[o] : 0.f);
+            float vv = acc[o] + (bias && o < co ? bias[o] : 0.f);
             if (RELU) vv = fmaxf(vv, 0.f);
-            y[v * ys + yo + o] = f2bf(vv);
+            o16[o] = f2bf(vv);
+        }
+        uint16_t* yv = y + v * ys + yo;
+        if (V16) {
+            // the voxel's 16 channels as two 16-byte stores (one 2-byte store per channel
+            // would scatter 64 voxels 32 bytes apart per instruction)
+            uint4 p0, p1;
+            __builtin_memcpy(&p0, o16, 16);
+            __builtin_memcpy(&p1, o16 + 8, 16);
+            *(uint4*)yv = p0;
+            *(uint4*)(yv + 8) = p1;
+        } else {
+#pragma unroll
+            for (int o = 0; o < 16; ++o)
+                if (o < co) yv[o] = o16[o];
         }
     }
 }
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-constexpr int kMarchBY = 8, kMarchNW = 4;  // output rows per workgroup, waves per workgroup
+// output rows per workgroup (by input channels) and waves per workgroup
+constexpr int march_rows(int ci) { return ci == 64 ? 8 : 8; }
+constexpr int kMarchNW = 4;
 
 template <int CI, int BY, int NW>
 struct MarchCfg {
-    static constexpr int CIP = CI + 8;              // LDS voxel stride (bf16)
     static constexpr int RY = BY + 2, RZ = 18;      // a plane slot: rows x columns of voxels
-    static constexpr int SLOT = RY * RZ * CIP;      // bf16 per slot
+    static constexpr int NV = RY * RZ;              // voxels per slot
+    static constexpr int SLOT = NV * CI;            // bf16 per slot (part-major)
     static constexpr int CH = CI / 8;               // 16-byte parts per voxel
     static constexpr int NCH = RY * RZ * CH;        // 16-byte parts per plane
     static constexpr int NT = 64 * NW;              // threads
@@ -193,11 +217,13 @@ constexpr long march_lds_bytes()
     return 2L * 4 * MarchCfg<CI, BY, NW>::SLOT;
 }
 
-// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers.
+// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers; CI = 32
+// three waves per SIMD (three workgroups' slots fit the LDS).
 // Output channels past co (the last slice of a co that is not a multiple of 16) have zero
 // weights and are not written; OUT_F32 writes FP32 instead of BF16.
 template <int CI, int BY, int NW, bool RELU, bool OUT_F32>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : 2)))
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : (CI == 32 ? 3 : 2))))
 void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_t* __restrict__ w,
                    const float* __restrict__ bias, void* __restrict__ y, int ys, int yo, int co,
                    int X, int Y, int Z, int CX, int nsl)
@@ -236,7 +262,7 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
         const int ry = r / C::RZ, rz = r - ry * C::RZ;
         gy[u] = y0 - 1 + ry;
         gz[u] = z0 - 1 + rz;
-        soff[u] = r * C::CIP + 8 * c;
+        soff[u] = (c * C::NV + r) * 8;
         poff[u] = (min(max(gy[u], 0), Y - 1) * Z + min(max(gz[u], 0), Z - 1)) * xs + xo + 8 * c;
     }
     // every part is loaded from a clamped address; those outside the volume become zeros
@@ -280,14 +306,13 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
                 for (int dz = -1; dz <= 1; ++dz)
 #pragma unroll
                     for (int h = 0; h < C::H; ++h) {
-                        const uint16_t* a0 = sm + ((xc + dx) & 3) * C::SLOT +
-                                             (wave * C::MT * C::RZ + 1 + dz + m) * C::CIP + 32 * h +
-                                             8 * lg;
+                        const uint16_t* a0 =
+                            sm + ((xc + dx) & 3) * C::SLOT +
+                            ((4 * h + lg) * C::NV + wave * C::MT * C::RZ + 1 + dz + m) * 8;
                         v8bf a[C::MT + 2];
 #pragma unroll
                         for (int r = 0; r < C::MT + 2; ++r)
-                            a[r] = __builtin_bit_cast(v8bf,
-                                                      *(const uint4*)(a0 + r * C::RZ * C::CIP));
+                            a[r] = __builtin_bit_cast(v8bf, *(const uint4*)(a0 + r * C::RZ * 8));
 #pragma unroll
                         for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
@@ -304,12 +329,12 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
                 // past the last has zero weights and reads tap 26's values)
                 const int tp = min(2 * s + (lg >> 1), 26);
                 const int dx = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dz = tp % 3 - 1;
-                const uint16_t* a0 = sm + ((xc + dx) & 3) * C::SLOT +
-                                     ((1 + dy + wave * C::MT) * C::RZ + 1 + dz + m) * C::CIP +
-                                     8 * (lg & 1);
+                const uint16_t* a0 =
+                    sm + ((xc + dx) & 3) * C::SLOT +
+                    ((lg & 1) * C::NV + (1 + dy + wave * C::MT) * C::RZ + 1 + dz + m) * 8;
 #pragma unroll
                 for (int t = 0; t < C::MT; ++t) {
-                    const uint4 a = *(const uint4*)(a0 + t * C::RZ * C::CIP);
+                    const uint4 a = *(const uint4*)(a0 + t * C::RZ * 8);
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         __builtin_bit_cast(v8bf, a), bw[s], acc[t], 0, 0, 0);
                 }
@@ -632,12 +657,15 @@ int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const vo
     // one input channel (the network's first layer), co <= 16, BF16 output
     if (ci == 1 && co <= 16 && !out_f32 && kpad >= 27) {
         const long n = (long)nx * ny * nz;
-        if (relu)
-            hipLaunchKernelGGL(k_conv3_c1<true>, dim3(nblk(n)), dim3(256), 0, ctx->stream, xp, xs,
-                               xo, wp, kpad, bias, (uint16_t*)y, ys, yo, co, nx, ny, nz);
-        else
-            hipLaunchKernelGGL(k_conv3_c1<false>, dim3(nblk(n)), dim3(256), 0, ctx->stream, xp,
-                               xs, xo, wp, kpad, bias, (uint16_t*)y, ys, yo, co, nx, ny, nz);
+        const bool v16 = co == 16 && ys % 8 == 0 && yo % 8 == 0 && (uintptr_t)y % 16 == 0;
+#define TK_C1(R, V)                                                                            \
+    hipLaunchKernelGGL((k_conv3_c1<R, V>), dim3(nblk(n)), dim3(256), 0, ctx->stream, xp, xs, xo,  \
+                       wp, kpad, bias, (uint16_t*)y, ys, yo, co, nx, ny, nz)
+        if (relu && v16) TK_C1(true, true);
+        else if (relu) TK_C1(true, false);
+        else if (v16) TK_C1(false, true);
+        else TK_C1(false, false);
+#undef TK_C1
         TK_LAUNCHED();
         return TK_OK;
     }
@@ -647,14 +675,16 @@ int tk_nn_conv3d_fwd(tk_ctx ctx, const void* x, int xs, int xo, int ci, const vo
     if ((ci == 16 || ci == 32 || ci == 64) && nz % 16 == 0 && xs % 8 == 0 && xo % 8 == 0 &&
         kpad == 27 * ci && (uintptr_t)x % 16 == 0 && (uintptr_t)w % 16 == 0 &&
         (uintptr_t)y % (out_f32 ? 4 : 2) == 0) {
-        constexpr int BY = kMarchBY, NW = kMarchNW;
+        constexpr int NW = kMarchNW;
+        const int BY = march_rows(ci);
         const int nsl = (co + 15) / 16;
         const long nyz = (long)(nz / 16) * ((ny + BY - 1) / BY) * nsl;
         int cx = (int)std::max(2L, ((long)nx * nyz + 2L * ctx->num_cu - 1) / (2L * ctx->num_cu));
         cx = std::min(cx, nx);
         const dim3 grid(nz / 16, (ny + BY - 1) / BY, (unsigned)(((nx + cx - 1) / cx) * nsl));
 #define TK_MARCH(CI_, R, F)                                                                    \
-    hipLaunchKernelGGL((k_conv3_march<CI_, BY, NW, R, F>), grid, dim3(64 * NW), 0, ctx->stream,  \
+    hipLaunchKernelGGL((k_conv3_march<CI_, march_rows(CI_), NW, R, F>), grid, dim3(64 * NW), 0,  \
+                       ctx->stream,                                                              \
                        xp, xs, xo, wp, bias, y, ys, yo, co, nx, ny, nz, cx, nsl)
 #define TK_MARCH_CI(CI_)                                                                       \
     do {                                                                                       \

@@ -92,7 +92,10 @@ def kpad_of(ci):
                           # a partial last slice, one FP32 output channel (the output layer)
                           ((3, 4, 16), 16, 20, True, False, 0, 0),
                           ((4, 6, 32), 16, 1, False, True, 0, 0),
-                          ((3, 5, 16), 32, 1, True, True, 0, 0)])
+                          ((3, 5, 16), 32, 1, True, True, 0, 0),
+                          # one input channel, 16 output channels in 16-byte stores
+                          ((5, 6, 18), 1, 16, True, False, 0, 0),
+                          ((4, 3, 7), 1, 12, False, False, 2, 0)])
 def test_conv3d_fwd(tk, ctx, rng, shape, ci, co, relu, out_f32, xs_extra, ys_extra):
     say("conv3d fwd", shape, ci, co, relu, out_f32)
     X, Y, Z = shape
