@@ -193,8 +193,9 @@ __global__ __launch_bounds__(256) void k_conv3_c1(const uint16_t* __restrict__ x
 }
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-// output rows per workgroup (by input channels) and waves per workgroup
-constexpr int march_rows(int ci) { return ci == 64 ? 8 : 8; }
+// output rows per workgroup (16 rows for CI = 32 leave one workgroup per CU: 20.9 against
+// 16.1 us at 64^3) and waves per workgroup
+constexpr int march_rows(int) { return 8; }
 constexpr int kMarchNW = 4;
 
 template <int CI, int BY, int NW>
@@ -217,13 +218,12 @@ constexpr long march_lds_bytes()
     return 2L * 4 * MarchCfg<CI, BY, NW>::SLOT;
 }
 
-// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers; CI = 32
-// three waves per SIMD (three workgroups' slots fit the LDS).
+// CI = 64 holds 54 B fragments (216 registers): one wave per SIMD, all 512 registers.  (CI =
+// 32 bounded to three waves per SIMD, 165 registers: 17.5 against 16.0 us at 64^3.)
 // Output channels past co (the last slice of a co that is not a multiple of 16) have zero
 // weights and are not written; OUT_F32 writes FP32 instead of BF16.
 template <int CI, int BY, int NW, bool RELU, bool OUT_F32>
-__global__ __launch_bounds__(64 * NW)
-__attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : (CI == 32 ? 3 : 2))))
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(CI == 64 ? 1 : 2)))
 void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_t* __restrict__ w,
                    const float* __restrict__ bias, void* __restrict__ y, int ys, int yo, int co,
                    int X, int Y, int Z, int CX, int nsl)
