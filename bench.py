@@ -184,6 +184,74 @@ def bench_warmstart(tk, a):
     print(json.dumps(out), flush=True)
 
 
+def bench_unet(tk, a):
+    """The warm start's network on the matrix cores: tk_unet_predict of the shipped 64^3
+    network (29.9 GFLOP per forward pass), and single 3x3x3 convolution layers on random BF16
+    data (2 * 27 * ci * co FLOP per voxel), timed over `steps` calls after `warmup`."""
+    ws = importlib.import_module("3dimagetoolkit_amd.warmstart")
+    ctx = tk.Context(0)
+    rng = np.random.default_rng(SEED)
+    steps, warm = max(a.steps, 1), max(a.warmup, 1)
+    peak = 2.5e15
+
+    def timed(f):
+        for _ in range(warm):
+            f()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f()
+        ctx.sync()
+        return (time.perf_counter() - t0) / steps
+
+    def bf16(x):
+        u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+    def dev(arr=None, nbytes=None):
+        nb = arr.nbytes if arr is not None else nbytes
+        v = ctx.volume((int(nb), 1, 1), np.uint8)
+        if arr is not None:
+            v.upload(np.frombuffer(np.ascontiguousarray(arr).tobytes(), np.uint8).reshape(-1, 1, 1))
+        return v, tk.lib.tk_vol_device_ptr(v.h)
+
+    layers = []
+    for shape, ci, co in (((64, 64, 64), 1, 16), ((64, 64, 64), 16, 16), ((64, 64, 64), 32, 16),
+                          ((32, 32, 32), 64, 32), ((128, 128, 128), 32, 16)):
+        n = int(np.prod(shape))
+        kp = (27 * ci + 15) // 16 * 16
+        x, xp = dev(bf16(rng.standard_normal((n, ci))))
+        w, wp = dev(bf16(rng.standard_normal((co, kp)) / np.sqrt(27 * ci)))
+        b, bp = dev(rng.standard_normal(co).astype(np.float32))
+        y, yp = dev(nbytes=2 * n * co)
+        t = timed(lambda: tk._check(tk.lib.tk_nn_conv3d_fwd(
+            ctx.h, xp, ci, 0, ci, wp, kp, bp, yp, co, 0, co,
+            *shape, 1, 0)))
+        fl = 2.0 * 27 * ci * co * n
+        layers.append({"shape": list(shape), "ci": ci, "co": co, "us": round(t * 1e6, 2),
+                       "TFLOPs": round(fl / t / 1e12, 1), "frac_bf16_peak": round(fl / t / peak, 4)})
+        for v in (x, w, b, y):
+            v.free()
+    net = ws.load(ctx)
+    rhs = ctx.from_numpy(rng.standard_normal(net.dims))
+    guess = ctx.from_numpy(np.zeros(net.dims))
+    t = timed(lambda: net.predict(rhs, guess))
+    fl = 29.9e9
+    out = {"metric": "U-Net forward pass @64^3 (BF16 on the matrix cores)", "value": fl / t / 1e12,
+           "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+           "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)",
+           "data": "synthetic (random rhs; the shipped 64^3 weights)",
+           "config": {"workload": "tk_unet_predict, 3-level U-Net 16/32/64 channels, 64^3; "
+                                  "single 3x3x3 layers on random BF16 data",
+                      "forward_frac_bf16_peak": round(fl / t / peak, 4), "layers": layers},
+           "roofline": {"bound": "mfma", "achieved": round(fl / t / 1e12, 1), "peak": 2500.0,
+                        "unit": "TFLOP/s", "frac": round(fl / t / peak, 4), "traffic": None},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def bench_c0(tk, a):
     """Config 0, time to converged mask: 64^3 two-ellipsoid phantom (seed of the committed
     oracle fixture), rescaled to [0,255], sigma 3, FP64, every Poisson step gated on the
@@ -429,10 +497,11 @@ def main():
                     help="time the headline mode only (not the two lexicographic modes)")
     ap.add_argument("--gs", choices=["default", "rb", "lex"], default="default",
                     help="Gauss-Seidel ordering (default: red-black for f32, lexicographic for f64)")
-    ap.add_argument("--workload", choices=["c1", "batch", "warmstart", "c0"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "batch", "warmstart", "c0", "unet"], default="c1",
                     help="c1: one 256^3 volume (default); batch: 64 uneven volumes; "
                          "warmstart: learned vs plain Poisson start (64^3, residual 1e-6); "
-                         "c0: time to converged mask, 64^3 FP64 gated at 1e-8")
+                         "c0: time to converged mask, 64^3 FP64 gated at 1e-8; "
+                         "unet: the warm start's network forward pass and layers")
     ap.add_argument("--no-graph", action="store_true", help="batch: eager launches")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 512 MiB volume copy (k_copy16): the known-size kernel "
@@ -455,6 +524,9 @@ def main():
         return
     if a.workload == "c0":
         bench_c0(tk, a)
+        return
+    if a.workload == "unet":
+        bench_unet(tk, a)
         return
     n = a.size
     shape = (n, n, n)
