@@ -467,12 +467,12 @@ def cc_seeded(ctx, img, seed, threshold, full, out):
 
 
 def poisson_rb_autotune(ctx, rhs):
-    """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16, ms8 and
-    ms16 of the two-sweep launch, 0 where it does not apply))."""
+    """Time the red-black tile variants on rhs's shape; returns (rows, (ms8, ms16, ms32 of
+    one sweep, ms8 and ms16 of the two-sweep launch; 0 where a variant does not apply))."""
     rows = C.c_int()
-    ms = (C.c_double * 4)()
+    ms = (C.c_double * 5)()
     _check(lib.tk_poisson_rb_autotune(ctx.h, rhs.h, C.byref(rows), ms))
-    return rows.value, (ms[0], ms[1], ms[2], ms[3])
+    return rows.value, tuple(ms)
 
 
 def poisson_rb_set_pair(ctx, enable):

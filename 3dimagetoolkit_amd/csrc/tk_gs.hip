@@ -949,8 +949,9 @@ inline long long rb2_key(int X, int Y, int Z)
     return rb_key(X, Y, Z, sizeof(T) == 4 ? 15 : 14);
 }
 
-// tile rows of the column kernel for n (1 or 2) sweeps per launch on rhs's shape: 8 or 16,
-// or 0 where it does not serve: rows not 16-byte aligned; for two sweeps also fewer than 3
+// tile rows of the column kernel for n (1 or 2) sweeps per launch on rhs's shape: 8 or 16
+// (one sweep in FP32 also 32), or 0 where it does not serve: rows not 16-byte aligned; for
+// two sweeps also fewer than 3
 // interior planes, two-sweep launches switched off, an untuned shape, or one where the
 // autotuner found them slower than two one-sweep launches
 template <class T>
@@ -960,7 +961,8 @@ static int rbc_rows(tk_ctx ctx, tk_vol rhs, int n)
     if (Z % Vec16<T>::n != 0) return 0;
     if (n == 1) {
         const auto it = ctx->rb_rows.find(rb_key(X, Y, Z, (int)sizeof(T)));
-        return it != ctx->rb_rows.end() && it->second == 16 ? 16 : 8;
+        if (it == ctx->rb_rows.end()) return 8;
+        return it->second == 16 || (sizeof(T) == 4 && it->second == 32) ? it->second : 8;
     }
     if (!ctx->rb_pair || X < 5 || Y < 3) return 0;
     const auto it = ctx->rb_rows.find(rb2_key<T>(X, Y, Z));
@@ -983,6 +985,11 @@ static unsigned rbc_blocks(tk_ctx ctx, tk_vol rhs, int n, int rows)
         else if (rows == 16) rbc_grid<T, 2, 16>(ctx, X, Y, Z, &grid, &CI);
         else rbc_grid<T, 2, 8>(ctx, X, Y, Z, &grid, &CI);
     } else {
+        if constexpr (sizeof(T) == 4)
+            if (rows == 32) {
+                rbc_grid<T, 1, 32>(ctx, X, Y, Z, &grid, &CI);
+                return grid.x * grid.y * grid.z;
+            }
         if (rows == 16) rbc_grid<T, 1, 16>(ctx, X, Y, Z, &grid, &CI);
         else rbc_grid<T, 1, 8>(ctx, X, Y, Z, &grid, &CI);
     }
@@ -1004,6 +1011,8 @@ static int rbc_launch(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const 
                               : launch_rbc<T, 2, 8>(ctx, src, dst, rhs, c, s, gated, g, unit);
     }
     TimedLaunch tl(ctx, TK_K_GS);
+    if constexpr (sizeof(T) == 4)
+        if (rows == 32) return launch_rbc<T, 1, 32>(ctx, src, dst, rhs, c, s, gated, g, unit);
     return rows == 16 ? launch_rbc<T, 1, 16>(ctx, src, dst, rhs, c, s, gated, g, unit)
                       : launch_rbc<T, 1, 8>(ctx, src, dst, rhs, c, s, gated, g, unit);
 }
@@ -1087,8 +1096,9 @@ static int rb_sweep_pp(tk_ctx ctx, const void* src, void* dst, tk_vol rhs, const
     return TK_OK;
 }
 
-// time the 8- and 16-row tile variants of the aligned sweep on rhs's shape (ungated sweeps
-// of a zero field into a scratch volume); keep the faster for this context
+// time the 8- and 16-row tile variants of the aligned sweep on rhs's shape, and 32 rows in
+// FP32 (ungated sweeps of a zero field into a scratch volume); keep the fastest for this
+// context
 template <class T>
 static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
 {
@@ -1099,7 +1109,7 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
     }
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    double ms[2] = {0, 0};
+    double ms[3] = {0, 0, 0};
     int rc = TK_OK;
     const GsCoef c = gs_coef(rhs, 1.0);
     const bool unit = c.hx2 == 1. && c.hy2 == 1. && c.hz2 == 1.;
@@ -1110,8 +1120,10 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
         TK_HIP(hipMemsetAsync(a, 0, rhs->bytes(), ctx->stream));
         TK_HIP(hipEventCreate(&e0));
         TK_HIP(hipEventCreate(&e1));
-        for (int v = 0; v < 2; ++v) {
+        for (int v = 0; v < (sizeof(T) == 4 ? 3 : 2); ++v) {
             auto launch = [&]() {
+                if constexpr (sizeof(T) == 4)
+                    if (v == 2) return launch_rbc<T, 1, 32>(ctx, a, b, rhs, c, 0, 0, g, unit);
                 return v == 0 ? launch_rbc<T, 1, 8>(ctx, a, b, rhs, c, 0, 0, g, unit)
                               : launch_rbc<T, 1, 16>(ctx, a, b, rhs, c, 0, 0, g, unit);
             };
@@ -1149,7 +1161,8 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
                 ms2[v] = t / 3.0;
             }
             // kept only where one launch of two sweeps beats two launches of one
-            const double best1 = std::min(ms[0], ms[1]);
+            const double best1 = sizeof(T) == 4 ? std::min(std::min(ms[0], ms[1]), ms[2])
+                                                : std::min(ms[0], ms[1]);
             const double best2 = sizeof(T) == 8 ? ms2[0] : std::min(ms2[0], ms2[1]);
             ctx->rb_rows[rb2_key<T>(X, Y, Z)] =
                 best2 < 2 * best1 ? (sizeof(T) == 4 && ms2[1] < ms2[0] ? 16 : 8) : 0;
@@ -1161,14 +1174,16 @@ static int rb_autotune(tk_ctx ctx, tk_vol rhs, int* rows_out, double* ms_out)
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     if (rc != TK_OK) return rc;
-    const int rows = ms[1] < ms[0] ? 16 : 8;
+    int rows = ms[1] < ms[0] ? 16 : 8;
+    if (sizeof(T) == 4 && ms[2] < std::min(ms[0], ms[1])) rows = 32;
     ctx->rb_rows[rb_key(X, Y, Z, (int)sizeof(T))] = rows;
     if (rows_out) *rows_out = rows;
     if (ms_out) {
         ms_out[0] = ms[0];
         ms_out[1] = ms[1];
-        ms_out[2] = ms2[0];
-        ms_out[3] = ms2[1];
+        ms_out[2] = ms[2];
+        ms_out[3] = ms2[0];
+        ms_out[4] = ms2[1];
     }
     return TK_OK;
 }
@@ -1706,11 +1721,13 @@ int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows)
 {
     TK_CHECK(ctx, TK_ERR_ARG, "tk_poisson_rb_set_tile: null context");
     TK_TRY(check_gs(rhs, rhs));
-    TK_CHECK(tile_rows == 8 || tile_rows == 16, TK_ERR_ARG,
-             "tk_poisson_rb_set_tile: tile_rows must be 8 or 16");
+    TK_CHECK(tile_rows == 8 || tile_rows == 16 || (tile_rows == 32 && rhs->dtype == TK_F32),
+             TK_ERR_ARG, "tk_poisson_rb_set_tile: tile_rows must be 8 or 16 (or 32 in FP32)");
     ctx->rb_rows[rb_key(rhs->nx, rhs->ny, rhs->nz, (int)dtype_size(rhs->dtype))] = tile_rows;
+    // the two-sweep launch has 8- and 16-row tiles
     ctx->rb_rows[rhs->dtype == TK_F32 ? rb2_key<float>(rhs->nx, rhs->ny, rhs->nz)
-                                      : rb2_key<double>(rhs->nx, rhs->ny, rhs->nz)] = tile_rows;
+                                      : rb2_key<double>(rhs->nx, rhs->ny, rhs->nz)] =
+        tile_rows > 16 ? 16 : tile_rows;
     return TK_OK;
 }
 

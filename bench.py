@@ -61,6 +61,7 @@ _TRAFFIC_256_F32_RB = {
     "conv_x2": 406943456,   # 1.011
     "gs8": 223585296,       # 1.111
     "gs16": 217623808,      # 1.081
+    "gs32": 217134464,      # 1.079
     "sbdiv": 900299952,     # 1.220
 }
 MEASURED_TRAFFIC = {(256, "f32", "rb", k): v for k, v in _TRAFFIC_256_F32_RB.items()}

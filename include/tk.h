@@ -199,12 +199,13 @@ int tk_poisson_solve_residual(tk_ctx ctx, tk_vol phi, tk_vol rhs, double dt, dou
                               int* sweeps_out, double* res_out);
 /* autotuner of the red-black sweep (no reference counterpart: the reference has one fixed
    lexicographic loop): times the 8-row and 16-row tile variants on rhs's shape and type,
-   keeps the faster for every later red-black sweep of that shape on ctx, returns the rows
-   in *tile_rows and the times in ms[0..1] (either may be null); it also times the two-sweep
-   launch with both tile heights (ms[2..3], 0 where the volume has fewer than 3 interior
-   planes; ms holds 4 doubles) and keeps the faster, or one launch per sweep where that is
-   faster than the pairs.  All variants give identical results.  Runs synchronously;
-   tk_rsfe_create calls it for red-black runs. */
+   and 32 rows in FP32, keeps the fastest for every later red-black sweep of that shape on
+   ctx, returns the rows in *tile_rows and the times in ms[0..2] (8, 16, 32 rows; ms[2] is
+   0 in FP64; either pointer may be null); it also times the two-sweep launch with 8- and
+   16-row tiles (ms[3..4], 0 where the volume has fewer than 3 interior planes or, for 16
+   rows, in FP64; ms holds 5 doubles) and keeps the faster, or one launch per sweep where
+   that is faster than the pairs.  All variants give identical results.  Runs
+   synchronously; tk_rsfe_create calls it for red-black runs. */
 int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms);
 /* red-black sweeps run two per launch where the volume allows (16-byte rows: z a multiple
    of 4 in FP32, of 2 in FP64; the result is bitwise that of one launch per sweep) and the
@@ -214,8 +215,8 @@ int tk_poisson_rb_autotune(tk_ctx ctx, tk_vol rhs, int* tile_rows, double* ms);
    stop at the same sweep with the same field: enable 1 (the default) or 0 for one launch
    per sweep, on every later solve on ctx */
 int tk_poisson_rb_set_pair(tk_ctx ctx, int enable);
-/* set the red-black tile rows (8 or 16) for rhs's shape and type on ctx (the one-sweep and
-   the two-sweep launch) */
+/* set the red-black tile rows (8 or 16, or 32 in FP32) for rhs's shape and type on ctx
+   (the one-sweep launch; the two-sweep launch takes 16 rows for 32) */
 int tk_poisson_rb_set_tile(tk_ctx ctx, tk_vol rhs, int tile_rows);
 
 /* ---- R6, R10-R13: pointwise and stencil operators ----------------------------- */

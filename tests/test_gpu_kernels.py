@@ -301,8 +301,8 @@ def rb_sweep_f32(phi, rhs):
     return rb_sweep(phi, rhs, np.float32)
 
 
-@pytest.mark.parametrize("rows", [8, 16])
-@pytest.mark.parametrize("shape", [(16, 18, 20), (33, 17, 9), (40, 37, 136)])
+@pytest.mark.parametrize("rows", [8, 16, 32])
+@pytest.mark.parametrize("shape", [(16, 18, 20), (33, 17, 9), (40, 37, 136), (20, 75, 72)])
 def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rows, rng):
     say("gs rb", shape, "tile rows", rows)
     phi = rng.random(shape).astype(np.float32)
@@ -316,7 +316,7 @@ def test_gs_redblack_f32_bitwise_vs_numpy(tk, ctx, shape, rows, rng):
     assert np.array_equal(vp.numpy(), ref)
 
 
-@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("rows", [8, 16, 32])
 @pytest.mark.parametrize("shape,sweeps,h", [((16, 18, 20), 2, 1.0), ((40, 37, 136), 5, 1.0),
                                             ((9, 70, 64), 4, 1.0), ((70, 45, 200), 3, 0.7),
                                             ((128, 96, 256), 5, 1.0)])
@@ -413,16 +413,21 @@ def test_gs_redblack_autotune(tk, ctx, rng):
     shape = (64, 66, 128)
     rhs = ctx.from_numpy(rng.standard_normal(shape).astype(np.float32))
     rows, ms = tk.poisson_rb_autotune(ctx, rhs)
-    say("rb autotune ms (8 rows, 16 rows; two-sweep launch 8, 16):", ms, "->", rows)
-    assert rows in (8, 16) and min(ms) > 0
-    assert rows == (16 if ms[1] < ms[0] else 8)
+    say("rb autotune ms (8, 16, 32 rows; two-sweep launch 8, 16):", ms, "->", rows)
+    assert rows in (8, 16, 32) and min(ms) > 0
+    assert rows == (8, 16, 32)[int(np.argmin(ms[:3]))]
+    r64 = ctx.from_numpy(rng.standard_normal(shape))
+    rows64, ms64 = tk.poisson_rb_autotune(ctx, r64)
+    assert rows64 in (8, 16) and ms64[2] == 0 and ms64[4] == 0 and min(ms64[:2]) > 0
+    with pytest.raises(tk.TkError, match="32 in FP32"):
+        tk.poisson_rb_set_tile(ctx, r64, 32)
     phi = rng.random(shape).astype(np.float32)
     vp = ctx.from_numpy(phi)
     tk.poisson_sweeps(ctx, vp, rhs, 2, 1.0, 0.0, tk.TK_GS_REDBLACK)
     assert np.array_equal(vp.numpy(), rb_sweep_f32(rb_sweep_f32(phi, rhs.numpy()), rhs.numpy()))
 
 
-@pytest.mark.parametrize("rows", [8, 16])
+@pytest.mark.parametrize("rows", [8, 16, 32])
 def test_gs_redblack_gate_matches_replay(tk, ctx, rows, rng):
     shape = (20, 22, 24)
     phi = rng.random(shape).astype(np.float32)
