@@ -218,10 +218,11 @@ def bench_unet(tk, a):
 
     layers = []
     for shape, ci, co in (((64, 64, 64), 1, 16), ((64, 64, 64), 16, 16), ((64, 64, 64), 32, 16),
-                          ((32, 32, 32), 64, 32), ((128, 128, 128), 32, 16)):
+                          ((32, 32, 32), 64, 32), ((128, 128, 128), 32, 16),
+                          ((256, 256, 256), 32, 16)):
         n = int(np.prod(shape))
         kp = (27 * ci + 15) // 16 * 16
-        x, xp = dev(bf16(rng.standard_normal((n, ci))))
+        x, xp = dev(bf16(rng.standard_normal((n, ci), dtype=np.float32)))
         w, wp = dev(bf16(rng.standard_normal((co, kp)) / np.sqrt(27 * ci)))
         b, bp = dev(rng.standard_normal(co).astype(np.float32))
         y, yp = dev(nbytes=2 * n * co)
