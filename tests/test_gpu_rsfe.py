@@ -227,10 +227,12 @@ def test_c0_s5_f32_redblack_vs_golden(tk, ctx):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape,h", [((24, 20, 28), 1.0), ((33, 21, 64), 1.0), ((9, 70, 68), 0.7),
-                                     ((96, 40, 136), 1.0)])
+                                     ((96, 40, 136), 1.0), ((3, 20, 28), 1.0), ((4, 18, 24), 1.0),
+                                     ((5, 26, 32), 1.0)])
 def test_fused_update_and_div_equal_two_passes(tk, ctx, dtype, shape, h):
-    """the update and div in one pass (tiles with recomputed rings, the first and last planes,
-    rows and columns, general spacing) against the two passes: phi and rhs (which holds the
+    """the update and div in one pass (tiles with recomputed rings, the first and last planes
+    of volumes of 3, 4, 5 and more planes, rows and columns, general spacing) against the
+    two passes: phi and rhs (which holds the
     divergence of the iteration before) bitwise equal after several iterations; the
     tolerance, whose sums are added in another order, to rounding"""
     say("fused update", shape, h, dtype.__name__)
