@@ -890,7 +890,8 @@ template <class T>
 int launch_sb_div(tk_ctx ctx, tk_vol phi, tk_vol phiold, tk_vol g, tk_vol* b_in, tk_vol* b_out,
                   tk_vol divdb, double a0, double b0, int* nblocks, double* part)
 {
-    // 8-row tiles: 16 rows measured 213 us and 4 rows 226 us against 184 us at 256^3 FP32
+    // 8-row tiles: 16 rows measured 189 us, 12 rows 283 us and 4 rows 226 us against 178 us
+    // at 256^3 FP32
     constexpr int TJ = 8;
     constexpr int NT = sd_threads<T, TJ>();
     const int X = phi->nx, Y = phi->ny, Z = phi->nz;
