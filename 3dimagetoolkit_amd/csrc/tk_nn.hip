@@ -118,8 +118,9 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const uint16_t* __restrict__ 
 // workgroup of NW waves owns BY rows (y) x 16 voxels (z) x 16 output channels (one of
 // ceil(co / 16) slices) and marches in x over a chunk of CX planes.  The input planes x-1, x,
 // x+1 with a one-voxel halo in y and z ((BY + 2) x 18 voxels, stored part-major: the
-// 8-channel part c of every voxel of the slot contiguous, so that eight lanes reading part c
-// of eight consecutive voxels read 128 contiguous bytes, every bank once) sit in three slots
+// 8-channel part c of every voxel of the slot contiguous, the parts a multiple of 256 bytes
+// apart, so that the lanes reading part c of 16 consecutive voxels read 256 contiguous
+// bytes with no bank conflict) sit in three slots
 // of a ring of four; plane x+2 is loaded into registers before the step's products and
 // written to the fourth slot after them, so one barrier per step suffices.  The weights are
 // B fragments held in registers for the whole march.  v_mfma_f32_16x16x32_bf16 (gfx950): K =
@@ -202,7 +203,13 @@ template <int CI, int BY, int NW>
 struct MarchCfg {
     static constexpr int RY = BY + 2, RZ = 18;      // a plane slot: rows x columns of voxels
     static constexpr int NV = RY * RZ;              // voxels per slot
-    static constexpr int SLOT = NV * CI;            // bf16 per slot (part-major)
+    // part stride in voxels: a multiple of 16, so that the 16-byte parts of the 4 lane
+    // groups of an A-fragment read (16 consecutive voxels each, one part per group) start
+    // a multiple of 256 bytes apart; at 180 voxels (2880 bytes) every read took 3.8 bank
+    // conflict cycles (an LDS probe of that access: none for strides of 2816, 3072 or 4096
+    // bytes, whatever the offset of the runs)
+    static constexpr int NVP = (NV + 15) / 16 * 16;
+    static constexpr int SLOT = NVP * CI;           // bf16 per slot (part-major)
     static constexpr int CH = CI / 8;               // 16-byte parts per voxel
     static constexpr int NCH = RY * RZ * CH;        // 16-byte parts per plane
     static constexpr int NT = 64 * NW;              // threads
@@ -254,6 +261,9 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
     }
     const float bo = bias && nok ? bias[m] : 0.f;
     // 16-byte part u of this thread in a plane: voxel r = row * RZ + col of the slot, part c
+    // (the parts of a voxel on consecutive lanes, which load whole voxels: with 16 lanes on
+    // one part of 16 voxels instead, the LDS writes have no bank conflicts, but the layers
+    // took 2-12% longer)
     int poff[C::PF], soff[C::PF], gy[C::PF], gz[C::PF];
 #pragma unroll
     for (int u = 0; u < C::PF; ++u) {
@@ -262,7 +272,7 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
         const int ry = r / C::RZ, rz = r - ry * C::RZ;
         gy[u] = y0 - 1 + ry;
         gz[u] = z0 - 1 + rz;
-        soff[u] = (c * C::NV + r) * 8;
+        soff[u] = (c * C::NVP + r) * 8;
         poff[u] = (min(max(gy[u], 0), Y - 1) * Z + min(max(gz[u], 0), Z - 1)) * xs + xo + 8 * c;
     }
     // every part is loaded from a clamped address; those outside the volume become zeros
@@ -308,7 +318,7 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
                     for (int h = 0; h < C::H; ++h) {
                         const uint16_t* a0 =
                             sm + ((xc + dx) & 3) * C::SLOT +
-                            ((4 * h + lg) * C::NV + wave * C::MT * C::RZ + 1 + dz + m) * 8;
+                            ((4 * h + lg) * C::NVP + wave * C::MT * C::RZ + 1 + dz + m) * 8;
                         v8bf a[C::MT + 2];
 #pragma unroll
                         for (int r = 0; r < C::MT + 2; ++r)
@@ -331,7 +341,7 @@ void k_conv3_march(const uint16_t* __restrict__ x, int xs, int xo, const uint16_
                 const int dx = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dz = tp % 3 - 1;
                 const uint16_t* a0 =
                     sm + ((xc + dx) & 3) * C::SLOT +
-                    ((lg & 1) * C::NV + (1 + dy + wave * C::MT) * C::RZ + 1 + dz + m) * 8;
+                    ((lg & 1) * C::NVP + (1 + dy + wave * C::MT) * C::RZ + 1 + dz + m) * 8;
 #pragma unroll
                 for (int t = 0; t < C::MT; ++t) {
                     const uint4 a = *(const uint4*)(a0 + t * C::RZ * 8);
