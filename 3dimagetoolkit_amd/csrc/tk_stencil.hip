@@ -660,10 +660,7 @@ void k_sb_div(SdArgs<T> a)
     auto ldo = [&](const T* f, int p, long o) {
         return ldrow(f + (long)min(max(p, 0), X - 1) * sX + o);
     };
-    // phi of plane p: planes past the last read plane p - 4 (clamped), so that at the last
-    // plane the ring slot of plane q + 2 holds plane X - 3, the one-sided derivative's third
-    // row; no other use reads a plane past the last (see the x rows below)
-    auto ldphi = [&](int p) { return ldo(a.phi, p < X ? p : max(p - 4, 0), off); };
+    auto ld = [&](const T* f, int p) { return ldo(f, p, off); };
     auto cl = [&](T x) { return clampv(x, a.lo, a.hi); };
     auto id = [](T x) { return x; };
     // y rows of a staged plane for row class cj (central: r+1, r-1; first: r, r+1, r+2;
@@ -691,7 +688,7 @@ void k_sb_div(SdArgs<T> a)
     // q in use and q+1 in flight; q_x of planes q-3 .. q
     Row<T> ph[K], rb0[K], rb1[K], rb2[K], rg[K], rpo[K], qx[K];
 #pragma unroll
-    for (int d = -1; d <= 2; ++d) ph[(d + K) % K] = ldphi(qs + d);
+    for (int d = -1; d <= 2; ++d) ph[(d + K) % K] = ld(a.phi, qs + d);
 #pragma unroll
     for (int d = 0; d <= 1; ++d) {
         rb0[d] = ldo(a.bin[0], qs + d, offq);
@@ -713,26 +710,21 @@ void k_sb_div(SdArgs<T> a)
         // ---- update of plane q: cut, tolerance sums, grad, b += d, shrink, b -= d, d -= b
         if (q >= 0 && q <= X - 1 && qunit) {
             const T* pl = Ph[q & 1];
-            // x rows, all from the ring (whose slots are only ever indexed by constants, so
-            // it stays in registers): planes q+1, q-1 inside; the first plane's one-sided
-            // rows 0, 1, 2 are the slots of q, q+1, q+2; the last plane's X-1, X-2, X-3 the
-            // slots of q, q-1 and q+2 (plane X+1, read as X-3).  No load sits in a branch:
-            // a register filled by a conditional load made the compiler wait for every load
-            // in flight (the ring's prefetches too) at its next use on every step.  The ring
-            // may hold phi before or after its cut (another workgroup's, or this one's, own
-            // voxels); the derivative clamps every row, and the cut is idempotent.
+            // x rows: the ring's planes q+1, q-1 inside; the first and last planes read their
+            // one-sided rows from memory (the ring's slots are only ever indexed by
+            // constants, so it stays in registers)
             DRows<T> rx;
             rx.c = plane_class(q, X);
             rx.r[0] = ph[sl(1)];
             rx.r[1] = ph[sl(-1)];
             if (rx.c == 0) {
-                rx.r[0] = ph[sl(0)];
-                rx.r[1] = ph[sl(1)];
-                rx.r[2] = ph[sl(2)];
+                rx.r[0] = ld(a.phi, 0);
+                rx.r[1] = ld(a.phi, 1);
+                rx.r[2] = ld(a.phi, 2);
             } else if (rx.c == 2) {
-                rx.r[0] = ph[sl(0)];
-                rx.r[1] = ph[sl(-1)];
-                rx.r[2] = ph[sl(2)];
+                rx.r[0] = ld(a.phi, X - 1);
+                rx.r[1] = ld(a.phi, X - 2);
+                rx.r[2] = ld(a.phi, X - 3);
             }
             const DRows<T> ry = yrows(pl);
             const Row<T> cc = ph[sl(0)];
@@ -863,7 +855,7 @@ void k_sb_div(SdArgs<T> a)
             strow(a.divdb + (long)j * Z + k0, o);
         }
         // refill the slots this step was the last to use (phi q-1, the fields of q)
-        ph[sl(3)] = ldphi(q + 3);
+        ph[sl(3)] = ld(a.phi, q + 3);
         rb0[sl(2)] = ldo(a.bin[0], q + 2, offq);
         rb1[sl(2)] = ldo(a.bin[1], q + 2, offq);
         rb2[sl(2)] = ldo(a.bin[2], q + 2, offq);
