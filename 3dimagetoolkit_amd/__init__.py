@@ -140,6 +140,7 @@ def _load():
         "tk_rsfe_default_params": (i, [C.POINTER(RsfeParams)]),
         "tk_rsfe_create": (i, [vp, vp, vp, C.POINTER(RsfeParams), C.POINTER(vp)]),
         "tk_rsfe_iterate": (i, [vp, i, C.POINTER(IterStats), ip]),
+        "tk_rsfe_onthego": (i, [vp, C.POINTER(RsfeParams), i, i]),
         "tk_rsfe_phi": (vp, [vp]),
         "tk_rsfe_destroy": (i, [vp]),
         "tk_nn_conv3d_fwd": (i, [vp, vp, i, i, i, vp, i, vp, vp, i, i, i, i, i, i, i, i]),
@@ -518,6 +519,11 @@ class Rsfe:
                      res_guess=st[q].res_guess, cc_extracted=st[q].cc_extracted,
                      coarse_sweeps=st[q].coarse_sweeps, tol_inf=st[q].tol_inf)
                 for q in range(done.value)]
+
+    def onthego(self, params, save_current=False, end_now=False):
+        """update_param_onthego: params for the iterations that follow (tk_rsfe_onthego)"""
+        _check(lib.tk_rsfe_onthego(self.h, C.byref(params), int(bool(save_current)),
+                                   int(bool(end_now))))
 
     def phi(self, dtype=np.float64):
         out = np.empty(self.shape, dtype)

@@ -278,6 +278,10 @@ struct tk_rsfe_s {
     // [a0, b0] (bc is 0 or a0), so they stay set for the run
     bool faces_set = false;
     bool final_dumped = false;  // the end-of-run dump has been written
+    bool ended = false;         // end_now set on the go (tk_rsfe_onthego): no more iterations
+    // edge_detector_sigma at the last g, held in a bool as the reference does (imp:1656,
+    // 1877-1882)
+    bool old_eds = false;
 };
 
 namespace {
@@ -792,6 +796,49 @@ int tk_rsfe_default_params(tk_rsfe_params* p)
     return TK_OK;
 }
 
+// gforshrink = edge_detect(myim) with the run's current parameters (imp:196-229): with
+// edge_detector_sigma != 0 the edge detector reads cv(myim) with a Gaussian of that sigma and
+// radius 3 (imp:214-227)
+static int compute_edge(tk_rsfe s)
+{
+    const tk_rsfe_params& p = s->p;
+    tk_ctx ctx = s->ctx;
+    const tk_dtype dt = s->dtype;
+    int r = TK_OK;
+    tk_vol edge_src = s->u0, blurred = nullptr;
+    tk_filter fe = nullptr;
+    if (p.edge_detector_sigma != 0) {
+        r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2],
+                               p.edge_detector_sigma, 3., p.gaussian_pixel_approach, dt, &fe);
+        if (r == TK_OK)
+            r = tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], dt, &blurred);
+        if (r == TK_OK)
+            r = p.acyclic ? tk_conv_acyclic(ctx, fe, blurred, s->u0)
+                          : tk_conv_cyclic(ctx, fe, blurred, s->u0);
+        edge_src = blurred;
+    }
+    if (r == TK_OK)
+        r = dt == TK_F64 ? launch_edge<double>(ctx, edge_src, p.beta, p.lamda, p.nu, s->g)
+                         : launch_edge<float>(ctx, edge_src, p.beta, p.lamda, p.nu, s->g);
+    if (fe || blurred) {
+        if (r == TK_OK) r = tk_ctx_sync(ctx);
+        if (blurred) tk_vol_free(blurred);
+        if (fe) tk_filter_free(fe);
+    }
+    return r;
+}
+
+// kones = cv(1), kmyim = cv(myim) in one complex transform with the run's filter
+// (imp:1648-1651)
+static int compute_ones_and_image(tk_rsfe s)
+{
+    return s->dtype == TK_F64
+               ? conv_passes_pair_ones<double>(s->ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
+                                               s->p.acyclic != 0)
+               : conv_passes_pair_ones<float>(s->ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
+                                              s->p.acyclic != 0);
+}
+
 int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, tk_rsfe* out)
 {
     TK_CHECK(ctx && u0 && p && out, TK_ERR_ARG, "tk_rsfe_create: null argument");
@@ -864,36 +911,10 @@ int tk_rsfe_create(tk_ctx ctx, tk_vol u0, tk_vol init, const tk_rsfe_params* p, 
     if (r == TK_OK && p->gs_order == TK_GS_REDBLACK)
         r = tk_poisson_rb_autotune(ctx, s->u0, nullptr, nullptr);
     // kones = cv(1), kmyim = cv(myim) in one complex transform (imp:1648-1651)
-    if (r == TK_OK) {
-        if (dt == TK_F64)
-            r = conv_passes_pair_ones<double>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
-                                              p->acyclic != 0);
-        else
-            r = conv_passes_pair_ones<float>(ctx, s->flt, s->kones, s->kmyim, s->u0, s->C,
-                                             p->acyclic != 0);
-    }
-    // gforshrink = edge_detect(myim) (imp:1655); with edge_detector_sigma != 0 the edge
-    // detector reads cv(myim) with a Gaussian of that sigma and radius 3 (imp:214-227)
-    tk_vol edge_src = s->u0, blurred = nullptr;
-    tk_filter fe = nullptr;
-    if (r == TK_OK && p->edge_detector_sigma != 0) {
-        r = tk_filter_gaussian(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2],
-                               p->edge_detector_sigma, 3., p->gaussian_pixel_approach, dt, &fe);
-        if (r == TK_OK)
-            r = tk_vol_alloc(ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], dt, &blurred);
-        if (r == TK_OK)
-            r = p->acyclic ? tk_conv_acyclic(ctx, fe, blurred, s->u0)
-                           : tk_conv_cyclic(ctx, fe, blurred, s->u0);
-        edge_src = blurred;
-    }
-    if (r == TK_OK)
-        r = dt == TK_F64 ? launch_edge<double>(ctx, edge_src, p->beta, p->lamda, p->nu, s->g)
-                         : launch_edge<float>(ctx, edge_src, p->beta, p->lamda, p->nu, s->g);
-    if (fe || blurred) {
-        if (r == TK_OK) r = tk_ctx_sync(ctx);
-        if (blurred) tk_vol_free(blurred);
-        if (fe) tk_filter_free(fe);
-    }
+    if (r == TK_OK) r = compute_ones_and_image(s);
+    // gforshrink = edge_detect(myim) (imp:1655)
+    if (r == TK_OK) r = compute_edge(s);
+    s->old_eds = p->edge_detector_sigma != 0;
     if (r == TK_OK) r = tk_ctx_sync(ctx);
     if (r != TK_OK) {
         free_state(s);
@@ -918,7 +939,7 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
     // whose read-out would wait for them.  Nothing is carried over to the next call.
     bool ahead = false;
     // for (t = 1; tollerance > tol && t <= maxiter; ++t)  (imp:1659)
-    while (done < n_iters && s->tollerance > p.tol && s->iter < p.maxiter) {
+    while (!s->ended && done < n_iters && s->tollerance > p.tol && s->iter < p.maxiter) {
         const bool spec = done + 1 < n_iters && s->iter + 1 < p.maxiter && p.tol <= 0 &&
                           !s->ctx->timing;
         tk_rsfe_iter_stats* st = stats ? &stats[done] : nullptr;
@@ -935,7 +956,8 @@ int tk_rsfe_iterate(tk_rsfe s, int n_iters, tk_rsfe_iter_stats* stats, int* n_do
         ++done;
     }
     // the loop has ended: the final dump (imp:1968-1978), once
-    if (p.dumpfrequency > 0 && !s->final_dumped && (s->tollerance <= p.tol || s->iter >= p.maxiter)) {
+    if (p.dumpfrequency > 0 && !s->final_dumped &&
+        (s->tollerance <= p.tol || s->iter >= p.maxiter || s->ended)) {
         s->final_dumped = true;
         const int r = tk_vol_write_mhd(
             s->phi, (std::string(p.outputname) + "_" + std::to_string(s->iter) + "_final").c_str());
@@ -1251,6 +1273,72 @@ extern "C" {
 
 tk_vol tk_rsfe_phi(tk_rsfe s) { return s ? s->phi : nullptr; }
 tk_vol tk_rsfe_rhs(tk_rsfe s) { return s ? s->rhs : nullptr; }
+
+int tk_rsfe_onthego(tk_rsfe s, const tk_rsfe_params* p, int save_current, int end_now)
+{
+    TK_CHECK(s && p, TK_ERR_ARG, "tk_rsfe_onthego: null argument");
+    const tk_rsfe_params old = s->p;
+    TK_CHECK(p->gs_order == old.gs_order && p->acyclic == old.acyclic &&
+                 p->neumann == old.neumann && p->warm_start == old.warm_start &&
+                 p->ls_res_tol == old.ls_res_tol && p->res_check_every == old.res_check_every &&
+                 p->ws_presweeps == old.ws_presweeps && p->ws_judge_sweeps == old.ws_judge_sweeps &&
+                 p->ws_continue == old.ws_continue && p->ws_coarse_sweeps == old.ws_coarse_sweeps,
+             TK_ERR_UNSUPPORTED,
+             "tk_rsfe_onthego: the sweep order, the convolution, the Neumann solver and the "
+             "warm start cannot change on the go");
+    TK_CHECK(p->sigma > 0 && p->maxiter >= 0 && p->ls_steps >= 0 && p->dumpfrequency >= 0 &&
+                 p->auto_extract_conn_comp_freq >= 0 && p->edge_detector_sigma >= 0 &&
+                 p->cc_threshold >= 0 && p->cc_threshold <= 1 &&
+                 (p->verbosity == 0 || p->verbosity == 1),
+             TK_ERR_ARG, "tk_rsfe_onthego: parameter out of range");
+    TK_CHECK(std::memchr(p->outputname, 0, sizeof p->outputname) != nullptr, TK_ERR_ARG,
+             "tk_rsfe_onthego: outputname is not terminated");
+    TK_CHECK(!s->in_batch, TK_ERR_UNSUPPORTED, "tk_rsfe_onthego: the run is part of a batch");
+    s->p = *p;
+    int r = TK_OK;
+    // a changed sigma: the filter (radius 3, as imp:1686-1689 builds it) and kones, kmyim
+    // for the next iteration (imp:1683-1696)
+    if (p->sigma != old.sigma) {
+        tk_filter f = nullptr;
+        r = tk_filter_gaussian(s->ctx, s->X, s->Y, s->Z, s->h[0], s->h[1], s->h[2], p->sigma,
+                               3., p->gaussian_pixel_approach, s->dtype, &f);
+        if (r == TK_OK) {
+            tk_filter_free(s->flt);
+            s->flt = f;
+            r = compute_ones_and_image(s);
+        }
+    }
+    // g again when the bool of the previous edge_detector_sigma differs from the current one
+    // (imp:1877-1882); with unchanged inputs that is the same g
+    if (r == TK_OK && (double)s->old_eds != p->edge_detector_sigma &&
+        (p->edge_detector_sigma != old.edge_detector_sigma || p->beta != old.beta ||
+         p->lamda != old.lamda || p->nu != old.nu ||
+         p->gaussian_pixel_approach != old.gaussian_pixel_approach))
+        r = compute_edge(s);
+    s->old_eds = p->edge_detector_sigma != 0;
+    // the Poisson boundary value (bc flag, a0) is set again by the next solve
+    if (p->a0 != old.a0 || p->b0 != old.b0 || p->bc_dir != old.bc_dir) s->faces_set = false;
+    if (r == TK_OK) r = tk_ctx_sync(s->ctx);
+    if (r != TK_OK) return r;
+    // this iteration's dump with the settings just read (imp:1939-1951): where the previous
+    // settings wrote one under the same name it is already there
+    const int t = s->iter;
+    auto dump_due = [t](const tk_rsfe_params& q) {
+        return q.dumpfrequency != 0 && t % q.dumpfrequency == 0 && t != q.maxiter;
+    };
+    const bool written = t > 0 && dump_due(old) && std::strcmp(old.outputname, p->outputname) == 0;
+    if (t > 0 && (dump_due(*p) || save_current) && !written)
+        TK_TRY(tk_vol_write_mhd(s->phi, (std::string(p->outputname) + "_" + std::to_string(t)).c_str()));
+    if (end_now) {
+        s->ended = true;
+        if (p->dumpfrequency > 0 && !s->final_dumped) {
+            s->final_dumped = true;
+            TK_TRY(tk_vol_write_mhd(
+                s->phi, (std::string(p->outputname) + "_" + std::to_string(t) + "_final").c_str()));
+        }
+    }
+    return TK_OK;
+}
 
 int tk_rsfe_set_warm_start(tk_rsfe s, tk_unet net)
 {

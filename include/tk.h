@@ -396,6 +396,20 @@ tk_vol tk_rsfe_phi(tk_rsfe s);
  * the forcing of the iteration it did not run, enqueued while the tollerance was read back;
  * phi, the statistics and the iteration count are those of the iterations that ran. */
 tk_vol tk_rsfe_rhs(tk_rsfe s);
+/* the on-the-go re-read (rsfe_splitbregman::update_param_onthego, imp:545-578, called at
+   1816-1829 every showfrequency iterations): between two tk_rsfe_iterate calls, *p becomes
+   the run's parameter set for the iterations that follow.  A changed sigma rebuilds the filter
+   (radius 3) and kones, kmyim (imp:1683-1696); g is computed again when the previous
+   edge_detector_sigma taken as a bool differs from the new value and the edge detector's
+   inputs changed (imp:1877-1882); a0, b0 and bc_dir set the Poisson faces again.
+   save_current writes phi as <outputname>_<t> (t: iterations so far), as does the
+   dumpfrequency just set when due at t (imp:1939-1951); end_now ends the run (later
+   tk_rsfe_iterate calls run nothing) and writes the final dump when dumpfrequency > 0.
+   The reference re-reads between the cut and the Split-Bregman update of iteration t, so a
+   changed edge detector reaches that update there; here, the next iteration's.  The
+   sweep order, cv, the Neumann solver and the warm start cannot change
+   (TK_ERR_UNSUPPORTED), nor can a batched run. */
+int tk_rsfe_onthego(tk_rsfe s, const tk_rsfe_params* p, int save_current, int end_now);
 /* R15: attach the learned warm start (created on the run's context for its dims).  The
    Poisson problem is linear, so the network that maps a right-hand side to its solution
    also maps the current residual r = rhs + lap phi to the correction.  The network is

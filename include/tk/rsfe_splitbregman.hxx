@@ -24,6 +24,7 @@
 #ifndef TK_RSFE_SPLITBREGMAN_HXX
 #define TK_RSFE_SPLITBREGMAN_HXX
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -158,7 +159,10 @@ public:
         else tk::warning("set_edge_detector_sigma: edge_detector_sigma must be greater than zero");
     }
     // parameter file (imp:807-813) read by apply (imp:1559-1571), GetPot section
-    // splitbregman/init/; showfrequency and onthego are kept but nothing is displayed
+    // splitbregman/init/, and every showfrequency iterations (not at maxiter) section
+    // splitbregman/onthego/ when onthego is true (update_param_onthego, imp:545-578, called at
+    // 1816-1829); nothing is displayed, and without a parameter file there is no interactive
+    // prompt
     void set_getpotfile(std::string const& name) { getpotfile_ = name; }
     void set_onthego(bool const onthego) { onthego_ = onthego; }
     void set_showfrequency(unsigned const& sf) { showfrequency_ = sf; }
@@ -178,9 +182,8 @@ public:
         if ((it = f("showfrequency", -1)) >= 0) set_showfrequency((unsigned)it);
         if ((it = f("dumpfrequency", -1)) >= 0) set_dumpfrequency((unsigned)it);
         if ((st = f("outputname", "")).size() != 0) set_outputname(st);
-        if (f("save_current", -1) != -1 || f("end_now", -1) != -1)
-            tk::warning("set_param_from_getpot: save_current and end_now are on-the-go "
-                        "controls; not built");
+        if ((it = f("save_current", -1)) != -1) save_current_ = it != 0;
+        if ((it = f("end_now", -1)) != -1) end_now_ = it != 0;
         if ((t = f("sigma", -1.)) != -1.) set_sigma((T)t);
         if ((t = f("lamda", -1.)) != -1.) set_lamda((T)t);
         if ((t = f("lamda1", -1.)) != -1.) set_lamda1((T)t);
@@ -274,11 +277,9 @@ private:
         else tk::warning(msg);
     }
 
-    void run(im3d::image3d<T> const& myim, tk_vol init)
+    // the device run's parameters from the current settings
+    tk_rsfe_params device_params() const
     {
-        // the parameter file's init section first (imp:1559-1571)
-        if (getpotfile_ != "") set_param_from_getpot("init/");
-        else onthego_ = false;
         tk_rsfe_params p = p_;
         p.gs_order = onestep_.getf() == &lapl::DirRedBlackGaussSeidel<T> ? TK_GS_REDBLACK
                                                                            : TK_GS_LEX;
@@ -287,6 +288,16 @@ private:
         p.verbosity = verbosity_ ? 1 : 0;
         // extract_connected_component's threshold (imp:481), in T as the reference
         p.cc_threshold = (double)((this->alpha - (T)p_.a0) / ((T)p_.b0 - (T)p_.a0));
+        return p;
+    }
+
+    void run(im3d::image3d<T> const& myim, tk_vol init)
+    {
+        // the parameter file's init section first (imp:1559-1571)
+        save_current_ = end_now_ = false;
+        if (getpotfile_ != "") set_param_from_getpot("init/");
+        else onthego_ = false;
+        tk_rsfe_params p = device_params();
         tk_rsfe s = nullptr;
         sweeps_.clear();
         tols_.clear();
@@ -295,16 +306,48 @@ private:
             return;
         }
         tk::check(tk_rsfe_create(myim.ctx(), myim.handle(), init, &p, &s), "apply");
-        const int n = p.maxiter > 0 ? p.maxiter : 0;
-        std::vector<tk_rsfe_iter_stats> st((size_t)(n > 0 ? n : 1));
-        int done = 0;
-        int r = tk_rsfe_iterate(s, n, st.data(), &done);
-        for (int t = 0; t < done; ++t) {
-            sweeps_.push_back(st[t].sweeps);
-            tols_.push_back(st[t].tollerance);
-            if (verbosity_) log_iteration(t + 1, st[t].sweeps, st[t].ls_err, st[t].tol_inf,
-                                          st[t].tollerance);
+        // with the on-the-go re-read, chunks of showfrequency iterations; after each one
+        // (t % showfrequency == 0, t != maxiter) section splitbregman/onthego/ of the file
+        const bool onthego = onthego_ && showfrequency_ > 0;
+        int r = TK_OK, t = 0;
+        for (;;) {
+            const int left = p.maxiter > t ? p.maxiter - t : 0;
+            const int want = onthego ? std::min((int)showfrequency_, left) : left;
+            std::vector<tk_rsfe_iter_stats> st((size_t)(want > 0 ? want : 1));
+            int done = 0;
+            r = tk_rsfe_iterate(s, want, st.data(), &done);
+            for (int q = 0; q < done; ++q) {
+                sweeps_.push_back(st[q].sweeps);
+                tols_.push_back(st[q].tollerance);
+                if (verbosity_) log_iteration(t + q + 1, st[q].sweeps, st[q].ls_err,
+                                              st[q].tol_inf, st[q].tollerance);
+            }
+            t += done;
+            if (r != TK_OK || !onthego || done < want || want == 0 || t == p.maxiter) break;
+            if (verbosity_)
+                std::clog << "-- Updating parameters of algorithm from file " << getpotfile_
+                          << " (section splitbregman/onthego/)" << std::endl;
+            set_param_from_getpot("onthego/");
+            tk_rsfe_params q = device_params();
+            if (q.gs_order != p.gs_order || q.acyclic != p.acyclic || q.neumann != p.neumann) {
+                tk::warning("update_param_onthego: the Poisson solver and cv of a running apply "
+                            "are kept");
+                q.gs_order = p.gs_order;
+                q.acyclic = p.acyclic;
+                q.neumann = p.neumann;
+            }
+            p = q;
+            const bool end = end_now_;
+            r = tk_rsfe_onthego(s, &p, save_current_ ? 1 : 0, end ? 1 : 0);
+            save_current_ = false;
+            if (r != TK_OK) break;
+            if (end) {
+                // iterations end here; the final dump is written (imp:1659, 1968-1978)
+                end_now_ = false;
+                break;
+            }
         }
+
         if (r == TK_OK) {
             this->phi = im3d::image3d<T>(myim.getdimx(), myim.getdimy(), myim.getdimz(),
                                          myim.gethx(), myim.gethy(), myim.gethz());
@@ -445,6 +488,7 @@ private:
     std::string getpotfile_ = "data";  // the constructor's default (imp:27)
     bool onthego_ = true;
     unsigned showfrequency_ = 0;
+    bool save_current_ = false, end_now_ = false;  // set by section splitbregman/onthego/
     conv::filtering<T> cv_;
     lapl::unsteady_poisson_functor<T> onestep_;
     std::vector<int> sweeps_;

@@ -963,25 +963,22 @@ static int auto_extract_cc(std::vector<double>& phi, int X, int Y, int Z, double
    cv = cyclic_fftw_convolution or acyclic_fftw_convolution (p->acyclic), onestep_poisson =
    DirGaussSeidel or NeuGaussSeidel (p->neumann), edge detector GRAD (on the image blurred
    with edge_detector_sigma when that is not 0), the automatic connected component
-   (p->auto_cc); no display / dump / on-the-go. */
-int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, double hx,
-                  double hy, double hz, const or_params* p, double* phi_out,
-                  int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
-{
-    return or_rsfe_apply_ckpt(myim, init, X, Y, Z, hx, hy, hz, p, phi_out, sweeps_per_iter,
-                              tol_per_iter, st, 0, nullptr, nullptr);
-}
-
-int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int Z, double hx,
-                       double hy, double hz, const or_params* p, double* phi_out,
-                       int* sweeps_per_iter, double* tol_per_iter, or_stats* st, int n_ckpt,
-                       const int* ckpt_iters, double* ckpt_phi)
+   (p->auto_cc); no display / dump.  The on-the-go re-read (update_param_onthego,
+   imp:545-578, called at 1816-1829) is one switch of the parameters to *p2 after iteration
+   t_switch's cut and connected component (t_switch 0: none). */
+namespace {
+int apply_core(const double* myim, const double* init, int X, int Y, int Z, double hx,
+               double hy, double hz, const or_params* p1, int t_switch, const or_params* p2,
+               int end_now_at_switch, double* phi_out, int* sweeps_per_iter,
+               double* tol_per_iter, or_stats* st, int n_ckpt, const int* ckpt_iters,
+               double* ckpt_phi)
 {
     double t_start = now_s();
     g_t_fft = 0;
     const long n = (long)X * Y * Z;
     const int space_dim = (Z == 1) ? 2 : 3;
     typedef std::vector<double> V;
+    const or_params* p = p1;
 
     /* initialize_phi_with_init, same-dims case (imp:99-110) */
     V phi(init, init + n);
@@ -990,65 +987,84 @@ int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int
     V divdb(n, 0.), gforshrink(n), kones(n), kmyim(n);
     std::vector<V> b(space_dim, V(n, 0.)), d(3, V(n, 0.));
 
-    /* build_gaussian_filter(phi, sigma, 3, gpa) (imp:1641-1644) */
+    /* build_gaussian_filter(phi, sigma, radius, gpa) (imp:1641-1644) and the cv it feeds:
+       cyclic_fftw_convolution, or acyclic_fftw_convolution (the reference's default) */
     int nh[3], nc[3];
-    or_gaussian_dims(Z, hx, hy, hz, p->sigma, p->radius, p->gaussian_pixel_approach, nh);
-    V tf((long)nh[0] * nh[1] * nh[2]);
-    or_gaussian_time_filter(Z, hx, hy, hz, p->sigma, p->radius, p->gaussian_pixel_approach,
-                            tf.data());
-    /* cv: cyclic_fftw_convolution, or acyclic_fftw_convolution (the reference's default) */
-    if (p->acyclic) or_acyclic_dims(X, Y, Z, nh, nc);
-    else or_cyclic_dims(X, Y, Z, nh, nc);
-    V spec((long)nc[0] * nc[1] * nc[2]);
-    or_cyclic_spectrum(tf.data(), nh, nc, spec.data());
-
+    V spec;
+    auto build = [&](double sigma, double radius) {
+        or_gaussian_dims(Z, hx, hy, hz, sigma, radius, p->gaussian_pixel_approach, nh);
+        V tf((long)nh[0] * nh[1] * nh[2]);
+        or_gaussian_time_filter(Z, hx, hy, hz, sigma, radius, p->gaussian_pixel_approach,
+                                tf.data());
+        if (p->acyclic) or_acyclic_dims(X, Y, Z, nh, nc);
+        else or_cyclic_dims(X, Y, Z, nh, nc);
+        spec.assign((long)nc[0] * nc[1] * nc[2], 0.);
+        or_cyclic_spectrum(tf.data(), nh, nc, spec.data());
+    };
     auto cv = [&](V& out, const V& in) {
         if (p->acyclic)
             or_acyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
         else
             or_cyclic_conv(spec.data(), nc, in.data(), X, Y, Z, hx, hy, hz, out.data());
     };
-    {
+    auto ones_and_image = [&]() {
         V aux(n, 1.);
         cv(kones, aux);
         V u0(myim, myim + n);
         cv(kmyim, u0);
-    }
+    };
+    build(p->sigma, p->radius);
+    ones_and_image();
     /* edge_detect (imp:196-229, called at 1655): with edge_detector_sigma != 0 the image is
        first convolved by cv with a Gaussian of that sigma (radius 3); the main filter is
-       rebuilt afterwards, so kones / kmyim above are unaffected.  (imp:1877-1882 recomputes g
-       from the same inputs when sigma_ed is not 0 or 1; the result is the same g.) */
-    if (p->edge_detector_sigma != 0) {
-        int nhe[3], nce[3];
-        or_gaussian_dims(Z, hx, hy, hz, p->edge_detector_sigma, 3., p->gaussian_pixel_approach,
-                         nhe);
-        V tfe((long)nhe[0] * nhe[1] * nhe[2]);
-        or_gaussian_time_filter(Z, hx, hy, hz, p->edge_detector_sigma, 3.,
-                                p->gaussian_pixel_approach, tfe.data());
-        if (p->acyclic) or_acyclic_dims(X, Y, Z, nhe, nce);
-        else or_cyclic_dims(X, Y, Z, nhe, nce);
-        V spece((long)nce[0] * nce[1] * nce[2]), blurred(n);
-        or_cyclic_spectrum(tfe.data(), nhe, nce, spece.data());
-        if (p->acyclic)
-            or_acyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
-        else
-            or_cyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
-        or_edge_detect(blurred.data(), X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu,
-                       gforshrink.data());
-    } else {
-        or_edge_detect(myim, X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu, gforshrink.data());
-    }
+       rebuilt afterwards, so kones / kmyim above are unaffected */
+    auto edge = [&]() {
+        if (p->edge_detector_sigma != 0) {
+            int nhe[3], nce[3];
+            or_gaussian_dims(Z, hx, hy, hz, p->edge_detector_sigma, 3.,
+                             p->gaussian_pixel_approach, nhe);
+            V tfe((long)nhe[0] * nhe[1] * nhe[2]);
+            or_gaussian_time_filter(Z, hx, hy, hz, p->edge_detector_sigma, 3.,
+                                    p->gaussian_pixel_approach, tfe.data());
+            if (p->acyclic) or_acyclic_dims(X, Y, Z, nhe, nce);
+            else or_cyclic_dims(X, Y, Z, nhe, nce);
+            V spece((long)nce[0] * nce[1] * nce[2]), blurred(n);
+            or_cyclic_spectrum(tfe.data(), nhe, nce, spece.data());
+            if (p->acyclic)
+                or_acyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
+            else
+                or_cyclic_conv(spece.data(), nce, myim, X, Y, Z, hx, hy, hz, blurred.data());
+            or_edge_detect(blurred.data(), X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu,
+                           gforshrink.data());
+        } else {
+            or_edge_detect(myim, X, Y, Z, hx, hy, hz, p->beta, p->lamda, p->nu,
+                           gforshrink.data());
+        }
+    };
+    edge();
+    /* imp:1655-1656: the previous sigma, and the previous edge_detector_sigma held in a
+       bool (so a value other than 0 and 1 always differs from it, imp:1877-1882) */
+    double old_sigma = p->sigma;
+    bool old_eds = p->edge_detector_sigma != 0;
 
-    double chosen_bc = (p->bc_dir == 0) ? 0 : p->a0;
     double t_loop0 = now_s();
     double tollerance = 1000;
     int t;
     int iters = 0, cc_extractions = 0;
-    for (t = 1; tollerance > p->tol && t <= p->maxiter; ++t) {
+    bool end_now = false;
+    for (t = 1; tollerance > p->tol && t <= p->maxiter && !end_now; ++t) {
         V phiold(phi);
+        /* sigma changed on the go: the filter (radius 3), kones and kmyim again
+           (imp:1683-1696) */
+        if (old_sigma != p->sigma) {
+            old_sigma = p->sigma;
+            build(p->sigma, 3.);
+            ones_and_image();
+        }
         V kr2(n);
         forcing(myim, phi.data(), kones.data(), kmyim.data(), divdb.data(), n, p, cv, kr2.data());
-        /* Poisson loop, imp:1762-1797 */
+        /* Poisson loop, imp:1762-1797, with the bc value of imp:1735-1743 */
+        const double chosen_bc = (p->bc_dir == 0) ? 0 : p->a0;
         double ls_err;
         int s = or_poisson_loop_bc(phi.data(), kr2.data(), X, Y, Z, hx, hy, hz, p->dt, chosen_bc,
                                    p->ls_steps, p->ls_tol, &ls_err, p->neumann);
@@ -1064,6 +1080,11 @@ int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int
             auto_extract_cc(phi, X, Y, Z, hx, hy, hz, p)) {
             ++cc_extractions;
             if (p->cc_init_variables) init_variables = true;
+        }
+        /* update_param_onthego (imp:1816-1829) */
+        if (t == t_switch) {
+            p = p2;
+            if (end_now_at_switch) end_now = true;
         }
         /* imp:1832 */
         {
@@ -1084,12 +1105,16 @@ int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int
                     std::memcpy(ckpt_phi + (long)c * n, phi.data(), sizeof(double) * n);
             continue;
         }
-        /* imp:1868-1936: d = grad(phi); b += d; d = shrink(b, g); b -= d; d -= b; div */
+        /* imp:1868-1936: d = grad(phi); b += d; (g again when the bool of the previous
+           edge_detector_sigma differs from the current one, imp:1877-1882); d = shrink(b, g);
+           b -= d; d -= b; div */
         or_grad(phi.data(), X, Y, Z, hx, hy, hz, d[0].data(), d[1].data(), d[2].data());
         for (int a = 0; a < space_dim; ++a) {
 #pragma omp parallel for
             for (long q = 0; q < n; ++q) b[a][q] += d[a][q];
         }
+        if ((double)old_eds != p->edge_detector_sigma) edge();
+        old_eds = p->edge_detector_sigma != 0;
         for (int a = 0; a < space_dim; ++a) or_shrink(b[a].data(), gforshrink.data(), n, d[a].data());
         for (int a = 0; a < space_dim; ++a) {
 #pragma omp parallel for
@@ -1114,6 +1139,33 @@ int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int
         st->t_fft = g_t_fft;
     }
     return iters;
+}
+}  // namespace
+
+int or_rsfe_apply(const double* myim, const double* init, int X, int Y, int Z, double hx,
+                  double hy, double hz, const or_params* p, double* phi_out,
+                  int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
+{
+    return apply_core(myim, init, X, Y, Z, hx, hy, hz, p, 0, p, 0, phi_out, sweeps_per_iter,
+                      tol_per_iter, st, 0, nullptr, nullptr);
+}
+
+int or_rsfe_apply_ckpt(const double* myim, const double* init, int X, int Y, int Z, double hx,
+                       double hy, double hz, const or_params* p, double* phi_out,
+                       int* sweeps_per_iter, double* tol_per_iter, or_stats* st, int n_ckpt,
+                       const int* ckpt_iters, double* ckpt_phi)
+{
+    return apply_core(myim, init, X, Y, Z, hx, hy, hz, p, 0, p, 0, phi_out, sweeps_per_iter,
+                      tol_per_iter, st, n_ckpt, ckpt_iters, ckpt_phi);
+}
+
+int or_rsfe_apply_onthego(const double* myim, const double* init, int X, int Y, int Z,
+                          double hx, double hy, double hz, const or_params* p1, int t_switch,
+                          const or_params* p2, int end_now, double* phi_out,
+                          int* sweeps_per_iter, double* tol_per_iter, or_stats* st)
+{
+    return apply_core(myim, init, X, Y, Z, hx, hy, hz, p1, t_switch, p2, end_now, phi_out,
+                      sweeps_per_iter, tol_per_iter, st, 0, nullptr, nullptr);
 }
 
 int or_rsfe_time_iters(const double* u0, int nx, int ny, int nz, const or_params* p,

@@ -148,6 +148,17 @@ int or_rsfe_apply_ckpt(const double* u0, const double* init, int nx, int ny, int
                        int* sweeps_per_iter, double* tol_per_iter, or_stats* st, int n_ckpt,
                        const int* ckpt_iters, double* ckpt_phi);
 
+/* the loop with the reference's on-the-go re-read (update_param_onthego,
+   rsfe_splitbregman_imp.hxx:545-578, called at 1816-1829) once: after iteration t_switch's
+   cut and connected component the parameters become *p2, as if section
+   splitbregman/onthego/ of the parameter file held them; a changed sigma rebuilds the filter
+   (radius 3) and kones / kmyim at the next iteration's start (1683-1696); end_now stops the
+   loop after iteration t_switch */
+int or_rsfe_apply_onthego(const double* myim, const double* init, int nx, int ny, int nz,
+                          double hx, double hy, double hz, const or_params* p1, int t_switch,
+                          const or_params* p2, int end_now, double* phi_out,
+                          int* sweeps_per_iter, double* tol_per_iter, or_stats* st);
+
 /* time n outer iterations with a fixed sweep count per iteration (CPU baseline) */
 int or_rsfe_time_iters(const double* u0, int nx, int ny, int nz, const or_params* p,
                        int n_iters, double* phi_inout, or_stats* st);

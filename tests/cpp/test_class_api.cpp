@@ -480,6 +480,79 @@ static void run_cases()
                gp.get_iterations() == 3 ? 0 : 1, 0);
         std::remove(pot.c_str());
     }
+    // ---- rsfe_splitbregman: the on-the-go re-read of the parameter file ------------------
+    // (update_param_onthego, imp:545-578, every showfrequency iterations, imp:1816-1829)
+    {
+        const int SX = 18, SY = 16, SZ = 20;
+        const long m = (long)SX * SY * SZ;
+        std::vector<double> u0(m), init(m);
+        tk_gen_standard(SX, SY, SZ, 9, 1, 2, u0.data(), init.data());
+        im3d::image3d<double> myim(SX, SY, SZ), phi0(SX, SY, SZ);
+        myim.upload(u0);
+        phi0.upload(init);
+        myim.change_range_of_intensity(255.);
+        std::vector<double> u255(u0);
+        or_change_range_of_intensity(u255.data(), m, 255., 0.);
+        or_params p1;
+        or_default_params(&p1);
+        p1.maxiter = 5;
+        p1.sigma = 1.5;
+        p1.ls_steps = 3;
+        p1.tol = 0.;
+        const std::string pot = std::string(dir) + "/onthego.pot";
+        auto run = [&](const char* onthego_section, segm::rsfe_splitbregman<double>& gp) {
+            FILE* f = std::fopen(pot.c_str(), "w");
+            if (f) {
+                std::fputs("[splitbregman]\n  [./init]\n    verbosity = 0\n    maxiter = 5\n"
+                           "    showfrequency = 2\n    sigma = 1.5\n    ls_steps = 3\n"
+                           "    tol = 0\n  [../]\n  [./onthego]\n", f);
+                std::fputs(onthego_section, f);
+                std::fputs("  [../]\n[../]\n", f);
+                std::fclose(f);
+            }
+            gp.set_cv(conv::filtering<double>(conv::cyclic_fftw_convolution<double>));
+            gp.set_getpotfile(pot);
+            gp.apply(myim, phi0);
+        };
+        // after iteration 2 (and again, unchanged, after 4): sigma, lamda1, ls_steps, dt,
+        // lamda change; the filter, kones and kmyim are built again
+        segm::rsfe_splitbregman<double> ga;
+        run("    sigma = 1.2\n    lamda1 = 1.4e-5\n    ls_steps = 5\n    dt = 0.9\n"
+            "    lamda = 0.08\n", ga);
+        or_params p2 = p1;
+        p2.sigma = 1.2;
+        p2.lamda1 = 1.4e-5;
+        p2.ls_steps = 5;
+        p2.dt = 0.9;
+        p2.lamda = 0.08;
+        std::vector<double> ref(m);
+        std::vector<int> sw(5, 0);
+        or_rsfe_apply_onthego(u255.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p1, 2, &p2, 0,
+                              ref.data(), sw.data(), nullptr, nullptr);
+        report("on the go: sigma, lamda1, ls_steps, dt, lamda from section onthego/ FP64",
+               maxabs(ga.getoutput().download(), ref), 1e-9);
+        report("on the go: sweeps per iteration 3 3 5 5 5",
+               (ga.get_sweeps() == std::vector<int>{3, 3, 5, 5, 5} &&
+                sw == std::vector<int>{3, 3, 5, 5, 5}) ? 0 : 1, 0);
+        // save_current and end_now: phi written as <outputname>_2, the loop ends after 2
+        segm::rsfe_splitbregman<double> gb;
+        const std::string og = std::string(dir) + "/og";
+        run(("    save_current = 1\n    end_now = 1\n    outputname = '" + og + "'\n").c_str(),
+            gb);
+        or_params p3 = p1;
+        p3.maxiter = 2;
+        or_rsfe_apply(u255.data(), init.data(), SX, SY, SZ, 1., 1., 1., &p3, ref.data(), nullptr,
+                      nullptr, nullptr);
+        report("on the go: end_now ends the loop after the re-read", gb.get_iterations() == 2
+                                                                     ? 0 : 1, 0);
+        report("on the go: end_now result FP64", maxabs(gb.getoutput().download(), ref), 1e-9);
+        im3d::interface<double> sv(og + "_2.mhd");
+        im3d::image3d<double> sv_im;
+        sv.convert2image3d(sv_im);
+        report("on the go: save_current writes <outputname>_2", maxabs(sv_im.download(), ref),
+               1e-9);
+        std::remove(pot.c_str());
+    }
     image3d_operator_cases();
 
     // ---- conv::direct_convolution (convolution_imp.hxx:761-812) --------------------------

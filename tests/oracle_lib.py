@@ -281,6 +281,24 @@ def rsfe_apply(u0, init, params, h=(1.0, 1.0, 1.0)):
     return phi, sw[:n].copy(), tl[:n].copy(), st
 
 
+def rsfe_apply_onthego(u0, init, p1, t_switch, p2, end_now=False, h=(1.0, 1.0, 1.0)):
+    """rsfe_apply with the on-the-go re-read once: after iteration t_switch the parameters
+    become p2 (end_now: the loop stops after that iteration)"""
+    u0 = np.ascontiguousarray(u0, np.float64)
+    init = np.ascontiguousarray(init, np.float64)
+    phi = np.zeros_like(u0)
+    m = max(p1.maxiter, p2.maxiter, 1)
+    sw = np.zeros(m, np.int32)
+    tl = np.zeros(m, np.float64)
+    st = OrStats()
+    lib().or_rsfe_apply_onthego.restype = C.c_int
+    n = lib().or_rsfe_apply_onthego(dp(u0), dp(init), *[C.c_int(s) for s in u0.shape],
+                                    *[C.c_double(x) for x in h], C.byref(p1), C.c_int(t_switch),
+                                    C.byref(p2), C.c_int(int(end_now)), dp(phi), ip(sw), dp(tl),
+                                    C.byref(st))
+    return phi, sw[:n].copy(), tl[:n].copy(), st
+
+
 def rsfe_forcing(u0, phi, divdb, params, h=(1.0, 1.0, 1.0)):
     u0 = np.ascontiguousarray(u0, np.float64)
     phi = np.ascontiguousarray(phi, np.float64)

@@ -52,6 +52,52 @@ def test_loop_f64_matches_oracle_small(tk, orc, ctx, shape, ls_tol):
     assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
 
 
+@pytest.mark.parametrize("shape,sigma2", [((24, 20, 28), 1.6), ((32, 32, 32), 3.0)])
+def test_onthego_f64_matches_oracle(tk, orc, ctx, shape, sigma2, tmp_path):
+    """tk_rsfe_onthego, the on-the-go re-read (rsfe_splitbregman_imp.hxx:545-578, called at
+    1816-1829) after iteration 2: sigma (the filter, kones and kmyim again), lamda1,
+    ls_steps, dt and lamda change; FP64 lexicographic against the oracle's switch; then
+    save_current and end_now; a structural change is refused"""
+    say("onthego f64", shape, sigma2)
+    u0, phi0 = tk.gen_standard(shape, 5, 1, 2)
+    p1 = orc.default_params(sigma=2.0, maxiter=5, tol=0.0, ls_steps=4, ls_tol=0.0)
+    p2 = orc.default_params(sigma=sigma2, maxiter=5, tol=0.0, ls_steps=6, ls_tol=0.0,
+                            lamda1=1.4e-5, dt=0.9, lamda=0.08)
+    ref, sw, tl, _ = orc.rsfe_apply_onthego(orc.change_range(u0, 255.0, 0.0), phi0, p1, 2, p2)
+    vu = ctx.from_numpy(u0)
+    tk._check(tk.lib.tk_change_range_of_intensity(ctx.h, vu.h, 255.0, 0.0))
+    vi = ctx.from_numpy(phi0)
+    q1 = tk.default_params(sigma=2.0, maxiter=5, tol=0.0, ls_steps=4, ls_tol=0.0,
+                           gs_order=tk.TK_GS_LEX)
+    q2 = tk.default_params(sigma=sigma2, maxiter=5, tol=0.0, ls_steps=6, ls_tol=0.0,
+                           lamda1=1.4e-5, dt=0.9, lamda=0.08, gs_order=tk.TK_GS_LEX)
+    r = tk.Rsfe(ctx, vu, vi, q1)
+    st = r.iterate(2)
+    r.onthego(q2)
+    st += r.iterate(5)
+    phi = r.phi()
+    assert [s["sweeps"] for s in st] == list(sw) == [4, 4, 6, 6, 6]
+    err = np.max(np.abs(phi - ref))
+    say(f"onthego {shape}: max|phi-ref| = {err:.3e}")
+    assert err < 1e-9
+    assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
+    r.close()
+    # save_current writes <outputname>_2; end_now ends the run
+    out = str(tmp_path / "og")
+    r = tk.Rsfe(ctx, vu, vi, q1)
+    r.iterate(2)
+    q3 = tk.default_params(sigma=2.0, maxiter=5, tol=0.0, ls_steps=4, ls_tol=0.0,
+                           gs_order=tk.TK_GS_LEX, outputname=out)
+    r.onthego(q3, save_current=True, end_now=True)
+    assert r.iterate(3) == []
+    saved = ctx.read_mhd(out + "_2.mhd").numpy()
+    assert np.array_equal(saved, r.phi())
+    with pytest.raises(tk.TkError):
+        r.onthego(tk.default_params(sigma=2.0, maxiter=5, tol=0.0, ls_steps=4, ls_tol=0.0,
+                                    gs_order=tk.TK_GS_REDBLACK))
+    r.close()
+
+
 @pytest.mark.parametrize("shape,sigma", [((24, 20, 28), 2.0), ((32, 32, 32), 3.0),
                                          ((32, 28, 36), 3.0)])
 def test_loop_acyclic_f64_matches_oracle(tk, orc, ctx, shape, sigma):

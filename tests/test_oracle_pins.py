@@ -776,6 +776,62 @@ def test_apply_blurred_edge_detector_matches_composition(orc, tk, acyclic):
     assert np.array_equal(phi, phi_or)
 
 
+def test_apply_onthego_matches_composition(orc, tk):
+    """the on-the-go re-read (update_param_onthego, rsfe_splitbregman_imp.hxx:545-578, called
+    at 1816-1829): after iteration 2 the parameters change (sigma: the filter, kones and kmyim
+    again at the next iteration's start, 1683-1696; lamda1, ls_steps, dt; lamda, which g does
+    not see again: edge_detector_sigma stays 0, 1877-1882); then edge_detector_sigma becomes
+    1.5 (g again, blurred, with the new lamda, at the same iteration's update); end_now stops
+    the loop; p2 = p1 gives the plain run bitwise"""
+    shape = (16, 14, 12)
+    u0, phi0 = tk.gen_standard(shape, 7, 1, 2)
+    p1 = orc.default_params(sigma=1.5, maxiter=4, tol=0.0, ls_steps=3, ls_tol=0.0)
+    plain = orc.rsfe_apply(u0, phi0, p1)[0]
+    assert np.array_equal(orc.rsfe_apply_onthego(u0, phi0, p1, 2, p1)[0], plain)
+    stop = orc.rsfe_apply_onthego(u0, phi0, p1, 2, p1, end_now=True)
+    assert stop[3].iters == 2
+    p1b = orc.default_params(sigma=1.5, maxiter=2, tol=0.0, ls_steps=3, ls_tol=0.0)
+    assert np.array_equal(stop[0], orc.rsfe_apply(u0, phi0, p1b)[0])
+
+    def composed(p2, g2):
+        phi = orc.change_range(phi0, 1.0, 0.0)
+        divdb = np.zeros(shape)
+        b = [np.zeros(shape) for _ in range(3)]
+        g = orc.edge_detect(u0, p1.beta, p1.lamda, p1.nu)
+        sweeps = []
+        for t in range(1, 5):
+            p = p1 if t <= 2 else p2
+            rhs = orc.rsfe_forcing(u0, phi, divdb, p)
+            phi, s, _ = orc.poisson_loop(phi, rhs, p.ls_steps, 0.0, dt=p.dt)
+            sweeps.append(s)
+            phi = np.clip(phi, 0.0, 1.0)
+            if t == 2 and g2 is not None:
+                g = g2
+            d = orc.grad(phi)
+            for a in range(3):
+                b[a] = b[a] + d[a]
+            d = [orc.shrink(b[a], g) for a in range(3)]
+            for a in range(3):
+                b[a] = b[a] - d[a]
+                d[a] = d[a] - b[a]
+            divdb = orc.div(*d)
+        return phi, sweeps
+
+    p2 = orc.default_params(sigma=1.2, maxiter=4, tol=0.0, ls_steps=5, ls_tol=0.0,
+                            lamda1=1.4e-5, dt=0.9, lamda=0.08)
+    phi_or, sw, _, st = orc.rsfe_apply_onthego(u0, phi0, p1, 2, p2)
+    phi_c, sw_c = composed(p2, None)
+    assert st.iters == 4 and list(sw) == sw_c == [3, 3, 5, 5]
+    assert np.array_equal(phi_or, phi_c)
+    assert np.max(np.abs(phi_or - plain)) > 1e-6
+    p3 = orc.default_params(sigma=1.5, maxiter=4, tol=0.0, ls_steps=3, ls_tol=0.0,
+                            lamda=0.08, edge_detector_sigma=1.5)
+    tf, nh, spec, nc = orc.gaussian(shape, 1.5)
+    g3 = orc.edge_detect(orc.cyclic_conv(spec, nc, u0), p3.beta, p3.lamda, p3.nu)
+    phi_or = orc.rsfe_apply_onthego(u0, phi0, p1, 2, p3)[0]
+    assert np.array_equal(phi_or, composed(p3, g3)[0])
+
+
 def test_init_phi_cases(orc):
     """initialize_phi_with_init / _with_cube (rsfe_splitbregman_imp.hxx:48-170), pinned by
     closed forms: same dims = min-max rescale; one voxel short = edge padding (numpy) then
