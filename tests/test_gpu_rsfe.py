@@ -52,6 +52,31 @@ def test_loop_f64_matches_oracle_small(tk, orc, ctx, shape, ls_tol):
     assert np.allclose([s["tollerance"] for s in st], tl, rtol=1e-9)
 
 
+@pytest.mark.parametrize("dtype,sigma,acyclic", [(np.float64, 2.0, 0), (np.float64, 3.0, 1),
+                                                 (np.float32, 3.0, 0), (np.float32, 6.0, 0)])
+def test_forcing_term_vs_oracle(tk, orc, ctx, dtype, sigma, acyclic):
+    """R6 + R7 on their own: the forcing term of the first outer iteration (Heaviside,
+    the convolutions of H u0 and H, the region fit, the second pair of convolutions and the
+    forcing, rsfe_splitbregman_imp.hxx:1700-1759, with divdb = 0), read back with
+    tk_rsfe_rhs, against the oracle's or_rsfe_forcing"""
+    shape = (32, 28, 36)
+    say("forcing", np.dtype(dtype).name, sigma, "acyclic" if acyclic else "cyclic")
+    u0, phi0 = tk.gen_standard(shape, 11, 1, 2)
+    u255 = orc.change_range(u0, 255.0, 0.0)
+    p = orc.default_params(sigma=sigma, acyclic=acyclic)
+    ref = orc.rsfe_forcing(u255, orc.change_range(phi0, 1.0, 0.0), np.zeros(shape), p)
+    r = tk.Rsfe(ctx, ctx.from_numpy(u255.astype(dtype)), ctx.from_numpy(phi0.astype(dtype)),
+                tk.default_params(sigma=sigma, maxiter=1, tol=0.0, ls_steps=1, ls_tol=0.0,
+                                  gs_order=tk.TK_GS_LEX, acyclic=acyclic))
+    r.iterate(1)
+    rhs = r.rhs()
+    r.close()
+    err = np.max(np.abs(rhs - ref)) / np.max(np.abs(ref))
+    say(f"forcing: max |rhs - ref| / max |ref| = {err:.3e}")
+    # FP64: convolution rounding only; FP32: single-precision sums over 2 x 3 axes of taps
+    assert err < (1e-11 if dtype == np.float64 else 2e-5)
+
+
 @pytest.mark.parametrize("shape,sigma2", [((24, 20, 28), 1.6), ((32, 32, 32), 3.0)])
 def test_onthego_f64_matches_oracle(tk, orc, ctx, shape, sigma2, tmp_path):
     """tk_rsfe_onthego, the on-the-go re-read (rsfe_splitbregman_imp.hxx:545-578, called at
